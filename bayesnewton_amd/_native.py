@@ -1,0 +1,89 @@
+"""ctypes binding of libbn_hip.so (the C ABI declared in include/bn_hip.h).
+
+The library is the only compute path: if it is missing, every entry point
+raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libbn_hip.so")
+
+BN_MATERN12, BN_MATERN32, BN_MATERN52 = 1, 2, 3
+BN_RETURN_PREDICT = 0x1
+BN_RETURN_FULL = 0x2
+
+vp = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+u32 = ctypes.c_uint32
+f32 = ctypes.c_float
+f64 = ctypes.c_double
+u8 = ctypes.c_uint8
+
+
+class KernelDesc(ctypes.Structure):
+    _fields_ = [("n_comp", i32), ("kind", i32 * 2)]
+
+
+# name -> (restype, argtypes); every symbol include/bn_hip.h declares
+SIGNATURES = {
+    "bn_last_error": (ctypes.c_char_p, []),
+    "bn_version": (i32, []),
+    "bn_plan_create": (i32, [vp, i32, i32, ctypes.POINTER(vp)]),
+    "bn_plan_destroy": (i32, [vp]),
+    "bn_plan_num_slots": (i64, [vp]),
+    "bn_plan_num_chunks": (i32, [vp]),
+    "bn_plan_num_tiles": (i32, [vp]),
+    "bn_workspace_bytes": (i64, [vp, vp]),
+    "bn_plan_gather_f32": (i32, [vp, vp, vp, i32, f32, vp]),
+    "bn_plan_scatter_f32": (i32, [vp, vp, vp, i32, vp]),
+    "bn_plan_gather_f64": (i32, [vp, vp, vp, i32, f64, vp]),
+    "bn_plan_scatter_f64": (i32, [vp, vp, vp, i32, vp]),
+    "bn_plan_gather_u8": (i32, [vp, vp, vp, u8, vp]),
+    "bn_discretise": (i32, [vp, vp, vp, i64, vp, vp, vp, vp]),
+    "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    "bn_rts_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libbn_hip.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(
+                f"{LIB_PATH} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().bn_last_error().decode(errors="replace")
+        raise NativeError(f"libbn_hip error {rc}: {msg}")
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

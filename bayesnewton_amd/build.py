@@ -1,0 +1,45 @@
+"""Build recipe for libbn_hip.so (gfx950).  `python -m bayesnewton_amd.build`.
+
+Compiles the HIP sources in-tree with hipcc; the library lands in
+bayesnewton_amd/_lib/ so it travels with the source tree.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(OUT_DIR, "libbn_hip.so")
+SOURCES = ["bn_markov.hip"]
+ARCH = os.environ.get("BN_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions stay disabled (-packed-fp32-ops): this code is
+# FP64-dominated anyway and the packed forms are not trusted on this platform.
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose=True, extra=()):
+    os.makedirs(OUT_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + ["-o", tmp] + srcs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(extra=sys.argv[1:])

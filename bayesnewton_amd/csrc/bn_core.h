@@ -1,0 +1,653 @@
+// Per-lane math of the Markov-GP hot path: state-space discretisation, the
+// Kalman filter / RTS smoother step, and the associative elements with their
+// combine operators.  Everything here is __host__ __device__ so the same code
+// runs in the HIP kernels (one lane = one chunk of time steps) and in a host
+// build used to check the kernels' data flow on the CPU.
+//
+// State-space models are sums of Matern components (kernels.py:123-300,
+// Sum 1624-1663): the component "kind" is its state dimension (1 = Matern12,
+// 2 = Matern32, 3 = Matern52).  A, Q and Pinf are block diagonal, so every
+// product below skips the zero blocks at compile time.  Observations are
+// scalar per step (p = 1): H sums the first state of each component.
+#pragma once
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define BN_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define BN_HD inline
+#endif
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define BN_QTAB __constant__ static const
+#else
+#define BN_QTAB static const
+#endif
+#include "bn_qtaylor.h"
+
+namespace bn {
+
+#if defined(__HIPCC__)
+#define BN_UNROLL _Pragma("unroll")
+#else
+#define BN_UNROLL
+#endif
+
+static constexpr double kLog2Pi = 1.8378770664093453;
+#ifndef BN_SEMICHOL_RTOL
+#define BN_SEMICHOL_RTOL 1e-9
+#endif
+
+// ---------------------------------------------------------------------------
+// Model structure
+// ---------------------------------------------------------------------------
+template <int K1, int K2>
+struct Spec {
+  static constexpr int D = K1 + K2;
+  static constexpr int NS = D * (D + 1) / 2;  // packed symmetric size
+  static constexpr BN_HD int blk(int i) { return i < K1 ? 0 : 1; }
+  static constexpr BN_HD bool nz(int i, int j) { return blk(i) == blk(j); }
+  static constexpr BN_HD bool h(int i) { return i == 0 || (K2 > 0 && i == K1); }
+  static constexpr BN_HD int sidx(int i, int j) {  // packed upper index, i <= j
+    return i * D - (i * (i - 1)) / 2 + (j - i);
+  }
+};
+
+// Closed-form A = expm(F dt) of one component (kernels.py:156-163, 216-224, 273-286).
+template <int K>
+BN_HD void comp_transition(double lam, double dt, double* A) {
+  if (K == 1) {
+    A[0] = exp(-dt * lam);
+  } else if (K == 2) {
+    double e = exp(-dt * lam);
+    A[0] = e * (dt * lam + 1.0);
+    A[1] = e * dt;
+    A[2] = e * (-dt * lam * lam);
+    A[3] = e * (-dt * lam + 1.0);
+  } else {
+    double dl = dt * lam, e = exp(-dl);
+    A[0] = e * (dt * lam * (0.5 * dl + 1.0) + 1.0);
+    A[1] = e * (dt * (dl + 1.0));
+    A[2] = e * (dt * 0.5 * dt);
+    A[3] = e * (dt * (-0.5 * dl * lam * lam));
+    A[4] = e * (dt * lam * (1.0 - dl) + 1.0);
+    A[5] = e * (dt * (1.0 - 0.5 * dl));
+    A[6] = e * (dt * lam * lam * lam * (0.5 * dl - 1.0));
+    A[7] = e * (dt * lam * lam * (dl - 3.0));
+    A[8] = e * (dt * lam * (0.5 * dl - 2.0) + 1.0);
+  }
+}
+
+// Stationary covariance of one component (kernels.py:144-146, 207-210, 288-293).
+template <int K>
+BN_HD void comp_pinf(double ls, double var, double* P) {
+  if (K == 1) {
+    P[0] = var;
+  } else if (K == 2) {
+    P[0] = var; P[1] = 0.0; P[2] = 0.0; P[3] = 3.0 * var / (ls * ls);
+  } else {
+    double kap = 5.0 / 3.0 * var / (ls * ls);
+    P[0] = var; P[1] = 0.0; P[2] = -kap;
+    P[3] = 0.0; P[4] = kap; P[5] = 0.0;
+    P[6] = -kap; P[7] = 0.0; P[8] = 25.0 * var / (ls * ls * ls * ls);
+  }
+}
+
+template <int K>
+BN_HD double comp_lambda(double ls) {
+  return K == 1 ? 1.0 / ls : (K == 2 ? sqrt(3.0) / ls : sqrt(5.0) / ls);
+}
+
+// Row `idx` (packed upper index) of the Taylor or direct-form polynomial table.
+template <int K>
+BN_HD const double* qtab(int idx, bool small) {
+  if (K == 1) return small ? kQTaylor1[idx] : kQPoly1[idx];
+  if (K == 2) return small ? kQTaylor2[idx] : kQPoly2[idx];
+  return small ? kQTaylor3[idx] : kQPoly3[idx];
+}
+template <int K>
+BN_HD double qc(int idx) {
+  return K == 1 ? kQC1[idx] : (K == 2 ? kQC2[idx] : kQC3[idx]);
+}
+template <int K>
+BN_HD constexpr int qpolydeg() {
+  return K == 1 ? kQPolyDeg1 : (K == 2 ? kQPolyDeg2 : kQPolyDeg3);
+}
+
+// Per-series model: hypers and Pinf, in FP64.
+template <int K1, int K2>
+struct Model {
+  using S = Spec<K1, K2>;
+  static constexpr int D = S::D;
+  double lam1, lam2, var1, var2;
+  double Pinf[D][D];
+
+  BN_HD void init(double ls1, double v1, double ls2, double v2) {
+    var1 = v1;
+    var2 = v2;
+    lam1 = comp_lambda<K1>(ls1);
+    lam2 = K2 > 0 ? comp_lambda<(K2 > 0 ? K2 : 1)>(ls2) : 0.0;
+    double b1[9], b2[9];
+    comp_pinf<K1>(ls1, v1, b1);
+    if (K2 > 0) comp_pinf<(K2 > 0 ? K2 : 1)>(ls2, v2, b2);
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = 0; j < D; ++j) {
+        double v = 0.0;
+        if (i < K1 && j < K1) v = b1[i * K1 + j];
+        else if (i >= K1 && j >= K1) v = b2[(i - K1) * (K2 > 0 ? K2 : 1) + (j - K1)];
+        Pinf[i][j] = v;
+      }
+  }
+
+  // A(dt) and Q(dt) = Pinf - A Pinf A' (ops.py:149-151) in FP64, rounded to T.
+  // Q comes from the cancellation-free form Q_ij = var lam^(i+j) F_ij(lam dt)
+  // (bn_qtaylor.h): evaluating Pinf - A Pinf A' as written leaves only
+  // rounding noise in the O(dt^5) entries when dt << lengthscale, which breaks
+  // one-step scan elements.  Only the nonzero blocks are meaningful.
+  template <class T>
+  BN_HD void discretise(double dt, T A[D][D], T Q[D][D]) const {
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = 0; j < D; ++j) { A[i][j] = T(0); Q[i][j] = T(0); }
+    comp_block<K1, 0>(lam1, var1, dt, A, Q);
+    if (K2 > 0) comp_block<(K2 > 0 ? K2 : 1), K1>(lam2, var2, dt, A, Q);
+  }
+
+  template <int K, int O, class T>
+  BN_HD static void comp_block(double lam, double var, double dt, T A[D][D], T Q[D][D]) {
+    double a[9];
+    comp_transition<K>(lam, dt, a);
+    BN_UNROLL for (int i = 0; i < K; ++i)
+      BN_UNROLL for (int j = 0; j < K; ++j) A[O + i][O + j] = (T)a[i * K + j];
+    const double x = lam * dt;
+    const bool small = x < kQX0;
+    const double e = exp(-x);
+    const double e2 = e * e;
+    int idx = 0;
+    double lp_i = var;  // var * lam^(i + j), built incrementally
+    BN_UNROLL for (int i = 0; i < K; ++i) {
+      double lp = lp_i;
+      BN_UNROLL for (int j = i; j < K; ++j) {
+        // Horner over a table row read from constant memory (a runtime loop, so
+        // the coefficients are not hoisted into registers across time steps)
+        const double* tab = qtab<K>(idx, small);
+        const int nt = small ? kQTerms : qpolydeg<K>();
+        double v = tab[nt - 1];
+#if defined(__HIPCC__)
+#pragma unroll 1
+#endif
+        for (int n = nt - 2; n >= 0; --n) v = v * x + tab[n];
+        if (!small) v = qc<K>(idx) - e2 * v;
+        const T q = (T)(lp * v);
+        Q[O + i][O + j] = q;
+        Q[O + j][O + i] = q;
+        lp *= lam;
+        ++idx;
+      }
+      lp_i *= lam * lam;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Small dense helpers (fixed size, fully unrolled)
+// ---------------------------------------------------------------------------
+template <int D, class T>
+BN_HD void set_identity(T A[D][D]) {
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) A[i][j] = (i == j) ? T(1) : T(0);
+}
+
+template <int D, class T>
+BN_HD void set_zero(T A[D][D]) {
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) A[i][j] = T(0);
+}
+
+// C = A B
+template <int D, class T>
+BN_HD void mat_mul(const T A[D][D], const T B[D][D], T C[D][D]) {
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += A[i][k] * B[k][j];
+      C[i][j] = s;
+    }
+}
+
+// C = A B', symmetric result assumed: fills upper and mirrors
+template <int D, class T>
+BN_HD void mat_mul_bt_sym(const T A[D][D], const T B[D][D], T C[D][D]) {
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += A[i][k] * B[j][k];
+      C[i][j] = s;
+      C[j][i] = s;
+    }
+}
+
+// Semidefinite Cholesky in FP64: A = U U' with U lower; a pivot at or below
+// rtol * A[j][j] (rounding noise of an FP32 PSD matrix) becomes a zero column.
+template <int D>
+BN_HD void semichol(double A[D][D], double rtol) {
+  BN_UNROLL for (int j = 0; j < D; ++j) {
+    double s = A[j][j];
+    double d0 = s;
+    BN_UNROLL for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
+    bool ok = s > rtol * (d0 > 0.0 ? d0 : 0.0) && s > 0.0;
+    double l = ok ? sqrt(s) : 0.0;
+    double il = ok ? 1.0 / l : 0.0;
+    A[j][j] = l;
+    BN_UNROLL for (int i = j + 1; i < D; ++i) {
+      double t = A[i][j];
+      BN_UNROLL for (int k = 0; k < j; ++k) t -= A[i][k] * A[j][k];
+      A[i][j] = t * il;
+    }
+    BN_UNROLL for (int i = 0; i < j; ++i) A[i][j] = 0.0;
+  }
+}
+
+// Cholesky of an SPD matrix in FP64 (lower, in place).  Returns false when a
+// pivot is not positive (the reference then yields NaN; callers propagate it).
+template <int D>
+BN_HD bool chol_spd(double A[D][D]) {
+  bool ok = true;
+  BN_UNROLL for (int j = 0; j < D; ++j) {
+    double s = A[j][j];
+    BN_UNROLL for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
+    ok = ok && (s > 0.0);
+    double l = sqrt(s);
+    double il = 1.0 / l;
+    A[j][j] = l;
+    BN_UNROLL for (int i = j + 1; i < D; ++i) {
+      double t = A[i][j];
+      BN_UNROLL for (int k = 0; k < j; ++k) t -= A[i][k] * A[j][k];
+      A[i][j] = t * il;
+    }
+  }
+  return ok;
+}
+
+// X <- L^-1 X (forward substitution), L lower FP64, X has D rows and NC columns.
+template <int D, int NC, class T>
+BN_HD void lower_solve(const double L[D][D], T X[D][NC]) {
+  BN_UNROLL for (int c = 0; c < NC; ++c)
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      double s = (double)X[i][c];
+      BN_UNROLL for (int k = 0; k < i; ++k) s -= L[i][k] * (double)X[k][c];
+      X[i][c] = (T)(s / L[i][i]);
+    }
+}
+
+// X <- L'^-1 X (backward substitution with L').
+template <int D, int NC, class T>
+BN_HD void upper_t_solve(const double L[D][D], T X[D][NC]) {
+  BN_UNROLL for (int c = 0; c < NC; ++c)
+    BN_UNROLL for (int i = D - 1; i >= 0; --i) {
+      double s = (double)X[i][c];
+      BN_UNROLL for (int k = i + 1; k < D; ++k) s -= L[k][i] * (double)X[k][c];
+      X[i][c] = (T)(s / L[i][i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Associative filtering element (ops.py:183-219), kept as
+//   x_e | x_{s-1} ~ N(Phi x + beta, Sig),  log p(y_s..e | x_{s-1}) = -x'Jx/2 + eta'x + c
+// A "state" (a filtered marginal) is an element with Phi = 0, J = 0, eta = 0.
+// ---------------------------------------------------------------------------
+template <int D, class T>
+struct FElem {
+  T Phi[D][D];
+  T beta[D];
+  T Sig[D][D];
+  T J[D][D];
+  T eta[D];
+  static constexpr int NF = D * D + 2 * D + D * (D + 1);  // stored floats
+
+  BN_HD void set_identity_elem() {
+    set_identity<D>(Phi);
+    set_zero<D>(Sig);
+    set_zero<D>(J);
+    BN_UNROLL for (int i = 0; i < D; ++i) { beta[i] = T(0); eta[i] = T(0); }
+  }
+  // field f of the packed storage order: Phi, beta, Sig(upper), J(upper), eta
+  template <class Store>
+  BN_HD void store(Store st) const {
+    int f = 0;
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) st(f++, Phi[i][j]);
+    BN_UNROLL for (int i = 0; i < D; ++i) st(f++, beta[i]);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) st(f++, Sig[i][j]);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) st(f++, J[i][j]);
+    BN_UNROLL for (int i = 0; i < D; ++i) st(f++, eta[i]);
+  }
+  template <class Load>
+  BN_HD void load(Load ld) {
+    int f = 0;
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) Phi[i][j] = ld(f++);
+    BN_UNROLL for (int i = 0; i < D; ++i) beta[i] = ld(f++);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) { T v = ld(f++); Sig[i][j] = v; Sig[j][i] = v; }
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) { T v = ld(f++); J[i][j] = v; J[j][i] = v; }
+    BN_UNROLL for (int i = 0; i < D; ++i) eta[i] = ld(f++);
+  }
+};
+
+// Shared factorisation of T = (I + Sig1 J2)^-1 (the reference's
+// solve(inv(C1) + J2, inv(C1)), ops.py:208-209) without inverting Sig1:
+// Sig1 = U U', W = I + U' J2 U = Wc Wc', Z = Wc^-1 U', Y = Wc^-1 U' J2.  Then
+//   T X = X - Z'(Y X),  T' x = x - Y'(Z x),  T Sig1 = Z'Z,  J2 T = J2 - Y'Y.
+// W has eigenvalues >= 1, so both factorisations are well conditioned.
+template <int D, class T>
+BN_HD void combine_factor(const T Sig1[D][D], const T J2[D][D], T Z[D][D], T Y[D][D]) {
+  double U[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) U[i][j] = (double)Sig1[i][j];
+  semichol<D>(U, BN_SEMICHOL_RTOL);
+  // V = J2 U ; W = I + U' V
+  double V[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      double s = 0.0;
+      BN_UNROLL for (int k = j; k < D; ++k) s += (double)J2[i][k] * U[k][j];
+      V[i][j] = s;
+    }
+  double W[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      double s = (i == j) ? 1.0 : 0.0;
+      BN_UNROLL for (int k = i; k < D; ++k) s += U[k][i] * V[k][j];
+      W[i][j] = s;
+      W[j][i] = s;
+    }
+  chol_spd<D>(W);
+  // Z = Wc^-1 U', Y = Wc^-1 V'
+  double Zd[D][D], Yd[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) { Zd[i][j] = U[j][i]; Yd[i][j] = V[j][i]; }
+  lower_solve<D, D, double>(W, Zd);
+  lower_solve<D, D, double>(W, Yd);
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) { Z[i][j] = (T)Zd[i][j]; Y[i][j] = (T)Yd[i][j]; }
+}
+
+// e = e1 (earlier) followed by e2 (later): parallel_filtering_operator, ops.py:203-219.
+template <int D, class T>
+BN_HD void fcombine(const FElem<D, T>& e1, const FElem<D, T>& e2, FElem<D, T>& out) {
+  T Z[D][D], Y[D][D];
+  combine_factor<D, T>(e1.Sig, e2.J, Z, Y);
+  // T Phi1 = Phi1 - Z'(Y Phi1)
+  T YP[D][D], TP[D][D];
+  mat_mul<D, T>(Y, e1.Phi, YP);
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      T s = e1.Phi[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * YP[k][j];
+      TP[i][j] = s;
+    }
+  // w = beta1 + Sig1 eta2 ; Tw = w - Z'(Y w)
+  T w[D], yw[D], Tw[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = e1.beta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Sig[i][k] * e2.eta[k];
+    w[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = T(0);
+    BN_UNROLL for (int k = 0; k < D; ++k) s += Y[i][k] * w[k];
+    yw[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = w[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * yw[k];
+    Tw[i] = s;
+  }
+  // q = eta2 - J2 beta1 ; Ttq = q - Y'(Z q)
+  T q[D], zq[D], Ttq[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = e2.eta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s -= e2.J[i][k] * e1.beta[k];
+    q[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = T(0);
+    BN_UNROLL for (int k = 0; k < D; ++k) s += Z[i][k] * q[k];
+    zq[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = q[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s -= Y[k][i] * zq[k];
+    Ttq[i] = s;
+  }
+  // R = Phi2 Z' ; Sig = R R' + Sig2
+  T R[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += e2.Phi[i][k] * Z[j][k];
+      R[i][j] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = e2.Sig[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += R[i][k] * R[j][k];
+      out.Sig[i][j] = s;
+      out.Sig[j][i] = s;
+    }
+  // beta = Phi2 Tw + beta2 ; Phi = Phi2 TP
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = e2.beta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += e2.Phi[i][k] * Tw[k];
+    out.beta[i] = s;
+  }
+  mat_mul<D, T>(e2.Phi, TP, out.Phi);
+  // eta = Phi1' Ttq + eta1
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = e1.eta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Phi[k][i] * Ttq[k];
+    out.eta[i] = s;
+  }
+  // J = Phi1' J2 Phi1 - (Y Phi1)'(Y Phi1) + J1
+  T JP[D][D];
+  mat_mul<D, T>(e2.J, e1.Phi, JP);
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = e1.J[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Phi[k][i] * JP[k][j] - YP[k][i] * YP[k][j];
+      out.J[i][j] = s;
+      out.J[j][i] = s;
+    }
+}
+
+// (m, P) <- state (m, P) followed by element e: the combine with Phi1 = 0.
+template <int D, class T>
+BN_HD void fapply(T m[D], T P[D][D], const FElem<D, T>& e) {
+  T Z[D][D], Y[D][D];
+  combine_factor<D, T>(P, e.J, Z, Y);
+  T w[D], yw[D], Tw[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = m[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += P[i][k] * e.eta[k];
+    w[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = T(0);
+    BN_UNROLL for (int k = 0; k < D; ++k) s += Y[i][k] * w[k];
+    yw[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = w[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * yw[k];
+    Tw[i] = s;
+  }
+  T R[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += e.Phi[i][k] * Z[j][k];
+      R[i][j] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = e.beta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += e.Phi[i][k] * Tw[k];
+    m[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = e.Sig[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += R[i][k] * R[j][k];
+      P[i][j] = s;
+      P[j][i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Associative smoothing element (ops.py:318-335):
+//   x_s | x_{e+1}, y_{1:N} ~ N(E x_{e+1} + g, L).   A state has E = 0.
+// ---------------------------------------------------------------------------
+template <int D, class T>
+struct SElem {
+  T E[D][D];
+  T g[D];
+  T L[D][D];
+  static constexpr int NF = D * D + D + D * (D + 1) / 2;
+
+  BN_HD void set_identity_elem() {
+    set_identity<D>(E);
+    set_zero<D>(L);
+    BN_UNROLL for (int i = 0; i < D; ++i) g[i] = T(0);
+  }
+  template <class Store>
+  BN_HD void store(Store st) const {
+    int f = 0;
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) st(f++, E[i][j]);
+    BN_UNROLL for (int i = 0; i < D; ++i) st(f++, g[i]);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) st(f++, L[i][j]);
+  }
+  template <class Load>
+  BN_HD void load(Load ld) {
+    int f = 0;
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) E[i][j] = ld(f++);
+    BN_UNROLL for (int i = 0; i < D; ++i) g[i] = ld(f++);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = i; j < D; ++j) { T v = ld(f++); L[i][j] = v; L[j][i] = v; }
+  }
+};
+
+// out = a (earlier in time) composed with b (later): parallel_smoothing_operator
+// (ops.py:328-335) with the reference's elem1 = b, elem2 = a.
+template <int D, class T>
+BN_HD void scombine(const SElem<D, T>& a, const SElem<D, T>& b, SElem<D, T>& out) {
+  T EL[D][D];
+  mat_mul<D, T>(a.E, b.L, EL);
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = a.L[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += EL[i][k] * a.E[j][k];
+      out.L[i][j] = s;
+      out.L[j][i] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = a.g[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += a.E[i][k] * b.g[k];
+    out.g[i] = s;
+  }
+  mat_mul<D, T>(a.E, b.E, out.E);
+}
+
+// (m, P) <- element a applied to the later state (m, P).
+template <int D, class T>
+BN_HD void sapply(const SElem<D, T>& a, T m[D], T P[D][D]) {
+  T EL[D][D], nm[D];
+  mat_mul<D, T>(a.E, P, EL);
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = a.g[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += a.E[i][k] * m[k];
+    nm[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = a.L[i][j];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += EL[i][k] * a.E[j][k];
+      P[i][j] = s;
+      P[j][i] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
+}
+
+// ---------------------------------------------------------------------------
+// One step of the filter / smoother on the block-structured model
+// ---------------------------------------------------------------------------
+
+// mp = A m ; Pp = A P A' + Q   (ops.py:160-161)
+template <int K1, int K2, class T>
+BN_HD void predict(const T A[Spec<K1, K2>::D][Spec<K1, K2>::D], const T Q[Spec<K1, K2>::D][Spec<K1, K2>::D],
+                   const T m[Spec<K1, K2>::D], const T P[Spec<K1, K2>::D][Spec<K1, K2>::D],
+                   T mp[Spec<K1, K2>::D], T Pp[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  T AP[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = T(0);
+    BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * m[k];
+    mp[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * P[k][j];
+      AP[i][j] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = S::nz(i, j) ? Q[i][j] : T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(j, k)) s += AP[i][k] * A[j][k];
+      Pp[i][j] = s;
+      Pp[j][i] = s;
+    }
+}
+
+// Measurement update with a scalar observation y, variance r (ops.py:163-171).
+// Returns the innovation log-density log N(y; H mp, S) in FP64 (utils.py:376-396);
+// the caller drops it for masked steps.
+template <int K1, int K2, class T>
+BN_HD double update(const T mp[Spec<K1, K2>::D], const T Pp[Spec<K1, K2>::D][Spec<K1, K2>::D], T y, T r,
+                    T m[Spec<K1, K2>::D], T P[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  T hP[D];
+  BN_UNROLL for (int j = 0; j < D; ++j) {
+    T s = T(0);
+    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) s += Pp[i][j];
+    hP[j] = s;
+  }
+  double Sd = (double)r;
+  T hm = T(0);
+  BN_UNROLL for (int j = 0; j < D; ++j) if (S::h(j)) { Sd += (double)hP[j]; hm += mp[j]; }
+  double v = (double)y - (double)hm;
+  double iS = 1.0 / Sd;
+  T vS = (T)(v * iS);
+  T iSt = (T)iS;
+  BN_UNROLL for (int i = 0; i < D; ++i) m[i] = mp[i] + hP[i] * vS;
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = i; j < D; ++j) {
+      T s = Pp[i][j] - hP[i] * hP[j] * iSt;
+      P[i][j] = s;
+      P[j][i] = s;
+    }
+  return -0.5 * (v * v * iS + kLog2Pi + log(Sd));
+}
+
+// RTS gain G = fP A' pP^-1 from AfP = A fP and pP (ops.py:294-298), with the
+// Cholesky of pP in FP64.  Writes G; returns false if pP is not SPD.
+template <int D, class T>
+BN_HD bool rts_gain(const T AfP[D][D], const T pP[D][D], T G[D][D]) {
+  double Lc[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) Lc[i][j] = (double)pP[i][j];
+  bool ok = chol_spd<D>(Lc);
+  double X[D][D];  // X = pP^-1 AfP = G'
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) X[i][j] = (double)AfP[i][j];
+  lower_solve<D, D, double>(Lc, X);
+  upper_t_solve<D, D, double>(Lc, X);
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) G[i][j] = (T)X[j][i];
+  return ok;
+}
+
+}  // namespace bn

@@ -1,0 +1,551 @@
+// Per-lane bodies of the chunked filter / smoother kernels.
+//
+// Decomposition (one series = one reference model, bayesnewton/basemodels.py:625-706):
+//   series -> chunks of <= chunk_len consecutive steps -> tiles of <= 256 chunks.
+// One lane walks one chunk sequentially; chunk summaries are combined with the
+// associative operators of ops.py:203-219 (filter) and 328-335 (smoother) by a
+// two-level scan (within a tile, then across the tiles of a series).
+//
+//   filter_reduce   chunk -> filtering element (conditional filter from x_{s-1})
+//   [tile scan, series scan of the filtering elements]
+//   filter_pass     incoming filtered state -> filtered moments of every step,
+//                   log-likelihood partial, and the chunk's smoothing element
+//   [tile scan, series scan of the smoothing elements, in reverse]
+//   smoother_pass   incoming smoothed state -> smoothed moments of every step
+//
+// Everything here is __host__ __device__: the HIP kernels call these bodies with
+// their lane index, and a host build runs them in plain loops.
+#pragma once
+#include "bn_core.h"
+
+namespace bn {
+
+enum : uint8_t { CH_FIRST = 1, CH_LAST = 2 };
+enum : uint8_t { TL_FIRST = 1, TL_LAST = 2 };
+
+struct PlanView {
+  int32_t n_series, n_chunks, n_chunks_pad, n_tiles, chunk_len;
+  int32_t gshift;  // log2 of the lane-group width (64, or fewer for < 64 chunks)
+  int64_t n_slots;
+  const int32_t* chunk_series;
+  const int32_t* chunk_len_;
+  const uint8_t* chunk_flags;
+  const int32_t* chunk_tile;
+  const int64_t* chunk_start;
+  const int32_t* tile_first;
+  const int32_t* tile_count;
+  const int32_t* tile_series;
+  const uint8_t* tile_flags;
+  const int32_t* series_first_tile;
+  const int32_t* series_ntiles;
+  const int32_t* series_first_chunk;
+  const int32_t* series_nchunks;
+
+  BN_HD int64_t slot(int32_t j, int32_t k) const {
+    return (((int64_t)(j >> gshift) * chunk_len + k) << gshift) + (j & ((1 << gshift) - 1));
+  }
+};
+
+struct HyperView {
+  const double* ls;
+  const double* var;
+  int32_t stride;
+};
+
+template <int K1, int K2>
+BN_HD Model<K1, K2> load_model(const HyperView& hv, int32_t b) {
+  Model<K1, K2> m;
+  int64_t o = (int64_t)b * hv.stride;
+  m.init(hv.ls[o], hv.var[o], K2 > 0 ? hv.ls[o + 1] : 1.0, K2 > 0 ? hv.var[o + 1] : 1.0);
+  return m;
+}
+
+// Storage precision of the chunk/tile elements and of the filtered-moment
+// scratch.  FP64 for both: with FP32 storage the posterior variance of a
+// Matern32 + Matern52 sum (whose observed variance is a small difference of
+// large component entries) drifts to ~1e-4 relative from the FP64 oracle;
+// with FP64 storage it stays at ~1e-7.  The macros exist for that experiment.
+#ifndef BN_AGG_STORE
+#define BN_AGG_STORE double
+#endif
+#ifndef BN_STATE_STORE
+#define BN_STATE_STORE double
+#endif
+using agg_t = BN_AGG_STORE;
+using state_t = BN_STATE_STORE;
+
+// Field-major (structure-of-arrays) element storage: field f of item i at base[f * stride + i].
+struct SoA {
+  agg_t* base;
+  int64_t stride;
+};
+struct CSoA {
+  const agg_t* base;
+  int64_t stride;
+};
+
+template <class E>
+BN_HD void soa_store(const E& e, SoA s, int64_t i) {
+  e.store([&](int f, auto v) { s.base[(int64_t)f * s.stride + i] = (agg_t)v; });
+}
+template <class E>
+BN_HD void soa_load(E& e, CSoA s, int64_t i) {
+  e.load([&](int f) { return s.base[(int64_t)f * s.stride + i]; });
+}
+
+// Where the filtered moments go / come from.  packed: D mean planes then NS
+// packed-upper covariance planes; full: D mean planes + D*D covariance planes.
+struct StateIO {
+  state_t* mean;
+  state_t* cov;
+  bool packed;
+};
+struct CStateIO {
+  const state_t* mean;
+  const state_t* cov;
+  bool packed;
+};
+
+template <int K1, int K2, class T>
+BN_HD void write_state(const StateIO& io, int64_t n_slots, int64_t s, const T m[Spec<K1, K2>::D],
+                       const T P[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  BN_UNROLL for (int i = 0; i < D; ++i) io.mean[i * n_slots + s] = (state_t)m[i];
+  if (io.packed) {
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) io.cov[S::sidx(i, j) * n_slots + s] = (state_t)P[i][j];
+  } else {
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = 0; j < D; ++j) io.cov[(i * D + j) * n_slots + s] = (state_t)P[i][j];
+  }
+}
+
+template <int K1, int K2, class T>
+BN_HD void read_state(const CStateIO& io, int64_t n_slots, int64_t s, T m[Spec<K1, K2>::D],
+                      T P[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  BN_UNROLL for (int i = 0; i < D; ++i) m[i] = (T)io.mean[i * n_slots + s];
+  if (io.packed) {
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) {
+        T v = (T)io.cov[S::sidx(i, j) * n_slots + s];
+        P[i][j] = v;
+        P[j][i] = v;
+      }
+  } else {
+    // symmetrise the caller's full covariance
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) {
+        T v = T(0.5) * ((T)io.cov[(i * D + j) * n_slots + s] + (T)io.cov[(j * D + i) * n_slots + s]);
+        P[i][j] = v;
+        P[j][i] = v;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// filter_reduce: the chunk's filtering element, built by running the filter
+// conditionally on the (unknown) state x before the chunk: m = Phi x + beta,
+// P conditional covariance, and the information (J, eta) that y_chunk carries
+// about x.  The first chunk of a series starts from the prior (m0 = 0,
+// P0 = Pinf, ops.py:272-273) and yields a state element (Phi = 0).
+// ---------------------------------------------------------------------------
+template <int K1, int K2, class T>
+BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
+                              const float* __restrict__ y, const float* __restrict__ r, SoA agg, int32_t j) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  FElem<D, T> e;
+  e.set_identity_elem();
+  for (int32_t k = 0; k < L; ++k) {
+    const int64_t s = pl.slot(j, k);
+    T Phip[D][D], bp[D], Pp[D][D];
+    if (k == 0 && (fl & CH_FIRST)) {
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        bp[i] = T(0);
+        BN_UNROLL for (int c = 0; c < D; ++c) { Phip[i][c] = T(0); Pp[i][c] = (T)mdl.Pinf[i][c]; }
+      }
+    } else {
+      T A[D][D], Q[D][D];
+      mdl.discretise(dt[s], A, Q);
+      predict<K1, K2, T>(A, Q, e.beta, e.Sig, bp, Pp);
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) {
+          T t = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * e.Phi[q][c];
+          Phip[i][c] = t;
+        }
+    }
+    // u = H Phi-, hP = H P-, S = H P- H' + r, v = y - H beta-
+    T u[D], hP[D];
+    T hb = T(0);
+    BN_UNROLL for (int c = 0; c < D; ++c) {
+      T su = T(0), sp = T(0);
+      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { su += Phip[i][c]; sp += Pp[i][c]; }
+      u[c] = su;
+      hP[c] = sp;
+    }
+    double Sd = (double)r[s];
+    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { Sd += (double)hP[i]; hb += bp[i]; }
+    const double iS = 1.0 / Sd;
+    const T v = (T)((double)y[s] - (double)hb);
+    const T iSf = (T)iS;
+    const T vS = v * iSf;
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      e.eta[i] += u[i] * vS;
+      BN_UNROLL for (int c = i; c < D; ++c) {
+        T t = e.J[i][c] + u[i] * u[c] * iSf;
+        e.J[i][c] = t;
+        e.J[c][i] = t;
+      }
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      const T Ki = hP[i] * iSf;
+      e.beta[i] = bp[i] + Ki * v;
+      BN_UNROLL for (int c = 0; c < D; ++c) e.Phi[i][c] = Phip[i][c] - Ki * u[c];
+      BN_UNROLL for (int c = i; c < D; ++c) {
+        T t = Pp[i][c] - hP[i] * hP[c] * iSf;
+        e.Sig[i][c] = t;
+        e.Sig[c][i] = t;
+      }
+    }
+  }
+  soa_store(e, agg, j);
+}
+
+// Incoming filtered state of chunk j (not the first of its series): the
+// inclusive prefix of all earlier chunks.  `agg` holds tile-local inclusive
+// prefixes, `carry[t]` the prefix of all tiles of the series before tile t.
+template <int D, class T>
+BN_HD void filter_incoming(const PlanView& pl, CSoA agg, CSoA carry, int32_t j, T m[D], T P[D][D]) {
+  const int32_t t = pl.chunk_tile[j];
+  FElem<D, T> e;
+  if (pl.tile_first[t] == j) {
+    soa_load(e, carry, t);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] = e.beta[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = e.Sig[i][c];
+    }
+    return;
+  }
+  soa_load(e, agg, j - 1);
+  if (pl.tile_flags[t] & TL_FIRST) {
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] = e.beta[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = e.Sig[i][c];
+    }
+    return;
+  }
+  FElem<D, T> cst;
+  soa_load(cst, carry, t);
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    m[i] = cst.beta[i];
+    BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = cst.Sig[i][c];
+  }
+  fapply<D, T>(m, P, e);
+}
+
+// Accumulate the smoothing element of one step into the chunk's running
+// composition acc = e_s o ... o e_{k-1} (forward in time, see SElem):
+//   G = fP A' pP^-1, g = fm - G pm, L = fP - G A fP   (ops.py:318-325)
+template <int D, class T>
+BN_HD void smooth_accumulate(SElem<D, T>& acc, const T fm[D], const T fP[D][D], const T AfP[D][D],
+                             const T pm[D], const T pP[D][D]) {
+  T G[D][D];
+  rts_gain<D, T>(AfP, pP, G);
+  T g[D], Lk[D][D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = fm[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s -= G[i][k] * pm[k];
+    g[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int c = i; c < D; ++c) {
+      T s1 = T(0), s2 = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) { s1 += G[i][k] * AfP[k][c]; s2 += G[c][k] * AfP[k][i]; }
+      T s = fP[i][c] - T(0.5) * (s1 + s2);
+      Lk[i][c] = s;
+      Lk[c][i] = s;
+    }
+  // acc.g += acc.E g ; acc.L += acc.E Lk acc.E' ; acc.E = acc.E G
+  T EL[D][D];
+  mat_mul<D, T>(acc.E, Lk, EL);
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = acc.g[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += acc.E[i][k] * g[k];
+    acc.g[i] = s;
+    BN_UNROLL for (int c = i; c < D; ++c) {
+      T t = acc.L[i][c];
+      BN_UNROLL for (int k = 0; k < D; ++k) t += EL[i][k] * acc.E[c][k];
+      acc.L[i][c] = t;
+      acc.L[c][i] = t;
+    }
+  }
+  T EG[D][D];
+  mat_mul<D, T>(acc.E, G, EG);
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) acc.E[i][c] = EG[i][c];
+}
+
+// Terminal element of a series: the smoothed state at the last step is the
+// filtered one (ops.py:314-315): acc = acc o (0, fm, fP).
+template <int D, class T>
+BN_HD void smooth_terminal(SElem<D, T>& acc, const T fm[D], const T fP[D][D]) {
+  T EL[D][D];
+  mat_mul<D, T>(acc.E, fP, EL);
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = acc.g[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += acc.E[i][k] * fm[k];
+    acc.g[i] = s;
+    BN_UNROLL for (int c = i; c < D; ++c) {
+      T t = acc.L[i][c];
+      BN_UNROLL for (int k = 0; k < D; ++k) t += EL[i][k] * acc.E[c][k];
+      acc.L[i][c] = t;
+      acc.L[c][i] = t;
+    }
+  }
+  set_zero<D>(acc.E);
+}
+
+// predict() that also returns A P (needed for the RTS gain)
+template <int K1, int K2, class T>
+BN_HD void predict_ap(const T A[Spec<K1, K2>::D][Spec<K1, K2>::D], const T Q[Spec<K1, K2>::D][Spec<K1, K2>::D],
+                      const T m[Spec<K1, K2>::D], const T P[Spec<K1, K2>::D][Spec<K1, K2>::D],
+                      T mp[Spec<K1, K2>::D], T Pp[Spec<K1, K2>::D][Spec<K1, K2>::D],
+                      T AP[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    T s = T(0);
+    BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * m[k];
+    mp[i] = s;
+  }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int c = 0; c < D; ++c) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * P[k][c];
+      AP[i][c] = s;
+    }
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int c = i; c < D; ++c) {
+      T s = S::nz(i, c) ? Q[i][c] : T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(c, k)) s += AP[i][k] * A[c][k];
+      Pp[i][c] = s;
+      Pp[c][i] = s;
+    }
+}
+
+struct FilterPassOut {
+  StateIO state;       // filtered (or predicted) moments; state.mean == nullptr: not written
+  bool predict;        // write predicted instead of filtered moments
+  SoA sagg;            // smoothing element per chunk; sagg.base == nullptr: not built
+  double* ell;         // per-chunk log-likelihood partial
+};
+
+// filter_pass: the filter over one chunk from its true incoming state
+// (ops.py:156-175), writing moments, the log-likelihood partial and the
+// chunk's smoothing element (the smoother uses dt of the NEXT step,
+// basemodels.py:700).
+template <int K1, int K2, class T>
+BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
+                            const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
+                            CSoA fagg, CSoA fcarry, const FilterPassOut& out, int32_t j) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  T m[D], P[D][D];
+  if (!(fl & CH_FIRST)) {
+    filter_incoming<D, T>(pl, fagg, fcarry, j, m, P);
+  } else {
+    BN_UNROLL for (int i = 0; i < D; ++i) { m[i] = T(0); BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = T(0); }
+  }
+  const bool want_s = out.sagg.base != nullptr;
+  SElem<D, T> acc;
+  acc.set_identity_elem();
+  double ell = 0.0;
+  for (int32_t k = 0; k < L; ++k) {
+    const int64_t s = pl.slot(j, k);
+    T mp[D], Pp[D][D], AP[D][D];
+    if (k == 0 && (fl & CH_FIRST)) {
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        mp[i] = T(0);
+        BN_UNROLL for (int c = 0; c < D; ++c) Pp[i][c] = (T)mdl.Pinf[i][c];
+      }
+    } else {
+      T A[D][D], Q[D][D];
+      mdl.discretise(dt[s], A, Q);
+      predict_ap<K1, K2, T>(A, Q, m, P, mp, Pp, AP);
+      if (want_s && k > 0) smooth_accumulate<D, T>(acc, m, P, AP, mp, Pp);
+    }
+    const double lp = update<K1, K2, T>(mp, Pp, y[s], r[s], m, P);
+    if (!(mask != nullptr && mask[s])) ell += lp;
+    if (out.state.mean != nullptr) {
+      if (out.predict) write_state<K1, K2, T>(out.state, pl.n_slots, s, mp, Pp);
+      else write_state<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+    }
+  }
+  if (want_s) {
+    if (fl & CH_LAST) {
+      smooth_terminal<D, T>(acc, m, P);
+    } else {
+      T A[D][D], Q[D][D], mp[D], Pp[D][D], AP[D][D];
+      mdl.discretise(dt[pl.slot(j + 1, 0)], A, Q);
+      predict_ap<K1, K2, T>(A, Q, m, P, mp, Pp, AP);
+      smooth_accumulate<D, T>(acc, m, P, AP, mp, Pp);
+    }
+    soa_store(acc, out.sagg, j);
+  }
+  out.ell[j] = ell;
+}
+
+// smoother_reduce (standalone smoother API): the chunk's smoothing element
+// from given filtered moments and an explicit (already shifted) dt.
+template <int K1, int K2, class T>
+BN_HD void lane_smoother_reduce(const PlanView& pl, const HyperView& hv, const float* __restrict__ dts,
+                                const CStateIO& fin, SoA sagg, int32_t j) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  SElem<D, T> acc;
+  acc.set_identity_elem();
+  for (int32_t k = 0; k < L; ++k) {
+    const int64_t s = pl.slot(j, k);
+    T fm[D], fP[D][D];
+    read_state<K1, K2, T>(fin, pl.n_slots, s, fm, fP);
+    if (k == L - 1 && (fl & CH_LAST)) {
+      smooth_terminal<D, T>(acc, fm, fP);
+    } else {
+      T A[D][D], Q[D][D], mp[D], Pp[D][D], AP[D][D];
+      mdl.discretise(dts[s], A, Q);
+      predict_ap<K1, K2, T>(A, Q, fm, fP, mp, Pp, AP);
+      smooth_accumulate<D, T>(acc, fm, fP, AP, mp, Pp);
+    }
+  }
+  soa_store(acc, sagg, j);
+}
+
+// Incoming smoothed state of chunk j (not the last of its series): the
+// inclusive suffix of all later chunks.  `agg` holds tile-local inclusive
+// suffixes, `carry[t]` the suffix of all tiles of the series after tile t.
+template <int D, class T>
+BN_HD void smoother_incoming(const PlanView& pl, CSoA agg, CSoA carry, int32_t j, T m[D], T P[D][D]) {
+  const int32_t t = pl.chunk_tile[j];
+  SElem<D, T> e;
+  if (pl.tile_first[t] + pl.tile_count[t] - 1 == j) {
+    soa_load(e, carry, t);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] = e.g[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = e.L[i][c];
+    }
+    return;
+  }
+  soa_load(e, agg, j + 1);
+  if (pl.tile_flags[t] & TL_LAST) {
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] = e.g[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = e.L[i][c];
+    }
+    return;
+  }
+  SElem<D, T> cst;
+  soa_load(cst, carry, t);
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    m[i] = cst.g[i];
+    BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = cst.L[i][c];
+  }
+  sapply<D, T>(e, m, P);
+}
+
+template <class OT>
+struct SmootherPassIO {
+  CStateIO fin;        // filtered moments
+  const float* dt;     // step vector
+  bool shifted;        // dt is the smoother's own (shifted) vector; else use dt[k+1], 0 at the end
+  OT* out_mean;        // H m planes (1) or full (D) when full
+  OT* out_cov;         // H P H' (1) or full (D*D)
+  OT* gains;           // D*D planes or nullptr
+  bool full;
+};
+
+// smoother_pass: the RTS recursion over one chunk, backwards, from its true
+// incoming smoothed state (ops.py:290-305).
+template <int K1, int K2, class T, class OT>
+BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg, CSoA scarry,
+                              const SmootherPassIO<OT>& io, int32_t j) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  T sm[D], sP[D][D];
+  if (!(fl & CH_LAST)) smoother_incoming<D, T>(pl, sagg, scarry, j, sm, sP);
+  const int64_t ns = pl.n_slots;
+  for (int32_t k = L - 1; k >= 0; --k) {
+    const int64_t s = pl.slot(j, k);
+    T fm[D], fP[D][D], G[D][D];
+    read_state<K1, K2, T>(io.fin, ns, s, fm, fP);
+    const bool terminal = (k == L - 1) && (fl & CH_LAST);
+    double dtn;
+    if (io.shifted) dtn = io.dt[s];
+    else dtn = terminal ? 0.0 : (double)io.dt[k + 1 < L ? pl.slot(j, k + 1) : pl.slot(j + 1, 0)];
+    T A[D][D], Q[D][D], pm[D], pP[D][D], AfP[D][D];
+    mdl.discretise(dtn, A, Q);
+    predict_ap<K1, K2, T>(A, Q, fm, fP, pm, pP, AfP);
+    rts_gain<D, T>(AfP, pP, G);
+    if (terminal) {
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        sm[i] = fm[i];
+        BN_UNROLL for (int c = 0; c < D; ++c) sP[i][c] = fP[i][c];
+      }
+    } else {
+      T dm[D], nm[D], Dd[D][D], GD[D][D];
+      BN_UNROLL for (int i = 0; i < D; ++i) dm[i] = sm[i] - pm[i];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = fm[i];
+        BN_UNROLL for (int c = 0; c < D; ++c) t += G[i][c] * dm[c];
+        nm[i] = t;
+      }
+      BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) Dd[i][c] = sP[i][c] - pP[i][c];
+      mat_mul<D, T>(G, Dd, GD);
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = i; c < D; ++c) {
+          T t = fP[i][c];
+          BN_UNROLL for (int q = 0; q < D; ++q) t += GD[i][q] * G[c][q];
+          sP[i][c] = t;
+          sP[c][i] = t;
+        }
+      BN_UNROLL for (int i = 0; i < D; ++i) sm[i] = nm[i];
+    }
+    if (io.full) {
+      BN_UNROLL for (int i = 0; i < D; ++i) io.out_mean[i * ns + s] = (OT)sm[i];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) io.out_cov[(i * D + c) * ns + s] = (OT)sP[i][c];
+    } else {
+      T hm = T(0), hv2 = T(0);
+      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) {
+        hm += sm[i];
+        BN_UNROLL for (int c = 0; c < D; ++c) if (S::h(c)) hv2 += sP[i][c];
+      }
+      io.out_mean[s] = (OT)hm;
+      io.out_cov[s] = (OT)hv2;
+    }
+    if (io.gains != nullptr) {
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) io.gains[(i * D + c) * ns + s] = (OT)G[i][c];
+    }
+  }
+}
+
+}  // namespace bn

@@ -1,0 +1,688 @@
+// HIP kernels and C ABI of the chunked Kalman filter / RTS smoother
+// (bayesnewton/ops.py:149-380, basemodels.py:689-741) for gfx950.
+//
+// Launch structure of one fused filter + smoother pass (bn_filter_smoother_f32):
+//   k_filter_reduce   lane per chunk            -> filtering element per chunk
+//   k_fscan_tile      256-thread block per tile -> tile-local inclusive prefixes (LDS Hillis-Steele)
+//   k_fscan_series    block per series          -> exclusive tile carries
+//   k_filter_pass     lane per chunk            -> filtered moments (scratch), loglik partials,
+//                                                  smoothing element per chunk
+//   k_sscan_tile      block per tile            -> tile-local inclusive suffixes
+//   k_sscan_series    block per series          -> exclusive tile carries (from the end)
+//   k_smoother_pass   lane per chunk            -> posterior mean / variance
+//   k_series_sum      block per series          -> FP64 log-likelihood per series
+// A series that fits in one chunk skips both scans (the sequential filter).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/bn_hip.h"
+#include "bn_lanes.h"
+#include "bn_plan_host.h"
+
+using namespace bn;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define BN_HIP_CHECK(expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(BN_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBlock = 256;  // lane-per-chunk kernels
+constexpr int kScan = kTile;  // scan kernels: one thread per element of a tile
+using CT = double;  // compute type of the recurrences (see bn_lanes.h)
+}  // namespace
+
+struct bn_plan {
+  PlanHost host;
+  PlanView view;
+  void* dev = nullptr;  // one allocation holding every table
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_filter_reduce(PlanView pl, HyperView hv, const float* __restrict__ dt,
+                                                          const float* __restrict__ y, const float* __restrict__ r,
+                                                          SoA agg) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  if ((pl.chunk_flags[j] & (CH_FIRST | CH_LAST)) == (CH_FIRST | CH_LAST)) return;  // single-chunk series
+  lane_filter_reduce<K1, K2, CT>(pl, hv, dt, y, r, agg, j);
+}
+
+template <int D>
+__global__ __launch_bounds__(kScan) void k_fscan_tile(PlanView pl, SoA agg, SoA tagg) {
+  extern __shared__ agg_t lds[];
+  using E = FElem<D, CT>;
+  const int32_t t = blockIdx.x;
+  const int32_t tid = threadIdx.x;
+  const int32_t first = pl.tile_first[t];
+  const int32_t count = pl.tile_count[t];
+  if ((pl.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && count == 1) return;
+  E e;
+  if (tid < count) soa_load(e, CSoA{agg.base, agg.stride}, first + tid);
+  else e.set_identity_elem();
+  const SoA ls{lds, kScan};
+  const CSoA lc{lds, kScan};
+  for (int32_t off = 1; off < count; off <<= 1) {
+    soa_store(e, ls, tid);
+    __syncthreads();
+    if (tid >= off && tid < count) {
+      E p, o;
+      soa_load(p, lc, tid - off);
+      fcombine<D, CT>(p, e, o);
+      e = o;
+    }
+    __syncthreads();
+  }
+  if (tid < count) soa_store(e, agg, first + tid);
+  if (tid == count - 1) soa_store(e, tagg, t);
+}
+
+template <int D>
+__global__ __launch_bounds__(kScan) void k_fscan_series(PlanView pl, SoA tagg, SoA carry) {
+  extern __shared__ agg_t lds[];
+  using E = FElem<D, CT>;
+  const int32_t b = blockIdx.x;
+  const int32_t nt = pl.series_ntiles[b];
+  if (nt <= 1) return;
+  const int32_t ft = pl.series_first_tile[b];
+  const int32_t tid = threadIdx.x;
+  E e;
+  if (tid < nt) soa_load(e, CSoA{tagg.base, tagg.stride}, ft + tid);
+  else e.set_identity_elem();
+  const SoA ls{lds, kScan};
+  const CSoA lc{lds, kScan};
+  for (int32_t off = 1; off < nt; off <<= 1) {
+    soa_store(e, ls, tid);
+    __syncthreads();
+    if (tid >= off && tid < nt) {
+      E p, o;
+      soa_load(p, lc, tid - off);
+      fcombine<D, CT>(p, e, o);
+      e = o;
+    }
+    __syncthreads();
+  }
+  if (tid < nt - 1) soa_store(e, carry, ft + tid + 1);
+}
+
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_filter_pass(PlanView pl, HyperView hv, const float* __restrict__ dt,
+                                                        const float* __restrict__ y, const float* __restrict__ r,
+                                                        const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry,
+                                                        FilterPassOut out) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  lane_filter_pass<K1, K2, CT>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
+}
+
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_smoother_reduce(PlanView pl, HyperView hv, const float* __restrict__ dts,
+                                                            CStateIO fin, SoA sagg) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  lane_smoother_reduce<K1, K2, CT>(pl, hv, dts, fin, sagg, j);
+}
+
+template <int D>
+__global__ __launch_bounds__(kScan) void k_sscan_tile(PlanView pl, SoA agg, SoA tagg) {
+  extern __shared__ agg_t lds[];
+  using E = SElem<D, CT>;
+  const int32_t t = blockIdx.x;
+  const int32_t tid = threadIdx.x;
+  const int32_t first = pl.tile_first[t];
+  const int32_t count = pl.tile_count[t];
+  if ((pl.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && count == 1) return;
+  E e;
+  if (tid < count) soa_load(e, CSoA{agg.base, agg.stride}, first + tid);
+  else e.set_identity_elem();
+  const SoA ls{lds, kScan};
+  const CSoA lc{lds, kScan};
+  for (int32_t off = 1; off < count; off <<= 1) {
+    soa_store(e, ls, tid);
+    __syncthreads();
+    if (tid + off < count) {
+      E p, o;
+      soa_load(p, lc, tid + off);
+      scombine<D, CT>(e, p, o);
+      e = o;
+    }
+    __syncthreads();
+  }
+  if (tid < count) soa_store(e, agg, first + tid);
+  if (tid == 0) soa_store(e, tagg, t);
+}
+
+template <int D>
+__global__ __launch_bounds__(kScan) void k_sscan_series(PlanView pl, SoA tagg, SoA carry) {
+  extern __shared__ agg_t lds[];
+  using E = SElem<D, CT>;
+  const int32_t b = blockIdx.x;
+  const int32_t nt = pl.series_ntiles[b];
+  if (nt <= 1) return;
+  const int32_t ft = pl.series_first_tile[b];
+  const int32_t tid = threadIdx.x;
+  E e;
+  if (tid < nt) soa_load(e, CSoA{tagg.base, tagg.stride}, ft + tid);
+  else e.set_identity_elem();
+  const SoA ls{lds, kScan};
+  const CSoA lc{lds, kScan};
+  for (int32_t off = 1; off < nt; off <<= 1) {
+    soa_store(e, ls, tid);
+    __syncthreads();
+    if (tid + off < nt) {
+      E p, o;
+      soa_load(p, lc, tid + off);
+      scombine<D, CT>(e, p, o);
+      e = o;
+    }
+    __syncthreads();
+  }
+  if (tid >= 1 && tid < nt) soa_store(e, carry, ft + tid - 1);
+}
+
+template <int K1, int K2, class OT>
+__global__ __launch_bounds__(kBlock) void k_smoother_pass(PlanView pl, HyperView hv, CSoA sagg, CSoA scarry,
+                                                          SmootherPassIO<OT> io) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  lane_smoother_pass<K1, K2, CT, OT>(pl, hv, sagg, scarry, io, j);
+}
+
+// Deterministic per-series sum of a per-chunk FP64 array (fixed tree order).
+__global__ __launch_bounds__(kBlock) void k_series_sum(PlanView pl, const double* __restrict__ per_chunk,
+                                                       double* __restrict__ out) {
+  __shared__ double red[kBlock];
+  const int32_t b = blockIdx.x;
+  const int32_t c0 = pl.series_first_chunk[b];
+  const int32_t nc = pl.series_nchunks[b];
+  double s = 0.0;
+  for (int32_t i = threadIdx.x; i < nc; i += kBlock) s += per_chunk[c0 + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int32_t)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = red[0];
+}
+
+__device__ __forceinline__ bool slot_to_step(const PlanView& pl, int64_t slot, int64_t& csr) {
+  const int32_t lane = (int32_t)(slot & ((1 << pl.gshift) - 1));
+  const int64_t q = slot >> pl.gshift;
+  const int32_t k = (int32_t)(q % pl.chunk_len);
+  const int32_t j = (int32_t)(((q / pl.chunk_len) << pl.gshift) + lane);
+  if (j >= pl.n_chunks || k >= pl.chunk_len_[j]) return false;
+  csr = pl.chunk_start[j] + k;
+  return true;
+}
+
+__global__ void k_gather_f32(PlanView pl, const float* __restrict__ src, float* __restrict__ dst, int32_t width,
+                             float pad) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= pl.n_slots) return;
+  int64_t csr;
+  const bool ok = slot_to_step(pl, slot, csr);
+  for (int32_t f = 0; f < width; ++f) dst[f * pl.n_slots + slot] = ok ? src[csr * width + f] : pad;
+}
+
+__global__ void k_gather_f64(PlanView pl, const double* __restrict__ src, double* __restrict__ dst, int32_t width,
+                             double pad) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= pl.n_slots) return;
+  int64_t csr;
+  const bool ok = slot_to_step(pl, slot, csr);
+  for (int32_t f = 0; f < width; ++f) dst[f * pl.n_slots + slot] = ok ? src[csr * width + f] : pad;
+}
+
+__global__ void k_scatter_f64(PlanView pl, const double* __restrict__ src, double* __restrict__ dst, int32_t width) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= pl.n_slots) return;
+  int64_t csr;
+  if (!slot_to_step(pl, slot, csr)) return;
+  for (int32_t f = 0; f < width; ++f) dst[csr * width + f] = src[f * pl.n_slots + slot];
+}
+
+__global__ void k_gather_u8(PlanView pl, const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint8_t pad) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= pl.n_slots) return;
+  int64_t csr;
+  const bool ok = slot_to_step(pl, slot, csr);
+  dst[slot] = ok ? src[csr] : pad;
+}
+
+__global__ void k_scatter_f32(PlanView pl, const float* __restrict__ src, float* __restrict__ dst, int32_t width) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= pl.n_slots) return;
+  int64_t csr;
+  if (!slot_to_step(pl, slot, csr)) return;
+  for (int32_t f = 0; f < width; ++f) dst[csr * width + f] = src[f * pl.n_slots + slot];
+}
+
+template <int K1, int K2>
+__global__ void k_discretise(HyperView hv, int64_t n, const float* __restrict__ dt, float* __restrict__ A,
+                             float* __restrict__ Q) {
+  constexpr int D = K1 + K2;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, 0);
+  float a[D][D], q[D][D];
+  mdl.discretise((double)dt[i], a, q);
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) {
+      A[(i * D + r) * D + c] = a[r][c];
+      Q[(i * D + r) * D + c] = q[r][c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+int state_dim(const bn_kernel_desc* k) {
+  int d = 0;
+  for (int c = 0; c < k->n_comp; ++c) d += k->kind[c];
+  return d;
+}
+
+int check_desc(const bn_kernel_desc* k) {
+  if (k == nullptr) return fail(BN_EINVAL, "kernel descriptor is NULL");
+  if (k->n_comp < 1 || k->n_comp > 2) return fail(BN_EUNSUPPORTED, "kernel must have 1 or 2 Matern components");
+  for (int c = 0; c < k->n_comp; ++c)
+    if (k->kind[c] < 1 || k->kind[c] > 3) return fail(BN_EUNSUPPORTED, "component kind must be Matern12/32/52");
+  return BN_OK;
+}
+
+struct WsLayout {
+  int64_t fagg, ftagg, ftcarry, sagg, stagg, stcarry, fbuf, ell, total;
+};
+
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+WsLayout ws_layout(const PlanHost& p, int D) {
+  const int64_t FA = D * D + 2 * D + D * (D + 1);
+  const int64_t SA = D * D + D + D * (D + 1) / 2;
+  const int64_t NS = D * (D + 1) / 2;
+  const int64_t nt = p.n_tiles > 0 ? p.n_tiles : 1;
+  WsLayout w;
+  int64_t o = 0;
+  const int64_t ea = sizeof(agg_t), es = sizeof(state_t);
+  w.fagg = o;    o = align256(o + FA * p.n_chunks_pad * ea);
+  w.ftagg = o;   o = align256(o + FA * nt * ea);
+  w.ftcarry = o; o = align256(o + FA * nt * ea);
+  w.sagg = o;    o = align256(o + SA * p.n_chunks_pad * ea);
+  w.stagg = o;   o = align256(o + SA * nt * ea);
+  w.stcarry = o; o = align256(o + SA * nt * ea);
+  w.fbuf = o;    o = align256(o + (D + NS) * p.n_slots * es);
+  w.ell = o;     o = align256(o + p.n_chunks_pad * 8);
+  w.total = o;
+  return w;
+}
+
+struct Bufs {
+  SoA fagg, ftagg, ftcarry, sagg, stagg, stcarry;
+  state_t* fbuf;
+  double* ell;
+};
+
+Bufs ws_bufs(const PlanHost& p, int D, void* ws) {
+  WsLayout w = ws_layout(p, D);
+  char* base = (char*)ws;
+  const int64_t nt = p.n_tiles > 0 ? p.n_tiles : 1;
+  Bufs b;
+  b.fagg = SoA{(agg_t*)(base + w.fagg), p.n_chunks_pad};
+  b.ftagg = SoA{(agg_t*)(base + w.ftagg), nt};
+  b.ftcarry = SoA{(agg_t*)(base + w.ftcarry), nt};
+  b.sagg = SoA{(agg_t*)(base + w.sagg), p.n_chunks_pad};
+  b.stagg = SoA{(agg_t*)(base + w.stagg), nt};
+  b.stcarry = SoA{(agg_t*)(base + w.stcarry), nt};
+  b.fbuf = (state_t*)(base + w.fbuf);
+  b.ell = (double*)(base + w.ell);
+  return b;
+}
+
+inline CSoA cs(SoA s) { return CSoA{s.base, s.stride}; }
+
+inline dim3 lane_grid(const PlanHost& p) { return dim3((p.n_chunks + kBlock - 1) / kBlock); }
+
+bool needs_scan(const PlanHost& p) { return p.needs_scan; }
+
+// filter side: reduce + both scans
+template <int K1, int K2>
+void launch_filter_scan(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
+                        const Bufs& b, hipStream_t st) {
+  constexpr int D = K1 + K2;
+  const PlanHost& p = plan->host;
+  if (!needs_scan(p)) return;
+  const size_t ldsF = sizeof(agg_t) * FElem<D, CT>::NF * kScan;
+  hipLaunchKernelGGL((k_filter_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, b.fagg);
+  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsF, st, plan->view, b.fagg, b.ftagg);
+  hipLaunchKernelGGL((k_fscan_series<D>), dim3(p.n_series), dim3(kScan), ldsF, st, plan->view, b.ftagg, b.ftcarry);
+}
+
+template <int D>
+void launch_smoother_scan(const bn_plan* plan, const Bufs& b, hipStream_t st) {
+  const PlanHost& p = plan->host;
+  if (!needs_scan(p)) return;
+  const size_t ldsS = sizeof(agg_t) * SElem<D, CT>::NF * kScan;
+  hipLaunchKernelGGL((k_sscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsS, st, plan->view, b.sagg, b.stagg);
+  hipLaunchKernelGGL((k_sscan_series<D>), dim3(p.n_series), dim3(kScan), ldsS, st, plan->view, b.stagg, b.stcarry);
+}
+
+template <int K1, int K2>
+int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
+                        const uint8_t* mask, float* pmean, float* pvar, double* loglik, void* ws, hipStream_t st) {
+  constexpr int D = K1 + K2;
+  const PlanHost& p = plan->host;
+  Bufs b = ws_bufs(p, D, ws);
+  launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
+  FilterPassOut fo;
+  fo.state = StateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
+  fo.predict = false;
+  fo.sagg = b.sagg;
+  fo.ell = b.ell;
+  hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
+                     cs(b.fagg), cs(b.ftcarry), fo);
+  launch_smoother_scan<D>(plan, b, st);
+  SmootherPassIO<float> io;
+  io.fin = CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
+  io.dt = dt;
+  io.shifted = false;
+  io.out_mean = pmean;
+  io.out_cov = pvar;
+  io.gains = nullptr;
+  io.full = false;
+  hipLaunchKernelGGL((k_smoother_pass<K1, K2, float>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, cs(b.sagg),
+                     cs(b.stcarry), io);
+  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+template <int K1, int K2>
+int run_kalman_filter(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
+                      const uint8_t* mask, double* means, double* covs, double* loglik, uint32_t flags, void* ws,
+                      hipStream_t st) {
+  constexpr int D = K1 + K2;
+  const PlanHost& p = plan->host;
+  Bufs b = ws_bufs(p, D, ws);
+  launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
+  FilterPassOut fo;
+  fo.state = StateIO{means, covs, false};
+  fo.predict = (flags & BN_RETURN_PREDICT) != 0;
+  fo.sagg = SoA{nullptr, 0};
+  fo.ell = b.ell;
+  hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
+                     cs(b.fagg), cs(b.ftcarry), fo);
+  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+template <int K1, int K2>
+int run_rts_smoother(const bn_plan* plan, HyperView hv, const float* dts, const double* fmean, const double* fcov,
+                     double* means, double* covs, double* gains, uint32_t flags, void* ws, hipStream_t st) {
+  constexpr int D = K1 + K2;
+  const PlanHost& p = plan->host;
+  Bufs b = ws_bufs(p, D, ws);
+  CStateIO fin{fmean, fcov, false};
+  if (needs_scan(p)) {
+    hipLaunchKernelGGL((k_smoother_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dts, fin,
+                       b.sagg);
+    launch_smoother_scan<D>(plan, b, st);
+  }
+  SmootherPassIO<double> io;
+  io.fin = fin;
+  io.dt = dts;
+  io.shifted = true;
+  io.out_mean = means;
+  io.out_cov = covs;
+  io.gains = gains;
+  io.full = (flags & BN_RETURN_FULL) != 0;
+  hipLaunchKernelGGL((k_smoother_pass<K1, K2, double>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv,
+                     cs(b.sagg), cs(b.stcarry), io);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+template <int K1, int K2>
+int run_discretise(HyperView hv, int64_t n, const float* dt, float* A, float* Q, hipStream_t st) {
+  if (n <= 0) return BN_OK;
+  hipLaunchKernelGGL((k_discretise<K1, K2>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, hv, n, dt, A, Q);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+// Calls F<K1, K2>::run(args...) for the descriptor's component kinds.
+#define BN_DISPATCH(desc, FN, ...)                                              \
+  [&]() -> int {                                                                \
+    const int k1_ = (desc)->kind[0];                                            \
+    const int k2_ = (desc)->n_comp > 1 ? (desc)->kind[1] : 0;                   \
+    switch (k1_ * 4 + k2_) {                                                    \
+      case 1 * 4 + 0: return FN<1, 0>(__VA_ARGS__);                             \
+      case 2 * 4 + 0: return FN<2, 0>(__VA_ARGS__);                             \
+      case 3 * 4 + 0: return FN<3, 0>(__VA_ARGS__);                             \
+      case 1 * 4 + 1: return FN<1, 1>(__VA_ARGS__);                             \
+      case 1 * 4 + 2: return FN<1, 2>(__VA_ARGS__);                             \
+      case 1 * 4 + 3: return FN<1, 3>(__VA_ARGS__);                             \
+      case 2 * 4 + 1: return FN<2, 1>(__VA_ARGS__);                             \
+      case 2 * 4 + 2: return FN<2, 2>(__VA_ARGS__);                             \
+      case 2 * 4 + 3: return FN<2, 3>(__VA_ARGS__);                             \
+      case 3 * 4 + 1: return FN<3, 1>(__VA_ARGS__);                             \
+      case 3 * 4 + 2: return FN<3, 2>(__VA_ARGS__);                             \
+      case 3 * 4 + 3: return FN<3, 3>(__VA_ARGS__);                             \
+      default: return fail(BN_EUNSUPPORTED, "unsupported kernel combination");  \
+    }                                                                           \
+  }()
+
+int check_plan(const bn_plan* plan) {
+  if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
+  return BN_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* bn_last_error(void) { return g_err.c_str(); }
+
+int bn_version(void) { return 1; }
+
+int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_len, bn_plan** out) {
+  if (offsets_host == nullptr || out == nullptr) return fail(BN_EINVAL, "NULL argument to bn_plan_create");
+  *out = nullptr;
+  bn_plan* p = new bn_plan();
+  std::string err;
+  if (!build_plan(offsets_host, n_series, chunk_len, p->host, err)) {
+    delete p;
+    return fail(BN_EINVAL, err);
+  }
+  const PlanHost& h = p->host;
+  // one device allocation for every table, 256-byte aligned pieces
+  struct Piece { const void* src; size_t bytes; size_t off; };
+  Piece pcs[15] = {
+      {h.chunk_series.data(), h.chunk_series.size() * 4, 0}, {h.chunk_len_.data(), h.chunk_len_.size() * 4, 0},
+      {h.chunk_flags.data(), h.chunk_flags.size(), 0},       {h.chunk_tile.data(), h.chunk_tile.size() * 4, 0},
+      {h.chunk_start.data(), h.chunk_start.size() * 8, 0},   {h.tile_first.data(), h.tile_first.size() * 4, 0},
+      {h.tile_count.data(), h.tile_count.size() * 4, 0},     {h.tile_series.data(), h.tile_series.size() * 4, 0},
+      {h.tile_flags.data(), h.tile_flags.size(), 0},         {h.series_first_tile.data(), h.series_first_tile.size() * 4, 0},
+      {h.series_ntiles.data(), h.series_ntiles.size() * 4, 0}, {h.series_first_chunk.data(), h.series_first_chunk.size() * 4, 0},
+      {h.series_nchunks.data(), h.series_nchunks.size() * 4, 0}, {nullptr, 0, 0}, {nullptr, 0, 0}};
+  size_t total = 0;
+  for (auto& pc : pcs) {
+    pc.off = total;
+    total += (pc.bytes + 255) & ~size_t(255);
+  }
+  if (total == 0) total = 256;
+  hipError_t e = hipMalloc(&p->dev, total);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(BN_ENOMEM, std::string("hipMalloc(plan): ") + hipGetErrorString(e));
+  }
+  std::vector<char> staging(total, 0);
+  for (auto& pc : pcs)
+    if (pc.bytes) std::memcpy(staging.data() + pc.off, pc.src, pc.bytes);
+  e = hipMemcpy(p->dev, staging.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p->dev);
+    delete p;
+    return fail(BN_EHIP, std::string("hipMemcpy(plan): ") + hipGetErrorString(e));
+  }
+  char* d = (char*)p->dev;
+  PlanView& v = p->view;
+  v.n_series = h.n_series;
+  v.n_chunks = h.n_chunks;
+  v.n_chunks_pad = h.n_chunks_pad;
+  v.n_tiles = h.n_tiles;
+  v.chunk_len = h.chunk_len;
+  v.gshift = h.gshift;
+  v.n_slots = h.n_slots;
+  v.chunk_series = (const int32_t*)(d + pcs[0].off);
+  v.chunk_len_ = (const int32_t*)(d + pcs[1].off);
+  v.chunk_flags = (const uint8_t*)(d + pcs[2].off);
+  v.chunk_tile = (const int32_t*)(d + pcs[3].off);
+  v.chunk_start = (const int64_t*)(d + pcs[4].off);
+  v.tile_first = (const int32_t*)(d + pcs[5].off);
+  v.tile_count = (const int32_t*)(d + pcs[6].off);
+  v.tile_series = (const int32_t*)(d + pcs[7].off);
+  v.tile_flags = (const uint8_t*)(d + pcs[8].off);
+  v.series_first_tile = (const int32_t*)(d + pcs[9].off);
+  v.series_ntiles = (const int32_t*)(d + pcs[10].off);
+  v.series_first_chunk = (const int32_t*)(d + pcs[11].off);
+  v.series_nchunks = (const int32_t*)(d + pcs[12].off);
+  *out = p;
+  return BN_OK;
+}
+
+int bn_plan_destroy(bn_plan* plan) {
+  if (plan == nullptr) return BN_OK;
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
+  return BN_OK;
+}
+
+int64_t bn_plan_num_slots(const bn_plan* plan) { return plan ? plan->host.n_slots : -1; }
+int32_t bn_plan_num_chunks(const bn_plan* plan) { return plan ? plan->host.n_chunks : -1; }
+int32_t bn_plan_num_tiles(const bn_plan* plan) { return plan ? plan->host.n_tiles : -1; }
+
+int64_t bn_workspace_bytes(const bn_plan* plan, const bn_kernel_desc* kern) {
+  if (plan == nullptr || check_desc(kern) != BN_OK) return -1;
+  return ws_layout(plan->host, state_dim(kern)).total;
+}
+
+int bn_plan_gather_f32(const bn_plan* plan, const float* src, float* dst, int32_t width, float pad, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (width < 1) return fail(BN_EINVAL, "width must be >= 1");
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plan->view,
+                     src, dst, width, pad);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_plan_scatter_f32(const bn_plan* plan, const float* src, float* dst, int32_t width, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (width < 1) return fail(BN_EINVAL, "width must be >= 1");
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_scatter_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plan->view,
+                     src, dst, width);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_plan_gather_f64(const bn_plan* plan, const double* src, double* dst, int32_t width, double pad, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (width < 1) return fail(BN_EINVAL, "width must be >= 1");
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_gather_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plan->view,
+                     src, dst, width, pad);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_plan_scatter_f64(const bn_plan* plan, const double* src, double* dst, int32_t width, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (width < 1) return fail(BN_EINVAL, "width must be >= 1");
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_scatter_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plan->view,
+                     src, dst, width);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_plan_gather_u8(const bn_plan* plan, const uint8_t* src, uint8_t* dst, uint8_t pad, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_gather_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, plan->view,
+                     src, dst, pad);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_discretise(const bn_kernel_desc* kern, const double* lengthscale, const double* variance, int64_t n,
+                      const float* dt, float* A, float* Q, void* stream) {
+  if (int rc = check_desc(kern)) return rc;
+  HyperView hv{lengthscale, variance, 0};
+  return BN_DISPATCH(kern, run_discretise, hv, n, dt, A, Q, (hipStream_t)stream);
+}
+
+int bn_filter_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                           const double* variance, int32_t hyper_stride, const float* dt, const float* y,
+                           const float* r, const uint8_t* mask, float* post_mean, float* post_var, double* loglik,
+                           void* workspace, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !y || !r || !post_mean || !post_var || !loglik || !workspace || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_filter_smoother");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, workspace,
+                     (hipStream_t)stream);
+}
+
+int bn_kalman_filter(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                         const double* variance, int32_t hyper_stride, const float* dt, const float* y,
+                         const float* r, const uint8_t* mask, double* means, double* covs, double* loglik,
+                         uint32_t flags, void* workspace, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !y || !r || !means || !covs || !loglik || !workspace || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_kalman_filter");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, means, covs, loglik, flags, workspace,
+                     (hipStream_t)stream);
+}
+
+int bn_rts_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                        const double* variance, int32_t hyper_stride, const float* dt, const double* filter_mean,
+                        const double* filter_cov, double* means, double* covs, double* gains, uint32_t flags,
+                        void* workspace, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !filter_mean || !filter_cov || !means || !covs || !workspace || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_rts_smoother");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_rts_smoother, plan, hv, dt, filter_mean, filter_cov, means, covs, gains, flags,
+                     workspace, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""Drop-in replacements for bayesnewton/ops.py's filter and smoother.
+
+`kalman_filter` and `rauch_tung_striebel_smoother` keep the reference
+signatures (bayesnewton/ops.py:256-285 and 357-380) and return FP64 tensors of
+the reference's shapes; the work runs in libbn_hip.so.  `parallel=False`
+selects the sequential scan (one lane walks the whole series), `parallel=True`
+the chunked associative scan; both compute the same quantities.
+
+`filter_smoother` is the fused path of MarkovGaussianProcess.update_posterior
+(bayesnewton/basemodels.py:689-706) on a SeriesPlan: FP32 tiled pseudo
+observations in, FP32 tiled posterior mean/variance and FP64 per-series
+log-likelihoods out, no filtered state leaving the device scratch.
+"""
+import numpy as np
+import torch
+
+from . import _native as nat
+from .plan import SeriesPlan
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise nat.NativeError("bayesnewton_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_tensor(x, dtype, device):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype)
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+
+
+def discretise(kernel, dt):
+    """vmap(kernel.state_transition)(dt) and Q = Pinf - A Pinf A' (ops.py:274-279).
+
+    Returns (A [N,d,d], Q [N,d,d]) as FP32 device tensors computed in FP64.
+    """
+    dev = _device()
+    dt = _as_tensor(dt, torch.float32, dev).reshape(-1).contiguous()
+    n = dt.shape[0]
+    d = kernel.state_dim
+    A = torch.empty((n, d, d), dtype=torch.float32, device=dev)
+    Q = torch.empty((n, d, d), dtype=torch.float32, device=dev)
+    desc = kernel.desc()
+    ls, var = kernel.hypers(dev)
+    import ctypes
+    nat.check(nat.lib().bn_discretise(ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), n, nat.ptr(dt), nat.ptr(A),
+                                      nat.ptr(Q), nat.stream_ptr()))
+    return A, Q
+
+
+def _single_plan(n, parallel, dev):
+    return SeriesPlan(np.array([0, n], dtype=np.int64), parallel=parallel, device=dev)
+
+
+def kalman_filter(dt, kernel, y, noise_cov, mask=None, parallel=False, return_predict=False, plan=None):
+    """ops.py:256-285.  dt [N(,1)], y [N,1,1], noise_cov [N,1,1], mask [N,1,1] bool.
+
+    Returns (ell, (means [N,d,1], covs [N,d,d])), FP64 tensors on the GPU.
+    """
+    import ctypes
+    dev = _device()
+    dt = _as_tensor(dt, torch.float32, dev).reshape(-1)
+    N = dt.shape[0]
+    y = _as_tensor(y, torch.float32, dev).reshape(N, -1)
+    r = _as_tensor(noise_cov, torch.float32, dev).reshape(N, -1)
+    if y.shape[1] != 1 or r.shape[1] != 1:
+        raise NotImplementedError("the temporal HIP path takes scalar observations (y [N,1,1])")
+    if plan is None:
+        plan = _single_plan(N, parallel, dev)
+    d = kernel.state_dim
+    desc = kernel.desc()
+    ls, var = kernel.hypers(dev)
+    dt_t, y_t, r_t = plan.gather(dt), plan.gather(y), plan.gather(r, pad=1.0)
+    mk = plan.gather_mask(_as_tensor(mask, torch.bool, dev)) if mask is not None else None
+    means = torch.empty((d, plan.n_slots), dtype=torch.float64, device=dev)
+    covs = torch.empty((d * d, plan.n_slots), dtype=torch.float64, device=dev)
+    ll = torch.empty(plan.n_series, dtype=torch.float64, device=dev)
+    flags = nat.BN_RETURN_PREDICT if return_predict else 0
+    nat.check(nat.lib().bn_kalman_filter(plan.handle, ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), 0,
+                                         nat.ptr(dt_t), nat.ptr(y_t), nat.ptr(r_t), nat.ptr(mk), nat.ptr(means),
+                                         nat.ptr(covs), nat.ptr(ll), flags, nat.ptr(plan.workspace(desc)),
+                                         nat.stream_ptr()))
+    m = plan.scatter(means).reshape(N, d, 1)
+    P = plan.scatter(covs).reshape(N, d, d)
+    return ll.sum(), (m, P)
+
+
+def rauch_tung_striebel_smoother(dt, kernel, filter_mean, filter_cov, return_full=False, parallel=False, plan=None):
+    """ops.py:357-380.  dt is the smoother's (shifted) step vector.
+
+    Returns (means, covs, gains): [N,p,1], [N,p,p] (or [N,d,1], [N,d,d] with
+    return_full) and gains [N,d,d], FP64 tensors on the GPU.
+    """
+    import ctypes
+    dev = _device()
+    dt = _as_tensor(dt, torch.float32, dev).reshape(-1)
+    N = dt.shape[0]
+    d = kernel.state_dim
+    fm = _as_tensor(filter_mean, torch.float64, dev).reshape(N, d)
+    fP = _as_tensor(filter_cov, torch.float64, dev).reshape(N, d * d)
+    if plan is None:
+        plan = _single_plan(N, parallel, dev)
+    desc = kernel.desc()
+    ls, var = kernel.hypers(dev)
+    dt_t, fm_t, fP_t = plan.gather(dt), plan.gather(fm), plan.gather(fP)
+    w_m, w_P = (d, d * d) if return_full else (1, 1)
+    om = torch.empty((w_m, plan.n_slots), dtype=torch.float64, device=dev)
+    oP = torch.empty((w_P, plan.n_slots), dtype=torch.float64, device=dev)
+    G = torch.empty((d * d, plan.n_slots), dtype=torch.float64, device=dev)
+    flags = nat.BN_RETURN_FULL if return_full else 0
+    nat.check(nat.lib().bn_rts_smoother(plan.handle, ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), 0,
+                                        nat.ptr(dt_t), nat.ptr(fm_t), nat.ptr(fP_t), nat.ptr(om), nat.ptr(oP),
+                                        nat.ptr(G), flags, nat.ptr(plan.workspace(desc)), nat.stream_ptr()))
+    if return_full:
+        means = plan.scatter(om).reshape(N, d, 1)
+        covs = plan.scatter(oP).reshape(N, d, d)
+    else:
+        means = plan.scatter(om).reshape(N, 1, 1)
+        covs = plan.scatter(oP).reshape(N, 1, 1)
+    return means, covs, plan.scatter(G).reshape(N, d, d)
+
+
+def filter_smoother(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=None, hyper_stride=0,
+                    post_mean=None, post_var=None, loglik=None):
+    """Fused filter + smoother on tiled FP32 inputs (update_posterior, basemodels.py:689-706).
+
+    lengthscale / variance: FP64 device tensors [n_comp] (hyper_stride 0) or
+    [B, n_comp] (hyper_stride n_comp).  Returns (post_mean, post_var, loglik).
+    """
+    import ctypes
+    dev = plan.device
+    if post_mean is None:
+        post_mean = torch.empty(plan.n_slots, dtype=torch.float32, device=dev)
+    if post_var is None:
+        post_var = torch.empty(plan.n_slots, dtype=torch.float32, device=dev)
+    if loglik is None:
+        loglik = torch.empty(plan.n_series, dtype=torch.float64, device=dev)
+    nat.check(nat.lib().bn_filter_smoother(plan.handle, ctypes.byref(kernel_desc), nat.ptr(lengthscale),
+                                           nat.ptr(variance), hyper_stride, nat.ptr(dt), nat.ptr(y), nat.ptr(r),
+                                           nat.ptr(mask), nat.ptr(post_mean), nat.ptr(post_var), nat.ptr(loglik),
+                                           nat.ptr(plan.workspace(kernel_desc)), nat.stream_ptr()))
+    return post_mean, post_var, loglik

@@ -1,0 +1,143 @@
+/*
+ * bn_hip.h - C ABI of the MI355X-native Markov-GP hot path (libbn_hip.so).
+ *
+ * The reference (AaltoML/BayesNewton v1.3.4) is pure Python/JAX and has no FFI;
+ * its boundary for this path is the Python API
+ *   ops.kalman_filter(dt, kernel, y, noise_cov, mask, parallel, return_predict)
+ *                                                   bayesnewton/ops.py:256-285
+ *   ops.rauch_tung_striebel_smoother(dt, kernel, filter_mean, filter_cov,
+ *                                    return_full, parallel) bayesnewton/ops.py:357-380
+ *   MarkovGaussianProcess.update_posterior / compute_log_lik
+ *                                     bayesnewton/basemodels.py:689-706, 726-741
+ *   InferenceMixin.inference / <Newton|VariationalInference|
+ *   ExpectationPropagation>.update_variational_params / energy
+ *                                     bayesnewton/inference.py:65-325
+ * Each entry point below names the reference function it replaces.  The Python
+ * package bayesnewton_amd binds them with ctypes and re-exposes the reference
+ * signatures.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers, caller-allocated (e.g. torch
+ *    tensors); `stream` is a hipStream_t passed as void* (NULL = null stream).
+ *  - Per-step arrays live in the plan's TILED layout: the time steps of every
+ *    series are cut into chunks of at most `chunk_len` steps; chunk j's step k
+ *    sits at slot ((j / G) * chunk_len + k) * G + (j % G) with G = 64 (or the
+ *    next power of two >= the chunk count when there are fewer than 64 chunks),
+ *    so the 64 lanes of a wavefront that each walk one chunk read consecutive
+ *    words.  Use
+ *    bn_plan_gather_* / bn_plan_scatter_* to convert from/to the reference's
+ *    series-contiguous [N] layout.  Per-chunk and per-tile scratch lives in a
+ *    caller-provided workspace of bn_workspace_bytes() bytes.
+ *  - Kernels: a sum of at most two Matern components (kernels.py:123-300,
+ *    Sum 1624-1663).  Hyperparameters are natural-scale FP64 arrays
+ *    lengthscale[b * hyper_stride + c], variance[b * hyper_stride + c]
+ *    (hyper_stride = 0 shares one set across all series).
+ *  - Numerics: per-step inputs (dt, y, r) and the fused path's posterior
+ *    outputs are FP32; the state recurrences, the discretisation A(dt),
+ *    Q(dt) = Pinf - A Pinf A', the log-likelihood accumulation, the scan
+ *    elements and the filtered-moment scratch are FP64 (FP32 recurrences leave
+ *    the posterior variance of a Matern32 + Matern52 sum ~1e-3 relative from
+ *    the FP64 reference).  The ops.* mirrors return FP64 moments, as the
+ *    reference does.
+ *  - Return value: 0 on success, a negative BN_E* code otherwise; the message is
+ *    available from bn_last_error() (thread-local).  Numerical failure (a
+ *    non-PD matrix) is not an error: as in the reference, NaN propagates into the
+ *    affected series' outputs.
+ */
+#ifndef BN_HIP_H
+#define BN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BN_OK 0
+#define BN_EINVAL -1
+#define BN_EHIP -2
+#define BN_ENOMEM -3
+#define BN_EUNSUPPORTED -4
+
+#define BN_MATERN12 1
+#define BN_MATERN32 2
+#define BN_MATERN52 3
+
+/* flags */
+#define BN_RETURN_PREDICT 0x1u /* kalman_filter(return_predict=True), ops.py:172-175 */
+#define BN_RETURN_FULL 0x2u    /* rauch_tung_striebel_smoother(return_full=True), ops.py:302-305 */
+
+typedef struct bn_kernel_desc {
+  int32_t n_comp;  /* 1 or 2 */
+  int32_t kind[2]; /* BN_MATERN12 / BN_MATERN32 / BN_MATERN52, in Sum order */
+} bn_kernel_desc;
+
+typedef struct bn_plan bn_plan; /* opaque: chunk/tile tables on the device */
+
+const char *bn_last_error(void);
+int bn_version(void);
+
+/* --- plan: the series -> chunk -> tile decomposition ------------------------
+ * offsets_host[n_series + 1]: CSR offsets (host memory) of the series in the
+ * [N] layout, as the reference's per-model arrays concatenated.  chunk_len:
+ * steps walked sequentially by one lane (1..2^24).  Each series may span at
+ * most 128 * 128 chunks (level-1 tiles of 128 chunks, at most 128 tiles).  Replaces input_admin's bookkeeping (utils.py:234-265) for
+ * many series at once; the reference handles one series per model. */
+int bn_plan_create(const int64_t *offsets_host, int32_t n_series, int32_t chunk_len, bn_plan **out);
+int bn_plan_destroy(bn_plan *plan);
+int64_t bn_plan_num_slots(const bn_plan *plan);
+int32_t bn_plan_num_chunks(const bn_plan *plan);
+int32_t bn_plan_num_tiles(const bn_plan *plan);
+int64_t bn_workspace_bytes(const bn_plan *plan, const bn_kernel_desc *kern);
+
+/* [N] -> tiled (padding slots get `pad`) and tiled -> [N].  `width` fields are
+ * interleaved in the [N] array ([N, width] row-major) and stored as `width`
+ * tiled planes of num_slots words each. */
+int bn_plan_gather_f32(const bn_plan *plan, const float *src, float *dst, int32_t width, float pad, void *stream);
+int bn_plan_scatter_f32(const bn_plan *plan, const float *src, float *dst, int32_t width, void *stream);
+int bn_plan_gather_f64(const bn_plan *plan, const double *src, double *dst, int32_t width, double pad, void *stream);
+int bn_plan_scatter_f64(const bn_plan *plan, const double *src, double *dst, int32_t width, void *stream);
+int bn_plan_gather_u8(const bn_plan *plan, const uint8_t *src, uint8_t *dst, uint8_t pad, void *stream);
+
+/* --- discretisation: vmap(kernel.state_transition)(dt) and
+ * process_noise_covariance, ops.py:274-279 / kernels.py:216-224, 273-286.
+ * dt[n] -> A[n, d, d], Q[n, d, d] (row-major, FP32, computed in FP64). */
+int bn_discretise(const bn_kernel_desc *kern, const double *lengthscale, const double *variance, int64_t n,
+                  const float *dt, float *A, float *Q, void *stream);
+
+/* --- filter + smoother, fused: MarkovGaussianProcess.update_posterior,
+ * basemodels.py:689-706 (filter ops.py:154-253, shifted dt basemodels.py:700,
+ * smoother ops.py:288-354).  Inputs dt, y (pseudo mean), r (pseudo variance),
+ * mask (1 = missing, may be NULL) are tiled.  Outputs: post_mean / post_var
+ * (tiled, H m and H P H') and loglik[n_series] (FP64, the filter's log p(y),
+ * i.e. compute_log_lik, basemodels.py:726-741).  With chunk_len >= the longest
+ * series this is the sequential scan; otherwise the associative scan. */
+int bn_filter_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                       const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
+                       const uint8_t *mask, float *post_mean, float *post_var, double *loglik, void *workspace,
+                       void *stream);
+
+/* --- ops.kalman_filter, ops.py:256-285.  Writes the filtered (or, with
+ * BN_RETURN_PREDICT, predicted) means [d tiled planes] and covariances [d*d
+ * tiled planes, row-major], FP64, and loglik[n_series]. */
+int bn_kalman_filter(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                     const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
+                     const uint8_t *mask, double *means, double *covs, double *loglik, uint32_t flags,
+                     void *workspace, void *stream);
+
+/* --- ops.rauch_tung_striebel_smoother, ops.py:357-380.  dt is the smoother's
+ * (already shifted) step vector, tiled.  filter_mean [d planes], filter_cov
+ * [d*d planes], FP64.  Outputs (FP64): with BN_RETURN_FULL, means [d planes] /
+ * covs [d*d planes]; otherwise H m [1 plane] / H P H' [1 plane].  gains [d*d
+ * planes] or NULL.  The last step of each series returns the filtered moments, as the
+ * reference's associative form does (ops.py:314-315). */
+int bn_rts_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                    const double *variance, int32_t hyper_stride, const float *dt, const double *filter_mean,
+                    const double *filter_cov, double *means, double *covs, double *gains, uint32_t flags,
+                    void *workspace, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BN_HIP_H */

@@ -1,0 +1,294 @@
+"""FP64 Markov-GP models of the oracle (test infrastructure only).
+
+Restates the reference's model/inference stack for the Markov GP:
+bayesnewton/basemodels.py:52-262 (sites, BaseModel) and 625-764
+(MarkovGaussianProcess), bayesnewton/inference.py:21-325 (newton_update,
+InferenceMixin.inference, Newton/Laplace, VariationalInference,
+ExpectationPropagation) and bayesnewton/models.py:118-132, 216-231, 297-299.
+
+`DenseGP` restates the kernel-matrix GP (basemodels.py:265-330 with
+ops.py:52-80) and is used only to pin the Markov oracle.
+"""
+import numpy as np
+
+from . import ops
+from .utils import (inv, solve, diag, input_admin, mvn_logpdf, gaussian_expected_log_lik,
+                    compute_cavity, pep_constant, ensure_diagonal_positive_precision, LOG2PI)
+from .kernels import SpatioTemporalKernel
+
+
+def T(P):
+    return np.swapaxes(P, -1, -2)
+
+
+def reparametrise(param1, param2):
+    """basemodels.py:85-90: (mean, cov) <-> (nat1, nat2) with nat2 = cov^-1 (no -1/2)."""
+    return solve(param2, param1), inv(param2)
+
+
+def newton_update(mean, jacobian, hessian):
+    """inference.py:21-39"""
+    hessian = np.where(np.isnan(hessian), -1e-6, hessian)
+    jacobian = np.where(np.isnan(jacobian), hessian @ mean, jacobian)
+    return jacobian - hessian @ mean, -hessian
+
+
+def _diag_mat(v):
+    """[N, p] -> [N, p, p] diagonal."""
+    out = np.zeros(v.shape + (v.shape[-1],))
+    idx = np.arange(v.shape[-1])
+    out[..., idx, idx] = v
+    return out
+
+
+class MarkovGP:
+    """basemodels.py:625-764 with the inference mixins of inference.py.
+
+    inference: 'VI' (MarkovVariationalGP), 'Newton' (MarkovLaplaceGP /
+    MarkovNewtonGP) or 'EP' (MarkovExpectationPropagationGP).
+    """
+
+    def __init__(self, kernel, likelihood, X, Y, R=None, parallel=False, inference="VI", power=1.0,
+                 gh_points=20):
+        X, Y, R, dt = input_admin(X, Y, R)
+        self.X, self.Y, self.R, self.dt = X, Y, R, dt
+        self.kernel = kernel
+        self.likelihood = likelihood
+        self.parallel = parallel
+        self.inference_kind = inference
+        self.power = float(power)
+        self.gh_points = gh_points
+        H = kernel.measurement_model()
+        self.func_dim = H.shape[0]
+        self.num_data = X.shape[0]
+        self.obs_dim = Y.shape[1]
+        p = self.obs_dim
+        # basemodels.py:130-133: sites N(0, 1e2 I)
+        self.site_mean = np.zeros([self.num_data, p, 1])
+        self.site_cov = 1e2 * np.tile(np.eye(p), [self.num_data, 1, 1])
+        self.nat1, self.nat2 = reparametrise(self.site_mean, self.site_cov)
+        self.posterior_mean = np.zeros([self.num_data, self.func_dim, 1])
+        self.posterior_variance = np.tile(np.eye(self.func_dim), [self.num_data, 1, 1])
+        self.mask_y = np.isnan(self.Y).reshape(Y.shape[0], Y.shape[1])
+        self.spatio_temporal = bool(np.any(~np.isnan(self.R)))
+        if self.func_dim == self.obs_dim:
+            self.mask_pseudo_y = self.mask_y
+        else:
+            self.mask_pseudo_y = np.tile(self.mask_y, [1, p])
+        if self.func_dim != self.obs_dim and self.spatio_temporal:
+            self.mask_pseudo_y = None
+        self.last_filter_loglik = None
+
+    # ---- sites ----------------------------------------------------------------
+    def update_nat_params(self, nat1, nat2):
+        """basemodels.py:97-100"""
+        self.nat1, self.nat2 = nat1, nat2
+        self.site_mean, self.site_cov = reparametrise(nat1, nat2)
+
+    def _mask3(self):
+        if self.mask_pseudo_y is None:
+            return None
+        return self.mask_pseudo_y.reshape(self.num_data, -1, 1)
+
+    # ---- posterior ------------------------------------------------------------
+    def spatial_conditional(self):
+        return self.kernel.spatial_conditional(self.X, self.R)
+
+    def compute_full_pseudo_lik(self):
+        """basemodels.py:676-687"""
+        if self.spatio_temporal:
+            B, C = self.spatial_conditional()
+            nat1f = T(B) @ self.nat1
+            nat2f = T(B) @ self.nat2 @ B
+            Mi = nat2f.shape[-1]
+            pvar = inv(nat2f + 1e-12 * np.eye(Mi))
+            return pvar @ nat1f, pvar
+        return self.site_mean, self.site_cov
+
+    def compute_full_pseudo_nat(self):
+        """basemodels.py:667-674"""
+        if self.spatio_temporal:
+            B, C = self.spatial_conditional()
+            return T(B) @ self.nat1, T(B) @ self.nat2 @ B
+        return self.nat1, self.nat2
+
+    def update_posterior(self):
+        """basemodels.py:689-706"""
+        pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
+        ell, (fm, fP) = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var,
+                                          mask=self._mask3(), parallel=self.parallel)
+        self.last_filter_loglik = ell
+        dt = np.concatenate([self.dt[1:], np.array([0.0])])
+        sm, sP, _ = ops.rauch_tung_striebel_smoother(dt, self.kernel, fm, fP, parallel=self.parallel)
+        self.posterior_mean, self.posterior_variance = sm, sP
+
+    def compute_log_lik(self, pseudo_y=None, pseudo_var=None):
+        """basemodels.py:726-741"""
+        if pseudo_y is None:
+            pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
+        ell, _ = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var,
+                                   mask=self._mask3(), parallel=self.parallel)
+        return ell
+
+    def compute_kl(self):
+        """basemodels.py:708-724"""
+        pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
+        log_lik_pseudo = self.compute_log_lik(pseudo_y, pseudo_var)
+        ed = gaussian_expected_log_lik(pseudo_y, self.posterior_mean, self.posterior_variance,
+                                       pseudo_var, self._mask3())
+        return np.sum(ed) - log_lik_pseudo
+
+    def conditional_posterior_to_data(self, post_mean=None, post_cov=None):
+        """basemodels.py:743-764"""
+        if post_mean is None:
+            post_mean = self.posterior_mean
+        if post_cov is None:
+            post_cov = self.posterior_variance
+        if self.spatio_temporal:
+            B, C = self.spatial_conditional()
+            return B @ post_mean, B @ post_cov @ T(B) + C
+        return post_mean, post_cov
+
+    # ---- site maps ------------------------------------------------------------
+    def _variational_expectation(self, mean_f, cov_f):
+        """likelihoods.py:363-383 vmapped over time (inference.py:179-184)."""
+        Nn, p = mean_f.shape[0], mean_f.shape[1]
+        y = self.Y.reshape(-1)
+        m = mean_f.reshape(-1)
+        v = diag(cov_f).reshape(-1)
+        E, dE, d2E = self.likelihood.variational_expectation(y, m, v, self.gh_points)
+        return E.reshape(Nn, p), dE.reshape(Nn, p, 1), _diag_mat(d2E.reshape(Nn, p))
+
+    def _log_likelihood_gradients(self, mean_f):
+        """likelihoods.py:337-355 vmapped over time (inference.py:113-117)."""
+        Nn, p = mean_f.shape[0], mean_f.shape[1]
+        ll, J, H = self.likelihood.log_likelihood_gradients(self.Y.reshape(-1), mean_f.reshape(-1))
+        return ll.reshape(Nn, p), J.reshape(Nn, p, 1), _diag_mat(H.reshape(Nn, p))
+
+    def cavity_distribution(self, power):
+        """basemodels.py:230-245"""
+        nat1f, nat2f = self.compute_full_pseudo_nat()
+        return compute_cavity(self.posterior_mean, self.posterior_variance, nat1f, nat2f, power)
+
+    def _moment_match(self, cav_mean_f, cav_cov_f):
+        """likelihoods.py:401-412 vmapped over time (scalar sites only)."""
+        if cav_mean_f.shape[1] != 1:
+            raise NotImplementedError("EP moment matching is implemented for scalar sites")
+        y = self.Y.reshape(-1)
+        m = cav_mean_f.reshape(-1)
+        v = cav_cov_f.reshape(-1)
+        lZ, dlZ, d2lZ = self.likelihood.moment_match(y, m, v, self.power, self.gh_points)
+        Nn = cav_mean_f.shape[0]
+        return lZ.reshape(Nn), dlZ.reshape(Nn, 1, 1), d2lZ.reshape(Nn, 1, 1)
+
+    def update_variational_params(self, ensure_psd=True):
+        kind = self.inference_kind
+        if kind == "VI":
+            # inference.py:170-195
+            mean_f, cov_f = self.conditional_posterior_to_data()
+            _, dE, d2E = self._variational_expectation(mean_f, cov_f)
+            if ensure_psd:
+                d2E = -ensure_diagonal_positive_precision(-d2E)
+            return mean_f, dE, d2E
+        if kind == "Newton":
+            # inference.py:105-128
+            mean_f, _ = self.conditional_posterior_to_data()
+            _, J, H = self._log_likelihood_gradients(mean_f)
+            if ensure_psd:
+                H = -ensure_diagonal_positive_precision(-H)
+            return mean_f, J, H
+        if kind == "EP":
+            # inference.py:238-284
+            cav_mean, cav_cov = self.cavity_distribution(self.power)
+            cav_mean_f, cav_cov_f = self.conditional_posterior_to_data(cav_mean, cav_cov)
+            _, dlel, d2lel = self._moment_match(cav_mean_f, cav_cov_f)
+            cav_prec = inv(cav_cov_f)
+            scale_factor = cav_prec @ inv(d2lel + cav_prec) / self.power
+            dlel = scale_factor @ dlel
+            d2lel = scale_factor @ d2lel
+            if self.mask_pseudo_y is not None:
+                mask = self.mask_pseudo_y[..., None]
+                dlel = np.where(mask, np.nan, dlel)
+                d2m = np.where(mask | T(mask), 0.0, d2lel)
+                idx = np.arange(d2m.shape[-1])
+                dd = d2m[..., idx, idx]
+                d2m[..., idx, idx] = np.where(mask[..., 0], np.nan, dd)
+                d2lel = d2m
+            if ensure_psd:
+                d2lel = -ensure_diagonal_positive_precision(-d2lel)
+            return cav_mean_f, dlel, d2lel
+        raise NotImplementedError(kind)
+
+    def inference(self, lr=1.0, ensure_psd=True):
+        """inference.py:65-90"""
+        self.update_posterior()
+        mean, jac, hess = self.update_variational_params(ensure_psd=ensure_psd)
+        nat1_n, nat2_n = newton_update(mean, jac, hess)
+        diff1 = np.mean(np.abs(nat1_n - self.nat1))
+        diff2 = np.mean(np.abs(nat2_n - self.nat2))
+        self.update_nat_params((1 - lr) * self.nat1 + lr * nat1_n, (1 - lr) * self.nat2 + lr * nat2_n)
+        self.update_posterior()
+        return (mean, jac, hess), (diff1, diff2)
+
+    def energy(self):
+        kind = self.inference_kind
+        if kind == "VI":
+            # inference.py:197-222
+            mean_f, cov_f = self.conditional_posterior_to_data()
+            ell, _, _ = self._variational_expectation(mean_f, cov_f)
+            return -(np.nansum(ell) - self.compute_kl())
+        if kind == "Newton":
+            # inference.py:130-154
+            mean_f, _ = self.conditional_posterior_to_data()
+            ll, _, _ = self._log_likelihood_gradients(mean_f)
+            return -(np.nansum(ll) - self.compute_kl())
+        if kind == "EP":
+            # inference.py:286-325 with basemodels.py:247-262
+            power = self.power
+            cav_mean, cav_cov = self.cavity_distribution(power)
+            pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
+            mask = self._mask3()
+            lel_pseudo = mvn_logpdf(pseudo_y, cav_mean, pseudo_var / power + cav_cov, mask)
+            lel_pseudo = lel_pseudo + pep_constant(pseudo_var, power,
+                                                   None if mask is None else mask[..., 0])
+            lZ = self.compute_log_lik(pseudo_y, pseudo_var)
+            cav_mean_f, cav_cov_f = self.conditional_posterior_to_data(cav_mean, cav_cov)
+            lel, _, _ = self._moment_match(cav_mean_f, cav_cov_f)
+            lel = np.where(np.squeeze(self.mask_y), 0.0, np.squeeze(lel))
+            return -(lZ + 1.0 / power * (np.nansum(lel) - np.nansum(lel_pseudo)))
+        raise NotImplementedError(kind)
+
+
+class DenseGP:
+    """Kernel-matrix GP used to pin the Markov oracle (basemodels.py:265-330, ops.py:52-80).
+
+    Holds the same site parameterisation as MarkovGP; posterior and pseudo
+    log-normaliser come from dense Cholesky algebra, O(N^3).
+    """
+
+    def __init__(self, kernel, X, site_mean, site_var):
+        self.kernel = kernel
+        self.X = np.asarray(X, np.float64)
+        if self.X.ndim < 2:
+            self.X = self.X[:, None]
+        self.y = np.asarray(site_mean, np.float64).reshape(-1)
+        self.r = np.asarray(site_var, np.float64).reshape(-1)
+
+    def posterior(self):
+        """ops.py:52-80 gaussian_conditional."""
+        K = self.kernel(self.X, self.X)
+        Ky = K + np.diag(self.r)
+        L = np.linalg.cholesky(Ky)
+        A = np.linalg.solve(L, K)
+        alpha = np.linalg.solve(L.T, np.linalg.solve(L, self.y))
+        mean = K @ alpha
+        cov = K - A.T @ A
+        return mean, cov
+
+    def log_marginal(self):
+        """basemodels.py:307-330: log N(y | 0, K + R)."""
+        K = self.kernel(self.X, self.X)
+        Ky = K + np.diag(self.r)
+        L = np.linalg.cholesky(Ky)
+        z = np.linalg.solve(L, self.y)
+        return -0.5 * z @ z - np.sum(np.log(np.diag(L))) - 0.5 * len(self.y) * LOG2PI

@@ -1,0 +1,211 @@
+"""FP64 Kalman filter / RTS smoother of the oracle (test infrastructure only).
+
+Restates bayesnewton/ops.py:149-380 of the reference: the sequential
+(lax.scan) and associative-scan forms of the filter and the smoother.
+Shapes follow the reference: y [N,p,1], noise_cov [N,p,p], means [N,d,1],
+covs [N,d,d].
+"""
+import numpy as np
+
+from .utils import mvn_logpdf, solve, inv
+
+
+def T(P):
+    return np.swapaxes(P, -1, -2)
+
+
+def process_noise_covariance(A, Pinf):
+    """ops.py:149-151: Q = Pinf - A Pinf A'."""
+    return Pinf - A @ Pinf @ T(A)
+
+
+def discretise(kernel, dt):
+    """ops.py:274-279: As = vmap(state_transition)(dt), Qs = Pinf - A Pinf A'."""
+    Pinf = kernel.stationary_covariance()
+    As = kernel.state_transition_batch(dt)
+    Qs = process_noise_covariance(As, Pinf)
+    return Pinf, As, Qs
+
+
+def _sequential_kf(As, Qs, H, ys, noise_covs, m0, P0, masks, return_predict=False):
+    """ops.py:154-180"""
+    N = As.shape[0]
+    d = As.shape[1]
+    m, P, ell = m0, P0, 0.0
+    out_m = np.empty((N, d, 1))
+    out_P = np.empty((N, d, d))
+    for n in range(N):
+        A, Q = As[n], Qs[n]
+        m_ = A @ m
+        P_ = A @ P @ A.T + Q
+        obs_mean = H @ m_
+        HP = H @ P_
+        S = HP @ H.T + noise_covs[n]
+        ell += float(mvn_logpdf(ys[n], obs_mean, S, masks[n]))
+        K = solve(S, HP).T
+        m = m_ + K @ (ys[n] - obs_mean)
+        P = P_ - K @ HP
+        if return_predict:
+            out_m[n], out_P[n] = m_, P_
+        else:
+            out_m[n], out_P[n] = m, P
+    return ell, out_m, out_P
+
+
+def parallel_filtering_element(A, Q, H, noise_cov, y):
+    """ops.py:183-200 (vectorised over the leading axis)."""
+    HQ, HA = H @ Q, H @ A
+    S = HQ @ H.T + noise_cov
+    SinvH = solve(S, np.broadcast_to(H, S.shape[:-2] + H.shape))
+    K = Q @ T(SinvH)
+    AA = A - K @ HA
+    b = K @ y
+    C = Q - K @ HQ
+    SinvHA = T(SinvH @ A)
+    eta = SinvHA @ y
+    J = SinvHA @ HA
+    return AA, b, C, J, eta
+
+
+def parallel_filtering_operator(elem1, elem2):
+    """ops.py:203-219 (vectorised over the leading axis)."""
+    A1, b1, C1, J1, eta1 = elem1
+    A2, b2, C2, J2, eta2 = elem2
+    d = A1.shape[-1]
+    C1inv = inv(C1)
+    temp = np.linalg.solve(C1inv + J2, C1inv)
+    A2temp = A2 @ temp
+    AA = A2temp @ A1
+    b = A2temp @ (b1 + C1 @ eta2) + b2
+    C = A2temp @ C1 @ T(A2) + C2
+    A1temp = T(A1) @ T(temp)
+    eta = A1temp @ (eta2 - J2 @ b1) + eta1
+    J = A1temp @ J2 @ A1 + J1
+    return AA, b, C, J, eta
+
+
+def make_associative_filtering_elements(As, Qs, H, ys, noise_covs, m0, P0):
+    """ops.py:222-229"""
+    Qs = Qs.copy()
+    Qs[0] = P0
+    AA, b, C, J, eta = parallel_filtering_element(As, Qs, H, noise_covs, ys)
+    S = H @ Qs[0] @ H.T + noise_covs[0]
+    K0 = solve(S, H @ Qs[0]).T
+    b = b.copy()
+    b[0] = b[0] + m0 - K0 @ H @ m0
+    return AA, b, C, J, eta
+
+
+def associative_scan(op, elems, reverse=False):
+    """Inclusive scan with an associative operator (Hillis-Steele, vectorised).
+
+    Stands in for jax.lax.associative_scan (ops.py:6): the same prefix results
+    in exact arithmetic (the combine order differs, so results agree to
+    rounding).  op(a, b) combines a (earlier) with b (later), as in the
+    reference's filtering operator; with reverse=True the scan runs from the
+    end and op(a, b) receives the later element first, as in ops.py:348.
+    """
+    elems = [np.array(e, dtype=np.float64) for e in elems]
+    N = elems[0].shape[0]
+    if reverse:
+        elems = [e[::-1].copy() for e in elems]
+    shift = 1
+    while shift < N:
+        prev = [e[:-shift] for e in elems]
+        cur = [e[shift:] for e in elems]
+        comb = op(prev, cur)
+        elems = [np.concatenate([e[:shift], c], axis=0) for e, c in zip(elems, comb)]
+        shift *= 2
+    if reverse:
+        elems = [e[::-1].copy() for e in elems]
+    return elems
+
+
+def _parallel_kf(As, Qs, H, ys, noise_covs, m0, P0, masks, return_predict=False):
+    """ops.py:237-253"""
+    init = make_associative_filtering_elements(As, Qs, H, ys, noise_covs, m0, P0)
+    final = associative_scan(parallel_filtering_operator, init)
+    fms, fPs = final[1], final[2]
+    mpredict = As @ np.concatenate([m0[None], fms[:-1]])
+    Ppredict = As @ np.concatenate([P0[None], fPs[:-1]]) @ T(As) + Qs
+    loglik = float(np.sum(mvn_logpdf(ys, H @ mpredict, H @ Ppredict @ H.T + noise_covs, masks)))
+    if return_predict:
+        return loglik, mpredict, Ppredict
+    return loglik, fms, fPs
+
+
+def kalman_filter(dt, kernel, y, noise_cov, mask=None, parallel=False, return_predict=False):
+    """ops.py:256-285"""
+    y = np.asarray(y, dtype=np.float64)
+    noise_cov = np.asarray(noise_cov, dtype=np.float64)
+    if mask is None:
+        mask = np.zeros_like(y, dtype=bool)
+    mask = np.asarray(mask, dtype=bool).reshape(y.shape)
+    Pinf, As, Qs = discretise(kernel, dt)
+    minf = np.zeros([Pinf.shape[0], 1])
+    H = kernel.measurement_model()
+    f = _parallel_kf if parallel else _sequential_kf
+    ell, means, covs = f(As, Qs, H, y, noise_cov, minf, Pinf, mask, return_predict=return_predict)
+    return ell, (means, covs)
+
+
+def _sequential_rts(fms, fPs, As, Qs, H, return_full):
+    """ops.py:288-311"""
+    N, d = fms.shape[0], fms.shape[1]
+    sm, sP = fms[-1], fPs[-1]
+    sms = np.empty((N,) + (fms.shape[1:] if return_full else (H.shape[0], 1)))
+    sPs = np.empty((N,) + (fPs.shape[1:] if return_full else (H.shape[0], H.shape[0])))
+    gains = np.empty((N, d, d))
+    for n in range(N - 1, -1, -1):
+        fm, fP, A, Q = fms[n], fPs[n], As[n], Qs[n]
+        pm = A @ fm
+        AfP = A @ fP
+        pP = AfP @ A.T + Q
+        C = solve(pP, AfP).T
+        sm = fm + C @ (sm - pm)
+        sP = fP + C @ (sP - pP) @ C.T
+        gains[n] = C
+        if return_full:
+            sms[n], sPs[n] = sm, sP
+        else:
+            sms[n], sPs[n] = H @ sm, H @ sP @ H.T
+    return sms, sPs, gains
+
+
+def parallel_smoothing_element(A, Q, m, P):
+    """ops.py:318-325"""
+    Pp = A @ P @ T(A) + Q
+    E = T(solve(Pp, A @ P))
+    g = m - E @ A @ m
+    L = P - E @ Pp @ T(E)
+    return E, g, L
+
+
+def parallel_smoothing_operator(elem1, elem2):
+    """ops.py:328-335 - elem1 is the later element in time (reverse scan)."""
+    E1, g1, L1 = elem1
+    E2, g2, L2 = elem2
+    return E2 @ E1, E2 @ g1 + g2, E2 @ L1 @ T(E2) + L2
+
+
+def _parallel_rts(fms, fPs, As, Qs, H, return_full):
+    """ops.py:338-354"""
+    E, g, L = parallel_smoothing_element(As, Qs, fms, fPs)
+    gains = E.copy()
+    E = E.copy(); g = g.copy(); L = L.copy()
+    E[-1] = 0.0
+    g[-1] = fms[-1]
+    L[-1] = fPs[-1]
+    final = associative_scan(parallel_smoothing_operator, [E, g, L], reverse=True)
+    sms, sPs = final[1], final[2]
+    if return_full:
+        return sms, sPs, gains
+    return H @ sms, H @ sPs @ H.T, gains
+
+
+def rauch_tung_striebel_smoother(dt, kernel, filter_mean, filter_cov, return_full=False, parallel=False):
+    """ops.py:357-380 (dt is the already-shifted step vector)."""
+    Pinf, As, Qs = discretise(kernel, dt)
+    H = kernel.measurement_model()
+    f = _parallel_rts if parallel else _sequential_rts
+    return f(np.asarray(filter_mean, np.float64), np.asarray(filter_cov, np.float64), As, Qs, H, return_full)

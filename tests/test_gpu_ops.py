@@ -1,0 +1,137 @@
+"""GPU parity of the ops.kalman_filter / ops.rauch_tung_striebel_smoother
+drop-ins (bayesnewton/ops.py:256-285, 357-380) against the FP64 oracle.
+
+Tolerances (north star): <= 1e-4 relative on posterior means and variances,
+<= 1e-5 relative on the log-likelihood.  Means are compared relative to the
+series' scale (max |mean|), variances and log-likelihoods pointwise.  The GPU
+path takes FP32 per-step inputs, so both sides get the same FP32-rounded
+inputs: the comparison measures the kernels, not the input rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle import kernels as OK, ops as OO, cfast
+
+pytestmark = pytest.mark.gpu
+
+MEAN_TOL, VAR_TOL, LL_TOL = 1e-4, 1e-4, 1e-5
+
+
+def _kern_pair(spec):
+    from bayesnewton_amd import kernels as K
+    if spec == "m52":
+        return K.Matern52(variance=1.0, lengthscale=1.0), OK.Matern52(1.0, 1.0)
+    if spec == "m32":
+        return K.Matern32(variance=0.7, lengthscale=2.0), OK.Matern32(0.7, 2.0)
+    if spec == "m32+m52":
+        return (K.Sum([K.Matern32(variance=0.5, lengthscale=3.0), K.Matern52(variance=1.0, lengthscale=1.0)]),
+                OK.Sum([OK.Matern32(0.5, 3.0), OK.Matern52(1.0, 1.0)]))
+    if spec == "m52+m32":
+        return (K.Sum([K.Matern52(variance=1.0, lengthscale=2.0), K.Matern32(variance=0.5, lengthscale=0.7)]),
+                OK.Sum([OK.Matern52(1.0, 2.0), OK.Matern32(0.5, 0.7)]))
+    raise ValueError(spec)
+
+
+def _data(n, seed, span=100.0, mask_frac=0.0):
+    rng = np.random.default_rng(seed)
+    x = np.sort(rng.uniform(0, span, n))
+    y = np.sin(0.3 * x) + 0.5 * np.sin(1.7 * x) + 0.1 * rng.standard_normal(n)
+    dt = np.concatenate([[0.0], np.diff(x)])
+    r = 0.01 + 0.05 * rng.uniform(size=n)
+    mask = rng.uniform(size=n) < mask_frac
+    return _r32(dt), _r32(y), _r32(r), mask
+
+
+def _r32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("spec", ["m52", "m32", "m32+m52", "m52+m32"])
+@pytest.mark.parametrize("parallel", [False, True])
+def test_filter_smoother_vs_numpy_oracle(spec, parallel):
+    n = 700
+    dt, y, r, mask = _data(n, 11, mask_frac=0.05)
+    kern, okern = _kern_pair(spec)
+    from bayesnewton_amd import ops
+    ell, (fm, fP) = ops.kalman_filter(dt, kern, y[:, None, None], r[:, None, None], mask=mask[:, None, None],
+                                      parallel=parallel)
+    oell, (ofm, ofP) = OO.kalman_filter(dt, okern, y[:, None, None], r[:, None, None], mask=mask[:, None, None])
+    assert abs(float(ell) - oell) / abs(oell) < LL_TOL
+    fm, fP = fm.cpu().numpy(), fP.cpu().numpy()
+    # full filtered state: compare relative to the scale of each component
+    assert np.max(np.abs(fm - ofm)) / np.max(np.abs(ofm)) < MEAN_TOL
+    assert np.max(np.abs(fP - ofP)) / np.max(np.abs(ofP)) < VAR_TOL
+    dts = np.concatenate([dt[1:], [0.0]])
+    sm, sP, G = ops.rauch_tung_striebel_smoother(dts, kern, ofm, ofP, parallel=parallel)
+    osm, osP, oG = OO.rauch_tung_striebel_smoother(dts, okern, ofm, ofP)
+    sm, sP = sm.cpu().numpy().reshape(-1), sP.cpu().numpy().reshape(-1)
+    assert np.max(np.abs(sm - osm.reshape(-1))) / np.max(np.abs(osm)) < MEAN_TOL
+    assert np.max(np.abs(sP - osP.reshape(-1)) / np.abs(osP.reshape(-1))) < VAR_TOL
+    assert np.max(np.abs(G.cpu().numpy()[:-1] - oG[:-1])) < 1e-6 * max(1.0, np.max(np.abs(oG)))
+
+
+def test_return_predict_and_full():
+    n = 300
+    dt, y, r, _ = _data(n, 5)
+    kern, okern = _kern_pair("m52")
+    from bayesnewton_amd import ops
+    ell, (pm, pP) = ops.kalman_filter(dt, kern, y[:, None, None], r[:, None, None], return_predict=True, parallel=True)
+    oell, (opm, opP) = OO.kalman_filter(dt, okern, y[:, None, None], r[:, None, None], return_predict=True)
+    assert np.max(np.abs(pm.cpu().numpy() - opm)) / np.max(np.abs(opm)) < MEAN_TOL
+    assert np.max(np.abs(pP.cpu().numpy() - opP)) / np.max(np.abs(opP)) < VAR_TOL
+    _, (ofm, ofP) = OO.kalman_filter(dt, okern, y[:, None, None], r[:, None, None])
+    dts = np.concatenate([dt[1:], [0.0]])
+    sm, sP, _ = ops.rauch_tung_striebel_smoother(dts, kern, ofm, ofP, return_full=True, parallel=True)
+    osm, osP, _ = OO.rauch_tung_striebel_smoother(dts, okern, ofm, ofP, return_full=True)
+    assert np.max(np.abs(sm.cpu().numpy() - osm)) / np.max(np.abs(osm)) < MEAN_TOL
+    assert np.max(np.abs(sP.cpu().numpy() - osP)) / np.max(np.abs(osP)) < VAR_TOL
+
+
+def test_discretise_matches_closed_form():
+    from bayesnewton_amd import ops
+    kern, okern = _kern_pair("m52+m32")
+    dt = _r32(np.array([0.0, 1e-3, 0.01, 0.3, 2.0, 10.0]))
+    A, Q = ops.discretise(kern, dt)
+    Pinf, oA, oQ = OO.discretise(okern, dt)
+    assert np.max(np.abs(A.cpu().numpy() - oA)) < 1e-6
+    assert np.max(np.abs(Q.cpu().numpy() - oQ)) < 1e-6 * np.max(np.abs(Pinf))
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 16, 64])
+def test_batched_uneven_fused_vs_c_oracle(chunk):
+    """Fused update_posterior path on many ragged series (plan + tiled layout)."""
+    import torch
+    from bayesnewton_amd import kernels as K, ops
+    from bayesnewton_amd.plan import SeriesPlan
+    rng = np.random.default_rng(100 + chunk)
+    lens = np.concatenate([[1, 2, 5], rng.integers(50, 900, size=37)])
+    off = np.concatenate([[0], np.cumsum(lens)])
+    N = int(off[-1])
+    dt = np.zeros(N); y = np.zeros(N)
+    for b in range(len(lens)):
+        n = lens[b]
+        x = np.sort(rng.uniform(0, 0.1 * n + 1, n))
+        dt[off[b]:off[b + 1]] = np.concatenate([[0.0], np.diff(x)])
+        y[off[b]:off[b + 1]] = np.sin(x) + 0.2 * rng.standard_normal(n)
+    r = 0.04 + 0.02 * rng.uniform(size=N)
+    dt, y, r = _r32(dt), _r32(y), _r32(r)
+    mask = rng.uniform(size=N) < 0.03
+    B = len(lens)
+    ls = np.stack([rng.uniform(0.5, 5.0, B), rng.uniform(0.5, 5.0, B)], axis=1)
+    var = np.stack([np.full(B, 1.0), np.full(B, 0.5)], axis=1)
+    kern = K.Sum([K.Matern52(), K.Matern32()])
+    dev = torch.device("cuda")
+    plan = SeriesPlan(off, chunk_len=chunk, device=dev)
+    t = lambda a, dtp=torch.float32: torch.as_tensor(a, dtype=dtp, device=dev)
+    dt_t, y_t, r_t = plan.gather(t(dt)), plan.gather(t(y)), plan.gather(t(r), pad=1.0)
+    mk = plan.gather_mask(t(mask, torch.bool))
+    pm, pv, ll = ops.filter_smoother(plan, kern.desc(), t(ls, torch.float64), t(var, torch.float64), dt_t, y_t, r_t,
+                                     mk, hyper_stride=2)
+    pm = plan.scatter(pm).cpu().numpy().reshape(-1)
+    pv = plan.scatter(pv).cpu().numpy().reshape(-1)
+    om, ov, oll = cfast.filter_smoother(["matern52", "matern32"], ls, var, off, dt, y, r, mask)
+    for b in range(B):
+        s = slice(off[b], off[b + 1])
+        assert np.max(np.abs(pm[s] - om[s])) / max(np.max(np.abs(om[s])), 1e-3) < MEAN_TOL, b
+    assert np.max(np.abs(pv - ov) / np.abs(ov)) < VAR_TOL
+    assert np.max(np.abs(ll.cpu().numpy() - oll) / np.abs(oll)) < LL_TOL
