@@ -26,6 +26,14 @@ class KernelDesc(ctypes.Structure):
     _fields_ = [("n_comp", i32), ("kind", i32 * 2)]
 
 
+class LikDesc(ctypes.Structure):
+    _fields_ = [("kind", i32), ("param0", f64), ("param1", f64)]
+
+
+BN_LIK_GAUSSIAN, BN_LIK_BERNOULLI_PROBIT, BN_LIK_POISSON, BN_LIK_STUDENTS_T = 1, 2, 3, 4
+BN_VI, BN_NEWTON, BN_EP = 0, 1, 2
+
+
 # name -> (restype, argtypes); every symbol include/bn_hip.h declares
 SIGNATURES = {
     "bn_last_error": (ctypes.c_char_p, []),
@@ -44,7 +52,17 @@ SIGNATURES = {
     "bn_discretise": (i32, [vp, vp, vp, i64, vp, vp, vp, vp]),
     "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    "bn_filter_loglik": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "bn_rts_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    "bn_site_update": (i32, [vp, vp, vp, i32, i32, f64, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                              vp, vp]),
+    "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_site_reparametrise": (i32, [vp, vp, vp, vp, vp, vp]),
+    "bn_gauss_hermite": (i32, [i32, vp, vp]),
+    "bn_timing_enable": (i32, [i32]),
+    "bn_timing_read": (i32, [i32, ctypes.POINTER(f64), ctypes.POINTER(i32)]),
+    "bn_kernel_name": (ctypes.c_char_p, [i32]),
+    "bn_num_kernel_ids": (i32, []),
 }
 
 _lib = None

@@ -312,14 +312,19 @@ BN_HD void smooth_terminal(SElem<D, T>& acc, const T fm[D], const T fP[D][D]) {
   set_zero<D>(acc.E);
 }
 
-// predict() that also returns A P (needed for the RTS gain)
+// Prediction without Q: A P A' + Q = A (P - Pinf) A' + Pinf exactly, since
+// Q = Pinf - A Pinf A' (ops.py:149-151).  Both forms carry rounding of order
+// eps |Pinf|, as the reference's own Q does; the filter and smoother passes
+// start from a filtered state, so only their absolute accuracy matters.  Also
+// returns A P (needed for the RTS gain).
 template <int K1, int K2, class T>
-BN_HD void predict_ap(const T A[Spec<K1, K2>::D][Spec<K1, K2>::D], const T Q[Spec<K1, K2>::D][Spec<K1, K2>::D],
+BN_HD void predict_ap(const T A[Spec<K1, K2>::D][Spec<K1, K2>::D], const double Pinf[Spec<K1, K2>::D][Spec<K1, K2>::D],
                       const T m[Spec<K1, K2>::D], const T P[Spec<K1, K2>::D][Spec<K1, K2>::D],
                       T mp[Spec<K1, K2>::D], T Pp[Spec<K1, K2>::D][Spec<K1, K2>::D],
                       T AP[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
   using S = Spec<K1, K2>;
   constexpr int D = S::D;
+  T AD[D][D];
   BN_UNROLL for (int i = 0; i < D; ++i) {
     T s = T(0);
     BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * m[k];
@@ -327,16 +332,36 @@ BN_HD void predict_ap(const T A[Spec<K1, K2>::D][Spec<K1, K2>::D], const T Q[Spe
   }
   BN_UNROLL for (int i = 0; i < D; ++i)
     BN_UNROLL for (int c = 0; c < D; ++c) {
-      T s = T(0);
-      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) s += A[i][k] * P[k][c];
+      T s = T(0), sp = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) {
+        s += A[i][k] * P[k][c];
+        if (S::nz(k, c)) sp += A[i][k] * (T)Pinf[k][c];
+      }
       AP[i][c] = s;
+      AD[i][c] = s - sp;
     }
   BN_UNROLL for (int i = 0; i < D; ++i)
     BN_UNROLL for (int c = i; c < D; ++c) {
-      T s = S::nz(i, c) ? Q[i][c] : T(0);
-      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(c, k)) s += AP[i][k] * A[c][k];
+      T s = S::nz(i, c) ? (T)Pinf[i][c] : T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(c, k)) s += AD[i][k] * A[c][k];
       Pp[i][c] = s;
       Pp[c][i] = s;
+    }
+}
+
+// A only (the passes never need Q)
+template <int K1, int K2, class T>
+BN_HD void transition_only(const Model<K1, K2>& mdl, double dt, T A[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
+  constexpr int D = Spec<K1, K2>::D;
+  double a1[9], a2[9];
+  comp_transition<K1>(mdl.lam1, dt, a1);
+  if (K2 > 0) comp_transition<(K2 > 0 ? K2 : 1)>(mdl.lam2, dt, a2);
+  BN_UNROLL for (int i = 0; i < D; ++i)
+    BN_UNROLL for (int j = 0; j < D; ++j) {
+      double v = 0.0;
+      if (i < K1 && j < K1) v = a1[i * K1 + j];
+      else if (i >= K1 && j >= K1) v = a2[(i - K1) * (K2 > 0 ? K2 : 1) + (j - K1)];
+      A[i][j] = (T)v;
     }
 }
 
@@ -349,8 +374,13 @@ struct FilterPassOut {
 
 // filter_pass: the filter over one chunk from its true incoming state
 // (ops.py:156-175), writing moments, the log-likelihood partial and the
-// chunk's smoothing element (the smoother uses dt of the NEXT step,
-// basemodels.py:700).
+// chunk's smoothing element.  The smoothing element x_s | x_{e+1}, y is built
+// by a fixed-point smoother run alongside the filter: with the chunk's first
+// step s, it tracks mu = E[x_s | y_1..k], V = Var(x_s | y_1..k) and
+// C = Cov(x_s, x_k | y_1..k) (rank-1 updates since p = 1); conditioning the
+// joint of (x_s, x_{e+1}) given y_1..e then gives E = C- Pp^-1, g = mu - E mp,
+// L = V - E C-', the element of ops.py:318-335 composed over the chunk, with
+// one solve per chunk.  The smoother uses dt of the NEXT step (basemodels.py:700).
 template <int K1, int K2, class T>
 BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                             const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
@@ -368,38 +398,108 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     BN_UNROLL for (int i = 0; i < D; ++i) { m[i] = T(0); BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = T(0); }
   }
   const bool want_s = out.sagg.base != nullptr;
-  SElem<D, T> acc;
-  acc.set_identity_elem();
+  T mu[D], V[D][D], C[D][D];  // fixed-point smoother of x_s
   double ell = 0.0;
   for (int32_t k = 0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
-    T mp[D], Pp[D][D], AP[D][D];
+    T mp[D], Pp[D][D], AP[D][D], Cm[D][D];
     if (k == 0 && (fl & CH_FIRST)) {
       BN_UNROLL for (int i = 0; i < D; ++i) {
         mp[i] = T(0);
         BN_UNROLL for (int c = 0; c < D; ++c) Pp[i][c] = (T)mdl.Pinf[i][c];
       }
     } else {
-      T A[D][D], Q[D][D];
-      mdl.discretise(dt[s], A, Q);
-      predict_ap<K1, K2, T>(A, Q, m, P, mp, Pp, AP);
-      if (want_s && k > 0) smooth_accumulate<D, T>(acc, m, P, AP, mp, Pp);
+      T A[D][D];
+      transition_only<K1, K2, T>(mdl, dt[s], A);
+      predict_ap<K1, K2, T>(A, mdl.Pinf, m, P, mp, Pp, AP);
+      if (want_s && k > 0) {
+        // C- = C A'
+        BN_UNROLL for (int i = 0; i < D; ++i)
+          BN_UNROLL for (int c = 0; c < D; ++c) {
+            T t = T(0);
+            BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
+            Cm[i][c] = t;
+          }
+      }
     }
-    const double lp = update<K1, K2, T>(mp, Pp, y[s], r[s], m, P);
+    const double lp = update<K1, K2, T>(mp, Pp, (T)y[s], (T)r[s], m, P);
     if (!(mask != nullptr && mask[s])) ell += lp;
+    if (want_s) {
+      if (k == 0) {
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          mu[i] = m[i];
+          BN_UNROLL for (int c = 0; c < D; ++c) { V[i][c] = P[i][c]; C[i][c] = P[i][c]; }
+        }
+      } else {
+        // ch = C- H', hP = H Pp, S = H Pp H' + r, v = y - H mp
+        T ch[D], hP[D];
+        double Sd = (double)r[s];
+        T hm = T(0);
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          T a = T(0), bq = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { a += Cm[i][q]; bq += Pp[q][i]; }
+          ch[i] = a;
+          hP[i] = bq;
+        }
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += mp[q]; }
+        const T iS = (T)(1.0 / Sd);
+        const T vS = ((T)y[s] - hm) * iS;
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          mu[i] += ch[i] * vS;
+          BN_UNROLL for (int c = i; c < D; ++c) {
+            const T t = V[i][c] - ch[i] * ch[c] * iS;
+            V[i][c] = t;
+            V[c][i] = t;
+          }
+          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = Cm[i][c] - ch[i] * hP[c] * iS;
+        }
+      }
+    }
     if (out.state.mean != nullptr) {
       if (out.predict) write_state<K1, K2, T>(out.state, pl.n_slots, s, mp, Pp);
       else write_state<K1, K2, T>(out.state, pl.n_slots, s, m, P);
     }
   }
   if (want_s) {
+    SElem<D, T> acc;
     if (fl & CH_LAST) {
-      smooth_terminal<D, T>(acc, m, P);
+      // terminal: x_s | y_1..N is a state (ops.py:314-315)
+      set_zero<D, T>(acc.E);
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        acc.g[i] = mu[i];
+        BN_UNROLL for (int c = 0; c < D; ++c) acc.L[i][c] = V[i][c];
+      }
     } else {
-      T A[D][D], Q[D][D], mp[D], Pp[D][D], AP[D][D];
-      mdl.discretise(dt[pl.slot(j + 1, 0)], A, Q);
-      predict_ap<K1, K2, T>(A, Q, m, P, mp, Pp, AP);
-      smooth_accumulate<D, T>(acc, m, P, AP, mp, Pp);
+      T A[D][D], mp[D], Pp[D][D], AP[D][D], Cn[D][D];
+      transition_only<K1, K2, T>(mdl, dt[pl.slot(j + 1, 0)], A);
+      predict_ap<K1, K2, T>(A, mdl.Pinf, m, P, mp, Pp, AP);
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) {
+          T t = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
+          Cn[i][c] = t;
+        }
+      // E = Cn Pp^-1  (E' = Pp^-1 Cn', Cholesky of Pp in FP64)
+      double Lc[D][D], X[D][D];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) { Lc[i][c] = (double)Pp[i][c]; X[i][c] = (double)Cn[c][i]; }
+      chol_spd<D>(Lc);
+      lower_solve<D, D, double>(Lc, X);
+      upper_t_solve<D, D, double>(Lc, X);
+      BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) acc.E[i][c] = (T)X[c][i];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = mu[i];
+        BN_UNROLL for (int q = 0; q < D; ++q) t -= acc.E[i][q] * mp[q];
+        acc.g[i] = t;
+      }
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = i; c < D; ++c) {
+          T t1 = T(0), t2 = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) { t1 += acc.E[i][q] * Cn[c][q]; t2 += acc.E[c][q] * Cn[i][q]; }
+          const T t = V[i][c] - T(0.5) * (t1 + t2);
+          acc.L[i][c] = t;
+          acc.L[c][i] = t;
+        }
     }
     soa_store(acc, out.sagg, j);
   }
@@ -426,9 +526,9 @@ BN_HD void lane_smoother_reduce(const PlanView& pl, const HyperView& hv, const f
     if (k == L - 1 && (fl & CH_LAST)) {
       smooth_terminal<D, T>(acc, fm, fP);
     } else {
-      T A[D][D], Q[D][D], mp[D], Pp[D][D], AP[D][D];
-      mdl.discretise(dts[s], A, Q);
-      predict_ap<K1, K2, T>(A, Q, fm, fP, mp, Pp, AP);
+      T A[D][D], mp[D], Pp[D][D], AP[D][D];
+      transition_only<K1, K2, T>(mdl, dts[s], A);
+      predict_ap<K1, K2, T>(A, mdl.Pinf, fm, fP, mp, Pp, AP);
       smooth_accumulate<D, T>(acc, fm, fP, AP, mp, Pp);
     }
   }
@@ -500,9 +600,9 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
     double dtn;
     if (io.shifted) dtn = io.dt[s];
     else dtn = terminal ? 0.0 : (double)io.dt[k + 1 < L ? pl.slot(j, k + 1) : pl.slot(j + 1, 0)];
-    T A[D][D], Q[D][D], pm[D], pP[D][D], AfP[D][D];
-    mdl.discretise(dtn, A, Q);
-    predict_ap<K1, K2, T>(A, Q, fm, fP, pm, pP, AfP);
+    T A[D][D], pm[D], pP[D][D], AfP[D][D];
+    transition_only<K1, K2, T>(mdl, dtn, A);
+    predict_ap<K1, K2, T>(A, mdl.Pinf, fm, fP, pm, pP, AfP);
     rts_gain<D, T>(AfP, pP, G);
     if (terminal) {
       BN_UNROLL for (int i = 0; i < D; ++i) {

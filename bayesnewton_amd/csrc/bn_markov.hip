@@ -17,10 +17,9 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
-#include "../../include/bn_hip.h"
-#include "bn_lanes.h"
-#include "bn_plan_host.h"
+#include "bn_internal.h"
 
 using namespace bn;
 
@@ -38,16 +37,46 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(BN_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+// ---- optional per-kernel timing (measurement only; not thread-safe) --------
+enum KernelId : int32_t {
+  KID_FILTER_REDUCE = 0, KID_FSCAN_TILE, KID_FSCAN_SERIES, KID_FILTER_PASS, KID_SMOOTHER_REDUCE,
+  KID_SSCAN_TILE, KID_SSCAN_SERIES, KID_SMOOTHER_PASS, KID_SERIES_SUM, KID_COUNT
+};
+const char* const kKernelNames[KID_COUNT] = {
+    "k_filter_reduce", "k_fscan_tile", "k_fscan_series", "k_filter_pass", "k_smoother_reduce",
+    "k_sscan_tile", "k_sscan_series", "k_smoother_pass", "k_series_sum"};
+struct TimingRec { int32_t kid; hipEvent_t a, b; };
+bool g_timing = false;
+std::vector<TimingRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t pool_event() {
+  if (!g_pool.empty()) { hipEvent_t e = g_pool.back(); g_pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets a launch with events on its stream when timing is enabled.
+struct TimedLaunch {
+  int32_t kid; hipStream_t st; hipEvent_t a = nullptr;
+  TimedLaunch(int32_t k, hipStream_t s) : kid(k), st(s) {
+    if (g_timing && (a = pool_event()) != nullptr) (void)hipEventRecord(a, st);
+  }
+  ~TimedLaunch() {
+    if (a == nullptr) return;
+    hipEvent_t b = pool_event();
+    if (b == nullptr) { g_pool.push_back(a); return; }
+    (void)hipEventRecord(b, st);
+    g_recs.push_back(TimingRec{kid, a, b});
+  }
+};
+#define BN_TIMED(kid, st) TimedLaunch bn_tl_##kid(kid, st)
+
 constexpr int kBlock = 256;  // lane-per-chunk kernels
 constexpr int kScan = kTile;  // scan kernels: one thread per element of a tile
 using CT = double;  // compute type of the recurrences (see bn_lanes.h)
 }  // namespace
-
-struct bn_plan {
-  PlanHost host;
-  PlanView view;
-  void* dev = nullptr;  // one allocation holding every table
-};
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -308,7 +337,7 @@ int check_desc(const bn_kernel_desc* k) {
 }
 
 struct WsLayout {
-  int64_t fagg, ftagg, ftcarry, sagg, stagg, stcarry, fbuf, ell, total;
+  int64_t fagg, ftagg, ftcarry, sagg, stagg, stcarry, fbuf, ell, sp1, sp2, total;
 };
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -329,6 +358,8 @@ WsLayout ws_layout(const PlanHost& p, int D) {
   w.stcarry = o; o = align256(o + SA * nt * ea);
   w.fbuf = o;    o = align256(o + (D + NS) * p.n_slots * es);
   w.ell = o;     o = align256(o + p.n_chunks_pad * 8);
+  w.sp1 = o;     o = align256(o + p.n_chunks_pad * 8);
+  w.sp2 = o;     o = align256(o + p.n_chunks_pad * 8);
   w.total = o;
   return w;
 }
@@ -369,9 +400,12 @@ void launch_filter_scan(const bn_plan* plan, HyperView hv, const float* dt, cons
   const PlanHost& p = plan->host;
   if (!needs_scan(p)) return;
   const size_t ldsF = sizeof(agg_t) * FElem<D, CT>::NF * kScan;
-  hipLaunchKernelGGL((k_filter_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, b.fagg);
-  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsF, st, plan->view, b.fagg, b.ftagg);
-  hipLaunchKernelGGL((k_fscan_series<D>), dim3(p.n_series), dim3(kScan), ldsF, st, plan->view, b.ftagg, b.ftcarry);
+  { BN_TIMED(KID_FILTER_REDUCE, st);
+  hipLaunchKernelGGL((k_filter_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, b.fagg); }
+  { BN_TIMED(KID_FSCAN_TILE, st);
+  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsF, st, plan->view, b.fagg, b.ftagg); }
+  { BN_TIMED(KID_FSCAN_SERIES, st);
+  hipLaunchKernelGGL((k_fscan_series<D>), dim3(p.n_series), dim3(kScan), ldsF, st, plan->view, b.ftagg, b.ftcarry); }
 }
 
 template <int D>
@@ -379,8 +413,10 @@ void launch_smoother_scan(const bn_plan* plan, const Bufs& b, hipStream_t st) {
   const PlanHost& p = plan->host;
   if (!needs_scan(p)) return;
   const size_t ldsS = sizeof(agg_t) * SElem<D, CT>::NF * kScan;
-  hipLaunchKernelGGL((k_sscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsS, st, plan->view, b.sagg, b.stagg);
-  hipLaunchKernelGGL((k_sscan_series<D>), dim3(p.n_series), dim3(kScan), ldsS, st, plan->view, b.stagg, b.stcarry);
+  { BN_TIMED(KID_SSCAN_TILE, st);
+  hipLaunchKernelGGL((k_sscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsS, st, plan->view, b.sagg, b.stagg); }
+  { BN_TIMED(KID_SSCAN_SERIES, st);
+  hipLaunchKernelGGL((k_sscan_series<D>), dim3(p.n_series), dim3(kScan), ldsS, st, plan->view, b.stagg, b.stcarry); }
 }
 
 template <int K1, int K2>
@@ -395,8 +431,11 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   fo.predict = false;
   fo.sagg = b.sagg;
   fo.ell = b.ell;
-  hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
-                     cs(b.fagg), cs(b.ftcarry), fo);
+  {
+    BN_TIMED(KID_FILTER_PASS, st);
+    hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
+                       cs(b.fagg), cs(b.ftcarry), fo);
+  }
   launch_smoother_scan<D>(plan, b, st);
   SmootherPassIO<float> io;
   io.fin = CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
@@ -406,9 +445,15 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   io.out_cov = pvar;
   io.gains = nullptr;
   io.full = false;
-  hipLaunchKernelGGL((k_smoother_pass<K1, K2, float>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, cs(b.sagg),
-                     cs(b.stcarry), io);
-  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  {
+    BN_TIMED(KID_SMOOTHER_PASS, st);
+    hipLaunchKernelGGL((k_smoother_pass<K1, K2, float>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv,
+                       cs(b.sagg), cs(b.stcarry), io);
+  }
+  {
+    BN_TIMED(KID_SERIES_SUM, st);
+    hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  }
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -496,6 +541,16 @@ int check_plan(const bn_plan* plan) {
 
 }  // namespace
 
+namespace bn {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+double* site_partials(const bn_plan* plan, const bn_kernel_desc* kern, void* workspace, int which) {
+  if (plan == nullptr || workspace == nullptr || check_desc(kern) != BN_OK) return nullptr;
+  WsLayout w = ws_layout(plan->host, state_dim(kern));
+  return (double*)((char*)workspace + (which == 0 ? w.sp1 : w.sp2));
+}
+}  // namespace bn
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -504,6 +559,36 @@ extern "C" {
 const char* bn_last_error(void) { return g_err.c_str(); }
 
 int bn_version(void) { return 1; }
+
+int bn_timing_enable(int32_t enable) {
+  for (auto& r : g_recs) { g_pool.push_back(r.a); g_pool.push_back(r.b); }
+  g_recs.clear();
+  g_timing = enable != 0;
+  return BN_OK;
+}
+
+int bn_timing_read(int32_t kernel_id, double* total_ms, int32_t* count) {
+  if (kernel_id < 0 || kernel_id >= KID_COUNT || !total_ms || !count) return fail(BN_EINVAL, "bad timing query");
+  double tot = 0.0;
+  int32_t n = 0;
+  for (auto& r : g_recs) {
+    if (r.kid != kernel_id) continue;
+    BN_HIP_CHECK(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    BN_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    tot += ms;
+    ++n;
+  }
+  *total_ms = tot;
+  *count = n;
+  return BN_OK;
+}
+
+const char* bn_kernel_name(int32_t kernel_id) {
+  return (kernel_id >= 0 && kernel_id < KID_COUNT) ? kKernelNames[kernel_id] : "";
+}
+
+int32_t bn_num_kernel_ids(void) { return KID_COUNT; }
 
 int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_len, bn_plan** out) {
   if (offsets_host == nullptr || out == nullptr) return fail(BN_EINVAL, "NULL argument to bn_plan_create");
@@ -669,6 +754,19 @@ int bn_kalman_filter(const bn_plan* plan, const bn_kernel_desc* kern, const doub
   HyperView hv{lengthscale, variance, hyper_stride};
   return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, means, covs, loglik, flags, workspace,
                      (hipStream_t)stream);
+}
+
+int bn_filter_loglik(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                     const double* variance, int32_t hyper_stride, const float* dt, const float* y, const float* r,
+                     const uint8_t* mask, double* loglik, void* workspace, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !y || !r || !loglik || !workspace || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_filter_loglik");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, (double*)nullptr, (double*)nullptr, loglik,
+                     0u, workspace, (hipStream_t)stream);
 }
 
 int bn_rts_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,

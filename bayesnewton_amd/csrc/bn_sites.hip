@@ -1,0 +1,335 @@
+// HIP kernels and C ABI of the site updates and energy terms of the Markov-GP
+// inference step (inference.py:21-325) on the plan's tiled layout.  One lane
+// walks one chunk (the same coalesced access pattern as the filter kernels)
+// and leaves per-chunk FP64 partial sums; a block per series reduces them in a
+// fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "bn_internal.h"
+#include "bn_sites.h"
+
+using namespace bn;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+int fail(int code, const std::string& m) { return set_error(code, m); }
+
+#define BN_HIP_CHECK2(expr)                                                                      \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(BN_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct SiteIO {
+  const float* y;
+  const float* pm;
+  const float* pv;
+  double* nat1;
+  double* nat2;
+  float* py;
+  float* pr;
+  float* o_mean;
+  float* o_jac;
+  float* o_hess;
+  double* part1;  // per chunk: sum |nat1_new - nat1| (update) or t1 (energy)
+  double* part2;
+};
+
+template <int METHOD>
+__global__ __launch_bounds__(kBlock) void k_site_update(PlanView pl, LikParams L, GHTable gh, SiteIO io, double lr,
+                                                        double power, int ensure_psd) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  const int32_t len = pl.chunk_len_[j];
+  double d1 = 0.0, d2 = 0.0;
+  for (int32_t k = 0; k < len; ++k) {
+    const int64_t s = pl.slot(j, k);
+    const double n1 = io.nat1[s], n2 = io.nat2[s];
+    SiteResult r = site_step<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], n1, n2, power,
+                                     ensure_psd != 0);
+    d1 += fabs(r.nat1 - n1);
+    d2 += fabs(r.nat2 - n2);
+    // damping (inference.py:84-87) and reparametrisation (basemodels.py:85-100)
+    const double a1 = (1.0 - lr) * n1 + lr * r.nat1;
+    const double a2 = (1.0 - lr) * n2 + lr * r.nat2;
+    io.nat1[s] = a1;
+    io.nat2[s] = a2;
+    io.pr[s] = (float)(1.0 / a2);
+    io.py[s] = (float)(a1 / a2);
+    if (io.o_mean != nullptr) {
+      io.o_mean[s] = (float)r.mean;
+      io.o_jac[s] = (float)r.jac;
+      io.o_hess[s] = (float)r.hess;
+    }
+  }
+  io.part1[j] = d1;
+  io.part2[j] = d2;
+}
+
+template <int METHOD>
+__global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams L, GHTable gh, SiteIO io,
+                                                         double power) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  const int32_t len = pl.chunk_len_[j];
+  double s1 = 0.0, s2 = 0.0;
+  for (int32_t k = 0; k < len; ++k) {
+    const int64_t s = pl.slot(j, k);
+    double t1, t2;
+    energy_terms<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], io.nat1[s], io.nat2[s], power,
+                         t1, t2);
+    s1 += t1;
+    s2 += t2;
+  }
+  io.part1[j] = s1;
+  io.part2[j] = s2;
+}
+
+__global__ void k_reparametrise(PlanView pl, const double* __restrict__ nat1, const double* __restrict__ nat2,
+                                float* __restrict__ py, float* __restrict__ pr) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= pl.n_slots) return;
+  pr[s] = (float)(1.0 / nat2[s]);
+  py[s] = (float)(nat1[s] / nat2[s]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_series_sum2(PlanView pl, const double* __restrict__ a,
+                                                        const double* __restrict__ b, double* __restrict__ oa,
+                                                        double* __restrict__ ob) {
+  __shared__ double ra[kBlock], rb[kBlock];
+  const int32_t s = blockIdx.x;
+  const int32_t c0 = pl.series_first_chunk[s];
+  const int32_t nc = pl.series_nchunks[s];
+  double x = 0.0, y = 0.0;
+  for (int32_t i = threadIdx.x; i < nc; i += kBlock) {
+    x += a[c0 + i];
+    y += b[c0 + i];
+  }
+  ra[threadIdx.x] = x;
+  rb[threadIdx.x] = y;
+  __syncthreads();
+  for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int32_t)threadIdx.x < w) {
+      ra[threadIdx.x] += ra[threadIdx.x + w];
+      rb[threadIdx.x] += rb[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    oa[s] = ra[0];
+    ob[s] = rb[0];
+  }
+}
+
+// Orthonormal Hermite polynomials for the weight exp(-t^2) at t:
+// p_0 = pi^-1/4, p_1 = sqrt(2) t p_0, p_{k+1} = sqrt(2/(k+1)) t p_k - sqrt(k/(k+1)) p_{k-1}.
+// Returns p_{n-1}(t), p_n(t) and sum_{k<n} p_k(t)^2.
+void hermite_orthonormal(int n, double t, double& pnm1, double& pn, double& sumsq) {
+  double pm = 0.0, p = 0.7511255444649425;  // pi^-1/4
+  sumsq = 0.0;
+  for (int k = 0; k < n; ++k) {
+    sumsq += p * p;
+    const double next = std::sqrt(2.0 / (k + 1)) * t * p - std::sqrt((double)k / (k + 1)) * pm;
+    pm = p;
+    p = next;
+  }
+  pnm1 = pm;
+  pn = p;
+}
+
+// Gauss-Hermite rule by Golub-Welsch: the nodes are the eigenvalues of the
+// Jacobi matrix of the weight exp(-t^2) (zero diagonal, off-diagonal sqrt(k/2)),
+// found with cyclic Jacobi rotations and polished by Newton steps on p_n; the
+// weights are the Christoffel numbers 1 / sum_{k<n} p_k(t_i)^2, which keep full
+// relative accuracy in the tails (eigenvector components would not).  Rescaled
+// for expectations under N(0, 1): x = sqrt(2) t, w = weight / sqrt(pi).
+bool golub_welsch(int n, double* x, double* w) {
+  if (n < 1 || n > kMaxGH) return false;
+  std::vector<double> A(n * n, 0.0);
+  for (int k = 1; k < n; ++k) A[(k - 1) * n + k] = A[k * n + k - 1] = std::sqrt(0.5 * k);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-36) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // A <- J' A J
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+      }
+  }
+  std::vector<double> t(n);
+  for (int i = 0; i < n; ++i) t[i] = A[i * n + i];
+  std::sort(t.begin(), t.end());
+  for (int i = 0; i < n; ++i) {
+    double ti = t[i], pnm1, pn, ss;
+    for (int it = 0; it < 3; ++it) {  // Newton on p_n, p_n' = sqrt(2n) p_{n-1}
+      hermite_orthonormal(n, ti, pnm1, pn, ss);
+      const double d = std::sqrt(2.0 * n) * pnm1;
+      if (d != 0.0) ti -= pn / d;
+    }
+    hermite_orthonormal(n, ti, pnm1, pn, ss);
+    x[i] = std::sqrt(2.0) * ti;
+    w[i] = 1.0 / (ss * std::sqrt(M_PI));
+  }
+  return true;
+}
+
+// The table for n nodes is computed once per thread and reused.
+int make_gh(int n, GHTable& gh) {
+  thread_local GHTable cache[kMaxGH + 1];
+  thread_local bool have[kMaxGH + 1] = {};
+  if (n < 1 || n > kMaxGH) return fail(BN_EINVAL, "gh_points must be in [1, 64]");
+  if (!have[n]) {
+    cache[n].n = n;
+    if (!golub_welsch(n, cache[n].x, cache[n].w)) return fail(BN_EINVAL, "Golub-Welsch failed");
+    have[n] = true;
+  }
+  gh = cache[n];
+  return BN_OK;
+}
+
+int make_lik_checked(const bn_lik_desc* d, LikParams& L) {
+  if (d == nullptr) return fail(BN_EINVAL, "likelihood descriptor is NULL");
+  switch (d->kind) {
+    case BN_LIK_GAUSSIAN:
+      if (!(d->param0 > 0)) return fail(BN_EINVAL, "Gaussian variance must be > 0");
+      break;
+    case BN_LIK_BERNOULLI_PROBIT:
+      break;
+    case BN_LIK_POISSON:
+      if (!(d->param0 > 0)) return fail(BN_EINVAL, "Poisson binsize must be > 0");
+      break;
+    case BN_LIK_STUDENTS_T:
+      if (!(d->param0 > 0) || !(d->param1 > 0)) return fail(BN_EINVAL, "Student-t scale and df must be > 0");
+      break;
+    default:
+      return fail(BN_EUNSUPPORTED, "unknown likelihood kind");
+  }
+  L = make_lik(d->kind, d->param0, d->param1);
+  return BN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bn_gauss_hermite(int32_t n, double* x, double* w) {
+  if (x == nullptr || w == nullptr) return fail(BN_EINVAL, "NULL output");
+  GHTable gh;
+  if (int rc = make_gh(n, gh)) return rc;
+  for (int i = 0; i < n; ++i) {
+    x[i] = gh.x[i];
+    w[i] = gh.w[i];
+  }
+  return BN_OK;
+}
+
+int bn_site_reparametrise(const bn_plan* plan, const double* nat1, const double* nat2, float* pseudo_y,
+                          float* pseudo_var, void* stream) {
+  if (plan == nullptr || !nat1 || !nat2 || !pseudo_y || !pseudo_var) return fail(BN_EINVAL, "NULL argument");
+  const int64_t n = plan->host.n_slots;
+  hipLaunchKernelGGL(k_reparametrise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     plan->view, nat1, nat2, pseudo_y, pseudo_var);
+  BN_HIP_CHECK2(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_site_update(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
+                   int32_t gh_points, double lr, double power, int32_t ensure_psd, const float* y,
+                   const float* post_mean, const float* post_var, double* nat1, double* nat2, float* pseudo_y,
+                   float* pseudo_var, float* out_mean, float* out_jac, float* out_hess, double* diff1, double* diff2,
+                   void* workspace, void* stream) {
+  if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
+  if (!y || !post_mean || !post_var || !nat1 || !nat2 || !pseudo_y || !pseudo_var || !diff1 || !diff2 || !workspace)
+    return fail(BN_EINVAL, "NULL array argument to bn_site_update");
+  if ((out_mean == nullptr) != (out_jac == nullptr) || (out_mean == nullptr) != (out_hess == nullptr))
+    return fail(BN_EINVAL, "out_mean / out_jac / out_hess must be all NULL or all set");
+  if (method == BN_EP && !(power > 0)) return fail(BN_EINVAL, "EP power must be > 0");
+  LikParams L;
+  if (int rc = make_lik_checked(lik, L)) return rc;
+  GHTable gh;
+  if (int rc = make_gh(gh_points, gh)) return rc;
+  double* p1 = site_partials(plan, kern, workspace, 0);
+  double* p2 = site_partials(plan, kern, workspace, 1);
+  if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
+  SiteIO io{y, post_mean, post_var, nat1, nat2, pseudo_y, pseudo_var, out_mean, out_jac, out_hess, p1, p2};
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
+  switch (method) {
+    case BN_VI:
+      hipLaunchKernelGGL(k_site_update<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power, ensure_psd);
+      break;
+    case BN_NEWTON:
+      hipLaunchKernelGGL(k_site_update<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power,
+                         ensure_psd);
+      break;
+    case BN_EP:
+      hipLaunchKernelGGL(k_site_update<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power, ensure_psd);
+      break;
+    default:
+      return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
+  }
+  hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, diff1, diff2);
+  BN_HIP_CHECK2(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
+                    int32_t gh_points, double power, const float* y, const float* post_mean, const float* post_var,
+                    const double* nat1, const double* nat2, double* term1, double* term2, void* workspace,
+                    void* stream) {
+  if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
+  if (!y || !post_mean || !post_var || !nat1 || !nat2 || !term1 || !term2 || !workspace)
+    return fail(BN_EINVAL, "NULL array argument to bn_energy_terms");
+  if (method == BN_EP && !(power > 0)) return fail(BN_EINVAL, "EP power must be > 0");
+  LikParams L;
+  if (int rc = make_lik_checked(lik, L)) return rc;
+  GHTable gh;
+  if (int rc = make_gh(gh_points, gh)) return rc;
+  double* p1 = site_partials(plan, kern, workspace, 0);
+  double* p2 = site_partials(plan, kern, workspace, 1);
+  if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
+  SiteIO io{y, post_mean, post_var, const_cast<double*>(nat1), const_cast<double*>(nat2), nullptr, nullptr,
+            nullptr, nullptr, nullptr, p1, p2};
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
+  switch (method) {
+    case BN_VI:
+      hipLaunchKernelGGL(k_energy_terms<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+      break;
+    case BN_NEWTON:
+      hipLaunchKernelGGL(k_energy_terms<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+      break;
+    case BN_EP:
+      hipLaunchKernelGGL(k_energy_terms<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+      break;
+    default:
+      return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
+  }
+  hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, term1, term2);
+  BN_HIP_CHECK2(hipGetLastError());
+  return BN_OK;
+}
+
+}  // extern "C"

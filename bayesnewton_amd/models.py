@@ -1,0 +1,277 @@
+"""Markov-GP models on the HIP path (mirrors bayesnewton/models.py:118-132,
+216-231, 297-299, 328 and their bases basemodels.py:625-764 /
+inference.py:65-325).
+
+A model holds every per-step array on the GPU in the plan's tiled layout:
+time steps, observations, the sites (natural parameters in FP64, pseudo
+observations in FP32) and the posterior.  `update_posterior` is one fused
+filter + smoother pass; `inference` is the reference's site step between two
+posterior updates; `energy` reuses the filter log-likelihood of the current
+sites (compute_log_lik) instead of running the filter again.
+
+Besides the reference's single-series constructor, `from_series` builds one
+model over many independent series of uneven lengths (one launch per pass for
+the whole batch), optionally with per-series kernel hyperparameters.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import cubature as cub
+from . import ops
+from .plan import SeriesPlan, default_chunk_len
+
+
+def input_admin(t, y):
+    """bayesnewton/utils.py:234-265 for the temporal case: sort by t, dt_0 = 0."""
+    t = np.asarray(t, dtype=np.float64).reshape(-1)
+    y = np.asarray(y, dtype=np.float64).reshape(-1)
+    if t.shape[0] != y.shape[0]:
+        raise ValueError("X and Y must have the same number of rows")
+    ind = np.argsort(t, kind="stable")
+    t, y = t[ind], y[ind]
+    dt = np.concatenate([[0.0], np.diff(t)])
+    return t, y, dt
+
+
+class PseudoLikelihood:
+    """The sites (GaussianDistribution, basemodels.py:52-100): nat2 = 1 / variance."""
+
+    def __init__(self, model):
+        self._m = model
+
+    @property
+    def nat1(self):
+        return self._m.plan.scatter(self._m._nat1[None]).reshape(-1, 1, 1)
+
+    @property
+    def nat2(self):
+        return self._m.plan.scatter(self._m._nat2[None]).reshape(-1, 1, 1)
+
+    @property
+    def mean(self):
+        return self.nat1 / self.nat2
+
+    @property
+    def covariance(self):
+        return 1.0 / self.nat2
+
+
+class MarkovGaussianProcess:
+    """Sum-of-Matern state-space GP with scalar sites (basemodels.py:625-764)."""
+    method = None
+
+    def __init__(self, kernel, likelihood, X, Y, R=None, parallel=None, power=1.0, chunk_len=None, device=None):
+        if R is not None or (np.ndim(X) > 1 and np.shape(X)[1] > 1):
+            raise NotImplementedError("spatio-temporal inputs: use SpatioTemporal models")
+        t, y, dt = input_admin(X, Y)
+        self._build(kernel, likelihood, [t], [y], [dt], None, None, parallel, power, chunk_len, device)
+
+    @classmethod
+    def from_series(cls, kernel, likelihood, xs, ys, lengthscale=None, variance=None, parallel=True, power=1.0,
+                    chunk_len=None, device=None):
+        """One model over B independent series (lists of per-series inputs / outputs).
+
+        lengthscale / variance: optional per-series hypers [B, n_comp] (natural
+        scale); default: the kernel's own hypers for every series."""
+        self = cls.__new__(cls)
+        ts, yy, dts = [], [], []
+        for x, y in zip(xs, ys):
+            t, yv, d = input_admin(x, y)
+            ts.append(t)
+            yy.append(yv)
+            dts.append(d)
+        self._build(kernel, likelihood, ts, yy, dts, lengthscale, variance, parallel, power, chunk_len, device)
+        return self
+
+    # ---- construction --------------------------------------------------------------
+    def _build(self, kernel, likelihood, ts, ys, dts, ls, var, parallel, power, chunk_len, device):
+        if not torch.cuda.is_available():
+            raise nat.NativeError("bayesnewton_amd models need a ROCm GPU")
+        self.kernel = kernel
+        self.likelihood = likelihood
+        self.power = float(power)
+        self.parallel = True if parallel is None else bool(parallel)
+        lens = np.array([len(t) for t in ts], dtype=np.int64)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        self.offsets = off
+        self.X = np.concatenate(ts)[:, None]
+        self.Y = np.concatenate(ys)[:, None]
+        self.dt = np.concatenate(dts)
+        self.num_data = int(off[-1])
+        self.num_series = len(ts)
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        if chunk_len is None:
+            chunk_len = default_chunk_len(lens, self.parallel)
+        self.plan = SeriesPlan(off, chunk_len=chunk_len, device=dev)
+        self._desc = kernel.desc()
+        n_comp = self._desc.n_comp
+        if ls is not None or var is not None:
+            B = self.num_series
+            base_ls, base_var = [c.lengthscale for c in kernel.components], [c.variance for c in kernel.components]
+            ls = np.broadcast_to(np.asarray(base_ls if ls is None else ls, np.float64).reshape(-1, n_comp), (B, n_comp))
+            var = np.broadcast_to(np.asarray(base_var if var is None else var, np.float64).reshape(-1, n_comp),
+                                  (B, n_comp))
+            self._hyper_stride = n_comp
+            self._ls = torch.tensor(np.ascontiguousarray(ls), dtype=torch.float64, device=dev)
+            self._var = torch.tensor(np.ascontiguousarray(var), dtype=torch.float64, device=dev)
+            self._per_series = True
+        else:
+            self._hyper_stride = 0
+            self._per_series = False
+            self._hyper_key = None
+        P = self.plan
+        self._dt = P.gather(torch.as_tensor(self.dt, dtype=torch.float32, device=dev))
+        y32 = torch.as_tensor(self.Y[:, 0], dtype=torch.float32, device=dev)
+        self._y = P.gather(y32, pad=float("nan"))[0]
+        self._mask = P.gather_mask(torch.isnan(y32))
+        # sites N(0, 1e2) (basemodels.py:130-133): nat1 = 0, nat2 = 1e-2
+        self._nat1 = torch.zeros(P.n_slots, dtype=torch.float64, device=dev)
+        self._nat2 = torch.full((P.n_slots,), 1e-2, dtype=torch.float64, device=dev)
+        self._py = torch.empty(P.n_slots, dtype=torch.float32, device=dev)
+        self._pr = torch.empty(P.n_slots, dtype=torch.float32, device=dev)
+        nat.check(nat.lib().bn_site_reparametrise(P.handle, nat.ptr(self._nat1), nat.ptr(self._nat2),
+                                                  nat.ptr(self._py), nat.ptr(self._pr), nat.stream_ptr()))
+        self._pm = torch.zeros(P.n_slots, dtype=torch.float32, device=dev)
+        self._pv = torch.ones(P.n_slots, dtype=torch.float32, device=dev)
+        self._loglik = torch.zeros(self.num_series, dtype=torch.float64, device=dev)
+        self._post_key = None  # state (hypers, sites) the stored posterior was computed from
+        self._ll_key = None    # state the stored log-likelihood was computed from
+        self._d1 = torch.empty(self.num_series, dtype=torch.float64, device=dev)
+        self._d2 = torch.empty(self.num_series, dtype=torch.float64, device=dev)
+        self.pseudo_likelihood = PseudoLikelihood(self)
+
+    # ---- hyperparameters ---------------------------------------------------------
+    def _hypers(self):
+        if self._per_series:
+            return self._ls, self._var
+        key = tuple((c.lengthscale, c.variance) for c in self.kernel.components)
+        if key != self._hyper_key:
+            self._ls, self._var = self.kernel.hypers(self.device)
+            self._hyper_key = key
+        return self._ls, self._var
+
+    def _state_key(self):
+        ls, var = self._hypers()
+        return (id(ls), id(var), self._sites_version)
+
+    _sites_version = 0
+
+    # ---- posterior -----------------------------------------------------------------
+    def update_posterior(self):
+        """basemodels.py:689-706: filter + smoother of the current sites."""
+        ls, var = self._hypers()
+        ops.filter_smoother(self.plan, self._desc, ls, var, self._dt, self._py, self._pr, self._mask,
+                            hyper_stride=self._hyper_stride, post_mean=self._pm, post_var=self._pv,
+                            loglik=self._loglik)
+        self._post_key = self._ll_key = self._state_key()
+
+    def _ensure_posterior(self):
+        # inference() starts with update_posterior() (inference.py:70); skipping it
+        # when the stored posterior already belongs to the current state is exact.
+        if self._post_key != self._state_key():
+            self.update_posterior()
+
+    def _ensure_loglik(self):
+        key = self._state_key()
+        if self._ll_key != key:
+            ls, var = self._hypers()
+            ops.filter_loglik(self.plan, self._desc, ls, var, self._dt, self._py, self._pr, self._mask,
+                              hyper_stride=self._hyper_stride, loglik=self._loglik)
+            self._ll_key = key
+
+    def compute_log_lik(self, per_series=False):
+        """basemodels.py:726-741: log normaliser of the pseudo model (filter log p(y))."""
+        self._ensure_loglik()
+        return self._loglik.clone() if per_series else self._loglik.sum()
+
+    @property
+    def posterior_mean(self):
+        return self.plan.scatter(self._pm[None]).to(torch.float64).reshape(-1, 1, 1)
+
+    @property
+    def posterior_variance(self):
+        return self.plan.scatter(self._pv[None]).to(torch.float64).reshape(-1, 1, 1)
+
+    # ---- inference (inference.py:65-90) ----------------------------------------------
+    def inference(self, lr=1.0, batch_ind=None, cubature=None, ensure_psd=True, return_terms=True, **kwargs):
+        if batch_ind is not None and len(batch_ind) != self.num_data:
+            raise NotImplementedError("mini-batched site updates are not implemented on the HIP path")
+        self._ensure_posterior()
+        P = self.plan
+        dev = self.device
+        if return_terms:
+            om = torch.empty(P.n_slots, dtype=torch.float32, device=dev)
+            oj = torch.empty_like(om)
+            oh = torch.empty_like(om)
+        else:
+            om = oj = oh = None
+        lik = self.likelihood.desc()
+        nat.check(nat.lib().bn_site_update(
+            P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, cub.num_points(cubature), float(lr),
+            self.power, 1 if ensure_psd else 0, nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv),
+            nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._py), nat.ptr(self._pr), nat.ptr(om),
+            nat.ptr(oj), nat.ptr(oh), nat.ptr(self._d1), nat.ptr(self._d2), nat.ptr(P.workspace(self._desc)),
+            nat.stream_ptr()))
+        self._sites_version += 1
+        self.update_posterior()
+        diff1 = self._d1.sum() / self.num_data
+        diff2 = self._d2.sum() / self.num_data
+        if return_terms:
+            terms = tuple(P.scatter(a[None]).to(torch.float64).reshape(-1, 1, 1) for a in (om, oj, oh))
+        else:
+            terms = (None, None, None)
+        return terms, (diff1, diff2)
+
+    # ---- energy -------------------------------------------------------------------
+    def energy_per_series(self, cubature=None):
+        """Energy of every series (the reference's energy() for each one).  As in the
+        reference, the stored posterior is used as is and only log Z (the filter
+        log-likelihood of the sites under the current hypers) is recomputed."""
+        self._ensure_loglik()
+        P = self.plan
+        lik = self.likelihood.desc()
+        t1 = torch.empty(self.num_series, dtype=torch.float64, device=self.device)
+        t2 = torch.empty_like(t1)
+        nat.check(nat.lib().bn_energy_terms(
+            P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, cub.num_points(cubature), self.power,
+            nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(self._nat1), nat.ptr(self._nat2),
+            nat.ptr(t1), nat.ptr(t2), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
+        logZ = self._loglik
+        if self.method == nat.BN_EP:
+            return -(logZ + (t1 - t2) / self.power)
+        return -(t1 - (t2 - logZ))
+
+    def energy(self, batch_ind=None, cubature=None, **kwargs):
+        """inference.py:130-154 (Newton), 197-222 (VI), 286-325 (EP)."""
+        if batch_ind is not None and len(batch_ind) != self.num_data:
+            raise NotImplementedError("mini-batched energies are not implemented on the HIP path")
+        return self.energy_per_series(cubature).sum()
+
+
+MarkovGP = MarkovGaussianProcess
+
+
+class MarkovVariationalGP(MarkovGaussianProcess):
+    """models.py:118-132 (VariationalInference + MarkovGaussianProcess)."""
+    method = nat.BN_VI
+
+
+class MarkovNewtonGP(MarkovGaussianProcess):
+    """models.py:297-299 (Newton + MarkovGaussianProcess)."""
+    method = nat.BN_NEWTON
+
+
+MarkovLaplaceGP = MarkovNewtonGP
+
+
+class MarkovExpectationPropagationGP(MarkovGaussianProcess):
+    """models.py:216-231 (ExpectationPropagation + MarkovGaussianProcess, power EP)."""
+    method = nat.BN_EP
+
+    def __init__(self, kernel, likelihood, X, Y, R=None, power=1.0, parallel=None, chunk_len=None, device=None):
+        super().__init__(kernel, likelihood, X, Y, R=R, parallel=parallel, power=power, chunk_len=chunk_len,
+                         device=device)

@@ -141,3 +141,15 @@ def filter_smoother(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=Non
                                            nat.ptr(mask), nat.ptr(post_mean), nat.ptr(post_var), nat.ptr(loglik),
                                            nat.ptr(plan.workspace(kernel_desc)), nat.stream_ptr()))
     return post_mean, post_var, loglik
+
+
+def filter_loglik(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=None, hyper_stride=0, loglik=None):
+    """Per-series filter log-likelihood on tiled FP32 inputs (compute_log_lik, basemodels.py:726-741)."""
+    import ctypes
+    if loglik is None:
+        loglik = torch.empty(plan.n_series, dtype=torch.float64, device=plan.device)
+    nat.check(nat.lib().bn_filter_loglik(plan.handle, ctypes.byref(kernel_desc), nat.ptr(lengthscale),
+                                         nat.ptr(variance), hyper_stride, nat.ptr(dt), nat.ptr(y), nat.ptr(r),
+                                         nat.ptr(mask), nat.ptr(loglik), nat.ptr(plan.workspace(kernel_desc)),
+                                         nat.stream_ptr()))
+    return loglik

@@ -1,0 +1,164 @@
+"""Seeded synthetic workloads of the benchmark configurations (FP64, host).
+
+Each generator returns plain NumPy arrays; values the configurations leave
+open are fixed here once:
+  C1  Matern52 (l=1, s2=1), Gaussian(0.01); N=10,000 sorted U[0,100];
+      y = sin(0.3x) + 0.5 sin(1.7x) + 0.1 eps.                         seed 1001
+  C2  Matern32 + Matern52 (l=10, s2=0.5 each); N=200,000 sorted U[0,1000];
+      f = 2 sin(0.05x) + GP draw; y = 1[u < Phi(f)].                     seed 1002
+  C3  B=4096 series, n_b = floor(exp(U[ln 256, ln 16384])), x_b sorted
+      U[0, 0.1 n_b]; Matern52 + Matern32 with one lengthscale per series
+      l_b ~ logU[0.5, 20], s2 = 1; y = GP draw + 0.2 t_3.                seed 1003
+  C4  Matern52 (l=1, s2=1); N=500,000 sorted U[0,5000]; y = GP draw + 0.1 eps.  seed 1004
+GP draws use the reference's state-space sampler (basemodels.py:833-879):
+m0 = chol(Pinf) e, m <- A m + chol(Q + 1e-8 I) e, f = H m.
+"""
+import math
+
+import numpy as np
+
+
+# ---- closed-form Matern blocks (kernels.py:175-300), vectorised over steps ----
+def _m32_A(lam, dt):
+    e = np.exp(-dt * lam)
+    A = np.empty(dt.shape + (2, 2))
+    A[..., 0, 0] = e * (dt * lam + 1.0)
+    A[..., 0, 1] = e * dt
+    A[..., 1, 0] = -e * dt * lam * lam
+    A[..., 1, 1] = e * (1.0 - dt * lam)
+    return A
+
+
+def _m52_A(lam, dt):
+    dl = dt * lam
+    e = np.exp(-dl)
+    A = np.empty(dt.shape + (3, 3))
+    A[..., 0, 0] = e * (dt * lam * (0.5 * dl + 1.0) + 1.0)
+    A[..., 0, 1] = e * dt * (dl + 1.0)
+    A[..., 0, 2] = e * 0.5 * dt * dt
+    A[..., 1, 0] = e * dt * (-0.5 * dl * lam * lam)
+    A[..., 1, 1] = e * (dt * lam * (1.0 - dl) + 1.0)
+    A[..., 1, 2] = e * dt * (1.0 - 0.5 * dl)
+    A[..., 2, 0] = e * dt * lam ** 3 * (0.5 * dl - 1.0)
+    A[..., 2, 1] = e * dt * lam ** 2 * (dl - 3.0)
+    A[..., 2, 2] = e * (dt * lam * (0.5 * dl - 2.0) + 1.0)
+    return A
+
+
+def _pinf(kind, ls, var):
+    ls = np.asarray(ls, np.float64)
+    var = np.asarray(var, np.float64)
+    if kind == "m32":
+        P = np.zeros(ls.shape + (2, 2))
+        P[..., 0, 0] = var
+        P[..., 1, 1] = 3.0 * var / ls ** 2
+        return P
+    kap = 5.0 / 3.0 * var / ls ** 2
+    P = np.zeros(ls.shape + (3, 3))
+    P[..., 0, 0] = var
+    P[..., 0, 2] = P[..., 2, 0] = -kap
+    P[..., 1, 1] = kap
+    P[..., 2, 2] = 25.0 * var / ls ** 4
+    return P
+
+
+def _trans(kind, ls, dt):
+    if kind == "m32":
+        return _m32_A(math.sqrt(3.0) / ls, dt)
+    return _m52_A(math.sqrt(5.0) / ls, dt)
+
+
+def _chol_small(Q):
+    """Batched Cholesky of 2x2 / 3x3 SPD matrices (closed form, lower)."""
+    k = Q.shape[-1]
+    L = np.zeros_like(Q)
+    L[:, 0, 0] = np.sqrt(Q[:, 0, 0])
+    L[:, 1, 0] = Q[:, 1, 0] / L[:, 0, 0]
+    L[:, 1, 1] = np.sqrt(Q[:, 1, 1] - L[:, 1, 0] ** 2)
+    if k == 3:
+        L[:, 2, 0] = Q[:, 2, 0] / L[:, 0, 0]
+        L[:, 2, 1] = (Q[:, 2, 1] - L[:, 2, 0] * L[:, 1, 0]) / L[:, 1, 1]
+        L[:, 2, 2] = np.sqrt(Q[:, 2, 2] - L[:, 2, 0] ** 2 - L[:, 2, 1] ** 2)
+    return L
+
+
+def gp_draw(kinds, ls, var, dt_list, rng):
+    """State-space prior draw of f = sum of components for each series.
+
+    kinds: component names ('m32'/'m52'); ls/var: [B, n_comp]; dt_list: list of
+    per-series step vectors (dt[0] = 0).  Vectorised over the series still
+    running at each time index (series are visited longest first).
+    """
+    B = len(dt_list)
+    ls = np.asarray(ls, np.float64).reshape(B, -1)
+    var = np.asarray(var, np.float64).reshape(B, -1)
+    lens = np.array([len(d) for d in dt_list])
+    order = np.argsort(-lens, kind="stable")
+    slens = lens[order]
+    T = int(slens[0])
+    dtm = np.zeros((B, T))
+    for i, b in enumerate(order):
+        dtm[i, :lens[b]] = dt_list[b]
+    f = np.zeros((B, T))
+    # number of series still running at time t (series sorted by length, descending)
+    active = np.searchsorted(-slens, -np.arange(1, T + 1), side="right")
+    for c, kind in enumerate(kinds):
+        lsc, vc = ls[order, c], var[order, c]
+        Pinf = _pinf(kind, lsc, vc)
+        k = Pinf.shape[-1]
+        m = np.einsum("bij,bj->bi", np.linalg.cholesky(Pinf), rng.standard_normal((B, k)))
+        jit = 1e-8 * np.eye(k)
+        for t in range(T):
+            a = int(active[t])
+            A = _trans(kind, lsc[:a], dtm[:a, t])
+            Q = Pinf[:a] - A @ Pinf[:a] @ np.swapaxes(A, -1, -2)
+            L = _chol_small(Q + jit)
+            m[:a] = np.einsum("bij,bj->bi", A, m[:a]) + np.einsum("bij,bj->bi", L, rng.standard_normal((a, k)))
+            f[:a, t] += m[:a, 0]
+    out = [None] * B
+    for i, b in enumerate(order):
+        out[b] = f[i, :lens[b]]
+    return out
+
+
+def _dt(x):
+    return np.concatenate([[0.0], np.diff(x)])
+
+
+def c1():
+    rng = np.random.default_rng(1001)
+    x = np.sort(rng.uniform(0, 100, 10_000))
+    y = np.sin(0.3 * x) + 0.5 * np.sin(1.7 * x) + 0.1 * rng.standard_normal(x.size)
+    return dict(x=x, y=y, kinds=["m52"], ls=[1.0], var=[1.0], noise=0.01)
+
+
+def c2():
+    rng = np.random.default_rng(1002)
+    x = np.sort(rng.uniform(0, 1000, 200_000))
+    f = 2 * np.sin(0.05 * x) + gp_draw(["m32", "m52"], [[10.0, 10.0]], [[0.5, 0.5]], [_dt(x)], rng)[0]
+    from scipy.special import ndtr
+    y = (rng.uniform(size=x.size) < ndtr(f)).astype(np.float64)
+    return dict(x=x, y=y, f=f, kinds=["m32", "m52"], ls=[10.0, 10.0], var=[0.5, 0.5])
+
+
+def c3(B=4096, seed=1003):
+    rng = np.random.default_rng(seed)
+    n = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), B))).astype(np.int64)
+    xs = [np.sort(rng.uniform(0, 0.1 * nb, nb)) for nb in n]
+    ls_b = np.exp(rng.uniform(np.log(0.5), np.log(20.0), B))
+    ls = np.stack([ls_b, ls_b], axis=1)
+    var = np.ones((B, 2))
+    dts = [_dt(x) for x in xs]
+    fs = gp_draw(["m52", "m32"], ls, var, dts, rng)
+    ys = [f + 0.2 * rng.standard_t(3.0, size=f.size) for f in fs]
+    off = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    return dict(offsets=off, x=np.concatenate(xs), dt=np.concatenate(dts), y=np.concatenate(ys),
+                kinds=["m52", "m32"], ls=ls, var=var, df=3.0, scale=0.2)
+
+
+def c4():
+    rng = np.random.default_rng(1004)
+    x = np.sort(rng.uniform(0, 5000, 500_000))
+    f = gp_draw(["m52"], [[1.0]], [[1.0]], [_dt(x)], rng)[0]
+    y = f + 0.1 * rng.standard_normal(x.size)
+    return dict(x=x, y=y, kinds=["m52"], ls=[1.0], var=[1.0], noise=0.01)

@@ -74,6 +74,24 @@ typedef struct bn_kernel_desc {
 
 typedef struct bn_plan bn_plan; /* opaque: chunk/tile tables on the device */
 
+/* likelihoods (likelihoods.py: Gaussian 684-803, Bernoulli/probit 806-871,
+ * Poisson 891-1008, StudentsT 1011-1044) */
+#define BN_LIK_GAUSSIAN 1         /* param0 = noise variance */
+#define BN_LIK_BERNOULLI_PROBIT 2 /* y in {0, 1}; link Phi(f)(1 - 2e-3) + 1e-3 */
+#define BN_LIK_POISSON 3          /* param0 = binsize; exp link */
+#define BN_LIK_STUDENTS_T 4       /* param0 = scale, param1 = degrees of freedom */
+
+/* inference schemes (inference.py: Newton 99-157, VariationalInference
+ * 160-222, ExpectationPropagation 225-325) */
+#define BN_VI 0
+#define BN_NEWTON 1
+#define BN_EP 2
+
+typedef struct bn_lik_desc {
+  int32_t kind;
+  double param0, param1;
+} bn_lik_desc;
+
 const char *bn_last_error(void);
 int bn_version(void);
 
@@ -125,6 +143,13 @@ int bn_kalman_filter(const bn_plan *plan, const bn_kernel_desc *kern, const doub
                      const uint8_t *mask, double *means, double *covs, double *loglik, uint32_t flags,
                      void *workspace, void *stream);
 
+/* --- log-likelihood only: MarkovGaussianProcess.compute_log_lik
+ * (basemodels.py:726-741), the filter's log p(pseudo_y) per series without
+ * writing any state. */
+int bn_filter_loglik(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                     const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
+                     const uint8_t *mask, double *loglik, void *workspace, void *stream);
+
 /* --- ops.rauch_tung_striebel_smoother, ops.py:357-380.  dt is the smoother's
  * (already shifted) step vector, tiled.  filter_mean [d planes], filter_cov
  * [d*d planes], FP64.  Outputs (FP64): with BN_RETURN_FULL, means [d planes] /
@@ -135,6 +160,56 @@ int bn_rts_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const doubl
                     const double *variance, int32_t hyper_stride, const float *dt, const double *filter_mean,
                     const double *filter_cov, double *means, double *covs, double *gains, uint32_t flags,
                     void *workspace, void *stream);
+
+/* --- site update: one inference step's site map for every time step,
+ * InferenceMixin.inference (inference.py:65-90) between its two
+ * update_posterior calls: update_variational_params (VI 170-195, Newton
+ * 105-128, EP 238-284 with the cavity of basemodels.py:230-245),
+ * ensure_diagonal_positive_precision (utils.py:89-96) when ensure_psd,
+ * newton_update (21-39), damping with lr and reparametrisation
+ * (basemodels.py:85-100).  Tiled arrays: y (observations, NaN = missing),
+ * post_mean / post_var (the current posterior), nat1 / nat2 (FP64 site
+ * natural parameters, updated in place; nat2 = 1 / site variance as in
+ * basemodels.py:56-57), pseudo_y / pseudo_var (written: the filter's inputs).
+ * out_mean / out_jac / out_hess: the (mean, jacobian, hessian) inference()
+ * returns, or all NULL.  diff1 / diff2 [n_series]: sums of |nat_new - nat_old|
+ * before damping (inference.py:79-81 divides by N).  gh_points: Gauss-Hermite
+ * nodes for the cubature likelihoods (reference default 20). */
+int bn_site_update(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
+                   int32_t gh_points, double lr, double power, int32_t ensure_psd, const float *y,
+                   const float *post_mean, const float *post_var, double *nat1, double *nat2, float *pseudo_y,
+                   float *pseudo_var, float *out_mean, float *out_jac, float *out_hess, double *diff1,
+                   double *diff2, void *workspace, void *stream);
+
+/* --- energy terms per series (inference.py:130-154, 197-222, 286-325):
+ * term1 = VI: sum E_q[log p(y|f)] | Newton: sum log p(y|m) | EP: sum log Z
+ * term2 = VI/Newton: sum E_q[log N(pseudo_y | f, pseudo_var)] (compute_kl,
+ *         basemodels.py:708-724) | EP: sum of lel_pseudo with pep_constant
+ *         (basemodels.py:247-262).  Masked sites contribute 0.
+ * With logZ the filter log-likelihood of the same sites: energy =
+ * -(term1 - (term2 - logZ)) for VI / Newton, -(logZ + (term1 - term2) / power)
+ * for EP. */
+int bn_energy_terms(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
+                    int32_t gh_points, double power, const float *y, const float *post_mean, const float *post_var,
+                    const double *nat1, const double *nat2, double *term1, double *term2, void *workspace,
+                    void *stream);
+
+/* pseudo_y = nat1 / nat2, pseudo_var = 1 / nat2 on every slot (basemodels.py:85-90). */
+int bn_site_reparametrise(const bn_plan *plan, const double *nat1, const double *nat2, float *pseudo_y,
+                          float *pseudo_var, void *stream);
+
+/* Gauss-Hermite nodes / weights for E_{z~N(0,1)} (cubature.py:56-84): x = sqrt(2)
+ * times the Hermite roots, w = weights / sqrt(pi), by Golub-Welsch on the host. */
+int bn_gauss_hermite(int32_t n, double *x, double *w);
+
+/* --- measurement hooks (no counterpart in the reference).  When enabled, each
+ * launch of the chunked filter/smoother kernels is bracketed by hipEvents on
+ * its own stream; bn_timing_read sums the elapsed time of one kernel id.
+ * Not thread-safe; for benchmarks only.  bn_timing_enable clears records. */
+int bn_timing_enable(int32_t enable);
+int bn_timing_read(int32_t kernel_id, double *total_ms, int32_t *count);
+const char *bn_kernel_name(int32_t kernel_id);
+int32_t bn_num_kernel_ids(void);
 
 #ifdef __cplusplus
 }

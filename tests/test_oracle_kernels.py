@@ -1,0 +1,143 @@
+"""Pins the oracle's state-space kernels (bayesnewton/kernels.py:123-300, 1624-1663)
+and the device's cancellation-free Q tables against closed forms.
+
+* A(dt) (closed form) == scipy.linalg.expm(F dt)
+* Pinf solves the continuous Lyapunov equation F P + P F' + L Qc L' = 0
+* Q = Pinf - A Pinf A' == the Van Loan integral of e^{Fs} L Qc L' e^{F's}
+* H e^{F|tau|} Pinf H' == the covariance function k(tau)
+* bn_qtaylor.h Q tables == 60-digit evaluation of Pinf - A Pinf A'
+"""
+import math
+import os
+import re
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import kernels as K
+
+KERNELS = [K.Matern12(1.3, 0.7), K.Matern32(0.8, 2.5), K.Matern52(1.7, 0.4),
+           K.Sum([K.Matern32(0.5, 3.0), K.Matern52(1.0, 1.0)])]
+DTS = [0.0, 1e-4, 0.013, 0.37, 2.2, 11.0]
+
+
+def _lyap_terms(k):
+    F = k.feedback_matrix()
+    L, Qc = k.noise_effect()
+    return F, L @ Qc @ L.T
+
+
+@pytest.mark.parametrize("k", KERNELS, ids=lambda k: type(k).__name__)
+def test_transition_is_expm(k):
+    F = k.feedback_matrix()
+    for dt in DTS:
+        np.testing.assert_allclose(k.state_transition(dt), scipy.linalg.expm(F * dt), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("k", KERNELS, ids=lambda k: type(k).__name__)
+def test_pinf_solves_lyapunov(k):
+    F, LQL = _lyap_terms(k)
+    P = k.stationary_covariance()
+    resid = F @ P + P @ F.T + LQL
+    assert np.max(np.abs(resid)) < 1e-10 * np.max(np.abs(LQL))
+    np.testing.assert_allclose(scipy.linalg.solve_continuous_lyapunov(F, -LQL), P, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", KERNELS, ids=lambda k: type(k).__name__)
+def test_process_noise_is_van_loan_integral(k):
+    from oracle import ops
+    F, LQL = _lyap_terms(k)
+    d = F.shape[0]
+    Pinf = k.stationary_covariance()
+    lam_max = np.max(np.abs(np.linalg.eigvals(F)))
+    for dt in DTS[2:]:
+        if lam_max * dt > 4.0:  # the Van Loan block exponential is ill-conditioned there
+            continue
+        M = np.zeros((2 * d, 2 * d))
+        M[:d, :d] = F
+        M[:d, d:] = LQL
+        M[d:, d:] = -F.T
+        E = scipy.linalg.expm(M * dt)
+        Qvl = E[:d, d:] @ E[:d, :d].T
+        Q = ops.process_noise_covariance(k.state_transition(dt), Pinf)
+        np.testing.assert_allclose(Q, Qvl, rtol=1e-8, atol=1e-10 * np.max(np.abs(Pinf)))
+
+
+@pytest.mark.parametrize("k", KERNELS, ids=lambda k: type(k).__name__)
+def test_state_space_reproduces_covariance_function(k):
+    F = k.feedback_matrix()
+    P = k.stationary_covariance()
+    H = np.concatenate([c.measurement_model() for c in (k.kernels if isinstance(k, K.Sum) else [k])], axis=1)
+    for tau in [0.0, 0.05, 0.4, 1.7, 6.0]:
+        kt = (H @ scipy.linalg.expm(F * tau) @ P @ H.T)[0, 0]
+        kd = k.K(np.array([[0.0]]), np.array([[tau]]))[0, 0]
+        assert abs(kt - kd) < 1e-10 * max(1.0, abs(kd))
+
+
+def _tables():
+    path = os.path.join(os.path.dirname(__file__), "..", "bayesnewton_amd", "csrc", "bn_qtaylor.h")
+    src = open(path).read()
+
+    def table(name):
+        m = re.search(name + r"\[(\d+)\]\[(\d+)\] = \{(.*?)\};", src, re.S)
+        rows = re.findall(r"\{([^{}]*)\}", m.group(3))
+        return [[float(v) for v in r.split(",")] for r in rows]
+
+    x0 = float(re.search(r"kQX0 = ([0-9.eE+-]+);", src).group(1))
+    return table, x0
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_cancellation_free_q_tables(kind):
+    """The device's Q_ij = s2 lam^(i+j) F_ij(lam dt) tables against 60-digit arithmetic."""
+    getcontext().prec = 60
+    table, x0 = _tables()
+    T = table(f"kQTaylor{kind}")
+    Pp = table(f"kQPoly{kind}")
+    comp = {1: K.Matern12, 2: K.Matern32, 3: K.Matern52}[kind](1.0, 1.0)
+    lam = {1: 1.0, 2: math.sqrt(3.0), 3: math.sqrt(5.0)}[kind]
+    # Pinf = s2 lam^(i+j) C (kernels.py:144-146, 207-210, 288-293), C exact
+    third = Decimal(1) / 3
+    C = {1: [[Decimal(1)]], 2: [[Decimal(1), Decimal(0)], [Decimal(0), Decimal(1)]],
+         3: [[Decimal(1), Decimal(0), -third], [Decimal(0), third, Decimal(0)], [-third, Decimal(0), Decimal(1)]]}[kind]
+    Pinf = comp.stationary_covariance()
+    for i in range(kind):
+        for j in range(kind):
+            assert abs(Pinf[i, j] - float(C[i][j]) * lam ** (i + j)) < 1e-12 * max(1.0, abs(Pinf[i, j]))
+    for xf in [1e-7, 1e-4, 0.03, 0.31, x0 * 0.999, x0, 1.3, 9.0]:
+        # exact F(x) = C - e^{-2x} a(x) C a(x)' with a = e^{x} A(x / lam) rescaled by lam^(j - i)
+        idx = 0
+        for i in range(kind):
+            for j in range(i, kind):
+                # exact value from a high-precision evaluation of the polynomial form
+                xd = Decimal(xf)
+                ex = (-xd).exp()
+                a = _poly_a(kind, xd)
+                P = sum(a[i][k] * C[k][l] * a[j][l] for k in range(kind) for l in range(kind))
+                exact = C[i][j] - ex * ex * P
+                if xf < x0:  # Horner, as on the device
+                    v = T[idx][-1]
+                    for c in reversed(T[idx][:-1]):
+                        v = v * xf + c
+                else:
+                    p = Pp[idx][-1]
+                    for c in reversed(Pp[idx][:-1]):
+                        p = p * xf + c
+                    v = float(C[i][j]) - math.exp(-2 * xf) * p
+                rel = abs(Decimal(v) - exact) / abs(exact)
+                assert rel < Decimal(1e-13), (kind, xf, i, j, float(rel))
+                idx += 1
+
+
+def _poly_a(kind, x):
+    """a_ij(x) = e^{x} lam^(j-i) A_ij(x / lam) (kernels.py:156-163, 216-224, 273-286)."""
+    h = Decimal(1) / 2
+    if kind == 1:
+        return [[Decimal(1)]]
+    if kind == 2:
+        return [[1 + x, x], [-x, 1 - x]]
+    return [[1 + x + h * x * x, x + x * x, h * x * x],
+            [-h * x * x, 1 + x - x * x, x - h * x * x],
+            [h * x * x - x, x * x - 3 * x, 1 - 2 * x + h * x * x]]
