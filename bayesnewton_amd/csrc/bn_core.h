@@ -99,12 +99,16 @@ BN_HD double comp_lambda(double ls) {
   return K == 1 ? 1.0 / ls : (K == 2 ? sqrt(3.0) / ls : sqrt(5.0) / ls);
 }
 
-// Row `idx` (packed upper index) of the Taylor or direct-form polynomial table.
+// Row `idx` (packed upper index) of the small-x series or direct-form polynomial table.
 template <int K>
 BN_HD const double* qtab(int idx, bool small) {
-  if (K == 1) return small ? kQTaylor1[idx] : kQPoly1[idx];
-  if (K == 2) return small ? kQTaylor2[idx] : kQPoly2[idx];
-  return small ? kQTaylor3[idx] : kQPoly3[idx];
+  if (K == 1) return small ? kQSeries1[idx] : kQPoly1[idx];
+  if (K == 2) return small ? kQSeries2[idx] : kQPoly2[idx];
+  return small ? kQSeries3[idx] : kQPoly3[idx];
+}
+template <int K>
+BN_HD int qlen(int idx) {
+  return K == 1 ? kQLen1[idx] : (K == 2 ? kQLen2[idx] : kQLen3[idx]);
 }
 template <int K>
 BN_HD double qc(int idx) {
@@ -169,15 +173,20 @@ struct Model {
       double lp = lp_i;
       BN_UNROLL for (int j = i; j < K; ++j) {
         // Horner over a table row read from constant memory (a runtime loop, so
-        // the coefficients are not hoisted into registers across time steps)
+        // the coefficients are not hoisted into registers across time steps).
+        // Small x: F_ij = x^(2K-1-i-j) G_ij(x); else F_ij = C_ij - e^{-2x} P_ij(x).
         const double* tab = qtab<K>(idx, small);
-        const int nt = small ? kQTerms : qpolydeg<K>();
+        const int nt = small ? qlen<K>(idx) : qpolydeg<K>();
         double v = tab[nt - 1];
 #if defined(__HIPCC__)
 #pragma unroll 1
 #endif
         for (int n = nt - 2; n >= 0; --n) v = v * x + tab[n];
-        if (!small) v = qc<K>(idx) - e2 * v;
+        if (small) {
+          BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) v *= x;
+        } else {
+          v = qc<K>(idx) - e2 * v;
+        }
         const T q = (T)(lp * v);
         Q[O + i][O + j] = q;
         Q[O + j][O + i] = q;

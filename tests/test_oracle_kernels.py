@@ -85,16 +85,22 @@ def _tables():
         rows = re.findall(r"\{([^{}]*)\}", m.group(3))
         return [[float(v) for v in r.split(",")] for r in rows]
 
+    def ints(name):
+        m = re.search(name + r"\[(\d+)\] = \{(.*?)\};", src, re.S)
+        return [int(v) for v in m.group(2).split(",")]
+
     x0 = float(re.search(r"kQX0 = ([0-9.eE+-]+);", src).group(1))
-    return table, x0
+    return table, ints, x0
 
 
 @pytest.mark.parametrize("kind", [1, 2, 3])
 def test_cancellation_free_q_tables(kind):
     """The device's Q_ij = s2 lam^(i+j) F_ij(lam dt) tables against 60-digit arithmetic."""
     getcontext().prec = 60
-    table, x0 = _tables()
-    T = table(f"kQTaylor{kind}")
+    table, ints, x0 = _tables()
+    T = table(f"kQSeries{kind}")
+    lens = ints(f"kQLen{kind}")
+    pows = ints(f"kQPow{kind}")
     Pp = table(f"kQPoly{kind}")
     comp = {1: K.Matern12, 2: K.Matern32, 3: K.Matern52}[kind](1.0, 1.0)
     lam = {1: 1.0, 2: math.sqrt(3.0), 3: math.sqrt(5.0)}[kind]
@@ -117,17 +123,21 @@ def test_cancellation_free_q_tables(kind):
                 a = _poly_a(kind, xd)
                 P = sum(a[i][k] * C[k][l] * a[j][l] for k in range(kind) for l in range(kind))
                 exact = C[i][j] - ex * ex * P
-                if xf < x0:  # Horner, as on the device
-                    v = T[idx][-1]
-                    for c in reversed(T[idx][:-1]):
+                if xf < x0:  # Horner on G_ij, times x^m, as on the device
+                    assert pows[idx] == 2 * kind - 1 - i - j
+                    row = T[idx][:lens[idx]]
+                    v = row[-1]
+                    for c in reversed(row[:-1]):
                         v = v * xf + c
+                    v *= xf ** pows[idx]
                 else:
                     p = Pp[idx][-1]
                     for c in reversed(Pp[idx][:-1]):
                         p = p * xf + c
                     v = float(C[i][j]) - math.exp(-2 * xf) * p
                 rel = abs(Decimal(v) - exact) / abs(exact)
-                assert rel < Decimal(1e-13), (kind, xf, i, j, float(rel))
+                # series region ~1e-16; the direct form just above X0 keeps ~1e-13
+                assert rel < Decimal(1e-12), (kind, xf, i, j, float(rel))
                 idx += 1
 
 
