@@ -30,6 +30,16 @@ class LikDesc(ctypes.Structure):
     _fields_ = [("kind", i32), ("param0", f64), ("param1", f64)]
 
 
+class StDims(ctypes.Structure):
+    _fields_ = [("n_runs", i32), ("n_steps", i32), ("n_loc", i32), ("n_inducing", i32), ("temporal_kind", i32)]
+
+
+class StHypers(ctypes.Structure):
+    _fields_ = [("ls_t", vp), ("var_t", vp), ("var_s", vp), ("stride", i32)]
+
+
+BN_ST_UPDATE, BN_ST_ENERGY = 0, 1
+
 BN_LIK_GAUSSIAN, BN_LIK_BERNOULLI_PROBIT, BN_LIK_POISSON, BN_LIK_STUDENTS_T = 1, 2, 3, 4
 BN_VI, BN_NEWTON, BN_EP = 0, 1, 2
 
@@ -53,12 +63,22 @@ SIGNATURES = {
     "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
     "bn_filter_loglik": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_grad_workspace_bytes": (i64, [vp, vp]),
+    "bn_filter_loglik_grad": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_rts_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
     "bn_site_update": (i32, [vp, vp, vp, i32, i32, f64, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                               vp, vp]),
     "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_site_reparametrise": (i32, [vp, vp, vp, vp, vp, vp]),
+    "bn_lik_param_grad": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp]),
     "bn_gauss_hermite": (i32, [i32, vp, vp]),
+    "bn_st_conditional": (i32, [vp, vp, f64, vp, vp, vp, vp, vp, vp]),
+    "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                          vp, vp]),
+    "bn_st_workspace_bytes": (i64, [vp]),
+    "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_timing_enable": (i32, [i32]),
     "bn_timing_read": (i32, [i32, ctypes.POINTER(f64), ctypes.POINTER(i32)]),
     "bn_kernel_name": (ctypes.c_char_p, [i32]),

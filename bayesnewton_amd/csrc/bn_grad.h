@@ -1,0 +1,357 @@
+// Derivative of the filter log-likelihood log Z = sum_k log N(y_k; H m-_k, S_k)
+// (ops.py:154-180, compute_log_lik basemodels.py:726-741) with respect to the
+// kernel hyperparameters (lengthscale, variance of every component), the
+// gradient objax.GradValues(model.energy) takes through the filter with the
+// sites and posterior held fixed (energy = ... - log Z, inference.py:197-222).
+//
+// Forward mode inside the chunked scan: tangents (dm, dP) obey a recursion
+// that is affine given the primal trajectory,
+//   z_k = J_k z_{k-1} + s_k^(theta),   dlogZ_k = g_k . z_{k-1} + c_k^(theta),
+// with one linear part J_k for every hyperparameter.  A lane propagates, over
+// its chunk, the NZ = D + D(D+1)/2 unit tangents (the chunk's linear map M and
+// functional f) and the NH hyperparameter tangents started at zero (sources o,
+// constants c); a per-series pass over the chunks then applies
+// z <- M_j z + o_j, grad += f_j . z + c_j.  Nothing is stored per time step.
+#pragma once
+#include "bn_lanes.h"
+
+namespace bn {
+
+// dA/dlambda of one component: A_ij = lam^(i-j) e^{-x} a_ij(x), x = lam dt, so
+//   dA_ij/dlam = (i-j) lam^(i-j-1) e^{-x} a_ij + lam^(i-j) e^{-x} dt (a_ij' - a_ij)
+template <int K>
+BN_HD void comp_dA_dlam(double lam, double dt, double* dA) {
+  const double x = lam * dt, e = exp(-x);
+  double a[9], da[9];
+  if (K == 1) {
+    a[0] = 1.0; da[0] = 0.0;
+  } else if (K == 2) {
+    a[0] = 1.0 + x; a[1] = x; a[2] = -x; a[3] = 1.0 - x;
+    da[0] = 1.0; da[1] = 1.0; da[2] = -1.0; da[3] = -1.0;
+  } else {
+    a[0] = 1.0 + x + 0.5 * x * x; a[1] = x + x * x;     a[2] = 0.5 * x * x;
+    a[3] = -0.5 * x * x;          a[4] = 1.0 + x - x * x; a[5] = x - 0.5 * x * x;
+    a[6] = 0.5 * x * x - x;       a[7] = x * x - 3.0 * x; a[8] = 1.0 - 2.0 * x + 0.5 * x * x;
+    da[0] = 1.0 + x; da[1] = 1.0 + 2.0 * x; da[2] = x;
+    da[3] = -x;      da[4] = 1.0 - 2.0 * x; da[5] = 1.0 - x;
+    da[6] = x - 1.0; da[7] = 2.0 * x - 3.0; da[8] = -2.0 + x;
+  }
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) {
+      const int p = i - j;
+      const double lp = pow(lam, (double)p);
+      dA[i * K + j] = (p != 0 ? p * lp / lam * e * a[i * K + j] : 0.0) + lp * e * dt * (da[i * K + j] - a[i * K + j]);
+    }
+}
+
+// dQ/dlambda of one component (packed upper, K(K+1)/2 entries) from
+// Q_ij = var lam^(i+j) F_ij(x):  dQ_ij/dlam = var (i+j) lam^(i+j-1) F + var lam^(i+j) dt F'(x),
+// with F = x^m G(x) (series, x < X0) or C - e^{-2x} P(x) (above), as in
+// Model::comp_block.  Also returns Q itself.
+template <int K>
+BN_HD void comp_dQ_dlam(double lam, double var, double dt, double* Qp, double* dQp) {
+  const double x = lam * dt;
+  const bool small = x < kQX0;
+  const double e2 = exp(-2.0 * x);
+  int idx = 0;
+  for (int i = 0; i < K; ++i)
+    for (int j = i; j < K; ++j) {
+      const double* tab = qtab<K>(idx, small);
+      const int nt = small ? qlen<K>(idx) : qpolydeg<K>();
+      double v = tab[nt - 1], dv = 0.0;
+      for (int n = nt - 2; n >= 0; --n) {
+        dv = dv * x + v;
+        v = v * x + tab[n];
+      }
+      double F, dF;
+      if (small) {
+        const int m = 2 * K - 1 - i - j;
+        double xm1 = 1.0;
+        for (int q = 0; q < m - 1; ++q) xm1 *= x;
+        F = xm1 * x * v;
+        dF = xm1 * (m * v + x * dv);
+      } else {
+        F = qc<K>(idx) - e2 * v;
+        dF = e2 * (2.0 * v - dv);
+      }
+      const double lp = pow(lam, (double)(i + j));
+      Qp[idx] = var * lp * F;
+      dQp[idx] = var * ((i + j) * lp / lam * F + lp * dt * dF);
+      ++idx;
+    }
+}
+
+// One linearised filter step applied to a tangent (dm, dP), with the primal
+// quantities of the step: A, m (previous filtered mean), P (previous filtered
+// covariance), mp = A m, Pp = A P A' + Q, s, v, b = Pp h.  Source terms
+// (dA, dQ) are added when `src` is set; at the first step of a series the
+// predicted moments are (0, Pinf) and the tangent is (0, dPinf).
+// Returns the step's d logZ; masked steps contribute 0 but still update.
+template <int K1, int K2>
+BN_HD double tangent_step(const double A[Spec<K1, K2>::D][Spec<K1, K2>::D], const double m[Spec<K1, K2>::D],
+                          const double P[Spec<K1, K2>::D][Spec<K1, K2>::D], const double b[Spec<K1, K2>::D],
+                          double s, double v, bool masked, bool first, const double* dA, const double* dQ,
+                          const double* dPinf, double* z) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  double dm[D], dP[D][D];
+  // unpack z = (dm, packed upper dP)
+  {
+    int f = D;
+    BN_UNROLL for (int i = 0; i < D; ++i) dm[i] = z[i];
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) { dP[i][j] = z[f]; dP[j][i] = z[f]; ++f; }
+  }
+  double dmp[D], dPp[D][D];
+  if (first) {
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      dmp[i] = 0.0;
+      BN_UNROLL for (int j = 0; j < D; ++j) dPp[i][j] = dPinf ? dPinf[i * D + j] : 0.0;
+    }
+  } else {
+    double AdP[D][D];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      double t = 0.0;
+      BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) t += A[i][k] * dm[k];
+      if (dA) BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) t += dA[i * D + k] * m[k];
+      dmp[i] = t;
+      BN_UNROLL for (int c = 0; c < D; ++c) {
+        double u = 0.0;
+        BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) u += A[i][k] * dP[k][c];
+        AdP[i][c] = u;
+      }
+    }
+    // dPp = A dP A' + (dA P A' + A P dA') + dQ; X = dA P A' so the middle term is X + X'
+    double X[D][D];
+    if (dA) {
+      double dAP[D][D];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) {
+          double u = 0.0;
+          BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(i, k)) u += dA[i * D + k] * P[k][c];
+          dAP[i][c] = u;
+        }
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = 0; c < D; ++c) {
+          double u = 0.0;
+          BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(c, k)) u += dAP[i][k] * A[c][k];
+          X[i][c] = u;
+        }
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int c = i; c < D; ++c) {
+        double t = dQ ? dQ[i * D + c] : 0.0;
+        BN_UNROLL for (int k = 0; k < D; ++k) if (S::nz(c, k)) t += AdP[i][k] * A[c][k];
+        if (dA) t += X[i][c] + X[c][i];
+        dPp[i][c] = t;
+        dPp[c][i] = t;
+      }
+  }
+  // ds = h' dPp h, dv = -h' dmp, db = dPp h
+  double db[D];
+  double ds = 0.0, dv = 0.0;
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    double t = 0.0;
+    BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += dPp[i][q];
+    db[i] = t;
+  }
+  BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { ds += db[q]; dv -= dmp[q]; }
+  const double is = 1.0 / s;
+  BN_UNROLL for (int i = 0; i < D; ++i) z[i] = dmp[i] + (db[i] * v + b[i] * dv) * is - b[i] * v * ds * is * is;
+  {
+    int f = D;
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) {
+        z[f++] = dPp[i][j] - (db[i] * b[j] + b[i] * db[j]) * is + b[i] * b[j] * ds * is * is;
+      }
+  }
+  if (masked) return 0.0;
+  return -v * dv * is + 0.5 * (v * v * is - 1.0) * ds * is;
+}
+
+}  // namespace bn
+
+namespace bn {
+
+template <int K1, int K2>
+struct GradDims {
+  static constexpr int D = K1 + K2;
+  static constexpr int NZ = D + D * (D + 1) / 2;  // tangent size (dm, packed dP)
+  static constexpr int NH = K2 > 0 ? 4 : 2;       // (lengthscale, variance) per component
+  static constexpr int NF = NZ * NZ + NH * NZ + NZ + NH;  // stored doubles per chunk
+};
+
+struct GradOut {
+  double* agg;     // field-major per chunk: M (NZ*NZ, column-major), o (NH*NZ), f (NZ), c (NH)
+  int64_t stride;  // n_chunks_pad
+  double* ell;     // per-chunk log-likelihood partial (primal)
+};
+
+// Source terms of one hyperparameter column at one step, as full D x D arrays.
+template <int K1, int K2>
+struct StepSources {
+  static constexpr int D = K1 + K2;
+  double dA[GradDims<K1, K2>::NH][D * D];
+  double dQ[GradDims<K1, K2>::NH][D * D];
+  double dPinf[GradDims<K1, K2>::NH][D * D];
+};
+
+template <int K, int O, int D>
+BN_HD void comp_sources(double lam, double var, double ls, double dt, const double* Pinf_full, int h0, double (*dA)[D * D],
+                        double (*dQ)[D * D], double (*dPinf)[D * D], double A[D][D], double Q[D][D]) {
+  double a[9], da[9], q[6], dq[6];
+  comp_transition<K>(lam, dt, a);
+  comp_dA_dlam<K>(lam, dt, da);
+  comp_dQ_dlam<K>(lam, var, dt, q, dq);
+  const double dlam_dls = -lam / ls;  // lam = c / ls
+  int idx = 0;
+  for (int i = 0; i < K; ++i)
+    for (int j = i; j < K; ++j) {
+      Q[O + i][O + j] = Q[O + j][O + i] = q[idx];
+      const double dl = dq[idx] * dlam_dls, dv = q[idx] / var;
+      dQ[h0][(O + i) * D + O + j] = dQ[h0][(O + j) * D + O + i] = dl;
+      dQ[h0 + 1][(O + i) * D + O + j] = dQ[h0 + 1][(O + j) * D + O + i] = dv;
+      ++idx;
+    }
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) {
+      A[O + i][O + j] = a[i * K + j];
+      dA[h0][(O + i) * D + O + j] = da[i * K + j] * dlam_dls;
+      // Pinf_ij = var lam^(i+j) C_ij: d/dlam = (i+j) Pinf / lam, d/dvar = Pinf / var
+      const double p = Pinf_full[(O + i) * D + O + j];
+      dPinf[h0][(O + i) * D + O + j] = (i + j) * p / lam * dlam_dls;
+      dPinf[h0 + 1][(O + i) * D + O + j] = p / var;
+    }
+}
+
+// One lane: the chunk's tangent aggregate (see the header comment).
+template <int K1, int K2>
+BN_HD void lane_grad_chunk(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
+                           const float* __restrict__ y, const float* __restrict__ r,
+                           const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry, const GradOut& out, int32_t j) {
+  using S = Spec<K1, K2>;
+  using GD = GradDims<K1, K2>;
+  constexpr int D = S::D, NZ = GD::NZ, NH = GD::NH;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  const bool first_chunk = (fl & CH_FIRST) != 0;
+  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  const int64_t ho = (int64_t)b * hv.stride;
+  const double ls1 = hv.ls[ho], var1 = hv.var[ho];
+  const double ls2 = K2 > 0 ? hv.ls[ho + 1] : 1.0, var2 = K2 > 0 ? hv.var[ho + 1] : 1.0;
+  double Pinf_full[D * D];
+  for (int i = 0; i < D; ++i)
+    for (int c = 0; c < D; ++c) Pinf_full[i * D + c] = mdl.Pinf[i][c];
+  double m[D], P[D][D];
+  if (!first_chunk) {
+    filter_incoming<D, double>(pl, fagg, fcarry, j, m, P);
+  } else {
+    for (int i = 0; i < D; ++i) { m[i] = 0.0; for (int c = 0; c < D; ++c) P[i][c] = 0.0; }
+  }
+  // columns: NZ unit tangents (the chunk's linear map; not needed for a series'
+  // first chunk, whose incoming tangent is zero) then NH hyperparameter tangents
+  const int c0 = first_chunk ? NZ : 0;
+  double Z[NZ + NH][NZ];
+  double acc[NZ + NH];
+  for (int c = 0; c < NZ + NH; ++c) {
+    acc[c] = 0.0;
+    for (int q = 0; q < NZ; ++q) Z[c][q] = (c < NZ && c == q) ? 1.0 : 0.0;
+  }
+  StepSources<K1, K2> src;
+  double ell = 0.0;
+  for (int32_t k = 0; k < L; ++k) {
+    const int64_t s = pl.slot(j, k);
+    const bool first = (k == 0) && first_chunk;
+    double A[D][D], Q[D][D];
+    for (int h = 0; h < NH; ++h)
+      for (int q = 0; q < D * D; ++q) { src.dA[h][q] = 0.0; src.dQ[h][q] = 0.0; src.dPinf[h][q] = 0.0; }
+    for (int i = 0; i < D; ++i)
+      for (int c = 0; c < D; ++c) { A[i][c] = 0.0; Q[i][c] = 0.0; }
+    const double dts = first ? 0.0 : (double)dt[s];
+    comp_sources<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, 0, src.dA, src.dQ, src.dPinf, A, Q);
+    if (K2 > 0) comp_sources<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, 2, src.dA, src.dQ, src.dPinf, A, Q);
+    // primal prediction
+    double mp[D], Pp[D][D];
+    if (first) {
+      for (int i = 0; i < D; ++i) { mp[i] = 0.0; for (int c = 0; c < D; ++c) Pp[i][c] = mdl.Pinf[i][c]; }
+    } else {
+      double AP[D][D];
+      for (int i = 0; i < D; ++i) {
+        double t = 0.0;
+        for (int q = 0; q < D; ++q) t += A[i][q] * m[q];
+        mp[i] = t;
+        for (int c = 0; c < D; ++c) {
+          double u = 0.0;
+          for (int q = 0; q < D; ++q) u += A[i][q] * P[q][c];
+          AP[i][c] = u;
+        }
+      }
+      for (int i = 0; i < D; ++i)
+        for (int c = i; c < D; ++c) {
+          double t = Q[i][c];
+          for (int q = 0; q < D; ++q) t += AP[i][q] * A[c][q];
+          Pp[i][c] = Pp[c][i] = t;
+        }
+    }
+    double bb[D];
+    double sd = (double)r[s], hm = 0.0;
+    for (int i = 0; i < D; ++i) {
+      double t = 0.0;
+      for (int q = 0; q < D; ++q) if (S::h(q)) t += Pp[i][q];
+      bb[i] = t;
+    }
+    for (int q = 0; q < D; ++q) if (S::h(q)) { sd += bb[q]; hm += mp[q]; }
+    const double v = (double)y[s] - hm;
+    const bool masked = mask != nullptr && mask[s];
+    if (!masked) ell += -0.5 * (v * v / sd + kLog2Pi + log(sd));
+    for (int c = c0; c < NZ + NH; ++c) {
+      const int h = c - NZ;
+      acc[c] += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, h >= 0 ? src.dA[h] : nullptr,
+                                     h >= 0 ? src.dQ[h] : nullptr, h >= 0 ? src.dPinf[h] : nullptr, Z[c]);
+    }
+    // primal update
+    for (int i = 0; i < D; ++i) m[i] = mp[i] + bb[i] * v / sd;
+    for (int i = 0; i < D; ++i)
+      for (int c = i; c < D; ++c) P[i][c] = P[c][i] = Pp[i][c] - bb[i] * bb[c] / sd;
+  }
+  // store: M column-major (column c = Z[c]), o, f, c
+  const int64_t st = out.stride;
+  int f = 0;
+  for (int c = 0; c < NZ; ++c)
+    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(f++) * st + j] = first_chunk ? 0.0 : Z[c][q];
+  for (int h = 0; h < NH; ++h)
+    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(f++) * st + j] = Z[NZ + h][q];
+  for (int c = 0; c < NZ; ++c) out.agg[(int64_t)(f++) * st + j] = first_chunk ? 0.0 : acc[c];
+  for (int h = 0; h < NH; ++h) out.agg[(int64_t)(f++) * st + j] = acc[NZ + h];
+  out.ell[j] = ell;
+}
+
+// One thread per (series, hyperparameter): apply the chunk maps in order.
+template <int K1, int K2>
+BN_HD void lane_grad_series(const PlanView& pl, const double* __restrict__ agg, int64_t stride, int32_t b, int32_t h,
+                            double* __restrict__ grad) {
+  using GD = GradDims<K1, K2>;
+  constexpr int NZ = GD::NZ, NH = GD::NH;
+  double z[NZ];
+  for (int q = 0; q < NZ; ++q) z[q] = 0.0;
+  double g = 0.0;
+  const int32_t c0 = pl.series_first_chunk[b], nc = pl.series_nchunks[b];
+  for (int32_t jj = 0; jj < nc; ++jj) {
+    const int64_t j = c0 + jj;
+    const int fo = NZ * NZ + h * NZ, ff = NZ * NZ + NH * NZ, fc = ff + NZ + h;
+    double t = agg[(int64_t)fc * stride + j];
+    for (int q = 0; q < NZ; ++q) t += agg[(int64_t)(ff + q) * stride + j] * z[q];
+    g += t;
+    double nz[NZ];
+    for (int p = 0; p < NZ; ++p) {
+      double u = agg[(int64_t)(fo + p) * stride + j];
+      for (int q = 0; q < NZ; ++q) u += agg[(int64_t)(q * NZ + p) * stride + j] * z[q];
+      nz[p] = u;
+    }
+    for (int p = 0; p < NZ; ++p) z[p] = nz[p];
+  }
+  grad[(int64_t)b * NH + h] = g;
+}
+
+}  // namespace bn

@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "bn_grad.h"
 #include "bn_internal.h"
 
 using namespace bn;
@@ -37,41 +38,6 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(BN_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-// ---- optional per-kernel timing (measurement only; not thread-safe) --------
-enum KernelId : int32_t {
-  KID_FILTER_REDUCE = 0, KID_FSCAN_TILE, KID_FSCAN_SERIES, KID_FILTER_PASS, KID_SMOOTHER_REDUCE,
-  KID_SSCAN_TILE, KID_SSCAN_SERIES, KID_SMOOTHER_PASS, KID_SERIES_SUM, KID_COUNT
-};
-const char* const kKernelNames[KID_COUNT] = {
-    "k_filter_reduce", "k_fscan_tile", "k_fscan_series", "k_filter_pass", "k_smoother_reduce",
-    "k_sscan_tile", "k_sscan_series", "k_smoother_pass", "k_series_sum"};
-struct TimingRec { int32_t kid; hipEvent_t a, b; };
-bool g_timing = false;
-std::vector<TimingRec> g_recs;
-std::vector<hipEvent_t> g_pool;
-
-hipEvent_t pool_event() {
-  if (!g_pool.empty()) { hipEvent_t e = g_pool.back(); g_pool.pop_back(); return e; }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-// Brackets a launch with events on its stream when timing is enabled.
-struct TimedLaunch {
-  int32_t kid; hipStream_t st; hipEvent_t a = nullptr;
-  TimedLaunch(int32_t k, hipStream_t s) : kid(k), st(s) {
-    if (g_timing && (a = pool_event()) != nullptr) (void)hipEventRecord(a, st);
-  }
-  ~TimedLaunch() {
-    if (a == nullptr) return;
-    hipEvent_t b = pool_event();
-    if (b == nullptr) { g_pool.push_back(a); return; }
-    (void)hipEventRecord(b, st);
-    g_recs.push_back(TimingRec{kid, a, b});
-  }
-};
-#define BN_TIMED(kid, st) TimedLaunch bn_tl_##kid(kid, st)
 
 constexpr int kBlock = 256;  // lane-per-chunk kernels
 constexpr int kScan = kTile;  // scan kernels: one thread per element of a tile
@@ -229,6 +195,24 @@ __global__ __launch_bounds__(kBlock) void k_smoother_pass(PlanView pl, HyperView
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
   lane_smoother_pass<K1, K2, CT, OT>(pl, hv, sagg, scarry, io, j);
+}
+
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv, const float* __restrict__ dt,
+                                                       const float* __restrict__ y, const float* __restrict__ r,
+                                                       const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry,
+                                                       GradOut out) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  lane_grad_chunk<K1, K2>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
+}
+
+template <int K1, int K2>
+__global__ void k_grad_series(PlanView pl, const double* __restrict__ agg, int64_t stride, double* __restrict__ grad) {
+  constexpr int NH = GradDims<K1, K2>::NH;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)pl.n_series * NH) return;
+  lane_grad_series<K1, K2>(pl, agg, stride, (int32_t)(t / NH), (int32_t)(t % NH), grad);
 }
 
 // Deterministic per-series sum of a per-chunk FP64 array (fixed tree order).
@@ -479,6 +463,24 @@ int run_kalman_filter(const bn_plan* plan, HyperView hv, const float* dt, const 
 }
 
 template <int K1, int K2>
+int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
+                    const uint8_t* mask, double* loglik, double* grad, void* ws, void* gws, hipStream_t st) {
+  constexpr int D = K1 + K2;
+  const PlanHost& p = plan->host;
+  Bufs b = ws_bufs(p, D, ws);
+  launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
+  GradOut go{(double*)gws, p.n_chunks_pad, b.ell};
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
+                     cs(b.fagg), cs(b.ftcarry), go);
+  const int64_t nthreads = (int64_t)p.n_series * GradDims<K1, K2>::NH;
+  hipLaunchKernelGGL((k_grad_series<K1, K2>), dim3((unsigned)((nthreads + 63) / 64)), dim3(64), 0, st, plan->view,
+                     (const double*)gws, (int64_t)p.n_chunks_pad, grad);
+  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+template <int K1, int K2>
 int run_rts_smoother(const bn_plan* plan, HyperView hv, const float* dts, const double* fmean, const double* fcov,
                      double* means, double* covs, double* gains, uint32_t flags, void* ws, hipStream_t st) {
   constexpr int D = K1 + K2;
@@ -548,6 +550,39 @@ double* site_partials(const bn_plan* plan, const bn_kernel_desc* kern, void* wor
   if (plan == nullptr || workspace == nullptr || check_desc(kern) != BN_OK) return nullptr;
   WsLayout w = ws_layout(plan->host, state_dim(kern));
   return (double*)((char*)workspace + (which == 0 ? w.sp1 : w.sp2));
+}
+}  // namespace bn
+
+// ---- per-kernel timing (measurement only; not thread-safe) -----------------
+namespace bn {
+namespace {
+const char* const kKernelNames[KID_COUNT] = {
+    "k_filter_reduce", "k_fscan_tile", "k_fscan_series", "k_filter_pass", "k_smoother_reduce",
+    "k_sscan_tile", "k_sscan_series", "k_smoother_pass", "k_series_sum", "k_st_conditional",
+    "k_st_sites", "k_st_filter", "k_st_smoother", "k_st_sum"};
+struct TimingRec { int32_t kid; hipEvent_t a, b; };
+bool g_timing = false;
+std::vector<TimingRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t pool_event() {
+  if (!g_pool.empty()) { hipEvent_t e = g_pool.back(); g_pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+TimedLaunch::TimedLaunch(int32_t k, hipStream_t s) : kid(k), st(s) {
+  if (g_timing && (a = pool_event()) != nullptr) (void)hipEventRecord(a, st);
+}
+
+TimedLaunch::~TimedLaunch() {
+  if (a == nullptr) return;
+  hipEvent_t b = pool_event();
+  if (b == nullptr) { g_pool.push_back(a); return; }
+  (void)hipEventRecord(b, st);
+  g_recs.push_back(TimingRec{kid, a, b});
 }
 }  // namespace bn
 
@@ -767,6 +802,27 @@ int bn_filter_loglik(const bn_plan* plan, const bn_kernel_desc* kern, const doub
   HyperView hv{lengthscale, variance, hyper_stride};
   return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, (double*)nullptr, (double*)nullptr, loglik,
                      0u, workspace, (hipStream_t)stream);
+}
+
+int64_t bn_grad_workspace_bytes(const bn_plan* plan, const bn_kernel_desc* kern) {
+  if (plan == nullptr || check_desc(kern) != BN_OK) return -1;
+  const int D = state_dim(kern);
+  const int64_t NZ = D + D * (D + 1) / 2, NH = 2 * kern->n_comp;
+  return align256((NZ * NZ + NH * NZ + NZ + NH) * plan->host.n_chunks_pad * 8);
+}
+
+int bn_filter_loglik_grad(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                          const double* variance, int32_t hyper_stride, const float* dt, const float* y,
+                          const float* r, const uint8_t* mask, double* loglik, double* grad, void* workspace,
+                          void* grad_workspace, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !y || !r || !loglik || !grad || !workspace || !grad_workspace || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_filter_loglik_grad");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_loglik_grad, plan, hv, dt, y, r, mask, loglik, grad, workspace, grad_workspace,
+                     (hipStream_t)stream);
 }
 
 int bn_rts_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,

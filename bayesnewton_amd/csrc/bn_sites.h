@@ -14,6 +14,7 @@
 // Sites are scalar (one per time step).  A NaN observation is a missing one.
 #pragma once
 #include "bn_core.h"
+#include "../../include/bn_hip.h"
 
 namespace bn {
 
@@ -162,6 +163,31 @@ BN_HD void moment_match(const LikParams& L, const GHTable& gh, double y, double 
   d2lZ = -dlZ * dlZ + iZ * d2Z;
 }
 
+// d log p(y | f) / d(likelihood hyperparameter): Gaussian variance, Student-t
+// scale; 0 for the likelihoods without one (Bernoulli, Poisson with fixed binsize).
+BN_HD double dlog_lik_dparam(const LikParams& L, double y, double f) {
+  if (L.kind == LIK_GAUSSIAN) return -0.5 / L.p0 + 0.5 * (y - f) * (y - f) / (L.p0 * L.p0);
+  if (L.kind == LIK_STUDENTS_T) {
+    const double r = y - f, s = L.p0, df = L.p1;
+    return -1.0 / s + (df + 1.0) * r * r / (s * (df * s * s + r * r));
+  }
+  return 0.0;
+}
+
+// d term1 / d(likelihood hyperparameter) of one site (see energy_terms), with the
+// posterior held fixed: VI E_q[d log p / d phi] (closed form for the Gaussian),
+// Newton d log p(y | m) / d phi.  Masked sites contribute 0.
+template <int METHOD>
+BN_HD double dterm1_dparam(const LikParams& L, const GHTable& gh, double y, double pm, double pv) {
+  if (y != y) return 0.0;
+  if (METHOD == M_NEWTON) return dlog_lik_dparam(L, y, pm);
+  if (L.kind == LIK_GAUSSIAN) return -0.5 / L.p0 + 0.5 * ((y - pm) * (y - pm) + pv) / (L.p0 * L.p0);
+  const double sv = sqrt(pv);
+  double e = 0.0;
+  for (int i = 0; i < gh.n; ++i) e += gh.w[i] * dlog_lik_dparam(L, y, pm + sv * gh.x[i]);
+  return e;
+}
+
 struct SiteResult {
   double mean, jac, hess;  // terms returned by inference() (hess after ensure_psd)
   double nat1, nat2;       // newton_update of those terms
@@ -248,5 +274,10 @@ BN_HD void energy_terms(const LikParams& L, const GHTable& gh, double y, double 
     t2 = -0.5 * (kLog2Pi + log(pr) + r * r / pr) - 0.5 * pv / pr;
   }
 }
+
+// Host helpers shared by the translation units that launch site kernels:
+// Gauss-Hermite table for n nodes, and a validated likelihood descriptor.
+int make_gh_table(int n, GHTable& gh);
+int check_lik(const bn_lik_desc* d, LikParams& L);
 
 }  // namespace bn

@@ -92,6 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams 
   io.part2[j] = s2;
 }
 
+template <int METHOD>
+__global__ __launch_bounds__(kBlock) void k_lik_grad(PlanView pl, LikParams L, GHTable gh, SiteIO io) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  const int32_t len = pl.chunk_len_[j];
+  double s1 = 0.0;
+  for (int32_t k = 0; k < len; ++k) {
+    const int64_t s = pl.slot(j, k);
+    s1 += dterm1_dparam<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s]);
+  }
+  io.part1[j] = s1;
+  io.part2[j] = 0.0;
+}
+
 __global__ void k_reparametrise(PlanView pl, const double* __restrict__ nat1, const double* __restrict__ nat2,
                                 float* __restrict__ py, float* __restrict__ pr) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,6 +246,11 @@ int make_lik_checked(const bn_lik_desc* d, LikParams& L) {
 
 }  // namespace
 
+namespace bn {
+int make_gh_table(int n, GHTable& gh) { return make_gh(n, gh); }
+int check_lik(const bn_lik_desc* d, LikParams& L) { return make_lik_checked(d, L); }
+}  // namespace bn
+
 extern "C" {
 
 int bn_gauss_hermite(int32_t n, double* x, double* w) {
@@ -328,6 +347,34 @@ int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_li
       return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
   }
   hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, term1, term2);
+  BN_HIP_CHECK2(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_lik_param_grad(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
+                      int32_t gh_points, const float* y, const float* post_mean, const float* post_var, double* grad,
+                      void* workspace, void* stream) {
+  if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
+  if (!y || !post_mean || !post_var || !grad || !workspace) return fail(BN_EINVAL, "NULL array argument");
+  if (method == BN_EP) return fail(BN_EUNSUPPORTED, "likelihood-hyperparameter gradients are implemented for VI and Newton");
+  LikParams L;
+  if (int rc = make_lik_checked(lik, L)) return rc;
+  GHTable gh;
+  if (int rc = make_gh(gh_points, gh)) return rc;
+  double* p1 = site_partials(plan, kern, workspace, 0);
+  double* p2 = site_partials(plan, kern, workspace, 1);
+  if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
+  SiteIO io{y, post_mean, post_var, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, p1, p2};
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
+  if (method == BN_VI)
+    hipLaunchKernelGGL(k_lik_grad<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io);
+  else if (method == BN_NEWTON)
+    hipLaunchKernelGGL(k_lik_grad<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io);
+  else
+    return fail(BN_EINVAL, "method must be BN_VI or BN_NEWTON");
+  // reuse the pairwise reducer; the second output is scratch
+  hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, grad, p2);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;
 }

