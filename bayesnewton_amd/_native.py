@@ -38,7 +38,7 @@ class StHypers(ctypes.Structure):
     _fields_ = [("ls_t", vp), ("var_t", vp), ("var_s", vp), ("stride", i32)]
 
 
-BN_ST_UPDATE, BN_ST_ENERGY = 0, 1
+BN_ST_UPDATE, BN_ST_ENERGY, BN_ST_PROJECT = 0, 1, 2
 
 BN_LIK_GAUSSIAN, BN_LIK_BERNOULLI_PROBIT, BN_LIK_POISSON, BN_LIK_STUDENTS_T = 1, 2, 3, 4
 BN_VI, BN_NEWTON, BN_EP = 0, 1, 2
@@ -72,7 +72,7 @@ SIGNATURES = {
     "bn_site_reparametrise": (i32, [vp, vp, vp, vp, vp, vp]),
     "bn_lik_param_grad": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp]),
     "bn_gauss_hermite": (i32, [i32, vp, vp]),
-    "bn_st_conditional": (i32, [vp, vp, f64, vp, vp, vp, vp, vp, vp]),
+    "bn_st_conditional": (i32, [vp, vp, i32, f64, vp, vp, vp, vp, vp, vp]),
     "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                           vp, vp]),
     "bn_st_workspace_bytes": (i64, [vp]),

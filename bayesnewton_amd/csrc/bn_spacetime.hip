@@ -194,12 +194,23 @@ __device__ void temporal_AQ(double ls, double var, double dt, double A[KT * KT],
     for (int j = 0; j < KT; ++j) { A[i * KT + j] = Am[i][j]; Q[i * KT + j] = Qm[i][j]; }
 }
 
+// Matern correlation at scaled distance r (kernels.py:130-133, 182-186, 240-245).
+__device__ inline double spatial_corr(int32_t kind, double r) {
+  if (kind == BN_MATERN12) return exp(-r);
+  if (kind == BN_MATERN32) {
+    const double a = sqrt(3.0) * r;
+    return (1.0 + a) * exp(-a);
+  }
+  const double a = sqrt(5.0) * r;
+  return (1.0 + a + 5.0 / 3.0 * r * r) * exp(-a);
+}
+
 // ---------------------------------------------------------------------------
 // k_st_conditional: B_t rows (b_i = Lzz^-1 k(z, r_i)) and diag C_t = var_t (var_s - |b_i|^2)
 // kernels.py:486-506 (K = Krz Kzz^-1, B = K Lzz = Krz Lzz^-T; C = k_t(0)(Krr - K Krz')).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kT) void k_st_conditional(StDims dm, StHyp hy, const double* __restrict__ Lzz,
-                                                       const double* __restrict__ z, double ls_s,
+                                                       const double* __restrict__ z, int32_t skind, double ls_s,
                                                        const float* __restrict__ r, double* __restrict__ B,
                                                        double* __restrict__ cdiag) {
   __shared__ double sL[kMaxM * kMaxM];
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(kT) void k_st_conditional(StDims dm, StHyp hy, cons
   for (int e = tid; e < 2 * M; e += kT) sz[e] = z[e];
   __syncthreads();
   const double vt = hy.var_t[k * hy.stride], vs = hy.var_s[k * hy.stride];
-  const double s3 = sqrt(3.0) / ls_s;
+  const double il = 1.0 / ls_s;
   const int64_t base = ((int64_t)k * dm.T + t) * dm.R;
   for (int i = tid; i < dm.R; i += kT) {
     const double rx = r[(base + i) * 2], ry = r[(base + i) * 2 + 1];
@@ -218,8 +229,8 @@ __global__ __launch_bounds__(kT) void k_st_conditional(StDims dm, StHyp hy, cons
     double nb = 0.0;
     for (int j = 0; j < M; ++j) {
       const double dx = rx - sz[2 * j], dy = ry - sz[2 * j + 1];
-      const double d = sqrt(fmax(dx * dx + dy * dy, 1e-36)) * s3;   // kernels.py:101-104
-      double s = vs * (1.0 + d) * exp(-d);
+      const double r1 = sqrt(fmax((dx * dx + dy * dy) * il * il, 1e-36));   // kernels.py:101-104
+      double s = vs * spatial_corr(skind, r1);
       for (int q = 0; q < j; ++q) s = fma(-sL[j * M + q], b[q], s);
       b[j] = s / sL[j * M + j];
       nb = fma(b[j], b[j], nb);
@@ -236,6 +247,7 @@ __global__ __launch_bounds__(kT) void k_st_conditional(StDims dm, StHyp hy, cons
 //   UPDATE: site step of METHOD at (mean_f, var_f), damping with lr, then the whitened
 //           pseudo observation L L' = B' diag(nat2) B + 1e-12 I, z = L^-1 B' nat1,
 //           ldl = sum log diag L.  part[0] = sum |dnat1|, part[1] = sum |dnat2|.
+//   PROJECT: only the whitened pseudo observation of the current sites.
 //   ENERGY: part[0] = sum of site terms t1 (inference.py:130-222),
 //           part[1] = E_q[log N(pseudo_y | f, pseudo_var)] in whitened form
 //           (utils.py:510-531): -1/2(|z - L'm|^2 + M log 2pi - 2 ldl) - 1/2 tr(L' P L).
@@ -260,7 +272,7 @@ struct StSiteIO {
 
 constexpr int kRB = 32;  // rows of B staged per LDS chunk in the projection
 
-template <int METHOD, bool UPDATE>
+template <int METHOD, int MODE>
 __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
                                                  int ensure_psd) {
   extern __shared__ double smem[];
@@ -273,12 +285,16 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
   double* red = sn + 2 * kRB;           // 4
   const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int64_t kt = (int64_t)k * dm.T + t;
+  constexpr bool UPDATE = MODE == BN_ST_UPDATE;
+  constexpr bool PROJECT = MODE != BN_ST_ENERGY;
   const double* pP = io.pP + kt * M * M;
-  for (int e = tid; e < M * M; e += kT) sP[e] = pP[e];
-  for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
+  if (MODE != BN_ST_PROJECT) {
+    for (int e = tid; e < M * M; e += kT) sP[e] = pP[e];
+    for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
+  }
   __syncthreads();
   double acc0 = 0.0, acc1 = 0.0;
-  for (int i = tid; i < R; i += kT) {
+  for (int i = tid; i < (MODE == BN_ST_PROJECT ? 0 : R); i += kT) {
     const int64_t s = kt * R + i;
     const double* b = io.B + s * M;
     double bl[kMaxM];
@@ -309,9 +325,11 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
       acc0 += t1;
     }
   }
-  if (UPDATE) {
-    const double d1 = wg_sum(acc0, red), d2 = wg_sum(acc1, red);
-    if (tid == 0) { io.part0[kt] = d1; io.part1[kt] = d2; }
+  if (PROJECT) {
+    if (UPDATE) {
+      const double d1 = wg_sum(acc0, red), d2 = wg_sum(acc1, red);
+      if (tid == 0) { io.part0[kt] = d1; io.part1[kt] = d2; }
+    }
     // W = B' diag(nat2) B + 1e-12 I (into sP), v = B' nat1; thread owns W entries e, e+kT, ...
     double wacc[(kMaxM * kMaxM) / kT];
     double vacc = 0.0;
@@ -693,15 +711,17 @@ int set_smem_limits() {
 
 extern "C" {
 
-int bn_st_conditional(const bn_st_dims* dims, const bn_st_hypers* hyp, double ls_s, const double* z,
+int bn_st_conditional(const bn_st_dims* dims, const bn_st_hypers* hyp, int32_t spatial_kind, double ls_s, const double* z,
                       const double* Lzz, const float* r, double* B, double* cdiag, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
   if (!hyp || !z || !Lzz || !r || !B || !cdiag) return st_fail(BN_EINVAL, "NULL argument to bn_st_conditional");
   if (!(ls_s > 0)) return st_fail(BN_EINVAL, "spatial lengthscale must be > 0");
+  if (spatial_kind < BN_MATERN12 || spatial_kind > BN_MATERN52)
+    return st_fail(BN_EUNSUPPORTED, "spatial kernel must be Matern12, Matern32 or Matern52");
   hipStream_t st = (hipStream_t)stream;
   BN_TIMED(KID_ST_CONDITIONAL, st);
-  hipLaunchKernelGGL(k_st_conditional, dim3(dm.T, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), Lzz, z, ls_s, r, B,
+  hipLaunchKernelGGL(k_st_conditional, dim3(dm.T, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), Lzz, z, spatial_kind, ls_s, r, B,
                      cdiag);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
@@ -719,7 +739,8 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
     return st_fail(BN_EINVAL, "NULL argument to bn_st_sites");
   if (method != BN_VI && method != BN_NEWTON)
     return st_fail(BN_EUNSUPPORTED, "space-time sites support BN_VI and BN_NEWTON");
-  if (mode != BN_ST_UPDATE && mode != BN_ST_ENERGY) return st_fail(BN_EINVAL, "mode must be UPDATE or ENERGY");
+  if (mode != BN_ST_UPDATE && mode != BN_ST_ENERGY && mode != BN_ST_PROJECT)
+    return st_fail(BN_EINVAL, "mode must be BN_ST_UPDATE, BN_ST_ENERGY or BN_ST_PROJECT");
   LikParams L;
   if (int rc = check_lik(lik, L)) return rc;
   GHTable gh;
@@ -728,19 +749,18 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(dm.T, dm.K);
   BN_TIMED(KID_ST_SITES, st);
-  if (mode == BN_ST_UPDATE) {
-    if (method == BN_VI)
-      hipLaunchKernelGGL((k_st_sites<M_VI, true>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd);
-    else
-      hipLaunchKernelGGL((k_st_sites<M_NEWTON, true>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
-                         ensure_psd);
+#define ST_SITES(MT, MD) \
+  hipLaunchKernelGGL((k_st_sites<MT, MD>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd)
+  if (method == BN_VI) {
+    if (mode == BN_ST_UPDATE) ST_SITES(M_VI, BN_ST_UPDATE);
+    else if (mode == BN_ST_ENERGY) ST_SITES(M_VI, BN_ST_ENERGY);
+    else ST_SITES(M_VI, BN_ST_PROJECT);
   } else {
-    if (method == BN_VI)
-      hipLaunchKernelGGL((k_st_sites<M_VI, false>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd);
-    else
-      hipLaunchKernelGGL((k_st_sites<M_NEWTON, false>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
-                         ensure_psd);
+    if (mode == BN_ST_UPDATE) ST_SITES(M_NEWTON, BN_ST_UPDATE);
+    else if (mode == BN_ST_ENERGY) ST_SITES(M_NEWTON, BN_ST_ENERGY);
+    else ST_SITES(M_NEWTON, BN_ST_PROJECT);
   }
+#undef ST_SITES
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

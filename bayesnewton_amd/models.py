@@ -21,6 +21,8 @@ import torch
 from . import _native as nat
 from . import cubature as cub
 from . import ops
+from . import spacetime
+from .kernels import SpatioTemporalKernel
 from .plan import SeriesPlan, default_chunk_len
 
 
@@ -63,9 +65,22 @@ class MarkovGaussianProcess:
     """Sum-of-Matern state-space GP with scalar sites (basemodels.py:625-764)."""
     method = None
 
+    def __new__(cls, kernel=None, likelihood=None, X=None, Y=None, R=None, *args, **kwargs):
+        # basemodels.py:646-650: a spatio-temporal kernel with locations R selects the
+        # space-time branch of the model (bayesnewton_amd.spacetime)
+        if isinstance(kernel, SpatioTemporalKernel):
+            if R is None:
+                raise ValueError("a SpatioTemporalKernel needs the locations R [T, n_loc, 2]")
+            st_cls = {nat.BN_VI: spacetime.SpaceTimeVariationalGP,
+                      nat.BN_NEWTON: spacetime.SpaceTimeLaplaceGP}.get(cls.method)
+            if st_cls is None:
+                raise NotImplementedError("space-time models support VI and Newton / Laplace")
+            return st_cls(kernel, likelihood, X, Y, R, device=kwargs.get("device"))
+        return super().__new__(cls)
+
     def __init__(self, kernel, likelihood, X, Y, R=None, parallel=None, power=1.0, chunk_len=None, device=None):
         if R is not None or (np.ndim(X) > 1 and np.shape(X)[1] > 1):
-            raise NotImplementedError("spatio-temporal inputs: use SpatioTemporal models")
+            raise NotImplementedError("spatio-temporal inputs need a SpatioTemporalKernel")
         t, y, dt = input_admin(X, Y)
         self._build(kernel, likelihood, [t], [y], [dt], None, None, parallel, power, chunk_len, device)
 

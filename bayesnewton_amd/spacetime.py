@@ -1,0 +1,278 @@
+"""Space-time Markov GPs: MarkovGaussianProcess with a SpatioTemporalKernel
+(bayesnewton/basemodels.py:625-764 on its `spatio_temporal` branch, kernels.py:385-586).
+
+The reference builds the model as MarkovVariationalGP(kernel, likelihood, X=t, R=r, Y=Y)
+with t [T], r [T, R, 2] and Y [T, R] (NaN = missing); the same call works here.
+`from_runs` puts K independent runs (same T and R, same hypers) into one model so
+that the per-run filter/smoother workgroups fill the GPU.
+
+Data flow of one inference step (inference.py:65-90), all on the device:
+  k_st_sites(UPDATE)   posterior -> data (B m, diag(B P B') + diag C), new sites,
+                       whitened projection L L' = B' diag(nat2) B + 1e-12 I, z = L^-1 B' nat1
+  k_st_filter          Kalman filter of the projected pseudo observations (state 2M)
+  k_st_smoother        posterior H m, H P H' at every step
+energy() (inference.py:130-154, 197-222) uses the stored posterior, the current sites
+and the current hypers: -(sum E_q[log p(y|f)] - (sum E_q[log N(pseudo_y|f, pseudo_var)] - log Z)).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import cubature as cub
+from .kernels import SpatioTemporalKernel
+
+
+def _admin(t, r, y):
+    """utils.py:234-265 for one run: sort by time, dt[0] = 0."""
+    t = np.asarray(t, dtype=np.float64).reshape(-1)
+    r = np.asarray(r, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    T = t.shape[0]
+    if r.ndim != 3 or r.shape[0] != T or r.shape[2] != 2:
+        raise ValueError("R must be [T, n_loc, 2]")
+    if y.shape != r.shape[:2]:
+        raise ValueError("Y must be [T, n_loc]")
+    ind = np.argsort(t, kind="stable")
+    t, r, y = t[ind], r[ind], y[ind]
+    dt = np.concatenate([[0.0], np.diff(t)])
+    return t, r, y, dt
+
+
+class SpaceTimeMarkovGP:
+    """Markov GP over a SpatioTemporalKernel, K independent runs per model."""
+    method = None
+
+    def __init__(self, kernel, likelihood, X, Y, R, parallel=None, device=None):
+        self._build(kernel, likelihood, [X], [R], [Y], device)
+        self._single = True
+
+    @classmethod
+    def from_runs(cls, kernel, likelihood, Xs, Rs, Ys, device=None):
+        """K runs (lists of t [T], R [T, n_loc, 2], Y [T, n_loc]) under one set of hypers."""
+        self = cls.__new__(cls)
+        self._build(kernel, likelihood, Xs, Rs, Ys, device)
+        self._single = False
+        return self
+
+    # ---- construction ----------------------------------------------------------------
+    def _build(self, kernel, likelihood, Xs, Rs, Ys, device):
+        if not isinstance(kernel, SpatioTemporalKernel):
+            raise TypeError("kernel must be a SpatioTemporalKernel")
+        if not torch.cuda.is_available():
+            raise nat.NativeError("bayesnewton_amd models need a ROCm GPU")
+        self.kernel = kernel
+        self.likelihood = likelihood
+        runs = [_admin(t, r, y) for t, r, y in zip(Xs, Rs, Ys)]
+        T, nr = runs[0][2].shape
+        for run in runs:
+            if run[2].shape != (T, nr):
+                raise ValueError("all runs need the same number of steps and locations")
+        K, M = len(runs), kernel.M
+        KT = kernel.temporal_kernel.state_dim
+        self.K, self.T, self.n_loc, self.M, self.D = K, T, nr, M, KT * M
+        self.X = np.stack([run[0] for run in runs])
+        self.R = np.stack([run[1] for run in runs])
+        self.Y = np.stack([run[2] for run in runs])
+        self.num_data = T
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        self._dims = nat.StDims(K, T, nr, M, kernel.temporal_kernel.kind)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self._dt = torch.tensor(np.stack([run[3] for run in runs]), **f64)
+        self._r = torch.tensor(self.R, dtype=torch.float32, device=dev).contiguous()
+        self._y = torch.tensor(self.Y, dtype=torch.float32, device=dev).contiguous()
+        self._z = torch.tensor(kernel.z, **f64).contiguous()
+        # sites N(0, 1e2 I) (basemodels.py:130-133)
+        self._nat1 = torch.zeros((K, T, nr), **f64)
+        self._nat2 = torch.full((K, T, nr), 1e-2, **f64)
+        self._B = torch.empty((K, T, nr, M), **f64)
+        self._cdiag = torch.empty((K, T, nr), **f64)
+        self._Lw = torch.empty((K, T, M, M), **f64)
+        self._zw = torch.empty((K, T, M), **f64)
+        self._ldl = torch.empty((K, T), **f64)
+        self._U0 = torch.empty((K, T, self.D, M), **f64)
+        self._Yf = torch.empty((K, T, M, M), **f64)
+        self._w = torch.empty((K, T, M), **f64)
+        self._Hm = torch.empty((K, T, M), **f64)
+        self._ell = torch.empty((K, T), **f64)
+        self._loglik = torch.zeros(K, **f64)
+        # posterior initialised as in basemodels.py:134-135 (zeros, identity)
+        self._pm = torch.zeros((K, T, M), **f64)
+        self._pP = torch.eye(M, **f64).expand(K, T, M, M).contiguous()
+        self._p0 = torch.empty((K, T), **f64)
+        self._p1 = torch.empty((K, T), **f64)
+        self._s0 = torch.empty(K, **f64)
+        self._s1 = torch.empty(K, **f64)
+        nb = nat.lib().bn_st_workspace_bytes(ctypes.byref(self._dims))
+        nat.check(0 if nb > 0 else -1)
+        self._ws = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+        self._hyp_key = None
+        self._cond_key = None
+        self._proj_key = None
+        self._post_key = None
+        self._ll_key = None
+        self._sites_version = 0
+
+    # ---- hyperparameters ---------------------------------------------------------------
+    def _hyp(self):
+        kt, ks = self.kernel.temporal_kernel, self.kernel.spatial_kernel
+        key = (kt.lengthscale, kt.variance, ks.lengthscale, ks.variance)
+        if key != self._hyp_key:
+            dev = self.device
+            self._hls = torch.tensor([key[0]], dtype=torch.float64, device=dev)
+            self._hvt = torch.tensor([key[1]], dtype=torch.float64, device=dev)
+            self._hvs = torch.tensor([key[3]], dtype=torch.float64, device=dev)
+            self._hyp_key = key
+        h = nat.StHypers(nat.ptr(self._hls), nat.ptr(self._hvt), nat.ptr(self._hvs), 0)
+        return h, key
+
+    def _ensure_conditional(self):
+        h, key = self._hyp()
+        ckey = (key[1], key[2], key[3], self.kernel.conditional)
+        if ckey == self._cond_key:
+            return h
+        _, Lzz = self.kernel.inducing_precision()
+        self._Lzz = torch.tensor(np.ascontiguousarray(Lzz), dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().bn_st_conditional(
+            ctypes.byref(self._dims), ctypes.byref(h), self.kernel.spatial_kernel.kind, float(key[2]),
+            nat.ptr(self._z), nat.ptr(self._Lzz), nat.ptr(self._r), nat.ptr(self._B), nat.ptr(self._cdiag),
+            nat.stream_ptr()))
+        if self.kernel.conditional == "DTC":
+            self._cdiag.zero_()
+        self._cond_key = ckey
+        return h
+
+    # ---- sites ---------------------------------------------------------------------------
+    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None):
+        lik = self.likelihood.desc()
+        om, oj, oh = terms if terms is not None else (None, None, None)
+        nat.check(nat.lib().bn_st_sites(
+            ctypes.byref(self._dims), ctypes.byref(lik), self.method, mode, cub.num_points(cubature), float(lr),
+            1 if ensure_psd else 0, nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag), nat.ptr(self._pm),
+            nat.ptr(self._pP), nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw), nat.ptr(self._zw),
+            nat.ptr(self._ldl), nat.ptr(self._p0), nat.ptr(self._p1), nat.ptr(om), nat.ptr(oj), nat.ptr(oh),
+            nat.stream_ptr()))
+
+    def _sum(self, per_step, out):
+        nat.check(nat.lib().bn_st_sum(self.K, self.T, nat.ptr(per_step), nat.ptr(out), nat.stream_ptr()))
+        return out
+
+    def _ensure_projection(self):
+        self._ensure_conditional()
+        key = (self._cond_key, self._sites_version)
+        if key != self._proj_key:
+            self._sites(nat.BN_ST_PROJECT)
+            self._proj_key = key
+
+    def _run_filter(self):
+        h, key = self._hyp()
+        nat.check(nat.lib().bn_st_filter(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt), nat.ptr(self._Lw), nat.ptr(self._zw),
+            nat.ptr(self._ldl), nat.ptr(self._U0), nat.ptr(self._Yf), nat.ptr(self._w), nat.ptr(self._Hm),
+            nat.ptr(self._ell), nat.ptr(self._loglik), None, nat.ptr(self._ws), nat.stream_ptr()))
+        self._ll_key = (self._proj_key, key[0], key[1])
+        return h
+
+    # ---- posterior (basemodels.py:689-706) ----------------------------------------------
+    def update_posterior(self):
+        self._ensure_projection()
+        h = self._run_filter()
+        nat.check(nat.lib().bn_st_smoother(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt), nat.ptr(self._U0), nat.ptr(self._Yf),
+            nat.ptr(self._w), nat.ptr(self._Hm), nat.ptr(self._pm), nat.ptr(self._pP), nat.ptr(self._ws),
+            nat.stream_ptr()))
+        self._post_key = self._ll_key
+
+    def _current_key(self):
+        _, key = self._hyp()
+        ckey = (key[1], key[2], key[3], self.kernel.conditional)
+        return ((ckey, self._sites_version), key[0], key[1])
+
+    def _ensure_posterior(self):
+        if self._post_key != self._current_key():
+            self.update_posterior()
+
+    def _ensure_loglik(self):
+        self._ensure_projection()
+        if self._ll_key != self._current_key():
+            self._run_filter()
+
+    def compute_log_lik(self):
+        """basemodels.py:726-741 (per run when the model holds several)."""
+        self._ensure_loglik()
+        return self._loglik.clone() if not self._single else self._loglik[0].clone()
+
+    def _out(self, a):
+        return a[0] if self._single else a
+
+    @property
+    def posterior_mean(self):
+        """H m of the smoothed state, [T, M, 1] ([K, T, M, 1] for several runs)."""
+        return self._out(self._pm[..., None])
+
+    @property
+    def posterior_variance(self):
+        """H P H' of the smoothed state, [T, M, M] ([K, T, M, M])."""
+        return self._out(self._pP)
+
+    @property
+    def site_nat(self):
+        """Current sites (nat1, diagonal of nat2), [T, n_loc] each ([K, T, n_loc])."""
+        return self._out(self._nat1), self._out(self._nat2)
+
+    def set_sites(self, nat1, nat2):
+        """Replace the sites (update_nat_params, basemodels.py:97-100); nat2 is the diagonal."""
+        self._nat1.copy_(torch.as_tensor(nat1, dtype=torch.float64).reshape(self._nat1.shape))
+        self._nat2.copy_(torch.as_tensor(nat2, dtype=torch.float64).reshape(self._nat2.shape))
+        self._sites_version += 1
+
+    # ---- inference (inference.py:65-90) --------------------------------------------------
+    def inference(self, lr=1.0, batch_ind=None, cubature=None, ensure_psd=True, return_terms=True, **kwargs):
+        """One site update + posterior update.  Returns ((mean, jacobian, hessian), (diff1,
+        diff2)) with the site terms per location, [T, n_loc] each (the hessian's diagonal),
+        and the mean absolute change of the natural parameters."""
+        if batch_ind is not None and len(batch_ind) != self.num_data:
+            raise NotImplementedError("mini-batched site updates are not implemented on the HIP path")
+        self._ensure_conditional()
+        self._ensure_posterior()
+        terms = None
+        if return_terms:
+            terms = tuple(torch.empty((self.K, self.T, self.n_loc), dtype=torch.float32, device=self.device)
+                          for _ in range(3))
+        self._sites(nat.BN_ST_UPDATE, lr, cubature, ensure_psd, terms)
+        self._sites_version += 1
+        self._proj_key = (self._cond_key, self._sites_version)
+        # inference.py:79-80 averages over nat1 [T, R, 1] and nat2 [T, R, R] (whose
+        # off-diagonal entries are zero)
+        n = float(self.T * self.n_loc)
+        d1 = self._sum(self._p0, self._s0) / n
+        d2 = self._sum(self._p1, self._s1) / (n * self.n_loc)
+        self.update_posterior()
+        out_terms = tuple(self._out(a.to(torch.float64)) for a in terms) if terms else (None, None, None)
+        return out_terms, (self._out(d1.clone()), self._out(d2.clone()))
+
+    # ---- energy ------------------------------------------------------------------------------
+    def energy_per_run(self, cubature=None):
+        self._ensure_loglik()
+        self._sites(nat.BN_ST_ENERGY, cubature=cubature)
+        ell = self._sum(self._p0, self._s0).clone()
+        ed = self._sum(self._p1, self._s1).clone()
+        return -(ell - (ed - self._loglik))
+
+    def energy(self, batch_ind=None, cubature=None, **kwargs):
+        if batch_ind is not None and len(batch_ind) != self.num_data:
+            raise NotImplementedError("mini-batched energies are not implemented on the HIP path")
+        e = self.energy_per_run(cubature)
+        return e[0] if self._single else e.sum()
+
+
+class SpaceTimeVariationalGP(SpaceTimeMarkovGP):
+    """MarkovVariationalGP with a SpatioTemporalKernel (models.py:118-132)."""
+    method = nat.BN_VI
+
+
+class SpaceTimeLaplaceGP(SpaceTimeMarkovGP):
+    """MarkovLaplaceGP / MarkovNewtonGP with a SpatioTemporalKernel (models.py:297-299, 328)."""
+    method = nat.BN_NEWTON

@@ -245,13 +245,16 @@ typedef struct bn_st_hypers {
 
 #define BN_ST_UPDATE 0 /* bn_st_sites: site update + projected pseudo observation */
 #define BN_ST_ENERGY 1 /* bn_st_sites: energy terms at the stored sites */
+#define BN_ST_PROJECT 2 /* bn_st_sites: projected pseudo observation of the current sites only */
 
 /* B_t = Krz Kzz^-1 Lzz (rows b_i = Lzz^-1 k_s(z, r_i)) and diag C_t =
  * var_t (var_s - |b_i|^2) of SpatioTemporalKernel.spatial_conditional
  * (kernels.py:486-506) for every step of every run.  z [M,2], Lzz [M,M] =
  * chol(k_s(z, z)) (inducing_precision, kernels.py:508-515), r [K,T,R,2];
- * B [K,T,R,M], cdiag [K,T,R].  Spatial kernel: Matern32 with lengthscale ls_s. */
-int bn_st_conditional(const bn_st_dims *dims, const bn_st_hypers *hyp, double ls_s, const double *z,
+ * B [K,T,R,M], cdiag [K,T,R].  Spatial kernel: spatial_kind (BN_MATERN12/32/52)
+ * with lengthscale ls_s and variance hyp->var_s. */
+int bn_st_conditional(const bn_st_dims *dims, const bn_st_hypers *hyp, int32_t spatial_kind, double ls_s,
+                      const double *z,
                       const double *Lzz, const float *r, double *B, double *cdiag, void *stream);
 
 /* Site map of one inference step or energy evaluation on every (run, step):
@@ -265,7 +268,8 @@ int bn_st_conditional(const bn_st_dims *dims, const bn_st_hypers *hyp, double ls
  * mode BN_ST_ENERGY (inference.py:130-154 / 197-222): part0 = per-step sum of
  * E_q[log p(y|f)] (VI) or log p(y|m) (Newton); part1 = per-step
  * gaussian_expected_log_lik of the pseudo observation (utils.py:510-531) from
- * Lw, zw, ldl.  Methods BN_VI and BN_NEWTON.  out_* optional [K,T,R]. */
+ * Lw, zw, ldl.  mode BN_ST_PROJECT: Lw, zw, ldl of the current sites only
+ * (post_* unused).  Methods BN_VI and BN_NEWTON.  out_* optional [K,T,R]. */
 int bn_st_sites(const bn_st_dims *dims, const bn_lik_desc *lik, int32_t method, int32_t mode, int32_t gh_points,
                 double lr, int32_t ensure_psd, const float *y, const double *B, const double *cdiag,
                 const double *post_mean, const double *post_cov, double *nat1, double *nat2, double *Lw, double *zw,
