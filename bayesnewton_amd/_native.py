@@ -39,6 +39,7 @@ class StHypers(ctypes.Structure):
 
 
 BN_ST_UPDATE, BN_ST_ENERGY, BN_ST_PROJECT = 0, 1, 2
+BN_WS_FEATURES, BN_WS_PARAMS = 8, 34434
 
 BN_LIK_GAUSSIAN, BN_LIK_BERNOULLI_PROBIT, BN_LIK_POISSON, BN_LIK_STUDENTS_T = 1, 2, 3, 4
 BN_VI, BN_NEWTON, BN_EP = 0, 1, 2
@@ -79,6 +80,13 @@ SIGNATURES = {
     "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
+    "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_ws_features": (i32, [vp, vp, vp, vp, f64, f64, f64, f64, vp, vp]),
+    "bn_mlp_forward": (i32, [i64, vp, vp, vp, vp, vp, vp]),
+    "bn_mlp_grad_workspace_bytes": (i64, [i64]),
+    "bn_mlp_grad_step": (i32, [i64, vp, vp, vp, vp, vp, vp, vp, i32, f32, vp, vp, vp, vp]),
+    "bn_mlp_to_bf16": (i32, [vp, vp, vp]),
+    "bn_ws_sites": (i32, [i64, vp, vp, vp, vp, vp, vp]),
     "bn_timing_enable": (i32, [i32]),
     "bn_timing_read": (i32, [i32, ctypes.POINTER(f64), ctypes.POINTER(i32)]),
     "bn_kernel_name": (ctypes.c_char_p, [i32]),

@@ -403,6 +403,38 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
 }
 
 // ---------------------------------------------------------------------------
+// k_st_marginals: q(f) at every data location, mean_f = B m, var_f = diag(B P B') + diag C
+// (conditional_posterior_to_data, basemodels.py:743-764, diagonal only).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_st_marginals(StDims dm, const double* __restrict__ B,
+                                                     const double* __restrict__ cdiag, const double* __restrict__ pm,
+                                                     const double* __restrict__ pP, double* __restrict__ mean_f,
+                                                     double* __restrict__ var_f) {
+  __shared__ double sP[kMaxM * kMaxM];
+  __shared__ double sm[kMaxM];
+  const int M = dm.M, R = dm.R;
+  const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  for (int e = tid; e < M * M; e += kT) sP[e] = pP[kt * M * M + e];
+  for (int e = tid; e < M; e += kT) sm[e] = pm[kt * M + e];
+  __syncthreads();
+  for (int i = tid; i < R; i += kT) {
+    const int64_t s = kt * R + i;
+    const double* b = B + s * M;
+    double mf = 0.0, vf = cdiag[s];
+    for (int j = 0; j < M; ++j) {
+      const double bj = b[j];
+      mf = fma(bj, sm[j], mf);
+      double u = 0.0;
+      for (int q = 0; q < M; ++q) u = fma(sP[j * M + q], b[q], u);
+      vf = fma(bj, u, vf);
+    }
+    mean_f[s] = mf;
+    var_f[s] = vf;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_st_filter: one workgroup per run, sequential over T (ops.py:154-180 in whitened form).
 // Per step outputs for the smoother: U0 = P-[:, :M] (D x M), Y (M x M), w (M), H m- (M).
 // ---------------------------------------------------------------------------
@@ -820,6 +852,19 @@ int bn_st_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double
     hipLaunchKernelGGL(k_st_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
   else
     hipLaunchKernelGGL(k_st_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_st_marginals(const bn_st_dims* dims, const double* B, const double* cdiag, const double* post_mean,
+                    const double* post_cov, double* mean_f, double* var_f, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!B || !cdiag || !post_mean || !post_cov || !mean_f || !var_f)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_marginals");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_st_marginals, dim3(dm.T, dm.K), dim3(kT), 0, st, dm, B, cdiag, post_mean, post_cov, mean_f,
+                     var_f);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

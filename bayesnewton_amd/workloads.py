@@ -176,7 +176,7 @@ def _sqdist(a, b):
     return np.sum(d * d, axis=-1)
 
 
-def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize=2.0):
+def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize=2.0, device=None):
     """Space-time log-Gaussian Cox process on [0,1]^2 x {0..T-1}.
 
     Inducing inputs z: grid x grid points of linspace(0,1,grid)^2.  Each step draws R
@@ -185,6 +185,9 @@ def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize
     Matern-3/2 temporal states u (whitened inducing process), f_t = B_t u_t + e_t,
     e_t ~ N(0, C_t), B_t = Krz Kzz^-1 Lzz, C_t = k_t(0) (Krr - Krz Kzz^-1 Kzr).
     Counts y ~ Poisson(binsize exp(f)) (mean log(binsize) folded into the binsize).
+    All random draws come from numpy's PCG64 stream of `seed`; with `device` (a torch
+    device) the per-step linear algebra runs there in FP64 (same draws, same data up
+    to rounding).
     """
     rng = np.random.default_rng(seed)
     g = np.linspace(0.0, 1.0, grid)
@@ -207,19 +210,49 @@ def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize
     for k in range(1, T):
         x = x @ A.T + rng.standard_normal((M, 2)) @ LQ.T
         u[k] = x[:, 0]
-    Krz = _m32_cov(np.sqrt(np.maximum(_sqdist(r, z[None]), 1e-36)), ls_s)          # [T, R, M]
-    Bt = np.linalg.solve(Lzz, np.swapaxes(Krz, 1, 2))                             # Lzz^-1 Kzr  [T, M, R]
-    f = np.einsum("tmr,tm->tr", Bt, u)
     eps = rng.standard_normal((T, R))
-    jit = 1e-8 * np.eye(R)
-    for k in range(T):
-        Krr = _m32_cov(np.sqrt(np.maximum(_sqdist(r[k], r[k]), 1e-36)), ls_s)
-        C = var_t * (Krr - Bt[k].T @ Bt[k])
-        try:
-            Lc = np.linalg.cholesky(C + jit)
-            f[k] += Lc @ eps[k]
-        except np.linalg.LinAlgError:
-            w, V = np.linalg.eigh(C)
-            f[k] += V @ (np.sqrt(np.maximum(w, 0.0)) * (V.T @ eps[k]))
+    if device is None:
+        Krz = _m32_cov(np.sqrt(np.maximum(_sqdist(r, z[None]), 1e-36)), ls_s)          # [T, R, M]
+        Bt = np.linalg.solve(Lzz, np.swapaxes(Krz, 1, 2))                             # Lzz^-1 Kzr  [T, M, R]
+        f = np.einsum("tmr,tm->tr", Bt, u)
+        jit = 1e-8 * np.eye(R)
+        for k in range(T):
+            Krr = _m32_cov(np.sqrt(np.maximum(_sqdist(r[k], r[k]), 1e-36)), ls_s)
+            C = var_t * (Krr - Bt[k].T @ Bt[k])
+            try:
+                f[k] += np.linalg.cholesky(C + jit) @ eps[k]
+            except np.linalg.LinAlgError:
+                w, V = np.linalg.eigh(C)
+                f[k] += V @ (np.sqrt(np.maximum(w, 0.0)) * (V.T @ eps[k]))
+    else:
+        import torch
+        dev = torch.device(device)
+        tr = torch.tensor(r, dtype=torch.float64, device=dev)
+        tz = torch.tensor(z, dtype=torch.float64, device=dev)
+        tL = torch.tensor(Lzz, dtype=torch.float64, device=dev)
+        s3 = math.sqrt(3.0) / ls_s
+
+        def m32(d2):
+            d = torch.sqrt(torch.clamp(d2, min=1e-36)) * s3
+            return (1.0 + d) * torch.exp(-d)
+
+        Krz = m32(((tr[:, :, None, :] - tz[None, None]) ** 2).sum(-1))                 # [T, R, M]
+        Bt = torch.linalg.solve_triangular(tL, Krz.transpose(1, 2), upper=False)        # [T, M, R]
+        fk = torch.einsum("tmr,tm->tr", Bt, torch.tensor(u, dtype=torch.float64, device=dev))
+        te = torch.tensor(eps, dtype=torch.float64, device=dev)
+        jit = 1e-8 * torch.eye(R, dtype=torch.float64, device=dev)
+        for k0 in range(0, T, 250):
+            k1 = min(T, k0 + 250)
+            rr = tr[k0:k1]
+            Krr = m32(((rr[:, :, None, :] - rr[:, None, :, :]) ** 2).sum(-1))
+            C = var_t * (Krr - Bt[k0:k1].transpose(1, 2) @ Bt[k0:k1])
+            Lc, info = torch.linalg.cholesky_ex(C + jit)
+            bad = torch.nonzero(info).reshape(-1).tolist()
+            fk[k0:k1] += (Lc @ te[k0:k1, :, None])[..., 0]
+            for b in bad:  # not PD even with jitter: square root from the spectrum
+                w, V = torch.linalg.eigh(C[b])
+                fk[k0 + b] = (Bt[k0 + b].T @ torch.tensor(u[k0 + b], dtype=torch.float64, device=dev)
+                              + V @ (torch.sqrt(torch.clamp(w, min=0.0)) * (V.T @ te[k0 + b])))
+        f = fk.cpu().numpy()
     y = rng.poisson(binsize * np.exp(f)).astype(np.float64)
     return dict(t=t, r=r, y=y, f=f, z=z, ls_t=ls_t, ls_s=ls_s, var_t=var_t, binsize=binsize)

@@ -56,7 +56,115 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
+    ap.add_argument("--workload", default="batched", choices=["batched", "spacetime"],
+                    help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
+                         "time to converge of the d=128 Cox process, learned warm start vs plain")
+    ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
+    ap.add_argument("--st-steps", type=int, default=2000, help="spacetime: time steps per run")
+    ap.add_argument("--st-tol", type=float, default=1e-7, help="spacetime: relative energy change to stop")
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_spacetime(args):
+    """Space-time log-Gaussian Cox process (d = 128: M = 64 inducing points on an 8x8 grid,
+    Matern32 in time (lengthscale 10) x Matern32 in space (lengthscale 0.2), T steps of
+    R = 256 locations, Poisson(binsize 2)), MarkovVariationalGP.  Trains the warm-start
+    network on the converged sites of --st-train plain runs (seeds 2000, 2001, ...) and
+    compares plain vs warm start on the held-out run of seed 1005: iterations and wall time
+    to the same convergence test (relative energy change <= --st-tol), final energies."""
+    import numpy as np
+    import torch
+    from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, spacetime as S
+    from bayesnewton_amd import warmstart as W, workloads
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    T = args.st_steps
+    g = np.linspace(0.0, 1.0, 8)
+    za, zb = np.meshgrid(g, g)
+    z = np.stack([za.reshape(-1), zb.reshape(-1)], 1)
+
+    def kern():
+        return K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=10.0),
+                                      K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
+
+    lik = L.Poisson(binsize=2.0)
+    t0 = time.perf_counter()
+    train = [workloads.c5(seed=2000 + k, T=T, device=dev) for k in range(args.st_train)]
+    log(f"[spacetime] {args.st_train} training runs generated in {time.perf_counter() - t0:.1f}s")
+    tm = S.SpaceTimeVariationalGP.from_runs(kern(), lik, [d["t"] for d in train], [d["r"] for d in train],
+                                            [d["y"] for d in train])
+    del train
+    L_ = nat.lib()
+    L_.bn_timing_enable(1)
+    it_tr, e_tr, sec_tr = W.run_to_convergence(tm, tol=args.st_tol, max_iter=200)
+    kern_ms = {}
+    for kid in range(L_.bn_num_kernel_ids()):
+        tot, cnt = ctypes.c_double(), ctypes.c_int32()
+        nat.check(L_.bn_timing_read(kid, ctypes.byref(tot), ctypes.byref(cnt)))
+        if cnt.value:
+            kern_ms[L_.bn_kernel_name(kid).decode()] = tot.value / cnt.value
+    L_.bn_timing_enable(0)
+    log(f"[spacetime] plain training batch: K={tm.K} converged in {it_tr} iterations, {sec_tr:.1f}s; "
+        f"per-launch ms {kern_ms}")
+    steps_per_pass = tm.K * tm.T
+    pass_ms = kern_ms.get("k_st_filter", 0.0) + kern_ms.get("k_st_smoother", 0.0)
+    X = W.site_features(tm)
+    Y = W.site_marginals(tm)
+    del tm
+    torch.cuda.empty_cache()
+    n_fit = min(X.shape[0], 4_000_000)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    idx = torch.randperm(X.shape[0], generator=gen, device=dev)[:n_fit]
+    Xs, Ys = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+    del X, Y
+    net = W.WarmStartMLP(seed=0)
+    t0 = time.perf_counter()
+    hist = net.fit(Xs, Ys, epochs=3, batch=8192, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    del Xs, Ys
+    torch.cuda.empty_cache()
+    # held-out run (seed 1005): plain vs warm start, same convergence test
+    d = workloads.c5(seed=1005, T=T, device=dev)
+    plain = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
+    it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=200)
+    log(f"[spacetime] plain: {it_p} iterations, {sec_p:.2f}s, energy {float(e_p[0]):.6f}")
+    del plain
+    warm = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    net.init_sites(warm)
+    torch.cuda.synchronize()
+    init_s = time.perf_counter() - c0
+    it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=200)
+    sec_w += init_s
+    log(f"[spacetime] warm: {it_w} iterations, {sec_w:.2f}s (network {init_s:.3f}s), energy {float(e_w[0]):.6f}")
+    out = {
+        "metric": "time-to-converge, learned warm start vs plain (space-time Cox process, d=128)",
+        "value": sec_p / sec_w,
+        "unit": "x (plain wall time / warm-start wall time)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64 filter/smoother/sites, bf16 weights + f32 accumulation in the network",
+        "data": "synthetic (seeded prior draws of the model, Poisson counts)",
+        "config": {"workload": f"space-time Cox process: T={T}, R=256 locations/step, M=64 inducing (8x8), "
+                               "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), MarkovVariationalGP",
+                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol},
+        "plain": {"iterations": it_p, "seconds": sec_p, "energy": float(e_p[0])},
+        "warm": {"iterations": it_w, "seconds": sec_w, "network_seconds": init_s, "energy": float(e_w[0])},
+        "energy_rel_diff": abs(float(e_w[0]) - float(e_p[0])) / abs(float(e_p[0])),
+        "train": {"plain_iterations": it_tr, "plain_seconds": sec_tr, "fit_seconds": fit_s, "loss": hist},
+        "batched_pass": {"runs": steps_per_pass // T, "filter_plus_smoother_ms": pass_ms,
+                         "steps_per_s": steps_per_pass / (pass_ms / 1e3) if pass_ms else None,
+                         "kernel_ms_per_launch": kern_ms},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def copy_calibration():
@@ -89,6 +197,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "spacetime":
+        bench_spacetime(args)
         return
     import numpy as np
     import torch

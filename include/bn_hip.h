@@ -300,6 +300,47 @@ int bn_st_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double
 /* per_run[k] = sum_t per_step[k, t] (FP64, fixed order). */
 int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *per_run, void *stream);
 
+/* q(f) at every data location: mean_f = B m, var_f = diag(B P B') + diag C, [K,T,R]
+ * (conditional_posterior_to_data, basemodels.py:743-764, diagonal). */
+int bn_st_marginals(const bn_st_dims *dims, const double *B, const double *cdiag, const double *post_mean,
+                    const double *post_cov, double *mean_f, double *var_f, void *stream);
+
+/* --- learned warm start of the space-time sites.  No counterpart in the reference:
+ * it replaces the fixed site initialisation N(0, 1e2 I) (basemodels.py:130-133) via
+ * update_nat_params (basemodels.py:97-100); the plain iteration then runs to the same
+ * convergence test.  A 3 x 128 SiLU MLP (BF16 weights, FP32 accumulation) maps
+ * BN_WS_FEATURES features per site to the converged marginal (m, log v) of f there.
+ * Parameters: flat FP32 array of BN_WS_PARAMS (W1[F][128], b1, W2[128][128], b2,
+ * W3[128][128], b3, W4[128][2], b4) and its BF16 copy (same layout).  norm: 2F + 4
+ * floats = feature means, feature std devs, output means, output std devs. */
+#define BN_WS_FEATURES 8
+#define BN_WS_PARAMS 34434
+
+/* Features of every site [K,T,R,F]: log(1 + y), three kernel-smoothed log rates over a
+ * +-6 step window (bandwidths (0.07, 1), (0.15, 2.5), (0.3, 6) in space and time),
+ * and the logs of ls_t, ls_s, var_t, binsize.  t [K,T] (FP64 times). */
+int bn_ws_features(const bn_st_dims *dims, const float *r, const float *y, const double *t, double ls_t,
+                   double ls_s, double var_t, double binsize, float *X, void *stream);
+
+/* out[n, 2] = MLP(X[n, F]) on the original output scale. */
+int bn_mlp_forward(int64_t n_rows, const float *X, const uint16_t *w_bf16, const float *params, const float *norm,
+                   float *out, void *stream);
+
+/* One mini-batch of L = sum (out - target)^2 / (2 n) on standardised outputs: gradient
+ * (optional, grad[BN_WS_PARAMS]), loss (optional, one float) and, for step >= 1, one
+ * Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on params with the BF16 copy
+ * refreshed.  step = 0: gradient only. */
+int64_t bn_mlp_grad_workspace_bytes(int64_t n_rows);
+int bn_mlp_grad_step(int64_t n_rows, const float *X, const float *target, float *params, uint16_t *w_bf16,
+                     const float *norm, float *adam_m, float *adam_v, int32_t step, float lr, float *grad,
+                     float *loss, void *workspace, void *stream);
+int bn_mlp_to_bf16(const float *params, uint16_t *w_bf16, void *stream);
+
+/* Initial sites from predicted marginals pred[n] = (m, log v): the VI site step at
+ * q(f) = N(m, v) (inference.py:160-195, 21-39).  nat1 / nat2 [n] FP64. */
+int bn_ws_sites(int64_t n_sites, const bn_lik_desc *lik, const float *y, const float *pred, double *nat1,
+                double *nat2, void *stream);
+
 /* --- measurement hooks (no counterpart in the reference).  When enabled, each
  * launch of the chunked filter/smoother kernels is bracketed by hipEvents on
  * its own stream; bn_timing_read sums the elapsed time of one kernel id.

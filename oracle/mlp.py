@@ -1,0 +1,120 @@
+"""FP64 restatement of the warm-start network and its features (test infrastructure only).
+
+The warm start has no counterpart in the reference (it replaces the site initialisation
+of basemodels.py:130-133), so this module restates the build's own definitions:
+the flat parameter layout, the 3 x 128 SiLU network with BF16-rounded weights and FP32
+biases, the standardised mean-squared-error loss and its gradient, and the site
+features.  Pinned by central finite differences of the loss (tests/test_oracle_mlp.py).
+"""
+import math
+
+import numpy as np
+
+F, H = 8, 128
+P = F * H + H + 2 * (H * H + H) + 2 * H + 2
+
+
+def _slices():
+    o = 0
+    out = {}
+    for name, shp in (("W1", (F, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, H)), ("b3", (H,)),
+                      ("W4", (H, 2)), ("b4", (2,))):
+        n = int(np.prod(shp))
+        out[name] = (o, shp)
+        o += n
+    assert o == P
+    return out
+
+
+SL = _slices()
+
+
+def bf16_round(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(params, round_weights=True):
+    params = np.asarray(params, dtype=np.float64)
+    d = {}
+    for name, (o, shp) in SL.items():
+        v = params[o:o + int(np.prod(shp))].reshape(shp)
+        if round_weights and name.startswith("W"):
+            v = bf16_round(v.astype(np.float32))
+        d[name] = v
+    return d
+
+
+def _sig(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def forward(params, norm, X):
+    """Network output on the original scale (rows of X are raw features)."""
+    p = unpack(params)
+    norm = np.asarray(norm, np.float64)
+    x = (np.asarray(X, np.float64) - norm[:F]) / norm[F:2 * F]
+    z1 = x @ p["W1"] + p["b1"]
+    a1 = z1 * _sig(z1)
+    z2 = a1 @ p["W2"] + p["b2"]
+    a2 = z2 * _sig(z2)
+    z3 = a2 @ p["W3"] + p["b3"]
+    a3 = z3 * _sig(z3)
+    out = a3 @ p["W4"] + p["b4"]
+    return out * norm[2 * F + 2:] + norm[2 * F:2 * F + 2], (x, z1, a1, z2, a2, z3, a3, out)
+
+
+def loss_and_grad(params, norm, X, Y):
+    """L = sum (out_std - y_std)^2 / (2 n) and dL/dparams (weights at their BF16 values)."""
+    norm = np.asarray(norm, np.float64)
+    _, (x, z1, a1, z2, a2, z3, a3, out) = forward(params, norm, X)
+    p = unpack(params)
+    n = x.shape[0]
+    yt = (np.asarray(Y, np.float64) - norm[2 * F:2 * F + 2]) / norm[2 * F + 2:]
+    d = (out - yt) / n
+    L = 0.5 * np.sum((out - yt) ** 2) / n
+
+    def dsilu(z):
+        s = _sig(z)
+        return s * (1.0 + z * (1.0 - s))
+
+    g = {"W4": a3.T @ d, "b4": d.sum(0)}
+    dz3 = (d @ p["W4"].T) * dsilu(z3)
+    g["W3"], g["b3"] = a2.T @ dz3, dz3.sum(0)
+    dz2 = (dz3 @ p["W3"].T) * dsilu(z2)
+    g["W2"], g["b2"] = a1.T @ dz2, dz2.sum(0)
+    dz1 = (dz2 @ p["W2"].T) * dsilu(z1)
+    g["W1"], g["b1"] = x.T @ dz1, dz1.sum(0)
+    flat = np.zeros(P)
+    for name, (o, shp) in SL.items():
+        flat[o:o + int(np.prod(shp))] = g[name].reshape(-1)
+    return L, flat
+
+
+KW = 6
+HS = (0.07, 0.15, 0.30)
+HT = (1.0, 2.5, 6.0)
+
+
+def features(t, r, y, ls_t, ls_s, var_t, binsize):
+    """Site features of one run (t [T], r [T, R, 2], y [T, R]; NaN = missing) -> [T, R, F]."""
+    T_, R_ = y.shape
+    X = np.zeros((T_, R_, F))
+    ok = np.isfinite(y)
+    for k in range(T_):
+        lo, hi = max(0, k - KW), min(T_ - 1, k + KW)
+        rr = r[lo:hi + 1].reshape(-1, 2)
+        yy = y[lo:hi + 1].reshape(-1)
+        mm = ok[lo:hi + 1].reshape(-1)
+        dtq = np.repeat(t[lo:hi + 1] - t[k], R_)
+        for i in range(R_):
+            d2 = np.sum((rr - r[k, i]) ** 2, axis=1)
+            X[k, i, 0] = math.log1p(y[k, i] if ok[k, i] else 0.0)
+            for b in range(3):
+                w = np.exp(-0.5 * dtq ** 2 / HT[b] ** 2 - 0.5 * d2 / HS[b] ** 2) * mm
+                X[k, i, 1 + b] = math.log((np.sum(w * np.where(mm, yy, 0.0)) + 1.0) / (np.sum(w) + 1.0) / binsize)
+            X[k, i, 4:] = np.log([ls_t, ls_s, var_t, binsize])
+    return X
