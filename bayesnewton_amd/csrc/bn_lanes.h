@@ -121,6 +121,22 @@ BN_HD void write_state(const StateIO& io, int64_t n_slots, int64_t s, const T m[
   }
 }
 
+// write_state from a packed covariance (NS entries, upper triangle by rows).
+template <int K1, int K2, class T>
+BN_HD void write_state_packed(const StateIO& io, int64_t n_slots, int64_t s, const T m[Spec<K1, K2>::D],
+                              const T P[Spec<K1, K2>::NS]) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  BN_UNROLL for (int i = 0; i < D; ++i) io.mean[i * n_slots + s] = (state_t)m[i];
+  if (io.packed) {
+    BN_UNROLL for (int q = 0; q < S::NS; ++q) io.cov[q * n_slots + s] = (state_t)P[q];
+  } else {
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = 0; j < D; ++j)
+        io.cov[(i * D + j) * n_slots + s] = (state_t)P[i <= j ? S::sidx(i, j) : S::sidx(j, i)];
+  }
+}
+
 template <int K1, int K2, class T>
 BN_HD void read_state(const CStateIO& io, int64_t n_slots, int64_t s, T m[Spec<K1, K2>::D],
                       T P[Spec<K1, K2>::D][Spec<K1, K2>::D]) {
@@ -155,68 +171,116 @@ BN_HD void read_state(const CStateIO& io, int64_t n_slots, int64_t s, T m[Spec<K
 template <int K1, int K2, class T>
 BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                               const float* __restrict__ y, const float* __restrict__ r, SoA agg, int32_t j) {
+  // The element is kept with packed symmetric Sig / J (NS entries each) and updated
+  // in place, which keeps the live FP64 state at D*D + 2*NS + 2*D values.
   using S = Spec<K1, K2>;
   constexpr int D = S::D;
+  constexpr int NS = S::NS;
   const int32_t b = pl.chunk_series[j];
   const int32_t L = pl.chunk_len_[j];
   const uint8_t fl = pl.chunk_flags[j];
-  Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
-  FElem<D, T> e;
-  e.set_identity_elem();
-  for (int32_t k = 0; k < L; ++k) {
+  const int64_t o = (int64_t)b * hv.stride;
+  const double ls1 = hv.ls[o], v1 = hv.var[o];
+  const double ls2 = K2 > 0 ? hv.ls[o + 1] : 1.0, v2 = K2 > 0 ? hv.var[o + 1] : 1.0;
+  T Phi[D][D], beta[D], Sig[NS], J[NS], eta[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) {
+    beta[i] = T(0);
+    eta[i] = T(0);
+    BN_UNROLL for (int c = 0; c < D; ++c) Phi[i][c] = T(i == c ? 1 : 0);
+  }
+  BN_UNROLL for (int q = 0; q < NS; ++q) { Sig[q] = T(0); J[q] = T(0); }
+  int32_t k0 = 0;
+  if (fl & CH_FIRST) {
+    // the series' first step: predicted state (0, Pinf) and no dependence on the
+    // incoming state (Phi- = 0)
+    Model<K1, K2> mdl;
+    mdl.init(ls1, v1, ls2, v2);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      BN_UNROLL for (int c = i; c < D; ++c) Sig[S::sidx(i, c)] = (T)mdl.Pinf[i][c];
+      BN_UNROLL for (int c = 0; c < D; ++c) Phi[i][c] = T(0);
+    }
+    k0 = 0;
+  }
+  Model<K1, K2> mdl;
+  mdl.lam1 = comp_lambda<K1>(ls1);
+  mdl.lam2 = K2 > 0 ? comp_lambda<(K2 > 0 ? K2 : 1)>(ls2) : 0.0;
+  mdl.var1 = v1;
+  mdl.var2 = v2;
+  for (int32_t k = k0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
-    T Phip[D][D], bp[D], Pp[D][D];
-    if (k == 0 && (fl & CH_FIRST)) {
-      BN_UNROLL for (int i = 0; i < D; ++i) {
-        bp[i] = T(0);
-        BN_UNROLL for (int c = 0; c < D; ++c) { Phip[i][c] = T(0); Pp[i][c] = (T)mdl.Pinf[i][c]; }
-      }
-    } else {
+    if (!(k == 0 && (fl & CH_FIRST))) {
       T A[D][D], Q[D][D];
       mdl.discretise(dt[s], A, Q);
-      predict<K1, K2, T>(A, Q, e.beta, e.Sig, bp, Pp);
-      BN_UNROLL for (int i = 0; i < D; ++i)
-        BN_UNROLL for (int c = 0; c < D; ++c) {
+      // beta = A beta, Phi = A Phi (block-diagonal A; new rows of a block from old ones)
+      T nb[D];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = T(0);
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * beta[q];
+        nb[i] = t;
+      }
+      BN_UNROLL for (int i = 0; i < D; ++i) beta[i] = nb[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) {
+        T col[D];
+        BN_UNROLL for (int i = 0; i < D; ++i) {
           T t = T(0);
-          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * e.Phi[q][c];
-          Phip[i][c] = t;
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * Phi[q][c];
+          col[i] = t;
         }
+        BN_UNROLL for (int i = 0; i < D; ++i) Phi[i][c] = col[i];
+      }
+      // Sig = A Sig A' + Q on the packed upper triangle
+      T ns[NS];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = i; c < D; ++c) {
+          T t = S::nz(i, c) ? Q[i][c] : T(0);
+          BN_UNROLL for (int p = 0; p < D; ++p) {
+            if (!S::nz(i, p)) continue;
+            T u = T(0);
+            BN_UNROLL for (int q = 0; q < D; ++q)
+              if (S::nz(c, q)) u += Sig[p <= q ? S::sidx(p, q) : S::sidx(q, p)] * A[c][q];
+            t += A[i][p] * u;
+          }
+          ns[S::sidx(i, c)] = t;
+        }
+      BN_UNROLL for (int q = 0; q < NS; ++q) Sig[q] = ns[q];
     }
     // u = H Phi-, hP = H P-, S = H P- H' + r, v = y - H beta-
     T u[D], hP[D];
     T hb = T(0);
     BN_UNROLL for (int c = 0; c < D; ++c) {
       T su = T(0), sp = T(0);
-      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { su += Phip[i][c]; sp += Pp[i][c]; }
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        if (S::h(i)) {
+          su += Phi[i][c];
+          sp += Sig[i <= c ? S::sidx(i, c) : S::sidx(c, i)];
+        }
       u[c] = su;
       hP[c] = sp;
     }
     double Sd = (double)r[s];
-    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { Sd += (double)hP[i]; hb += bp[i]; }
+    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { Sd += (double)hP[i]; hb += beta[i]; }
     const double iS = 1.0 / Sd;
     const T v = (T)((double)y[s] - (double)hb);
     const T iSf = (T)iS;
     const T vS = v * iSf;
     BN_UNROLL for (int i = 0; i < D; ++i) {
-      e.eta[i] += u[i] * vS;
+      eta[i] += u[i] * vS;
+      const T ki = hP[i] * iSf;
+      beta[i] += ki * v;
       BN_UNROLL for (int c = i; c < D; ++c) {
-        T t = e.J[i][c] + u[i] * u[c] * iSf;
-        e.J[i][c] = t;
-        e.J[c][i] = t;
+        J[S::sidx(i, c)] += u[i] * u[c] * iSf;
+        Sig[S::sidx(i, c)] -= hP[i] * hP[c] * iSf;
       }
-    }
-    BN_UNROLL for (int i = 0; i < D; ++i) {
-      const T Ki = hP[i] * iSf;
-      e.beta[i] = bp[i] + Ki * v;
-      BN_UNROLL for (int c = 0; c < D; ++c) e.Phi[i][c] = Phip[i][c] - Ki * u[c];
-      BN_UNROLL for (int c = i; c < D; ++c) {
-        T t = Pp[i][c] - hP[i] * hP[c] * iSf;
-        e.Sig[i][c] = t;
-        e.Sig[c][i] = t;
-      }
+      BN_UNROLL for (int c = 0; c < D; ++c) Phi[i][c] -= ki * u[c];
     }
   }
-  soa_store(e, agg, j);
+  // storage order of FElem: Phi, beta, Sig (upper), J (upper), eta
+  int f = 0;
+  BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) agg.base[(int64_t)(f++) * agg.stride + j] = (agg_t)Phi[i][c];
+  BN_UNROLL for (int i = 0; i < D; ++i) agg.base[(int64_t)(f++) * agg.stride + j] = (agg_t)beta[i];
+  BN_UNROLL for (int q = 0; q < NS; ++q) agg.base[(int64_t)(f++) * agg.stride + j] = (agg_t)Sig[q];
+  BN_UNROLL for (int q = 0; q < NS; ++q) agg.base[(int64_t)(f++) * agg.stride + j] = (agg_t)J[q];
+  BN_UNROLL for (int i = 0; i < D; ++i) agg.base[(int64_t)(f++) * agg.stride + j] = (agg_t)eta[i];
 }
 
 // Incoming filtered state of chunk j (not the first of its series): the
@@ -385,80 +449,114 @@ template <int K1, int K2, class T>
 BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                             const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
                             CSoA fagg, CSoA fcarry, const FilterPassOut& out, int32_t j) {
+  // Covariances P (filter) and V (fixed-point smoother) are kept packed (NS
+  // entries) and updated in place; C = Cov(x_s, x_k | y_1..k) is a full D x D.
   using S = Spec<K1, K2>;
   constexpr int D = S::D;
+  constexpr int NS = S::NS;
   const int32_t b = pl.chunk_series[j];
   const int32_t L = pl.chunk_len_[j];
   const uint8_t fl = pl.chunk_flags[j];
   Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
-  T m[D], P[D][D];
+  T m[D], P[NS];
   if (!(fl & CH_FIRST)) {
-    filter_incoming<D, T>(pl, fagg, fcarry, j, m, P);
+    T Pf[D][D];
+    filter_incoming<D, T>(pl, fagg, fcarry, j, m, Pf);
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = Pf[i][c];
   } else {
-    BN_UNROLL for (int i = 0; i < D; ++i) { m[i] = T(0); BN_UNROLL for (int c = 0; c < D; ++c) P[i][c] = T(0); }
+    BN_UNROLL for (int i = 0; i < D; ++i) m[i] = T(0);
+    BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = T(0);
   }
   const bool want_s = out.sagg.base != nullptr;
-  T mu[D], V[D][D], C[D][D];  // fixed-point smoother of x_s
+  T mu[D], V[NS], C[D][D];  // fixed-point smoother of x_s
   double ell = 0.0;
+  auto pk = [](int i, int c) { return i <= c ? S::sidx(i, c) : S::sidx(c, i); };
   for (int32_t k = 0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
-    T mp[D], Pp[D][D], AP[D][D], Cm[D][D];
     if (k == 0 && (fl & CH_FIRST)) {
       BN_UNROLL for (int i = 0; i < D; ++i) {
-        mp[i] = T(0);
-        BN_UNROLL for (int c = 0; c < D; ++c) Pp[i][c] = (T)mdl.Pinf[i][c];
+        m[i] = T(0);
+        BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = (T)mdl.Pinf[i][c];
       }
     } else {
+      // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'
       T A[D][D];
       transition_only<K1, K2, T>(mdl, dt[s], A);
-      predict_ap<K1, K2, T>(A, mdl.Pinf, m, P, mp, Pp, AP);
+      T nm[D];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = T(0);
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * m[q];
+        nm[i] = t;
+      }
+      BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
+      T np_[NS];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = i; c < D; ++c) {
+          T t = S::nz(i, c) ? (T)mdl.Pinf[i][c] : T(0);
+          BN_UNROLL for (int p = 0; p < D; ++p) {
+            if (!S::nz(i, p)) continue;
+            T u = T(0);
+            BN_UNROLL for (int q = 0; q < D; ++q)
+              if (S::nz(c, q)) u += (P[pk(p, q)] - (S::nz(p, q) ? (T)mdl.Pinf[p][q] : T(0))) * A[c][q];
+            t += A[i][p] * u;
+          }
+          np_[S::sidx(i, c)] = t;
+        }
+      BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = np_[q];
       if (want_s && k > 0) {
-        // C- = C A'
-        BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          T row[D];
           BN_UNROLL for (int c = 0; c < D; ++c) {
             T t = T(0);
             BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
-            Cm[i][c] = t;
+            row[c] = t;
           }
-      }
-    }
-    const double lp = update<K1, K2, T>(mp, Pp, (T)y[s], (T)r[s], m, P);
-    if (!(mask != nullptr && mask[s])) ell += lp;
-    if (want_s) {
-      if (k == 0) {
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          mu[i] = m[i];
-          BN_UNROLL for (int c = 0; c < D; ++c) { V[i][c] = P[i][c]; C[i][c] = P[i][c]; }
-        }
-      } else {
-        // ch = C- H', hP = H Pp, S = H Pp H' + r, v = y - H mp
-        T ch[D], hP[D];
-        double Sd = (double)r[s];
-        T hm = T(0);
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          T a = T(0), bq = T(0);
-          BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { a += Cm[i][q]; bq += Pp[q][i]; }
-          ch[i] = a;
-          hP[i] = bq;
-        }
-        BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += mp[q]; }
-        const T iS = (T)(1.0 / Sd);
-        const T vS = ((T)y[s] - hm) * iS;
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          mu[i] += ch[i] * vS;
-          BN_UNROLL for (int c = i; c < D; ++c) {
-            const T t = V[i][c] - ch[i] * ch[c] * iS;
-            V[i][c] = t;
-            V[c][i] = t;
-          }
-          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = Cm[i][c] - ch[i] * hP[c] * iS;
+          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = row[c];
         }
       }
     }
-    if (out.state.mean != nullptr) {
-      if (out.predict) write_state<K1, K2, T>(out.state, pl.n_slots, s, mp, Pp);
-      else write_state<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+    if (out.state.mean != nullptr && out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+    // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
+    T hP[D];
+    BN_UNROLL for (int c = 0; c < D; ++c) {
+      T t = T(0);
+      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
+      hP[c] = t;
     }
+    double Sd = (double)r[s];
+    T hm = T(0);
+    BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
+    const double vd = (double)y[s] - (double)hm;
+    const double iSd = 1.0 / Sd;
+    const T vS = (T)(vd * iSd);
+    const T iS = (T)iSd;
+    if (!(mask != nullptr && mask[s])) ell += -0.5 * (vd * vd * iSd + kLog2Pi + log(Sd));
+    if (want_s && k > 0) {
+      // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
+      T ch[D];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = T(0);
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += C[i][q];
+        ch[i] = t;
+      }
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        mu[i] += ch[i] * vS;
+        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ch[i] * ch[c] * iS;
+        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ch[i] * hP[c] * iS;
+      }
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] += hP[i] * vS;
+      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= hP[i] * hP[c] * iS;
+    }
+    if (want_s && k == 0) {
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        mu[i] = m[i];
+        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] = P[S::sidx(i, c)];
+        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = P[pk(i, c)];
+      }
+    }
+    if (out.state.mean != nullptr && !out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
   }
   if (want_s) {
     SElem<D, T> acc;
@@ -467,12 +565,13 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       set_zero<D, T>(acc.E);
       BN_UNROLL for (int i = 0; i < D; ++i) {
         acc.g[i] = mu[i];
-        BN_UNROLL for (int c = 0; c < D; ++c) acc.L[i][c] = V[i][c];
+        BN_UNROLL for (int c = 0; c < D; ++c) acc.L[i][c] = V[pk(i, c)];
       }
     } else {
-      T A[D][D], mp[D], Pp[D][D], AP[D][D], Cn[D][D];
+      T A[D][D], mp[D], Pp[D][D], AP[D][D], Cn[D][D], Pf[D][D];
+      BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) Pf[i][c] = P[pk(i, c)];
       transition_only<K1, K2, T>(mdl, dt[pl.slot(j + 1, 0)], A);
-      predict_ap<K1, K2, T>(A, mdl.Pinf, m, P, mp, Pp, AP);
+      predict_ap<K1, K2, T>(A, mdl.Pinf, m, Pf, mp, Pp, AP);
       BN_UNROLL for (int i = 0; i < D; ++i)
         BN_UNROLL for (int c = 0; c < D; ++c) {
           T t = T(0);
@@ -496,7 +595,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
         BN_UNROLL for (int c = i; c < D; ++c) {
           T t1 = T(0), t2 = T(0);
           BN_UNROLL for (int q = 0; q < D; ++q) { t1 += acc.E[i][q] * Cn[c][q]; t2 += acc.E[c][q] * Cn[i][q]; }
-          const T t = V[i][c] - T(0.5) * (t1 + t2);
+          const T t = V[S::sidx(i, c)] - T(0.5) * (t1 + t2);
           acc.L[i][c] = t;
           acc.L[c][i] = t;
         }
@@ -645,6 +744,135 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
       BN_UNROLL for (int i = 0; i < D; ++i)
         BN_UNROLL for (int c = 0; c < D; ++c) io.gains[(i * D + c) * ns + s] = (OT)G[i][c];
     }
+  }
+}
+
+// smoother_mbf: the smoothed marginals H m, H P H' of one chunk by the modified
+// Bryson-Frazier recursion (backwards over the chunk) from the filter's PREDICTED
+// moments (m-, P-) and the observations.  It carries the adjoint (lh, Lh) with
+//   m_s = m- - P- lt,  P_s = P- - P- Lt P-,
+//   lt = lh - h (k' lh + v / S),  Lt = Lh - h g' - g h' + (k' g + 1/S) h h',
+//   g = Lh k, k = P- h / S,  lh <- A' lt, Lh <- A' Lt A (A of the step),
+// which gives the posterior of ops.py:288-311 without factorising P- per step.
+// The chunk's incoming adjoint comes from the smoothed state (sm, sP) at the next
+// chunk's first step: lt = -P-^-1 (sm - m-), Lt = P-^-1 (P- - sP) P-^-1.
+template <int K1, int K2, class T>
+BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
+                             const float* __restrict__ y, const float* __restrict__ r, CSoA sagg, CSoA scarry,
+                             const CStateIO& fin, float* __restrict__ out_mean, float* __restrict__ out_var,
+                             int32_t j) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  constexpr int NS = S::NS;
+  const int32_t b = pl.chunk_series[j];
+  const int32_t L = pl.chunk_len_[j];
+  const uint8_t fl = pl.chunk_flags[j];
+  const int64_t ns = pl.n_slots;
+  const int64_t o = (int64_t)b * hv.stride;
+  Model<K1, K2> mdl;
+  mdl.lam1 = comp_lambda<K1>(hv.ls[o]);
+  mdl.lam2 = K2 > 0 ? comp_lambda<(K2 > 0 ? K2 : 1)>(hv.ls[o + 1]) : 0.0;
+  auto pk = [](int i, int c) { return i <= c ? S::sidx(i, c) : S::sidx(c, i); };
+  T lh[D], Lh[NS];
+  BN_UNROLL for (int i = 0; i < D; ++i) lh[i] = T(0);
+  BN_UNROLL for (int q = 0; q < NS; ++q) Lh[q] = T(0);
+  // propagate (lt, Lt) of a step back over its transition A: lh = A' lt, Lh = A' Lt A
+  auto back = [&](double dts, const T lt[D], const T Lt[NS]) {
+    T A[D][D];
+    transition_only<K1, K2, T>(mdl, dts, A);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T t = T(0);
+      BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(q, i)) t += A[q][i] * lt[q];
+      lh[i] = t;
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int c = i; c < D; ++c) {
+        T t = T(0);
+        BN_UNROLL for (int p = 0; p < D; ++p) {
+          if (!S::nz(p, i)) continue;
+          T u = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(q, c)) u += Lt[pk(p, q)] * A[q][c];
+          t += A[p][i] * u;
+        }
+        Lh[S::sidx(i, c)] = t;
+      }
+  };
+  if (!(fl & CH_LAST)) {
+    T sm[D], sP[D][D], pm[D], pP[D][D];
+    smoother_incoming<D, T>(pl, sagg, scarry, j, sm, sP);
+    const int64_t sn = pl.slot(j + 1, 0);
+    read_state<K1, K2, T>(fin, ns, sn, pm, pP);
+    double Lc[D][D], xv[D][1], X[D][D];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      xv[i][0] = (double)pm[i] - (double)sm[i];
+      BN_UNROLL for (int c = 0; c < D; ++c) {
+        Lc[i][c] = (double)pP[i][c];
+        X[i][c] = (double)pP[i][c] - (double)sP[i][c];
+      }
+    }
+    chol_spd<D>(Lc);
+    lower_solve<D, 1, double>(Lc, xv);
+    upper_t_solve<D, 1, double>(Lc, xv);
+    lower_solve<D, D, double>(Lc, X);
+    upper_t_solve<D, D, double>(Lc, X);   // X = P-^-1 (P- - sP)
+    double Xt[D][D];
+    BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = 0; c < D; ++c) Xt[i][c] = X[c][i];
+    lower_solve<D, D, double>(Lc, Xt);
+    upper_t_solve<D, D, double>(Lc, Xt);  // Xt = P-^-1 (P- - sP) P-^-1 (transposed; symmetric)
+    T lt[D], Lt[NS];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      lt[i] = (T)xv[i][0];
+      BN_UNROLL for (int c = i; c < D; ++c) Lt[S::sidx(i, c)] = (T)(0.5 * (Xt[i][c] + Xt[c][i]));
+    }
+    back((double)dt[sn], lt, Lt);
+  }
+  for (int32_t k = L - 1; k >= 0; --k) {
+    const int64_t s = pl.slot(j, k);
+    T m[D], P[NS];
+    BN_UNROLL for (int i = 0; i < D; ++i) m[i] = (T)fin.mean[i * ns + s];
+    BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = (T)fin.cov[q * ns + s];
+    T hP[D], kv[D];
+    BN_UNROLL for (int c = 0; c < D; ++c) {
+      T t = T(0);
+      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
+      hP[c] = t;
+    }
+    T hph = T(0), hm = T(0);
+    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { hph += hP[i]; hm += m[i]; }
+    const double Sd = (double)r[s] + (double)hph;
+    const T iS = (T)(1.0 / Sd);
+    const T vS = (T)(((double)y[s] - (double)hm) / Sd);
+    BN_UNROLL for (int i = 0; i < D; ++i) kv[i] = hP[i] * iS;
+    T kl = T(0);
+    BN_UNROLL for (int i = 0; i < D; ++i) kl += kv[i] * lh[i];
+    T lt[D], g[D], Lt[NS];
+    BN_UNROLL for (int i = 0; i < D; ++i) lt[i] = S::h(i) ? lh[i] - (kl + vS) : lh[i];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T t = T(0);
+      BN_UNROLL for (int q = 0; q < D; ++q) t += Lh[pk(i, q)] * kv[q];
+      g[i] = t;
+    }
+    T al = iS;
+    BN_UNROLL for (int i = 0; i < D; ++i) al += kv[i] * g[i];
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int c = i; c < D; ++c) {
+        T t = Lh[S::sidx(i, c)];
+        if (S::h(i)) t -= g[c];
+        if (S::h(c)) t -= g[i];
+        if (S::h(i) && S::h(c)) t += al;
+        Lt[S::sidx(i, c)] = t;
+      }
+    // H m_s = H m- - hP' lt ;  H P_s H' = H P- H' - hP' Lt hP
+    T om = hm, ov = hph;
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      om -= hP[i] * lt[i];
+      T t = T(0);
+      BN_UNROLL for (int c = 0; c < D; ++c) t += Lt[pk(i, c)] * hP[c];
+      ov -= hP[i] * t;
+    }
+    out_mean[s] = (float)om;
+    out_var[s] = (float)ov;
+    if (k > 0) back((double)dt[s], lt, Lt);
   }
 }
 

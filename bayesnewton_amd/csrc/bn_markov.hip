@@ -197,6 +197,18 @@ __global__ __launch_bounds__(kBlock) void k_smoother_pass(PlanView pl, HyperView
   lane_smoother_pass<K1, K2, CT, OT>(pl, hv, sagg, scarry, io, j);
 }
 
+// Fused path: smoothed marginals by the Bryson-Frazier recursion from the filter's
+// predicted moments (lane_smoother_mbf).
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView hv, const float* __restrict__ dt,
+                                                         const float* __restrict__ y, const float* __restrict__ r,
+                                                         CSoA sagg, CSoA scarry, CStateIO fin, float* pmean,
+                                                         float* pvar) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= pl.n_chunks) return;
+  lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, pmean, pvar, j);
+}
+
 template <int K1, int K2>
 __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                        const float* __restrict__ y, const float* __restrict__ r,
@@ -411,8 +423,10 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   Bufs b = ws_bufs(p, D, ws);
   launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
   FilterPassOut fo;
+  // the filter pass keeps the predicted moments: the smoother's Bryson-Frazier
+  // recursion runs on them without factorising a covariance per step
   fo.state = StateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
-  fo.predict = false;
+  fo.predict = true;
   fo.sagg = b.sagg;
   fo.ell = b.ell;
   {
@@ -421,18 +435,11 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
                        cs(b.fagg), cs(b.ftcarry), fo);
   }
   launch_smoother_scan<D>(plan, b, st);
-  SmootherPassIO<float> io;
-  io.fin = CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
-  io.dt = dt;
-  io.shifted = false;
-  io.out_mean = pmean;
-  io.out_cov = pvar;
-  io.gains = nullptr;
-  io.full = false;
   {
     BN_TIMED(KID_SMOOTHER_PASS, st);
-    hipLaunchKernelGGL((k_smoother_pass<K1, K2, float>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv,
-                       cs(b.sagg), cs(b.stcarry), io);
+    hipLaunchKernelGGL((k_smoother_mbf<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
+                       cs(b.sagg), cs(b.stcarry), CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true}, pmean,
+                       pvar);
   }
   {
     BN_TIMED(KID_SERIES_SUM, st);
