@@ -77,8 +77,8 @@ SIGNATURES = {
     "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                           vp, vp]),
     "bn_st_workspace_bytes": (i64, [vp]),
-    "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_ws_features": (i32, [vp, vp, vp, vp, f64, f64, f64, f64, vp, vp]),

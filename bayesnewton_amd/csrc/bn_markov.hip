@@ -3,7 +3,7 @@
 //
 // Launch structure of one fused filter + smoother pass (bn_filter_smoother_f32):
 //   k_filter_reduce   lane per chunk            -> filtering element per chunk
-//   k_fscan_tile      256-thread block per tile -> tile-local inclusive prefixes (LDS Hillis-Steele)
+//   k_fscan_tile      64-thread block per tile  -> tile-local inclusive prefixes (pairs + LDS Hillis-Steele)
 //   k_fscan_series    block per series          -> exclusive tile carries
 //   k_filter_pass     lane per chunk            -> filtered moments (scratch), loglik partials,
 //                                                  smoothing element per chunk
@@ -57,33 +57,63 @@ __global__ __launch_bounds__(kBlock) void k_filter_reduce(PlanView pl, HyperView
   lane_filter_reduce<K1, K2, CT>(pl, hv, dt, y, r, agg, j);
 }
 
+// Tile scan with two elements per thread: each thread combines its pair, a
+// Hillis-Steele scan runs over the <= 64 pair totals in LDS, and the first element
+// of each pair takes the previous pair's prefix.  About 3.5 combines per element
+// instead of log2(128) = 7 for a scan over the elements themselves.
 template <int D>
-__global__ __launch_bounds__(kScan) void k_fscan_tile(PlanView pl, SoA agg, SoA tagg) {
+__global__ __launch_bounds__(kScan / 2) void k_fscan_tile(PlanView pl, SoA agg, SoA tagg) {
   extern __shared__ agg_t lds[];
   using E = FElem<D, CT>;
+  constexpr int NP = kScan / 2;
   const int32_t t = blockIdx.x;
   const int32_t tid = threadIdx.x;
   const int32_t first = pl.tile_first[t];
   const int32_t count = pl.tile_count[t];
   if ((pl.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && count == 1) return;
-  E e;
-  if (tid < count) soa_load(e, CSoA{agg.base, agg.stride}, first + tid);
-  else e.set_identity_elem();
-  const SoA ls{lds, kScan};
-  const CSoA lc{lds, kScan};
-  for (int32_t off = 1; off < count; off <<= 1) {
-    soa_store(e, ls, tid);
+  const int32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+  const int32_t np = (count + 1) / 2;
+  E e0, pr;
+  if (i0 < count) soa_load(e0, CSoA{agg.base, agg.stride}, first + i0);
+  else e0.set_identity_elem();
+  if (i1 < count) {
+    E e1;
+    soa_load(e1, CSoA{agg.base, agg.stride}, first + i1);
+    fcombine<D, CT>(e0, e1, pr);
+  } else {
+    pr = e0;
+  }
+  const SoA ls{lds, NP};
+  const CSoA lc{lds, NP};
+  for (int32_t off = 1; off < np; off <<= 1) {
+    soa_store(pr, ls, tid);
     __syncthreads();
-    if (tid >= off && tid < count) {
+    if (tid >= off && tid < np) {
       E p, o;
       soa_load(p, lc, tid - off);
-      fcombine<D, CT>(p, e, o);
-      e = o;
+      fcombine<D, CT>(p, pr, o);
+      pr = o;
     }
     __syncthreads();
   }
-  if (tid < count) soa_store(e, agg, first + tid);
-  if (tid == count - 1) soa_store(e, tagg, t);
+  soa_store(pr, ls, tid);
+  __syncthreads();
+  if (i0 < count) {
+    E out0;
+    if (tid == 0) {
+      out0 = e0;
+    } else {
+      E p;
+      soa_load(p, lc, tid - 1);
+      fcombine<D, CT>(p, e0, out0);
+    }
+    soa_store(out0, agg, first + i0);
+    if (i0 == count - 1) soa_store(out0, tagg, t);
+  }
+  if (i1 < count) {
+    soa_store(pr, agg, first + i1);
+    if (i1 == count - 1) soa_store(pr, tagg, t);
+  }
 }
 
 template <int D>
@@ -399,7 +429,8 @@ void launch_filter_scan(const bn_plan* plan, HyperView hv, const float* dt, cons
   { BN_TIMED(KID_FILTER_REDUCE, st);
   hipLaunchKernelGGL((k_filter_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, b.fagg); }
   { BN_TIMED(KID_FSCAN_TILE, st);
-  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsF, st, plan->view, b.fagg, b.ftagg); }
+  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan / 2), ldsF / 2, st, plan->view, b.fagg,
+                     b.ftagg); }
   { BN_TIMED(KID_FSCAN_SERIES, st);
   hipLaunchKernelGGL((k_fscan_series<D>), dim3(p.n_series), dim3(kScan), ldsF, st, plan->view, b.ftagg, b.ftcarry); }
 }

@@ -16,9 +16,11 @@
 //                     M = I + L' Sh L = Mc Mc', Y = L Mc^-T, w = Mc^-1 (z - L' H m-),
 //                     V = P-[:, :M] Y, m = m- + V w, P = P- - V V', log N terms.
 //                     Equal to ops.py:159-171 with noise (W + 1e-12 I)^-1.
-//   k_st_smoother     block per run, loop back over T: modified Bryson-Frazier smoother
-//                     (adjoint Lambda, lambda), which yields the RTS posterior of
-//                     ops.py:288-311 without factorising P-: only GEMMs per step.
+//   k_st_smoother     block per run, loop back over T: the RTS smoother of ops.py:288-311
+//                     (G = P A' P-^-1 by a blocked Cholesky in LDS, P_s = P + G (P_s' - P-) G').
+//                     An adjoint (Bryson-Frazier) backward pass avoids the factorisation but
+//                     its covariance error grows geometrically over long runs with
+//                     precise sites, so the filtered moments are stored instead.
 //   k_st_sum          per-run FP64 sums of per-step partials.
 // Dense products are workgroup GEMMs on FP64 operands staged through LDS.
 #include <hip/hip_runtime.h>
@@ -37,6 +39,7 @@ namespace {
 constexpr int kT = 256;      // threads per workgroup
 constexpr int kMaxM = 64;    // inducing points per run
 constexpr int kMaxKT = 2;    // temporal state dim (Matern12 / Matern32)
+constexpr int kMaxD = kMaxKT * kMaxM;  // state dim
 constexpr int kPK = 16;      // GEMM K-panel
 constexpr int kPS = 68;      // LDS panel row stride (doubles)
 constexpr double kPseudoJitter = 1e-12;  // basemodels.py:683
@@ -445,13 +448,13 @@ struct StFilterIO {
   const double* ldl;  // [K,T]
   double* P;          // per-run scratch [K,D,D]
   double* V;          // per-run scratch [K,D,M]
-  double* U0;         // [K,T,D,M]
-  double* Y;          // [K,T,M,M]
-  double* w;          // [K,T,M]
-  double* Hm;         // [K,T,M]
+  double* fmean;      // [K,T,D] filtered means
+  double* fcov;       // [K,T,D(D+1)/2] filtered covariances, packed upper triangle by rows
   double* ell;        // [K,T] per-step log-likelihood terms
-  double* fmean;      // optional [K,T,D] filtered means
 };
+
+// packed upper-triangle index of (i, j), i <= j, in an n x n matrix
+__device__ inline int64_t pidx(int i, int j, int n) { return (int64_t)i * n - (int64_t)i * (i - 1) / 2 + (j - i); }
 
 template <int KT>
 __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterIO io) {
@@ -516,9 +519,6 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
     const double* L = io.Lw + kt * M * M;
     for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
     __syncthreads();
-    if (tid < M) io.Hm[kt * M + tid] = vm[tid];
-    double* U0 = io.U0 + kt * D * M;
-    for (int e = tid; e < D * M; e += kT) U0[e] = P[(int64_t)(e / M) * D + e % M];
     // X = Sh L -> b1 ; Mm = I + L' X -> b2 (stride M+1)
     wg_gemm(M, M, M, 1.0, MatRef{P, D, 1}, MatRef{sL, M, 1}, 0.0, b1, M, pan);
     wg_gemm(M, M, M, 1.0, MatRef{sL, 1, M}, MatRef{b1, M, 1}, 0.0, b2, M + 1, pan);
@@ -535,15 +535,7 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
       b1[tid * (M + 1) + M] = u;
     }
     wg_lower_solve(b2, M, M + 1, b1, M + 1, 1, M + 1);   // b1 = [Y' | w]
-    double* Yo = io.Y + kt * M * M;
-    for (int e = tid; e < M * M; e += kT) {
-      const int i = e / M, c = e % M;
-      Yo[e] = b1[c * (M + 1) + i];
-    }
-    if (tid < M) {
-      vq[tid] = b1[tid * (M + 1) + M];
-      io.w[kt * M + tid] = vq[tid];
-    }
+    if (tid < M) vq[tid] = b1[tid * (M + 1) + M];
     // log N: -1/2 (|w|^2 + M log 2pi + 2 sum log diag Mc - 2 ldl)
     const double wq = (tid < M) ? vq[tid] * vq[tid] + 2.0 * log(b2[tid * (M + 1) + tid]) : 0.0;
     const double s = wg_sum(wq, red);
@@ -558,133 +550,171 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
     }
     // P = P- - V V'
     wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{V, 1, M}, 1.0, P, D, pan);
-    if (io.fmean != nullptr)
-      for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
+    for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
+    double* fc = io.fcov + kt * ((int64_t)D * (D + 1) / 2);
+    for (int e = tid; e < D * D; e += kT) {
+      const int i = e / D, j = e % D;
+      if (j >= i) fc[pidx(i, j, D)] = P[e];
+    }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// k_st_smoother: modified Bryson-Frazier backward pass, one workgroup per run.
-//   Lt = Lh - E H' - H E' + H F H',  E = (Lh V) Y',  F = Y (V' Lh V + I) Y'
-//   lt = lh - H' Y (V' lh + w)
-//   posterior: H m = H m- - U0' lt,  H P H' = Sh - U0' Lt U0
-//   Lh <- A' Lt A, lh <- A' lt (A of the step, i.e. from t-1 to t)
+// k_st_smoother: RTS backward pass (ops.py:288-311), one workgroup per run.
+//   P- = A P A' + Q (A, Q of the next step's dt, basemodels.py:700), L L' = P-,
+//   G' = L^-T L^-1 (A P), m_s = m + G (m_s' - A m), P_s = P + G (P_s' - P-) G'.
+// Outputs H m_s and H P_s H' (the first M states).
 // ---------------------------------------------------------------------------
 struct StSmootherIO {
   const double* dt;
-  const double* U0;
-  const double* Y;
-  const double* w;
-  const double* Hm;
-  double* Lh;     // scratch [K,D,D]
-  double* G1;     // scratch [K,D,M]
-  double* G3;     // scratch [K,D,M]
-  double* E;      // scratch [K,D,M]
-  double* V;      // scratch [K,D,M]
-  double* pm;     // [K,T,M]
-  double* pP;     // [K,T,M,M]
+  const double* fmean;  // [K,T,D]
+  const double* fcov;   // [K,T,D(D+1)/2]
+  double* Ps;           // per-run scratch [K,D,D] each
+  double* F;
+  double* GT;
+  double* Dl;
+  double* T1;
+  double* pm;           // [K,T,M]
+  double* pP;           // [K,T,M,M]
 };
+
+constexpr int kSB = 16;  // row block of the triangular solves
+
+// X := L^-T L^-1 X for X (n x ncol, row stride ldx, global) with L lower (n x n, LDS, stride ld).
+__device__ void wg_chol_solve(const double* L, int n, int ld, double* X, int ldx, int ncol, double* pan) {
+  const int tid = threadIdx.x;
+  for (int rb = 0; rb < n; rb += kSB) {
+    const int nb = min(kSB, n - rb);
+    if (rb > 0) wg_gemm(nb, ncol, rb, -1.0, MatRef{L + (int64_t)rb * ld, ld, 1}, MatRef{X, ldx, 1}, 1.0,
+                        X + (int64_t)rb * ldx, ldx, pan);
+    for (int c = tid; c < ncol; c += kT) {
+      double x[kSB];
+#pragma unroll
+      for (int i = 0; i < kSB; ++i) {
+        if (i < nb) {
+          double v = X[(int64_t)(rb + i) * ldx + c];
+#pragma unroll
+          for (int q = 0; q < kSB; ++q)
+            if (q < i) v = fma(-L[(rb + i) * ld + rb + q], x[q], v);
+          x[i] = v / L[(rb + i) * ld + rb + i];
+          X[(int64_t)(rb + i) * ldx + c] = x[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int rb = ((n - 1) / kSB) * kSB; rb >= 0; rb -= kSB) {
+    const int nb = min(kSB, n - rb);
+    if (rb + nb < n)
+      wg_gemm(nb, ncol, n - rb - nb, -1.0, MatRef{L + (int64_t)(rb + nb) * ld + rb, 1, ld},
+              MatRef{X + (int64_t)(rb + nb) * ldx, ldx, 1}, 1.0, X + (int64_t)rb * ldx, ldx, pan);
+    for (int c = tid; c < ncol; c += kT) {
+      double x[kSB];
+#pragma unroll
+      for (int i = kSB - 1; i >= 0; --i) {
+        if (i < nb) {
+          double v = X[(int64_t)(rb + i) * ldx + c];
+#pragma unroll
+          for (int q = 0; q < kSB; ++q)
+            if (q > i && q < nb) v = fma(-L[(rb + q) * ld + rb + i], x[q], v);
+          x[i] = v / L[(rb + i) * ld + rb + i];
+          X[(int64_t)(rb + i) * ldx + c] = x[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
 
 template <int KT>
 __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmootherIO io) {
   extern __shared__ double smem[];
   const int M = dm.M, D = dm.D, tid = threadIdx.x;
   const int k = blockIdx.x;
-  double* pan = smem;                      // 2*kPK*kPS
-  double* sY = pan + 2 * kPK * kPS;        // M*M
-  double* sF = sY + kMaxM * kMaxM;         // M*M
-  double* sT = sF + kMaxM * kMaxM;         // M*M
-  double* lh = sT + kMaxM * kMaxM;         // D
-  double* lt = lh + kMaxKT * kMaxM;        // D
-  double* vq = lt + kMaxKT * kMaxM;        // M
-  double* vr = vq + kMaxM;                 // M
+  double* pan = smem;                        // 2*kPK*kPS
+  double* sL = pan + 2 * kPK * kPS;          // D*D
+  double* ms = sL + kMaxD * kMaxD;           // D
+  double* mf = ms + kMaxD;                   // D
+  double* mp = mf + kMaxD;                   // D
+  double* dv = mp + kMaxD;                   // D
   __shared__ double sA[KT * KT], sQ[KT * KT];
-  double* Lh = io.Lh + (int64_t)k * D * D;
-  double* G1 = io.G1 + (int64_t)k * D * M;
-  double* G3 = io.G3 + (int64_t)k * D * M;
-  double* E = io.E + (int64_t)k * D * M;
-  double* V = io.V + (int64_t)k * D * M;
+  const int64_t DD = (int64_t)D * D, NSD = (int64_t)D * (D + 1) / 2;
+  double* Ps = io.Ps + k * DD;
+  double* F = io.F + k * DD;
+  double* GT = io.GT + k * DD;
+  double* Dl = io.Dl + k * DD;
+  double* T1 = io.T1 + k * DD;
   const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
-  for (int64_t e = tid; e < (int64_t)D * D; e += kT) Lh[e] = 0.0;
-  for (int e = tid; e < D; e += kT) lh[e] = 0.0;
-  __syncthreads();
-  for (int t = dm.T - 1; t >= 0; --t) {
+  auto unpack = [&](const double* pk, double* out) {
+    for (int e = tid; e < D * D; e += kT) {
+      const int i = e / D, j = e % D;
+      out[e] = pk[i <= j ? pidx(i, j, D) : pidx(j, i, D)];
+    }
+  };
+  auto emit = [&](int64_t kt) {
+    for (int e = tid; e < M; e += kT) io.pm[kt * M + e] = ms[e];
+    for (int e = tid; e < M * M; e += kT) io.pP[kt * M * M + e] = Ps[(int64_t)(e / M) * D + e % M];
+  };
+  {
+    const int64_t kt = (int64_t)k * dm.T + dm.T - 1;
+    unpack(io.fcov + kt * NSD, Ps);
+    for (int e = tid; e < D; e += kT) ms[e] = io.fmean[kt * D + e];
+    __syncthreads();
+    emit(kt);
+  }
+  for (int t = dm.T - 2; t >= 0; --t) {
     const int64_t kt = (int64_t)k * dm.T + t;
-    const double* U0 = io.U0 + kt * D * M;
-    const double* Yg = io.Y + kt * M * M;
-    for (int e = tid; e < M * M; e += kT) sY[e] = Yg[e];
-    if (tid < M) vq[tid] = io.w[kt * M + tid];
+    unpack(io.fcov + kt * NSD, F);
+    for (int e = tid; e < D; e += kT) mf[e] = io.fmean[kt * D + e];
+    if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt + 1], sA, sQ);
     __syncthreads();
-    // V = U0 Y ; G1 = Lh V ; G3 = Lh U0
-    wg_gemm(D, M, M, 1.0, MatRef{U0, M, 1}, MatRef{sY, M, 1}, 0.0, V, M, pan);
-    wg_gemm(D, M, D, 1.0, MatRef{Lh, D, 1}, MatRef{V, M, 1}, 0.0, G1, M, pan);
-    wg_gemm(D, M, D, 1.0, MatRef{Lh, D, 1}, MatRef{U0, M, 1}, 0.0, G3, M, pan);
-    // sT = V' G1 + I ; E = G1 Y' ; sF = Y sT Y'
-    wg_gemm(M, M, D, 1.0, MatRef{V, 1, M}, MatRef{G1, M, 1}, 0.0, sT, M, pan);
-    for (int e = tid; e < M; e += kT) sT[e * M + e] += 1.0;
-    wg_gemm(D, M, M, 1.0, MatRef{G1, M, 1}, MatRef{sY, 1, M}, 0.0, E, M, pan);
-    wg_gemm(M, M, M, 1.0, MatRef{sT, M, 1}, MatRef{sY, 1, M}, 0.0, sF, M, pan);   // sF = sT Y'
-    wg_gemm(M, M, M, 1.0, MatRef{sY, M, 1}, MatRef{sF, M, 1}, 0.0, sT, M, pan);   // sT = Y (sT Y')
-    // F now in sT.  lt = lh - H' Y (V' lh + w)
-    if (tid < M) {
-      double u = vq[tid];
-      for (int i = 0; i < D; ++i) u = fma(V[(int64_t)i * M + tid], lh[i], u);
-      vr[tid] = u;
-    }
-    __syncthreads();
-    for (int i = tid; i < D; i += kT) {
-      double u = lh[i];
-      if (i < M)
-        for (int c = 0; c < M; ++c) u = fma(-sY[i * M + c], vr[c], u);
-      lt[i] = u;
-    }
-    __syncthreads();
-    // Z = Lt U0 = G3 - E U0[:M] - H'(E' U0) + H'(F U0[:M])  (into G3)
-    wg_gemm(D, M, M, -1.0, MatRef{E, M, 1}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
-    wg_gemm(M, M, D, -1.0, MatRef{E, 1, M}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
-    wg_gemm(M, M, M, 1.0, MatRef{sT, M, 1}, MatRef{U0, M, 1}, 1.0, G3, M, pan);
-    // posterior: H P H' = Sh - U0' Z ; H m = H m- - U0' lt
-    double* pPo = io.pP + kt * M * M;
-    for (int e = tid; e < M * M; e += kT) pPo[e] = U0[e];
-    __syncthreads();
-    wg_gemm(M, M, D, -1.0, MatRef{U0, 1, M}, MatRef{G3, M, 1}, 1.0, pPo, M, pan);
-    if (tid < M) {
-      double u = io.Hm[kt * M + tid];
-      for (int i = 0; i < D; ++i) u = fma(-U0[(int64_t)i * M + tid], lt[i], u);
-      io.pm[kt * M + tid] = u;
-    }
-    // Lt, then Lh <- (A (x) I)' Lt (A (x) I), lh <- (A (x) I)' lt
-    if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt], sA, sQ);
-    __syncthreads();
+    // AF = (A (x) I) F -> GT ;  P- = AF (A (x) I)' + Q (x) I -> sL ;  Dl = Ps' - P-
     for (int e = tid; e < M * M; e += kT) {
       const int a = e / M, c = e % M;
-      double blk[KT][KT];
+      double f[KT][KT], af[KT][KT];
+      for (int p = 0; p < KT; ++p)
+        for (int q = 0; q < KT; ++q) f[p][q] = F[(int64_t)(p * M + a) * D + q * M + c];
       for (int i = 0; i < KT; ++i)
-        for (int j = 0; j < KT; ++j) {
-          double v = Lh[(int64_t)(i * M + a) * D + j * M + c];
-          if (j == 0) v -= E[(int64_t)(i * M + a) * M + c];
-          if (i == 0) v -= E[(int64_t)(j * M + c) * M + a];
-          if (i == 0 && j == 0) v += sT[a * M + c];
-          blk[i][j] = v;
+        for (int q = 0; q < KT; ++q) {
+          double v = 0.0;
+          for (int p = 0; p < KT; ++p) v = fma(sA[i * KT + p], f[p][q], v);
+          af[i][q] = v;
+          GT[(int64_t)(i * M + a) * D + q * M + c] = v;
         }
       for (int i = 0; i < KT; ++i)
         for (int j = 0; j < KT; ++j) {
-          double s = 0.0;
-          for (int p = 0; p < KT; ++p)
-            for (int q = 0; q < KT; ++q) s = fma(sA[p * KT + i] * blk[p][q], sA[q * KT + j], s);
-          Lh[(int64_t)(i * M + a) * D + j * M + c] = s;
+          double v = (a == c) ? sQ[i * KT + j] : 0.0;
+          for (int q = 0; q < KT; ++q) v = fma(af[i][q], sA[j * KT + q], v);
+          const int64_t o = (int64_t)(i * M + a) * D + j * M + c;
+          sL[o] = v;
+          Dl[o] = Ps[o] - v;
         }
     }
     if (tid < M) {
       double x[KT];
-      for (int i = 0; i < KT; ++i) x[i] = lt[i * M + tid];
+      for (int i = 0; i < KT; ++i) x[i] = mf[i * M + tid];
       for (int i = 0; i < KT; ++i) {
-        double s = 0.0;
-        for (int p = 0; p < KT; ++p) s = fma(sA[p * KT + i], x[p], s);
-        lh[i * M + tid] = s;
+        double v = 0.0;
+        for (int p = 0; p < KT; ++p) v = fma(sA[i * KT + p], x[p], v);
+        mp[i * M + tid] = v;
       }
     }
+    wg_chol(sL, D, D);
+    wg_chol_solve(sL, D, D, GT, D, D, pan);  // GT = G'
+    // T1 = G Dl ; Ps = F + T1 G'
+    wg_gemm(D, D, D, 1.0, MatRef{GT, 1, D}, MatRef{Dl, D, 1}, 0.0, T1, D, pan);
+    for (int e = tid; e < D * D; e += kT) Ps[e] = F[e];
+    for (int e = tid; e < D; e += kT) dv[e] = ms[e] - mp[e];
+    __syncthreads();
+    wg_gemm(D, D, D, 1.0, MatRef{T1, D, 1}, MatRef{GT, D, 1}, 1.0, Ps, D, pan);
+    for (int i = tid; i < D; i += kT) {
+      double v = mf[i];
+      for (int q = 0; q < D; ++q) v = fma(GT[(int64_t)q * D + i], dv[q], v);
+      ms[i] = v;
+    }
+    __syncthreads();
+    emit(kt);
     __syncthreads();
   }
 }
@@ -701,8 +731,7 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
 
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 2 * kMaxM * (kMaxM + 1) +
                                                   kMaxKT * kMaxM + kMaxM + 4);
-constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + 3 * kMaxM * kMaxM + 2 * kMaxKT * kMaxM +
-                                                    2 * kMaxM);
+constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 4 * kMaxD);
 constexpr size_t kSitesSmem = sizeof(double) * (kMaxM * kMaxM + 2 * kMaxM + kRB * kMaxM + 2 * kRB + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
@@ -800,22 +829,22 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
 int64_t bn_st_workspace_bytes(const bn_st_dims* dims) {
   StDims dm;
   if (check_dims(dims, dm) != BN_OK) return -1;
-  // filter: P, V ; smoother: Lh, G1, G3, E, V  (per run)
-  const int64_t D = dm.D, M = dm.M;
-  return (int64_t)dm.K * (D * D + 4 * D * M) * (int64_t)sizeof(double) + 4096;
+  // per run: filter P (D x D) and V (D x M); smoother Ps, F, GT, Dl, T1 (5 x D x D)
+  const int64_t D = dm.D;
+  return (int64_t)dm.K * 5 * D * D * (int64_t)sizeof(double) + 4096;
 }
 
 int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* Lw,
-                 const double* zw, const double* ldl, double* U0, double* Y, double* w, double* Hm, double* ell_steps,
-                 double* loglik, double* filter_mean, void* workspace, void* stream) {
+                 const double* zw, const double* ldl, double* filter_mean, double* filter_cov, double* ell_steps,
+                 double* loglik, void* workspace, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
-  if (!hyp || !dt || !Lw || !zw || !ldl || !U0 || !Y || !w || !Hm || !ell_steps || !loglik || !workspace)
+  if (!hyp || !dt || !Lw || !zw || !ldl || !filter_mean || !filter_cov || !ell_steps || !loglik || !workspace)
     return st_fail(BN_EINVAL, "NULL argument to bn_st_filter");
   double* ws = (double*)workspace;
-  const int64_t D = dm.D, M = dm.M;
+  const int64_t D = dm.D;
   if (int rc = set_smem_limits()) return rc;
-  StFilterIO io{dt, Lw, zw, ldl, ws, ws + dm.K * D * D, U0, Y, w, Hm, ell_steps, filter_mean};
+  StFilterIO io{dt, Lw, zw, ldl, ws, ws + dm.K * D * D, filter_mean, filter_cov, ell_steps};
   hipStream_t st = (hipStream_t)stream;
   {
     BN_TIMED(KID_ST_FILTER, st);
@@ -830,22 +859,20 @@ int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* 
     hipLaunchKernelGGL(k_st_sum, dim3(dm.K), dim3(kT), 0, st, dm.T, ell_steps, loglik);
   }
   ST_HIP_CHECK(hipGetLastError());
-  (void)M;
   return BN_OK;
 }
 
-int bn_st_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* U0,
-                   const double* Y, const double* w, const double* Hm, double* post_mean, double* post_cov,
-                   void* workspace, void* stream) {
+int bn_st_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* filter_mean,
+                   const double* filter_cov, double* post_mean, double* post_cov, void* workspace, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
-  if (!hyp || !dt || !U0 || !Y || !w || !Hm || !post_mean || !post_cov || !workspace)
+  if (!hyp || !dt || !filter_mean || !filter_cov || !post_mean || !post_cov || !workspace)
     return st_fail(BN_EINVAL, "NULL argument to bn_st_smoother");
   if (int rc = set_smem_limits()) return rc;
   double* ws = (double*)workspace;
-  const int64_t D = dm.D, M = dm.M, K = dm.K;
-  StSmootherIO io{dt, U0, Y, w, Hm, ws, ws + K * D * D, ws + K * (D * D + D * M), ws + K * (D * D + 2 * D * M),
-                  ws + K * (D * D + 3 * D * M), post_mean, post_cov};
+  const int64_t D = dm.D, K = dm.K, DD = D * D;
+  StSmootherIO io{dt, filter_mean, filter_cov, ws, ws + K * DD, ws + 2 * K * DD, ws + 3 * K * DD, ws + 4 * K * DD,
+                  post_mean, post_cov};
   hipStream_t st = (hipStream_t)stream;
   BN_TIMED(KID_ST_SMOOTHER, st);
   if (dm.KT == 1)

@@ -10,7 +10,7 @@ Data flow of one inference step (inference.py:65-90), all on the device:
   k_st_sites(UPDATE)   posterior -> data (B m, diag(B P B') + diag C), new sites,
                        whitened projection L L' = B' diag(nat2) B + 1e-12 I, z = L^-1 B' nat1
   k_st_filter          Kalman filter of the projected pseudo observations (state 2M)
-  k_st_smoother        posterior H m, H P H' at every step
+  k_st_smoother        RTS smoother: posterior H m, H P H' at every step
 energy() (inference.py:130-154, 197-222) uses the stored posterior, the current sites
 and the current hypers: -(sum E_q[log p(y|f)] - (sum E_q[log N(pseudo_y|f, pseudo_var)] - log Z)).
 """
@@ -92,10 +92,8 @@ class SpaceTimeMarkovGP:
         self._Lw = torch.empty((K, T, M, M), **f64)
         self._zw = torch.empty((K, T, M), **f64)
         self._ldl = torch.empty((K, T), **f64)
-        self._U0 = torch.empty((K, T, self.D, M), **f64)
-        self._Yf = torch.empty((K, T, M, M), **f64)
-        self._w = torch.empty((K, T, M), **f64)
-        self._Hm = torch.empty((K, T, M), **f64)
+        self._fm = torch.empty((K, T, self.D), **f64)
+        self._fP = torch.empty((K, T, self.D * (self.D + 1) // 2), **f64)
         self._ell = torch.empty((K, T), **f64)
         self._loglik = torch.zeros(K, **f64)
         # posterior initialised as in basemodels.py:134-135 (zeros, identity)
@@ -170,8 +168,8 @@ class SpaceTimeMarkovGP:
         h, key = self._hyp()
         nat.check(nat.lib().bn_st_filter(
             ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt), nat.ptr(self._Lw), nat.ptr(self._zw),
-            nat.ptr(self._ldl), nat.ptr(self._U0), nat.ptr(self._Yf), nat.ptr(self._w), nat.ptr(self._Hm),
-            nat.ptr(self._ell), nat.ptr(self._loglik), None, nat.ptr(self._ws), nat.stream_ptr()))
+            nat.ptr(self._ldl), nat.ptr(self._fm), nat.ptr(self._fP), nat.ptr(self._ell), nat.ptr(self._loglik),
+            nat.ptr(self._ws), nat.stream_ptr()))
         self._ll_key = (self._proj_key, key[0], key[1])
         return h
 
@@ -180,9 +178,8 @@ class SpaceTimeMarkovGP:
         self._ensure_projection()
         h = self._run_filter()
         nat.check(nat.lib().bn_st_smoother(
-            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt), nat.ptr(self._U0), nat.ptr(self._Yf),
-            nat.ptr(self._w), nat.ptr(self._Hm), nat.ptr(self._pm), nat.ptr(self._pP), nat.ptr(self._ws),
-            nat.stream_ptr()))
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt), nat.ptr(self._fm), nat.ptr(self._fP),
+            nat.ptr(self._pm), nat.ptr(self._pP), nat.ptr(self._ws), nat.stream_ptr()))
         self._post_key = self._ll_key
 
     def _current_key(self):

@@ -281,21 +281,19 @@ int64_t bn_st_workspace_bytes(const bn_st_dims *dims);
 
 /* Kalman filter of the projected pseudo observations (kalman_filter,
  * ops.py:256-285, with y = (Lw Lw')^-1 Lw zw and noise (Lw Lw')^-1), one
- * workgroup per run.  dt [K,T] with dt[k,0] = 0.  Outputs for the smoother:
- * U0 [K,T,D,M] = P-[:, :M], Y [K,T,M,M], w [K,T,M], Hm [K,T,M] = H m-;
- * ell_steps [K,T] per-step log N terms, loglik [K] their sums;
- * filter_mean [K,T,D] optional (NULL to skip). */
+ * workgroup per run.  dt [K,T] with dt[k,0] = 0.  Outputs: filtered means
+ * filter_mean [K,T,D], filtered covariances filter_cov [K,T,D(D+1)/2] (upper
+ * triangle by rows), per-step log N terms ell_steps [K,T] and their sums loglik [K]. */
 int bn_st_filter(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *Lw,
-                 const double *zw, const double *ldl, double *U0, double *Y, double *w, double *Hm, double *ell_steps,
-                 double *loglik, double *filter_mean, void *workspace, void *stream);
+                 const double *zw, const double *ldl, double *filter_mean, double *filter_cov, double *ell_steps,
+                 double *loglik, void *workspace, void *stream);
 
 /* Smoothed posterior H m, H P H' at every step (rauch_tung_striebel_smoother,
- * ops.py:288-311, 357-380, as used by update_posterior, basemodels.py:689-706)
- * from the filter's outputs, by the modified Bryson-Frazier recursion.
- * post_mean [K,T,M], post_cov [K,T,M,M]. */
-int bn_st_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *U0,
-                   const double *Y, const double *w, const double *Hm, double *post_mean, double *post_cov,
-                   void *workspace, void *stream);
+ * ops.py:288-311, 357-380, as used by update_posterior, basemodels.py:689-706,
+ * with the next step's dt) from the filter's outputs.  post_mean [K,T,M],
+ * post_cov [K,T,M,M]. */
+int bn_st_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *filter_mean,
+                   const double *filter_cov, double *post_mean, double *post_cov, void *workspace, void *stream);
 
 /* per_run[k] = sum_t per_step[k, t] (FP64, fixed order). */
 int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *per_run, void *stream);

@@ -150,3 +150,21 @@ def test_spacetime_energy_keeps_posterior_under_new_hypers():
     ge, oe = float(gm.energy()), om.energy()
     assert abs(ge - oe) <= E_TOL * abs(oe), (ge, oe)
     _check_post(gm, om)
+
+
+def test_spacetime_precise_sites_stay_stable():
+    """Counts of a prior draw (seed 505, 60 steps x 64 locations, 4x4 inducing grid) drive
+    site precisions to ~1e4 within two VI steps; the smoothed covariance must still follow
+    the oracle's RTS recursion (an adjoint backward pass loses it geometrically here)."""
+    from bayesnewton_amd import models as M, workloads
+    d = workloads.c5(seed=505, T=60, R=64, grid=4)
+    r = d["r"].astype(np.float32).astype(np.float64)
+    gk, ok, gl, ol = _pair("VI", "poisson", 4, ls_t=10.0, ls_s=0.2)
+    gm = M.MarkovVariationalGP(gk, gl, d["t"], d["y"], R=r)
+    om = OM.MarkovGP(ok, ol, d["t"][:, None], d["y"], R=r, inference="VI")
+    for _ in range(4):
+        gm.inference(lr=1.0)
+        om.inference(lr=1.0)
+        _check_post(gm, om)
+        ge, oe = float(gm.energy()), om.energy()
+        assert abs(ge - oe) <= E_TOL * abs(oe), (ge, oe)
