@@ -13,11 +13,11 @@
 //                     pseudo observation in whitened form: W = B' diag(nat2) B + 1e-12 I = L L',
 //                     z = L^-1 B' nat1 (basemodels.py:676-687 without forming W^-1)
 //   k_st_filter       block per run, loop over T: predict, measurement update with
-//                     M = I + L' Sh L = Mc Mc', Y = L Mc^-T, w = Mc^-1 (z - L' H m-),
-//                     V = P-[:, :M] Y, m = m- + V w, P = P- - V V', log N terms.
+//                     U = P-[:, :M] L, Mm = I + L' Sh L (inverted in LDS by the sweep operator),
+//                     q = z - L' H m-, m = m- + U Mm^-1 q, P = P- - U Mm^-1 U', log N terms.
 //                     Equal to ops.py:159-171 with noise (W + 1e-12 I)^-1.
 //   k_st_smoother     block per run, loop back over T: the RTS smoother of ops.py:288-311
-//                     (G = P A' P-^-1 by a blocked Cholesky in LDS, P_s = P + G (P_s' - P-) G').
+//                     (G' = P-^-1 A P with P-^-1 by an in-LDS sweep, P_s = P + G (P_s' - P-) G').
 //                     An adjoint (Bryson-Frazier) backward pass avoids the factorisation but
 //                     its covariance error grows geometrically over long runs with
 //                     precise sites, so the filtered moments are stored instead.
@@ -59,65 +59,165 @@ struct MatRef {
   __device__ double at(int i, int j) const { return p[i * rs + j * cs]; }
 };
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 // C[i*ldc + j] = alpha * sum_k A(i,k) B(k,j) + beta * C[i*ldc + j], 0 <= i < m, 0 <= j < n.
-// The whole workgroup (kT threads) calls it; each thread owns a 4x4 piece of every
-// 64x64 output block.  Panels of A and B go through LDS (`sm`: 2 * kPK * kPS doubles).
+// The whole workgroup (kT threads = 4 waves) calls it; every 64x64 output block is
+// computed by FP64 matrix-core instructions, one 32x32 quadrant per wave.  Panels of A
+// and B go through LDS (`sm`: 2 * kPK * kPS doubles); the next panel is fetched from
+// memory into registers while the current one is multiplied.
 // Operands and C may live in global memory or LDS; C must not alias A or B.
 __device__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, double* C, int64_t ldc,
                         double* sm) {
   const int tid = threadIdx.x;
-  const int tr = tid >> 4, tc = tid & 15;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  __syncthreads();  // the caller's writes to the operands are complete
   double* As = sm;             // [kPK][kPS]: As[kk][ii] = A(i0+ii, k0+kk)
   double* Bs = sm + kPK * kPS;  // [kPK][kPS]: Bs[kk][jj] = B(k0+kk, j0+jj)
+  constexpr int NL = kPK * 64 / kT;  // panel elements loaded per thread
   for (int i0 = 0; i0 < m; i0 += 64) {
     for (int j0 = 0; j0 < n; j0 += 64) {
-      double acc[4][4];
+      f64x4 acc[2][2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
-      for (int k0 = 0; k0 < k; k0 += kPK) {
-        for (int e = tid; e < kPK * 64; e += kT) {
-          int ii, kk;
+        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+      // the next panel is fetched into registers while the current one is used
+      double ra[NL], rb[NL];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+          const int e = tid + q * kT;
+          int ii, kk, jj, kk2;
           if (A.cs == 1) { ii = e >> 4; kk = e & 15; } else { kk = e >> 6; ii = e & 63; }
-          const int gi = i0 + ii, gk = k0 + kk;
-          As[kk * kPS + ii] = (gi < m && gk < k) ? A.at(gi, gk) : 0.0;
-          int jj;
-          if (B.cs == 1) { kk = e >> 6; jj = e & 63; } else { jj = e >> 4; kk = e & 15; }
-          const int gj = j0 + jj, gk2 = k0 + kk;
-          Bs[kk * kPS + jj] = (gj < n && gk2 < k) ? B.at(gk2, gj) : 0.0;
+          if (B.cs == 1) { kk2 = e >> 6; jj = e & 63; } else { jj = e >> 4; kk2 = e & 15; }
+          const int gi = i0 + ii, gk = k0 + kk, gj = j0 + jj, gk2 = k0 + kk2;
+          ra[q] = (gi < m && gk < k) ? A.at(gi, gk) : 0.0;
+          rb[q] = (gj < n && gk2 < k) ? B.at(gk2, gj) : 0.0;
+        }
+      };
+      fetch(0);
+      for (int k0 = 0; k0 < k; k0 += kPK) {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+          const int e = tid + q * kT;
+          int ii, kk, jj, kk2;
+          if (A.cs == 1) { ii = e >> 4; kk = e & 15; } else { kk = e >> 6; ii = e & 63; }
+          if (B.cs == 1) { kk2 = e >> 6; jj = e & 63; } else { jj = e >> 4; kk2 = e & 15; }
+          As[kk * kPS + ii] = ra[q];
+          Bs[kk2 * kPS + jj] = rb[q];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < kPK; ++kk) {
-          double a[4], b[4];
+        if (k0 + kPK < k) fetch(k0 + kPK);
+        // wave w: 32 x 32 quadrant (rows (w/2)*32, cols (w%2)*32) as 2 x 2 blocks of
+        // v_mfma_f64_16x16x4f64: lane l holds A[l%16][k+l/16], B[k+l/16][l%16]
 #pragma unroll
-          for (int r = 0; r < 4; ++r) a[r] = As[kk * kPS + tr * 4 + r];
+        for (int kk = 0; kk < kPK; kk += 4) {
+          double af[2], bf[2];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) b[c] = Bs[kk * kPS + tc * 4 + c];
+          for (int bi = 0; bi < 2; ++bi) af[bi] = As[(kk + (lane >> 4)) * kPS + wr + bi * 16 + (lane & 15)];
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+          for (int bj = 0; bj < 2; ++bj) bf[bj] = Bs[(kk + (lane >> 4)) * kPS + wc + bj * 16 + (lane & 15)];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+          for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
         }
         __syncthreads();
       }
+      // output: block (bi, bj), register r -> row wr + bi*16 + lane/16 + 4r, column wc + bj*16 + lane%16
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + tr * 4 + r;
-        if (gi >= m) continue;
+      for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int gj = j0 + tc * 4 + c;
-          if (gj >= n) continue;
-          double v = alpha * acc[r][c];
-          if (beta != 0.0) v = fma(beta, C[gi * ldc + gj], v);
-          C[gi * ldc + gj] = v;
-        }
-      }
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gi = i0 + wr + bi * 16 + (lane >> 4) + 4 * r;
+            const int gj = j0 + wc + bj * 16 + (lane & 15);
+            if (gi < m && gj < n) {
+              double v = alpha * acc[bi][bj][r];
+              if (beta != 0.0) v = fma(beta, C[gi * ldc + gj], v);
+              C[gi * ldc + gj] = v;
+            }
+          }
     }
   }
   __syncthreads();
+}
+
+// In-place inverse of an n x n symmetric positive definite matrix a (LDS, stride ld) by
+// the sweep operator on every pivot: a := -a^-1.  Returns sum log pivots = log det a
+// (thread 0 only, when LOGDET).  Diagonal pivots of an SPD matrix stay positive, so no pivoting;
+// a non-positive pivot gives NaN, which propagates.  Thread t holds the BxB block at
+// rows (t / 16) B.., columns (t % 16) B.. in registers (n <= 16 B); per pivot the
+// owners of row k publish it to LDS (`row`: 2 x 16B doubles, double-buffered).
+template <int B, bool LOGDET>
+__device__ double wg_sweep_inv(double* a, int n, int ld, double* row) {
+  const int tid = threadIdx.x;
+  const int r0 = (tid >> 4) * B, c0 = (tid & 15) * B;
+  double v[B][B];
+  __syncthreads();  // the caller's writes to `a` are complete
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int gi = r0 + i, gj = c0 + j;
+      v[i][j] = (gi < n && gj < n) ? a[gi * ld + gj] : 0.0;
+    }
+  double logdet = 0.0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    double* rw = row + (k & 1) * (16 * B);
+    const bool own_row = k >= r0 && k < r0 + B;
+    const bool own_col = k >= c0 && k < c0 + B;
+    if (own_row) {
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        if (r0 + i == k) {
+#pragma unroll
+          for (int j = 0; j < B; ++j) rw[c0 + j] = v[i][j];
+        }
+    }
+    __syncthreads();
+    const double d = rw[k];
+    const double id = 1.0 / d;
+    if (LOGDET && tid == 0) logdet += log(d);
+    double ri[B], rj[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) { ri[i] = rw[r0 + i] * id; rj[i] = rw[c0 + i]; }
+    // generic rank-1 update, then the pivot row / column of the owners:
+    // a_kj := a_kj / d, a_ik := a_ik / d, a_kk := -1 / d
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) v[i][j] = fma(-ri[i], rj[j], v[i][j]);
+    if (own_row) {
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        if (r0 + i == k) {
+#pragma unroll
+          for (int j = 0; j < B; ++j) v[i][j] = rj[j] * id;
+        }
+    }
+    if (own_col) {
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (c0 + j == k) {
+#pragma unroll
+          for (int i = 0; i < B; ++i) v[i][j] = (r0 + i == k) ? -id : ri[i];
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int gi = r0 + i, gj = c0 + j;
+      if (gi < n && gj < n) a[gi * ld + gj] = v[i][j];
+    }
+  __syncthreads();
+  return logdet;
 }
 
 // In-place lower Cholesky of the n x n SPD matrix a (row stride ld, LDS), upper part zeroed.
@@ -447,6 +547,7 @@ struct StFilterIO {
   const double* zw;   // [K,T,M]
   const double* ldl;  // [K,T]
   double* P;          // per-run scratch [K,D,D]
+  double* U;          // per-run scratch [K,D,M]
   double* V;          // per-run scratch [K,D,M]
   double* fmean;      // [K,T,D] filtered means
   double* fcov;       // [K,T,D(D+1)/2] filtered covariances, packed upper triangle by rows
@@ -463,13 +564,14 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
   const int k = blockIdx.x;
   double* pan = smem;                        // 2*kPK*kPS
   double* sL = pan + 2 * kPK * kPS;          // M*M
-  double* b1 = sL + kMaxM * kMaxM;           // M*(M+1)
-  double* b2 = b1 + kMaxM * (kMaxM + 1);     // M*(M+1) (stride M+1)
-  double* vm = b2 + kMaxM * (kMaxM + 1);     // D mean
-  double* vq = vm + kMaxKT * kMaxM;          // M
+  double* sM = sL + kMaxM * kMaxM;           // M*M
+  double* row = sM + kMaxM * kMaxM;          // 2 M (pivot rows)
+  double* vm = row + 2 * kMaxM;              // D mean
+  double* vq = vm + kMaxD;                   // M
   double* red = vq + kMaxM;                  // 4
   __shared__ double sA[KT * KT], sQ[KT * KT];
   double* P = io.P + (int64_t)k * D * D;
+  double* U = io.U + (int64_t)k * D * M;
   double* V = io.V + (int64_t)k * D * M;
   const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
   // P0 = Pinf (x) I, m0 = 0
@@ -519,37 +621,33 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
     const double* L = io.Lw + kt * M * M;
     for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
     __syncthreads();
-    // X = Sh L -> b1 ; Mm = I + L' X -> b2 (stride M+1)
-    wg_gemm(M, M, M, 1.0, MatRef{P, D, 1}, MatRef{sL, M, 1}, 0.0, b1, M, pan);
-    wg_gemm(M, M, M, 1.0, MatRef{sL, 1, M}, MatRef{b1, M, 1}, 0.0, b2, M + 1, pan);
-    for (int e = tid; e < M; e += kT) b2[e * (M + 1) + e] += 1.0;
-    wg_chol(b2, M, M + 1);
-    // right-hand sides [L' | q] into b1 (stride M+1): q = z - L' H m-
-    for (int e = tid; e < M * M; e += kT) {
-      const int i = e / M, c = e % M;
-      b1[i * (M + 1) + c] = sL[c * M + i];
-    }
+    // U = P-[:, :M] L ;  Mm = I + L' Sh L = I + L' U[:M] ;  q = z - L' H m-
+    wg_gemm(D, M, M, 1.0, MatRef{P, D, 1}, MatRef{sL, M, 1}, 0.0, U, M, pan);
+    wg_gemm(M, M, M, 1.0, MatRef{sL, 1, M}, MatRef{U, M, 1}, 0.0, sM, M, pan);
+    for (int e = tid; e < M; e += kT) sM[e * M + e] += 1.0;
     if (tid < M) {
       double u = io.zw[kt * M + tid];
       for (int a = tid; a < M; ++a) u = fma(-sL[a * M + tid], vm[a], u);
-      b1[tid * (M + 1) + M] = u;
+      vq[tid] = u;
     }
-    wg_lower_solve(b2, M, M + 1, b1, M + 1, 1, M + 1);   // b1 = [Y' | w]
-    if (tid < M) vq[tid] = b1[tid * (M + 1) + M];
-    // log N: -1/2 (|w|^2 + M log 2pi + 2 sum log diag Mc - 2 ldl)
-    const double wq = (tid < M) ? vq[tid] * vq[tid] + 2.0 * log(b2[tid * (M + 1) + tid]) : 0.0;
-    const double s = wg_sum(wq, red);
-    if (tid == 0) io.ell[kt] = -0.5 * (s + M * kLog2Pi - 2.0 * io.ldl[kt]);
-    // V = U0 Y (Y(i,c) = b1[c*(M+1) + i])
-    wg_gemm(D, M, M, 1.0, MatRef{P, D, 1}, MatRef{b1, 1, M + 1}, 0.0, V, M, pan);
-    // m = m- + V w
+    const double ldM = wg_sweep_inv<kMaxM / 16, true>(sM, M, M, row);  // sM = -Mm^-1 (ldM: thread 0)
+    // log N = -1/2 (q' Mm^-1 q + M log 2pi + log det Mm - 2 ldl)
+    double qq = 0.0;
+    if (tid < M) {
+      double u = 0.0;
+      for (int c = 0; c < M; ++c) u = fma(sM[tid * M + c], vq[c], u);
+      qq = -vq[tid] * u;
+    }
+    const double quad = wg_sum(qq, red);
+    if (tid == 0) io.ell[kt] = -0.5 * (quad + M * kLog2Pi + ldM - 2.0 * io.ldl[kt]);
+    // V = U Mm^-1 ; m = m- + V q ; P = P- - V U'
+    wg_gemm(D, M, M, -1.0, MatRef{U, M, 1}, MatRef{sM, M, 1}, 0.0, V, M, pan);
     for (int i = tid; i < D; i += kT) {
       double s2 = vm[i];
       for (int c = 0; c < M; ++c) s2 = fma(V[(int64_t)i * M + c], vq[c], s2);
       vm[i] = s2;
     }
-    // P = P- - V V'
-    wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{V, 1, M}, 1.0, P, D, pan);
+    wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{U, 1, M}, 1.0, P, D, pan);
     for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
     double* fc = io.fcov + kt * ((int64_t)D * (D + 1) / 2);
     for (int e = tid; e < D * D; e += kT) {
@@ -579,54 +677,6 @@ struct StSmootherIO {
   double* pP;           // [K,T,M,M]
 };
 
-constexpr int kSB = 16;  // row block of the triangular solves
-
-// X := L^-T L^-1 X for X (n x ncol, row stride ldx, global) with L lower (n x n, LDS, stride ld).
-__device__ void wg_chol_solve(const double* L, int n, int ld, double* X, int ldx, int ncol, double* pan) {
-  const int tid = threadIdx.x;
-  for (int rb = 0; rb < n; rb += kSB) {
-    const int nb = min(kSB, n - rb);
-    if (rb > 0) wg_gemm(nb, ncol, rb, -1.0, MatRef{L + (int64_t)rb * ld, ld, 1}, MatRef{X, ldx, 1}, 1.0,
-                        X + (int64_t)rb * ldx, ldx, pan);
-    for (int c = tid; c < ncol; c += kT) {
-      double x[kSB];
-#pragma unroll
-      for (int i = 0; i < kSB; ++i) {
-        if (i < nb) {
-          double v = X[(int64_t)(rb + i) * ldx + c];
-#pragma unroll
-          for (int q = 0; q < kSB; ++q)
-            if (q < i) v = fma(-L[(rb + i) * ld + rb + q], x[q], v);
-          x[i] = v / L[(rb + i) * ld + rb + i];
-          X[(int64_t)(rb + i) * ldx + c] = x[i];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int rb = ((n - 1) / kSB) * kSB; rb >= 0; rb -= kSB) {
-    const int nb = min(kSB, n - rb);
-    if (rb + nb < n)
-      wg_gemm(nb, ncol, n - rb - nb, -1.0, MatRef{L + (int64_t)(rb + nb) * ld + rb, 1, ld},
-              MatRef{X + (int64_t)(rb + nb) * ldx, ldx, 1}, 1.0, X + (int64_t)rb * ldx, ldx, pan);
-    for (int c = tid; c < ncol; c += kT) {
-      double x[kSB];
-#pragma unroll
-      for (int i = kSB - 1; i >= 0; --i) {
-        if (i < nb) {
-          double v = X[(int64_t)(rb + i) * ldx + c];
-#pragma unroll
-          for (int q = 0; q < kSB; ++q)
-            if (q > i && q < nb) v = fma(-L[(rb + q) * ld + rb + i], x[q], v);
-          x[i] = v / L[(rb + i) * ld + rb + i];
-          X[(int64_t)(rb + i) * ldx + c] = x[i];
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
 template <int KT>
 __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmootherIO io) {
   extern __shared__ double smem[];
@@ -634,7 +684,8 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
   const int k = blockIdx.x;
   double* pan = smem;                        // 2*kPK*kPS
   double* sL = pan + 2 * kPK * kPS;          // D*D
-  double* ms = sL + kMaxD * kMaxD;           // D
+  double* row = sL + kMaxD * kMaxD;          // 2 D (pivot rows)
+  double* ms = row + 2 * kMaxD;              // D
   double* mf = ms + kMaxD;                   // D
   double* mp = mf + kMaxD;                   // D
   double* dv = mp + kMaxD;                   // D
@@ -700,17 +751,17 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
         mp[i * M + tid] = v;
       }
     }
-    wg_chol(sL, D, D);
-    wg_chol_solve(sL, D, D, GT, D, D, pan);  // GT = G'
-    // T1 = G Dl ; Ps = F + T1 G'
-    wg_gemm(D, D, D, 1.0, MatRef{GT, 1, D}, MatRef{Dl, D, 1}, 0.0, T1, D, pan);
+    (void)wg_sweep_inv<kMaxD / 16, false>(sL, D, D, row);  // sL = -(P-)^-1
+    // T1 = G' = (P-)^-1 A P ; GT = G (Ps' - P-) ; Ps = P + GT G'
+    wg_gemm(D, D, D, -1.0, MatRef{sL, D, 1}, MatRef{GT, D, 1}, 0.0, T1, D, pan);
+    wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Dl, D, 1}, 0.0, GT, D, pan);
     for (int e = tid; e < D * D; e += kT) Ps[e] = F[e];
     for (int e = tid; e < D; e += kT) dv[e] = ms[e] - mp[e];
     __syncthreads();
-    wg_gemm(D, D, D, 1.0, MatRef{T1, D, 1}, MatRef{GT, D, 1}, 1.0, Ps, D, pan);
+    wg_gemm(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{T1, D, 1}, 1.0, Ps, D, pan);
     for (int i = tid; i < D; i += kT) {
       double v = mf[i];
-      for (int q = 0; q < D; ++q) v = fma(GT[(int64_t)q * D + i], dv[q], v);
+      for (int q = 0; q < D; ++q) v = fma(T1[(int64_t)q * D + i], dv[q], v);
       ms[i] = v;
     }
     __syncthreads();
@@ -729,9 +780,8 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
   if (threadIdx.x == 0) out[k] = s;
 }
 
-constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 2 * kMaxM * (kMaxM + 1) +
-                                                  kMaxKT * kMaxM + kMaxM + 4);
-constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 4 * kMaxD);
+constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
+constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
 constexpr size_t kSitesSmem = sizeof(double) * (kMaxM * kMaxM + 2 * kMaxM + kRB * kMaxM + 2 * kRB + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
@@ -844,7 +894,8 @@ int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* 
   double* ws = (double*)workspace;
   const int64_t D = dm.D;
   if (int rc = set_smem_limits()) return rc;
-  StFilterIO io{dt, Lw, zw, ldl, ws, ws + dm.K * D * D, filter_mean, filter_cov, ell_steps};
+  StFilterIO io{dt, Lw, zw, ldl, ws, ws + dm.K * D * D, ws + dm.K * (D * D + D * dm.M), filter_mean, filter_cov,
+                ell_steps};
   hipStream_t st = (hipStream_t)stream;
   {
     BN_TIMED(KID_ST_FILTER, st);

@@ -152,10 +152,10 @@ class WarmStartMLP:
         nat.check(nat.lib().bn_mlp_to_bf16(nat.ptr(self.params), nat.ptr(self.wbf), nat.stream_ptr()))
 
 
-def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0):
+def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0, log=None):
     """inference(lr) + energy() until |E_k - E_{k-1}| <= tol |E_k| for every run.
     Returns (iterations, final energies [K], wall seconds); the GPU is synchronised
-    before the clock is read."""
+    before the clock is read.  `log(it, energies)` is called after every iteration."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prev = None
@@ -164,6 +164,8 @@ def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0):
         model.inference(lr=lr, return_terms=False)
         e = model.energy_per_run()
         it += 1
+        if log is not None:
+            log(it, e)
         if prev is not None and bool(torch.all(torch.abs(e - prev) <= tol * torch.abs(e))):
             break
         prev = e

@@ -229,7 +229,6 @@ def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize
         dev = torch.device(device)
         tr = torch.tensor(r, dtype=torch.float64, device=dev)
         tz = torch.tensor(z, dtype=torch.float64, device=dev)
-        tL = torch.tensor(Lzz, dtype=torch.float64, device=dev)
         s3 = math.sqrt(3.0) / ls_s
 
         def m32(d2):
@@ -237,7 +236,8 @@ def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize
             return (1.0 + d) * torch.exp(-d)
 
         Krz = m32(((tr[:, :, None, :] - tz[None, None]) ** 2).sum(-1))                 # [T, R, M]
-        Bt = torch.linalg.solve_triangular(tL, Krz.transpose(1, 2), upper=False)        # [T, M, R]
+        tLi = torch.tensor(np.linalg.solve(Lzz, np.eye(M)), dtype=torch.float64, device=dev)
+        Bt = torch.matmul(tLi, Krz.transpose(1, 2))                                     # Lzz^-1 Kzr [T, M, R]
         fk = torch.einsum("tmr,tm->tr", Bt, torch.tensor(u, dtype=torch.float64, device=dev))
         te = torch.tensor(eps, dtype=torch.float64, device=dev)
         jit = 1e-8 * torch.eye(R, dtype=torch.float64, device=dev)

@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
     ap.add_argument("--st-steps", type=int, default=2000, help="spacetime: time steps per run")
     ap.add_argument("--st-tol", type=float, default=1e-7, help="spacetime: relative energy change to stop")
+    ap.add_argument("--st-lr", type=float, default=0.2,
+                    help="spacetime: inference step size (lr_newton of the reference's 2-D Cox demo)")
     return ap.parse_args()
 
 
@@ -196,7 +198,9 @@ def bench_spacetime(args):
     del train
     L_ = nat.lib()
     L_.bn_timing_enable(1)
-    it_tr, e_tr, sec_tr = W.run_to_convergence(tm, tol=args.st_tol, max_iter=200)
+    it_tr, e_tr, sec_tr = W.run_to_convergence(
+        tm, tol=args.st_tol, max_iter=200, lr=args.st_lr,
+        log=lambda i, e: log(f"[spacetime] training batch iteration {i}: max energy {float(e.max()):.6f}"))
     kern_ms = {}
     for kid in range(L_.bn_num_kernel_ids()):
         tot, cnt = ctypes.c_double(), ctypes.c_int32()
@@ -228,7 +232,8 @@ def bench_spacetime(args):
     # held-out run (seed 1005): plain vs warm start, same convergence test
     d = workloads.c5(seed=1005, T=T, device=dev)
     plain = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
-    it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=200)
+    it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=200, lr=args.st_lr,
+                                            log=lambda i, e: log(f"[spacetime] plain iteration {i}: {float(e[0]):.8f}"))
     log(f"[spacetime] plain: {it_p} iterations, {sec_p:.2f}s, energy {float(e_p[0]):.6f}")
     del plain
     warm = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
@@ -237,7 +242,8 @@ def bench_spacetime(args):
     net.init_sites(warm)
     torch.cuda.synchronize()
     init_s = time.perf_counter() - c0
-    it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=200)
+    it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=200, lr=args.st_lr,
+                                            log=lambda i, e: log(f"[spacetime] warm iteration {i}: {float(e[0]):.8f}"))
     sec_w += init_s
     log(f"[spacetime] warm: {it_w} iterations, {sec_w:.2f}s (network {init_s:.3f}s), energy {float(e_w[0]):.6f}")
     out = {
@@ -250,7 +256,7 @@ def bench_spacetime(args):
         "data": "synthetic (seeded prior draws of the model, Poisson counts)",
         "config": {"workload": f"space-time Cox process: T={T}, R=256 locations/step, M=64 inducing (8x8), "
                                "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), MarkovVariationalGP",
-                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol},
+                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol, "lr": args.st_lr},
         "plain": {"iterations": it_p, "seconds": sec_p, "energy": float(e_p[0])},
         "warm": {"iterations": it_w, "seconds": sec_w, "network_seconds": init_s, "energy": float(e_w[0])},
         "energy_rel_diff": abs(float(e_w[0]) - float(e_p[0])) / abs(float(e_p[0])),
