@@ -373,112 +373,161 @@ struct StSiteIO {
   float* o_hess;
 };
 
-constexpr int kRB = 32;  // rows of B staged per LDS chunk in the projection
+// In-place lower Cholesky of an n x n SPD matrix a (LDS, stride ld), the upper part
+// zeroed; register blocks as in wg_sweep_inv, one barrier per pivot.  Returns
+// sum log diag L (thread 0 only).  A non-positive pivot gives NaN.
+template <int B>
+__device__ double wg_chol_reg(double* a, int n, int ld, double* row) {
+  const int tid = threadIdx.x;
+  const int r0 = (tid >> 4) * B, c0 = (tid & 15) * B;
+  double v[B][B];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int gi = r0 + i, gj = c0 + j;
+      v[i][j] = (gi < n && gj < n) ? a[gi * ld + gj] : 0.0;
+    }
+  double ldl = 0.0;
+  for (int k = 0; k < n; ++k) {
+    double* rw = row + (k & 1) * (16 * B);
+    if (k >= r0 && k < r0 + B) {
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        if (r0 + i == k) {
+#pragma unroll
+          for (int j = 0; j < B; ++j) rw[c0 + j] = v[i][j];
+        }
+    }
+    __syncthreads();
+    const double d = sqrt(rw[k]);
+    const double id = 1.0 / d;
+    if (tid == 0) ldl += log(d);
+    double ri[B], rj[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      ri[i] = (r0 + i > k) ? rw[r0 + i] * id : 0.0;
+      rj[i] = (c0 + i > k) ? rw[c0 + i] * id : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) v[i][j] = fma(-ri[i], rj[j], v[i][j]);
+    if (k >= c0 && k < c0 + B) {
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (c0 + j == k) {
+#pragma unroll
+          for (int i = 0; i < B; ++i)
+            if (r0 + i >= k) v[i][j] = (r0 + i == k) ? d : ri[i];
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int gi = r0 + i, gj = c0 + j;
+      if (gi < n && gj < n) a[gi * ld + gj] = (gj > gi) ? 0.0 : v[i][j];
+    }
+  __syncthreads();
+  return ldl;
+}
+
+constexpr int kRC = 64;  // rows of B per chunk
 
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
                                                  int ensure_psd) {
   extern __shared__ double smem[];
   const int M = dm.M, R = dm.R;
-  double* sP = smem;                    // M*M posterior covariance; later W / L
-  double* sm = sP + kMaxM * kMaxM;      // M posterior mean
+  double* pan = smem;                   // 2*kPK*kPS GEMM panels
+  double* sP = pan + 2 * kPK * kPS;     // M*M posterior covariance
+  double* sW = sP + kMaxM * kMaxM;      // M*M projection B' diag(nat2) B, then its factor
+  double* sC = sW + kMaxM * kMaxM;      // kRC*M rows of B
+  double* sX = sC + kRC * kMaxM;        // kRC*M rows of B P, then nat2-scaled rows of B
+  double* sm = sX + kRC * kMaxM;        // M posterior mean
   double* sv = sm + kMaxM;              // M: B' nat1 -> z
-  double* sB = sv + kMaxM;              // kRB*M staged rows of B
-  double* sn = sB + kRB * kMaxM;        // kRB*2 staged (nat1, nat2)
-  double* red = sn + 2 * kRB;           // 4
+  double* sn = sv + kMaxM;              // kRC: new nat1 of the chunk
+  double* row = sn + kRC;               // 2*kMaxM pivot rows
+  double* red = row + 2 * kMaxM;        // 4
   const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int64_t kt = (int64_t)k * dm.T + t;
   constexpr bool UPDATE = MODE == BN_ST_UPDATE;
   constexpr bool PROJECT = MODE != BN_ST_ENERGY;
-  const double* pP = io.pP + kt * M * M;
-  if (MODE != BN_ST_PROJECT) {
-    for (int e = tid; e < M * M; e += kT) sP[e] = pP[e];
+  constexpr bool SITES = MODE != BN_ST_PROJECT;
+  if (SITES) {
+    for (int e = tid; e < M * M; e += kT) sP[e] = io.pP[kt * M * M + e];
     for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
   }
-  __syncthreads();
+  for (int e = tid; e < M; e += kT) sv[e] = 0.0;
   double acc0 = 0.0, acc1 = 0.0;
-  for (int i = tid; i < (MODE == BN_ST_PROJECT ? 0 : R); i += kT) {
-    const int64_t s = kt * R + i;
-    const double* b = io.B + s * M;
-    double bl[kMaxM];
-    double mf = 0.0;
-    for (int j = 0; j < M; ++j) { bl[j] = b[j]; mf = fma(bl[j], sm[j], mf); }
-    double vf = io.cdiag[s];
-    for (int j = 0; j < M; ++j) {
-      double u = 0.0;
-      for (int q = 0; q < M; ++q) u = fma(sP[j * M + q], bl[q], u);
-      vf = fma(bl[j], u, vf);
-    }
-    const double yv = (double)io.y[s];
-    if (UPDATE) {
-      const double n1 = io.nat1[s], n2 = io.nat2[s];
-      SiteResult sr = site_step<METHOD>(lik, gh, yv, mf, vf, n1, n2, 1.0, ensure_psd != 0);
-      acc0 += fabs(sr.nat1 - n1);
-      acc1 += fabs(sr.nat2 - n2);
-      io.nat1[s] = (1.0 - lr) * n1 + lr * sr.nat1;
-      io.nat2[s] = (1.0 - lr) * n2 + lr * sr.nat2;
-      if (io.o_mean != nullptr) {
-        io.o_mean[s] = (float)sr.mean;
-        io.o_jac[s] = (float)sr.jac;
-        io.o_hess[s] = (float)sr.hess;
-      }
-    } else {
-      double t1, t2;
-      energy_terms<METHOD>(lik, gh, yv, mf, vf, 1.0, 1.0, 1.0, t1, t2);
-      acc0 += t1;
-    }
-  }
-  if (PROJECT) {
-    if (UPDATE) {
-      const double d1 = wg_sum(acc0, red), d2 = wg_sum(acc1, red);
-      if (tid == 0) { io.part0[kt] = d1; io.part1[kt] = d2; }
-    }
-    // W = B' diag(nat2) B + 1e-12 I (into sP), v = B' nat1; thread owns W entries e, e+kT, ...
-    double wacc[(kMaxM * kMaxM) / kT];
-    double vacc = 0.0;
-    constexpr int NW = (kMaxM * kMaxM) / kT;
-#pragma unroll
-    for (int q = 0; q < NW; ++q) wacc[q] = 0.0;
-    for (int i0 = 0; i0 < R; i0 += kRB) {
-      const int nr = min(kRB, R - i0);
-      __syncthreads();
-      for (int e = tid; e < nr * M; e += kT) sB[e] = io.B[(kt * R + i0) * M + e];
-      for (int e = tid; e < nr; e += kT) {
-        sn[2 * e] = io.nat1[kt * R + i0 + e];
-        sn[2 * e + 1] = io.nat2[kt * R + i0 + e];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        const int e = tid + q * kT;
-        const int a = e / kMaxM, c = e % kMaxM;
-        if (a < M && c < M && c <= a) {
-          double s = wacc[q];
-          for (int ii = 0; ii < nr; ++ii) s = fma(sB[ii * M + a] * sn[2 * ii + 1], sB[ii * M + c], s);
-          wacc[q] = s;
+  for (int i0 = 0; i0 < R; i0 += kRC) {
+    const int nr = min(kRC, R - i0);
+    __syncthreads();
+    for (int e = tid; e < kRC * M; e += kT) sC[e] = (e < nr * M) ? io.B[(kt * R + i0) * M + e] : 0.0;
+    if (SITES) {
+      // X = C P ; row i: mean_f = c_i . m, var_f = c_i . x_i + diag C
+      wg_gemm(kRC, M, M, 1.0, MatRef{sC, M, 1}, MatRef{sP, M, 1}, 0.0, sX, M, pan);
+      if (tid < nr) {
+        const int64_t s = kt * R + i0 + tid;
+        double mf = 0.0, vf = io.cdiag[s];
+        for (int j = 0; j < M; ++j) {
+          const double c = sC[tid * M + j];
+          mf = fma(c, sm[j], mf);
+          vf = fma(c, sX[tid * M + j], vf);
+        }
+        const double yv = (double)io.y[s];
+        if (UPDATE) {
+          const double n1 = io.nat1[s], n2 = io.nat2[s];
+          SiteResult sr = site_step<METHOD>(lik, gh, yv, mf, vf, n1, n2, 1.0, ensure_psd != 0);
+          acc0 += fabs(sr.nat1 - n1);
+          acc1 += fabs(sr.nat2 - n2);
+          const double a1 = (1.0 - lr) * n1 + lr * sr.nat1, a2 = (1.0 - lr) * n2 + lr * sr.nat2;
+          io.nat1[s] = a1;
+          io.nat2[s] = a2;
+          if (io.o_mean != nullptr) {
+            io.o_mean[s] = (float)sr.mean;
+            io.o_jac[s] = (float)sr.jac;
+            io.o_hess[s] = (float)sr.hess;
+          }
+        } else {
+          double t1, t2;
+          energy_terms<METHOD>(lik, gh, yv, mf, vf, 1.0, 1.0, 1.0, t1, t2);
+          acc0 += t1;
         }
       }
-      if (tid < M)
-        for (int ii = 0; ii < nr; ++ii) vacc = fma(sB[ii * M + tid], sn[2 * ii], vacc);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-      const int e = tid + q * kT;
-      const int a = e / kMaxM, c = e % kMaxM;
-      if (a < M && c < M && c <= a) {
-        const double w = wacc[q] + (a == c ? kPseudoJitter : 0.0);
-        sP[a * M + c] = w;
-        sP[c * M + a] = w;
+    if (PROJECT) {
+      __syncthreads();
+      // X = diag(nat2) C (rows of the chunk), v += C' nat1, W += C' X
+      for (int e = tid; e < kRC * M; e += kT) {
+        const int i = e / M;
+        sX[e] = (i < nr) ? io.nat2[kt * R + i0 + i] * sC[e] : 0.0;
       }
+      if (tid < kRC) sn[tid] = (tid < nr) ? io.nat1[kt * R + i0 + tid] : 0.0;
+      __syncthreads();
+      if (tid < M) {
+        double u = sv[tid];
+        for (int i = 0; i < nr; ++i) u = fma(sC[i * M + tid], sn[i], u);
+        sv[tid] = u;
+      }
+      wg_gemm(M, M, kRC, 1.0, MatRef{sC, 1, M}, MatRef{sX, M, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
     }
-    if (tid < M) sv[tid] = vacc;
-    wg_chol(sP, M, M);
-    wg_lower_solve(sP, M, M, sv, 1, 0, 1);
+  }
+  if (UPDATE) {
+    const double d1 = wg_sum(acc0, red), d2 = wg_sum(acc1, red);
+    if (tid == 0) { io.part0[kt] = d1; io.part1[kt] = d2; }
+  }
+  if (PROJECT) {
+    __syncthreads();
+    for (int e = tid; e < M; e += kT) sW[e * M + e] += kPseudoJitter;
+    const double ld = wg_chol_reg<kMaxM / 16>(sW, M, M, row);
+    wg_lower_solve(sW, M, M, sv, 1, 0, 1);
     double* Lo = io.Lw + kt * M * M;
-    for (int e = tid; e < M * M; e += kT) Lo[e] = sP[e];
+    for (int e = tid; e < M * M; e += kT) Lo[e] = sW[e];
     for (int e = tid; e < M; e += kT) io.zw[kt * M + e] = sv[e];
-    const double ld = wg_sum(tid < M ? log(sP[tid * M + tid]) : 0.0, red);
     if (tid == 0) io.ldl[kt] = ld;
   } else {
     const double e1 = wg_sum(acc0, red);
@@ -490,7 +539,6 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
       double u = z[tid];
       for (int a = tid; a < M; ++a) u = fma(-L[a * M + tid], sm[a], u);   // (z - L' m)_tid
       q2 = u * u;
-      // column tid of L: l; tr += l' P l
       for (int a = tid; a < M; ++a) {
         double pa = 0.0;
         for (int c = tid; c < M; ++c) pa = fma(sP[a * M + c], L[c * M + tid], pa);
@@ -782,7 +830,8 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
 
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
-constexpr size_t kSitesSmem = sizeof(double) * (kMaxM * kMaxM + 2 * kMaxM + kRB * kMaxM + 2 * kRB + 4);
+constexpr size_t kSitesSmem =
+    sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kMaxM + 2 * kMaxM + kRC + 2 * kMaxM + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
   if (d == nullptr) return st_fail(BN_EINVAL, "dims is NULL");
@@ -814,6 +863,16 @@ int set_smem_limits() {
                                    (int)kSmootherSmem));
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_smoother<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kSmootherSmem));
+#define ST_SITES_ATTR(MT, MD)                                                                              \
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_sites<MT, MD>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                   (int)kSitesSmem))
+  ST_SITES_ATTR(M_VI, BN_ST_UPDATE);
+  ST_SITES_ATTR(M_VI, BN_ST_ENERGY);
+  ST_SITES_ATTR(M_VI, BN_ST_PROJECT);
+  ST_SITES_ATTR(M_NEWTON, BN_ST_UPDATE);
+  ST_SITES_ATTR(M_NEWTON, BN_ST_ENERGY);
+  ST_SITES_ATTR(M_NEWTON, BN_ST_PROJECT);
+#undef ST_SITES_ATTR
   done = 1;
   return BN_OK;
 }
@@ -856,6 +915,7 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
   if (int rc = check_lik(lik, L)) return rc;
   GHTable gh;
   if (int rc = make_gh_table(gh_points, gh)) return rc;
+  if (int rc = set_smem_limits()) return rc;
   StSiteIO io{y, B, cdiag, post_mean, post_cov, nat1, nat2, Lw, zw, ldl, part0, part1, out_mean, out_jac, out_hess};
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(dm.T, dm.K);
