@@ -149,71 +149,62 @@ __device__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, d
 // In-place inverse of an n x n symmetric positive definite matrix a (LDS, stride ld) by
 // the sweep operator on every pivot: a := -a^-1.  Returns sum log pivots = log det a
 // (thread 0 only, when LOGDET).  Diagonal pivots of an SPD matrix stay positive, so no pivoting;
-// a non-positive pivot gives NaN, which propagates.  Thread t holds the BxB block at
-// rows (t / 16) B.., columns (t % 16) B.. in registers (n <= 16 B); per pivot the
-// owners of row k publish it to LDS (`row`: 2 x 16B doubles, double-buffered).
+// a non-positive pivot gives NaN, which propagates.  Thread t holds, in registers, the
+// B x B elements at rows t/16 + 16 i and columns t%16 + 16 j (cyclic, n <= 16 B): pivot
+// k = 16 ib + rr then sits in register row/column ib for every thread, a compile-time
+// index once ib is unrolled, so the pivot row and column fix-ups are predicated selects.
+// Per pivot the owners of row k publish it to LDS (`row`: 2 x 16B doubles,
+// double-buffered); by symmetry it is also column k.
 template <int B, bool LOGDET>
 __device__ double wg_sweep_inv(double* a, int n, int ld, double* row) {
   const int tid = threadIdx.x;
-  const int r0 = (tid >> 4) * B, c0 = (tid & 15) * B;
+  const int rg = tid >> 4, cg = tid & 15;
   double v[B][B];
   __syncthreads();  // the caller's writes to `a` are complete
 #pragma unroll
   for (int i = 0; i < B; ++i)
 #pragma unroll
     for (int j = 0; j < B; ++j) {
-      const int gi = r0 + i, gj = c0 + j;
+      const int gi = rg + 16 * i, gj = cg + 16 * j;
       v[i][j] = (gi < n && gj < n) ? a[gi * ld + gj] : 0.0;
     }
   double logdet = 0.0;
   __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    double* rw = row + (k & 1) * (16 * B);
-    const bool own_row = k >= r0 && k < r0 + B;
-    const bool own_col = k >= c0 && k < c0 + B;
-    if (own_row) {
+#pragma unroll
+  for (int ib = 0; ib < B; ++ib) {
+    const int kend = min(16, n - 16 * ib);
+    for (int rr = 0; rr < kend; ++rr) {
+      const int k = 16 * ib + rr;
+      double* rw = row + (k & 1) * (16 * B);
+      if (rg == rr) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) rw[cg + 16 * j] = v[ib][j];
+      }
+      __syncthreads();
+      const double d = rw[k];
+      const double id = 1.0 / d;
+      if (LOGDET && tid == 0) logdet += log(d);
+      double ri[B], rj[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) { ri[i] = rw[rg + 16 * i] * id; rj[i] = rw[cg + 16 * i]; }
+      // generic rank-1 update, then the pivot row / column of the owners:
+      // a_kj := a_kj / d, a_ik := a_ik / d, a_kk := -1 / d
 #pragma unroll
       for (int i = 0; i < B; ++i)
-        if (r0 + i == k) {
 #pragma unroll
-          for (int j = 0; j < B; ++j) rw[c0 + j] = v[i][j];
-        }
-    }
-    __syncthreads();
-    const double d = rw[k];
-    const double id = 1.0 / d;
-    if (LOGDET && tid == 0) logdet += log(d);
-    double ri[B], rj[B];
+        for (int j = 0; j < B; ++j) v[i][j] = fma(-ri[i], rj[j], v[i][j]);
+      const bool orow = rg == rr, ocol = cg == rr;
 #pragma unroll
-    for (int i = 0; i < B; ++i) { ri[i] = rw[r0 + i] * id; rj[i] = rw[c0 + i]; }
-    // generic rank-1 update, then the pivot row / column of the owners:
-    // a_kj := a_kj / d, a_ik := a_ik / d, a_kk := -1 / d
+      for (int j = 0; j < B; ++j) v[ib][j] = orow ? rj[j] * id : v[ib][j];
 #pragma unroll
-    for (int i = 0; i < B; ++i)
-#pragma unroll
-      for (int j = 0; j < B; ++j) v[i][j] = fma(-ri[i], rj[j], v[i][j]);
-    if (own_row) {
-#pragma unroll
-      for (int i = 0; i < B; ++i)
-        if (r0 + i == k) {
-#pragma unroll
-          for (int j = 0; j < B; ++j) v[i][j] = rj[j] * id;
-        }
-    }
-    if (own_col) {
-#pragma unroll
-      for (int j = 0; j < B; ++j)
-        if (c0 + j == k) {
-#pragma unroll
-          for (int i = 0; i < B; ++i) v[i][j] = (r0 + i == k) ? -id : ri[i];
-        }
+      for (int i = 0; i < B; ++i) v[i][ib] = ocol ? ((i == ib && orow) ? -id : ri[i]) : v[i][ib];
     }
   }
 #pragma unroll
   for (int i = 0; i < B; ++i)
 #pragma unroll
     for (int j = 0; j < B; ++j) {
-      const int gi = r0 + i, gj = c0 + j;
+      const int gi = rg + 16 * i, gj = cg + 16 * j;
       if (gi < n && gj < n) a[gi * ld + gj] = v[i][j];
     }
   __syncthreads();
