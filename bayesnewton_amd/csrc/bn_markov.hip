@@ -467,7 +467,7 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   }
   launch_smoother_scan<D>(plan, b, st);
   {
-    BN_TIMED(KID_SMOOTHER_PASS, st);
+    BN_TIMED(KID_SMOOTHER_MBF, st);
     hipLaunchKernelGGL((k_smoother_mbf<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
                        cs(b.sagg), cs(b.stcarry), CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true}, pmean,
                        pvar);
@@ -597,7 +597,7 @@ namespace {
 const char* const kKernelNames[KID_COUNT] = {
     "k_filter_reduce", "k_fscan_tile", "k_fscan_series", "k_filter_pass", "k_smoother_reduce",
     "k_sscan_tile", "k_sscan_series", "k_smoother_pass", "k_series_sum", "k_st_conditional",
-    "k_st_sites", "k_st_filter", "k_st_smoother", "k_st_sum"};
+    "k_st_sites", "k_st_filter", "k_st_smoother", "k_st_sum", "k_smoother_mbf"};
 struct TimingRec { int32_t kid; hipEvent_t a, b; };
 bool g_timing = false;
 std::vector<TimingRec> g_recs;

@@ -34,7 +34,7 @@ D = 5
 # (12 + 80 = 92 B), the smoother pass reads them back and writes mean / variance
 # (80 + 8 = 88 B), the chunk reduction reads dt, y, r (12 B).
 PASS_BYTES_PER_STEP = 180.0
-ALG_BYTES = {"k_filter_reduce": 12.0, "k_filter_pass": 92.0, "k_smoother_pass": 88.0}
+ALG_BYTES = {"k_filter_reduce": 12.0, "k_filter_pass": 92.0, "k_smoother_pass": 88.0, "k_smoother_mbf": 88.0}
 # Algorithmic FP64 flops per time step (dense d x d algebra, H a 1 x d row,
 # mul+add = 2), counted on the recurrence each kernel runs:
 #  k_filter_reduce (chunk filtering element, ops.py:183-219 folded step by step):
@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
