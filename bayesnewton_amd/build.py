@@ -16,9 +16,13 @@ ARCH = os.environ.get("BN_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions stay disabled (-packed-fp32-ops): this code is
 # FP64-dominated anyway and the packed forms are not trusted on this platform.
+# Matrix-core accumulators in VGPRs (-amdgpu-mfma-vgpr-form): with the default AGPR
+# form the register allocator copied the 64 accumulators of the 128 x 128 FP64 GEMM
+# between the two register files on every K panel.
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

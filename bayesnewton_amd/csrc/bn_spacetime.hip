@@ -40,6 +40,7 @@ constexpr int kT = 256;      // threads per workgroup
 constexpr int kMaxM = 64;    // inducing points per run
 constexpr int kMaxKT = 2;    // temporal state dim (Matern12 / Matern32)
 constexpr int kMaxD = kMaxKT * kMaxM;  // state dim
+static_assert(kT == 2 * kMaxD, "the smoother's mean update splits the workgroup in two halves of kMaxD");
 constexpr int kPK = 16;      // GEMM K-panel
 constexpr int kPS = 68;      // LDS panel row stride (doubles)
 constexpr double kPseudoJitter = 1e-12;  // basemodels.py:683
@@ -61,14 +62,101 @@ struct MatRef {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// C = alpha A B + beta Cin (m x n, inner k; Cin may be C, neither may alias A or B) for
+// m, n > 64: 128 x 128 output tiles, wave w owns the 64 x 64 quadrant (rows (w/2)*64,
+// columns (w%2)*64) as 4 x 4 blocks of v_mfma_f64_16x16x4f64.  K panels of 8 go
+// through LDS (row stride 136 doubles: the four k rows a fragment read touches fall on
+// disjoint bank halves), the next panel is fetched into registers while the current
+// one is used.  `sm` holds 2 * 8 * 136 doubles.
+constexpr int kBPK = 8, kBLD = 136;
+static_assert(2 * kBPK * kBLD <= 2 * kPK * kPS, "wg_gemm_128 panels fit the wg_gemm panel space");
+__device__ __noinline__ void wg_gemm_128(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, const double* Cin,
+                            int64_t ldci, double* C, int64_t ldc, double* sm) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  __syncthreads();  // the caller's writes to the operands are complete
+  double* As = sm;                // [kBPK][kBLD]: As[kk][ii] = A(i0+ii, k0+kk)
+  double* Bs = sm + kBPK * kBLD;  // [kBPK][kBLD]: Bs[kk][jj] = B(k0+kk, j0+jj)
+  constexpr int NL = kBPK * 128 / kT;
+  for (int i0 = 0; i0 < m; i0 += 128) {
+    for (int j0 = 0; j0 < n; j0 += 128) {
+      f64x4 acc[4][4];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f64x4{0.0, 0.0, 0.0, 0.0};
+      // per-thread panel elements: A(i0 + ai[q], k0 + ak[q]), B(k0 + bk[q], j0 + bj[q])
+      int ai[NL], ak[NL], bk[NL], bjx[NL];
+#pragma unroll
+      for (int q = 0; q < NL; ++q) {
+        const int e = tid + q * kT;
+        if (A.cs == 1) { ai[q] = e >> 3; ak[q] = e & 7; } else { ak[q] = e >> 7; ai[q] = e & 127; }
+        if (B.cs == 1) { bk[q] = e >> 7; bjx[q] = e & 127; } else { bjx[q] = e >> 3; bk[q] = e & 7; }
+      }
+      double ra[NL], rb[NL];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+          const int gi = i0 + ai[q], gk = k0 + ak[q], gj = j0 + bjx[q], gk2 = k0 + bk[q];
+          ra[q] = (gi < m && gk < k) ? A.p[gi * A.rs + gk * A.cs] : 0.0;
+          rb[q] = (gj < n && gk2 < k) ? B.p[gk2 * B.rs + gj * B.cs] : 0.0;
+        }
+      };
+      fetch(0);
+#pragma unroll 1
+      for (int k0 = 0; k0 < k; k0 += kBPK) {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+          As[ak[q] * kBLD + ai[q]] = ra[q];
+          Bs[bk[q] * kBLD + bjx[q]] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + kBPK < k) fetch(k0 + kBPK);
+#pragma unroll
+        for (int kk = 0; kk < kBPK; kk += 4) {
+          double af[4], bf[4];
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) af[bi] = As[(kk + (lane >> 4)) * kBLD + wr + bi * 16 + (lane & 15)];
+#pragma unroll
+          for (int bj = 0; bj < 4; ++bj) bf[bj] = Bs[(kk + (lane >> 4)) * kBLD + wc + bj * 16 + (lane & 15)];
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+              acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+        }
+        __syncthreads();
+      }
+      // block (bi, bj), register r -> row wr + bi*16 + lane/16 + 4r, column wc + bj*16 + lane%16
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gi = i0 + wr + bi * 16 + (lane >> 4) + 4 * r;
+            const int gj = j0 + wc + bj * 16 + (lane & 15);
+            if (gi < m && gj < n) {
+              double v = alpha * acc[bi][bj][r];
+              if (beta != 0.0) v = fma(beta, Cin[gi * ldci + gj], v);
+              C[gi * ldc + gj] = v;
+            }
+          }
+    }
+  }
+  __syncthreads();
+}
+
 // C[i*ldc + j] = alpha * sum_k A(i,k) B(k,j) + beta * C[i*ldc + j], 0 <= i < m, 0 <= j < n.
 // The whole workgroup (kT threads = 4 waves) calls it; every 64x64 output block is
-// computed by FP64 matrix-core instructions, one 32x32 quadrant per wave.  Panels of A
+// computed by FP64 matrix-core instructions, one 32x32 quadrant per wave
+// (wg_gemm dispatches m, n > 64 to wg_gemm_128).  Panels of A
 // and B go through LDS (`sm`: 2 * kPK * kPS doubles); the next panel is fetched from
 // memory into registers while the current one is multiplied.
 // Operands and C may live in global memory or LDS; C must not alias A or B.
-__device__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, double* C, int64_t ldc,
-                        double* sm) {
+__device__ void wg_gemm_small(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, double* C,
+                              int64_t ldc, double* sm) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -144,6 +232,16 @@ __device__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, d
     }
   }
   __syncthreads();
+}
+
+// Out-of-line entry for the filter and smoother: one call site per product keeps the
+// callers' register budgets free of the GEMM's accumulators.
+__device__ __noinline__ void wg_gemm(int m, int n, int k, double alpha, MatRef A, MatRef B, double beta, double* C,
+                                     int64_t ldc, double* sm) {
+  if (m > 64 && n > 64)
+    wg_gemm_128(m, n, k, alpha, A, B, beta, C, ldc, C, ldc, sm);
+  else
+    wg_gemm_small(m, n, k, alpha, A, B, beta, C, ldc, sm);
 }
 
 // In-place inverse of an n x n symmetric positive definite matrix a (LDS, stride ld) by
@@ -460,7 +558,7 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
     for (int e = tid; e < kRC * M; e += kT) sC[e] = (e < nr * M) ? io.B[(kt * R + i0) * M + e] : 0.0;
     if (SITES) {
       // X = C P ; row i: mean_f = c_i . m, var_f = c_i . x_i + diag C
-      wg_gemm(kRC, M, M, 1.0, MatRef{sC, M, 1}, MatRef{sP, M, 1}, 0.0, sX, M, pan);
+      wg_gemm_small(kRC, M, M, 1.0, MatRef{sC, M, 1}, MatRef{sP, M, 1}, 0.0, sX, M, pan);
       if (tid < nr) {
         const int64_t s = kt * R + i0 + tid;
         double mf = 0.0, vf = io.cdiag[s];
@@ -504,7 +602,7 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
         for (int i = 0; i < nr; ++i) u = fma(sC[i * M + tid], sn[i], u);
         sv[tid] = u;
       }
-      wg_gemm(M, M, kRC, 1.0, MatRef{sC, 1, M}, MatRef{sX, M, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
+      wg_gemm_small(M, M, kRC, 1.0, MatRef{sC, 1, M}, MatRef{sX, M, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
     }
   }
   if (UPDATE) {
@@ -689,9 +787,9 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
     wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{U, 1, M}, 1.0, P, D, pan);
     for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
     double* fc = io.fcov + kt * ((int64_t)D * (D + 1) / 2);
-    for (int e = tid; e < D * D; e += kT) {
-      const int i = e / D, j = e % D;
-      if (j >= i) fc[pidx(i, j, D)] = P[e];
+    for (int i = tid >> 6; i < D; i += kT / 64) {  // one wave per row
+      const int64_t o = pidx(i, i, D) - i;
+      for (int j = i + (tid & 63); j < D; j += 64) fc[o + j] = P[(int64_t)i * D + j];
     }
     __syncthreads();
   }
@@ -737,9 +835,8 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
   double* T1 = io.T1 + k * DD;
   const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
   auto unpack = [&](const double* pk, double* out) {
-    for (int e = tid; e < D * D; e += kT) {
-      const int i = e / D, j = e % D;
-      out[e] = pk[i <= j ? pidx(i, j, D) : pidx(j, i, D)];
+    for (int i = tid >> 6; i < D; i += kT / 64) {  // one wave per row
+      for (int j = tid & 63; j < D; j += 64) out[(int64_t)i * D + j] = pk[i <= j ? pidx(i, j, D) : pidx(j, i, D)];
     }
   };
   auto emit = [&](int64_t kt) {
@@ -794,14 +891,16 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
     // T1 = G' = (P-)^-1 A P ; GT = G (Ps' - P-) ; Ps = P + GT G'
     wg_gemm(D, D, D, -1.0, MatRef{sL, D, 1}, MatRef{GT, D, 1}, 0.0, T1, D, pan);
     wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Dl, D, 1}, 0.0, GT, D, pan);
-    for (int e = tid; e < D * D; e += kT) Ps[e] = F[e];
     for (int e = tid; e < D; e += kT) dv[e] = ms[e] - mp[e];
-    __syncthreads();
-    wg_gemm(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{T1, D, 1}, 1.0, Ps, D, pan);
-    for (int i = tid; i < D; i += kT) {
-      double v = mf[i];
-      for (int q = 0; q < D; ++q) v = fma(T1[(int64_t)q * D + i], dv[q], v);
-      ms[i] = v;
+    wg_gemm_128(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{T1, D, 1}, 1.0, F, D, Ps, D, pan);
+    {  // m_s = m + G dv, the two halves of the workgroup split q; mp is free again
+      const int i = tid & (kMaxD - 1), h = tid / kMaxD;
+      double v = 0.0;
+      if (i < D)
+        for (int q = h; q < D; q += 2) v = fma(T1[(int64_t)q * D + i], dv[q], v);
+      if (h == 1 && i < D) mp[i] = v;
+      __syncthreads();
+      if (h == 0 && i < D) ms[i] = mf[i] + v + mp[i];
     }
     __syncthreads();
     emit(kt);

@@ -154,8 +154,12 @@ class WarmStartMLP:
 
 def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0, log=None):
     """inference(lr) + energy() until |E_k - E_{k-1}| <= tol |E_k| for every run.
-    Returns (iterations, final energies [K], wall seconds); the GPU is synchronised
-    before the clock is read.  `log(it, energies)` is called after every iteration."""
+    A run whose energy is no longer finite has diverged (the plain iteration from the
+    N(0, 1e2) sites can overshoot past the range of exp on large counts): it leaves the
+    test and keeps iterating with the others; callers read it from the non-finite
+    energy.  Returns (iterations, final energies [K], wall seconds); the GPU is
+    synchronised before the clock is read.  `log(it, energies)` is called after every
+    iteration."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prev = None
@@ -166,8 +170,10 @@ def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0, log=None):
         it += 1
         if log is not None:
             log(it, e)
-        if prev is not None and bool(torch.all(torch.abs(e - prev) <= tol * torch.abs(e))):
-            break
+        if prev is not None:
+            done = (torch.abs(e - prev) <= tol * torch.abs(e)) | ~torch.isfinite(e)
+            if bool(torch.all(done)):
+                break
         prev = e
     torch.cuda.synchronize()
     return it, e, time.perf_counter() - t0

@@ -61,6 +61,11 @@ def parse():
                          "time to converge of the d=128 Cox process, learned warm start vs plain; gradient: "
                          "200 Adam steps on the energy of one 500k-step series")
     ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
+    ap.add_argument("--st-max-iter", type=int, default=600, help="spacetime: iteration cap of every run")
+    ap.add_argument("--st-save", default=None, help="spacetime: write the trained network and training record (.npz)")
+    ap.add_argument("--st-train-only", action="store_true", help="spacetime: stop after training (with --st-save)")
+    ap.add_argument("--st-load", default=None,
+                    help="spacetime: skip training, load the network and training record written by --st-save")
     ap.add_argument("--st-steps", type=int, default=2000, help="spacetime: time steps per run")
     ap.add_argument("--st-tol", type=float, default=1e-7, help="spacetime: relative energy change to stop")
     ap.add_argument("--st-lr", type=float, default=0.2,
@@ -186,53 +191,92 @@ def bench_spacetime(args):
                                       K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
 
     lik = L.Poisson(binsize=2.0)
-    t0 = time.perf_counter()
-    train = []
-    for k in range(args.st_train):
-        train.append(workloads.c5(seed=2000 + k, T=T, device=dev))
-        if (k + 1) % 20 == 0:
-            log(f"[spacetime] generated {k + 1}/{args.st_train} training runs ({time.perf_counter() - t0:.1f}s)")
-    log(f"[spacetime] {args.st_train} training runs generated in {time.perf_counter() - t0:.1f}s")
-    tm = S.SpaceTimeVariationalGP.from_runs(kern(), lik, [d["t"] for d in train], [d["r"] for d in train],
-                                            [d["y"] for d in train])
-    del train
-    L_ = nat.lib()
-    L_.bn_timing_enable(1)
-    it_tr, e_tr, sec_tr = W.run_to_convergence(
-        tm, tol=args.st_tol, max_iter=200, lr=args.st_lr,
-        log=lambda i, e: log(f"[spacetime] training batch iteration {i}: max energy {float(e.max()):.6f}"))
-    kern_ms = {}
-    for kid in range(L_.bn_num_kernel_ids()):
-        tot, cnt = ctypes.c_double(), ctypes.c_int32()
-        nat.check(L_.bn_timing_read(kid, ctypes.byref(tot), ctypes.byref(cnt)))
-        if cnt.value:
-            kern_ms[L_.bn_kernel_name(kid).decode()] = tot.value / cnt.value
-    L_.bn_timing_enable(0)
-    log(f"[spacetime] plain training batch: K={tm.K} converged in {it_tr} iterations, {sec_tr:.1f}s; "
-        f"per-launch ms {kern_ms}")
-    steps_per_pass = tm.K * tm.T
+    if args.st_load:
+        rec = np.load(args.st_load, allow_pickle=False)
+        net = W.WarmStartMLP(seed=0)
+        net.load_state_dict({"params": rec["params"], "norm": rec["norm"]})
+        it_tr, sec_tr, fit_s, n_div = int(rec["it_tr"]), float(rec["sec_tr"]), float(rec["fit_s"]), int(rec["n_div"])
+        hist = [float(v) for v in rec["hist"]]
+        kern_ms = dict(zip([str(k) for k in rec["kern_names"]], [float(v) for v in rec["kern_ms"]]))
+        steps_per_pass = int(rec["steps_per_pass"])
+        log(f"[spacetime] network and training record loaded from {args.st_load}")
+    else:
+        t0 = time.perf_counter()
+        train = []
+        for k in range(args.st_train):
+            train.append(workloads.c5(seed=2000 + k, T=T, device=dev))
+            if (k + 1) % 20 == 0:
+                log(f"[spacetime] generated {k + 1}/{args.st_train} training runs ({time.perf_counter() - t0:.1f}s)")
+        log(f"[spacetime] {args.st_train} training runs generated in {time.perf_counter() - t0:.1f}s")
+        tm = S.SpaceTimeVariationalGP.from_runs(kern(), lik, [d["t"] for d in train], [d["r"] for d in train],
+                                                [d["y"] for d in train])
+        del train
+        L_ = nat.lib()
+        L_.bn_timing_enable(1)
+        state = {"prev": None}
+
+        def tr_log(i, e):
+            ok = torch.isfinite(e)
+            msg = f"[spacetime] training batch iteration {i}: {int(ok.sum())}/{e.numel()} runs finite"
+            if state["prev"] is not None:
+                rel = torch.abs(e - state["prev"]) / torch.abs(e)
+                rel = rel[ok & torch.isfinite(rel)]
+                if rel.numel():
+                    msg += f", {int((rel <= args.st_tol).sum())} within tol, max rel change {float(rel.max()):.3e}"
+            state["prev"] = e
+            log(msg)
+
+        it_tr, e_tr, sec_tr = W.run_to_convergence(tm, tol=args.st_tol, max_iter=args.st_max_iter, lr=args.st_lr,
+                                                   log=tr_log)
+        conv = torch.isfinite(e_tr)
+        n_div = int((~conv).sum())
+        log(f"[spacetime] training batch: {n_div} of {tm.K} plain runs diverged (non-finite energy)")
+        kern_ms = {}
+        for kid in range(L_.bn_num_kernel_ids()):
+            tot, cnt = ctypes.c_double(), ctypes.c_int32()
+            nat.check(L_.bn_timing_read(kid, ctypes.byref(tot), ctypes.byref(cnt)))
+            if cnt.value:
+                kern_ms[L_.bn_kernel_name(kid).decode()] = tot.value / cnt.value
+        L_.bn_timing_enable(0)
+        log(f"[spacetime] plain training batch: K={tm.K} converged in {it_tr} iterations, {sec_tr:.1f}s; "
+            f"per-launch ms {kern_ms}")
+        steps_per_pass = tm.K * tm.T
+        X = W.site_features(tm)
+        Y = W.site_marginals(tm)
+        keep = conv.repeat_interleave(tm.T * tm.n_loc) & torch.isfinite(Y).all(1)  # converged runs only
+        X, Y = X[keep], Y[keep]
+        del tm
+        torch.cuda.empty_cache()
+        n_fit = min(X.shape[0], 4_000_000)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(0)
+        idx = torch.randperm(X.shape[0], generator=gen, device=dev)[:n_fit]
+        Xs, Ys = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+        del X, Y
+        net = W.WarmStartMLP(seed=0)
+        t0 = time.perf_counter()
+        hist = net.fit(Xs, Ys, epochs=3, batch=8192, lr=2e-3, log=log)
+        torch.cuda.synchronize()
+        fit_s = time.perf_counter() - t0
+        del Xs, Ys
+        torch.cuda.empty_cache()
+        if args.st_save:
+            sd = net.state_dict()
+            np.savez(args.st_save, params=sd["params"], norm=sd["norm"], it_tr=it_tr, sec_tr=sec_tr, fit_s=fit_s, n_div=n_div,
+                     hist=np.asarray(hist), kern_names=np.asarray(list(kern_ms.keys())),
+                     kern_ms=np.asarray(list(kern_ms.values())), steps_per_pass=steps_per_pass)
+            log(f"[spacetime] network and training record written to {args.st_save}")
+        if args.st_train_only:
+            print(json.dumps({"spacetime_training": {"runs": args.st_train, "diverged": n_div, "iterations": it_tr,
+                                                     "seconds": sec_tr,
+                                                     "fit_seconds": fit_s, "loss": hist,
+                                                     "kernel_ms_per_launch": kern_ms}}), flush=True)
+            return
     pass_ms = kern_ms.get("k_st_filter", 0.0) + kern_ms.get("k_st_smoother", 0.0)
-    X = W.site_features(tm)
-    Y = W.site_marginals(tm)
-    del tm
-    torch.cuda.empty_cache()
-    n_fit = min(X.shape[0], 4_000_000)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(0)
-    idx = torch.randperm(X.shape[0], generator=gen, device=dev)[:n_fit]
-    Xs, Ys = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
-    del X, Y
-    net = W.WarmStartMLP(seed=0)
-    t0 = time.perf_counter()
-    hist = net.fit(Xs, Ys, epochs=3, batch=8192, lr=2e-3, log=log)
-    torch.cuda.synchronize()
-    fit_s = time.perf_counter() - t0
-    del Xs, Ys
-    torch.cuda.empty_cache()
     # held-out run (seed 1005): plain vs warm start, same convergence test
     d = workloads.c5(seed=1005, T=T, device=dev)
     plain = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
-    it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=200, lr=args.st_lr,
+    it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=args.st_max_iter, lr=args.st_lr,
                                             log=lambda i, e: log(f"[spacetime] plain iteration {i}: {float(e[0]):.8f}"))
     log(f"[spacetime] plain: {it_p} iterations, {sec_p:.2f}s, energy {float(e_p[0]):.6f}")
     del plain
@@ -242,7 +286,7 @@ def bench_spacetime(args):
     net.init_sites(warm)
     torch.cuda.synchronize()
     init_s = time.perf_counter() - c0
-    it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=200, lr=args.st_lr,
+    it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=args.st_max_iter, lr=args.st_lr,
                                             log=lambda i, e: log(f"[spacetime] warm iteration {i}: {float(e[0]):.8f}"))
     sec_w += init_s
     log(f"[spacetime] warm: {it_w} iterations, {sec_w:.2f}s (network {init_s:.3f}s), energy {float(e_w[0]):.6f}")
@@ -256,11 +300,13 @@ def bench_spacetime(args):
         "data": "synthetic (seeded prior draws of the model, Poisson counts)",
         "config": {"workload": f"space-time Cox process: T={T}, R=256 locations/step, M=64 inducing (8x8), "
                                "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), MarkovVariationalGP",
-                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol, "lr": args.st_lr},
+                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol, "lr": args.st_lr,
+                   "max_iter": args.st_max_iter, "network_loaded_from": args.st_load},
         "plain": {"iterations": it_p, "seconds": sec_p, "energy": float(e_p[0])},
         "warm": {"iterations": it_w, "seconds": sec_w, "network_seconds": init_s, "energy": float(e_w[0])},
         "energy_rel_diff": abs(float(e_w[0]) - float(e_p[0])) / abs(float(e_p[0])),
-        "train": {"plain_iterations": it_tr, "plain_seconds": sec_tr, "fit_seconds": fit_s, "loss": hist},
+        "train": {"plain_iterations": it_tr, "plain_seconds": sec_tr, "plain_diverged": n_div, "fit_seconds": fit_s,
+                  "loss": hist},
         "batched_pass": {"runs": steps_per_pass // T, "filter_plus_smoother_ms": pass_ms,
                          "steps_per_s": steps_per_pass / (pass_ms / 1e3) if pass_ms else None,
                          "kernel_ms_per_launch": kern_ms},
