@@ -206,11 +206,25 @@ BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const flo
   mdl.lam2 = K2 > 0 ? comp_lambda<(K2 > 0 ? K2 : 1)>(ls2) : 0.0;
   mdl.var1 = v1;
   mdl.var2 = v2;
+  // the next step's inputs are loaded one step ahead, off the recurrence's critical path
+  float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;
+  if (k0 < L) {
+    const int64_t s0 = pl.slot(j, k0);
+    nx_dt = dt[s0];
+    nx_y = y[s0];
+    nx_r = r[s0];
+  }
   for (int32_t k = k0; k < L; ++k) {
-    const int64_t s = pl.slot(j, k);
+    const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+    if (k + 1 < L) {
+      const int64_t s1 = pl.slot(j, k + 1);
+      nx_dt = dt[s1];
+      nx_y = y[s1];
+      nx_r = r[s1];
+    }
     if (!(k == 0 && (fl & CH_FIRST))) {
       T A[D][D], Q[D][D];
-      mdl.discretise(dt[s], A, Q);
+      mdl.discretise(dtk, A, Q);
       // beta = A beta, Phi = A Phi (block-diagonal A; new rows of a block from old ones)
       T nb[D];
       BN_UNROLL for (int i = 0; i < D; ++i) {
@@ -257,10 +271,10 @@ BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const flo
       u[c] = su;
       hP[c] = sp;
     }
-    double Sd = (double)r[s];
+    double Sd = (double)rk;
     BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { Sd += (double)hP[i]; hb += beta[i]; }
     const double iS = 1.0 / Sd;
-    const T v = (T)((double)y[s] - (double)hb);
+    const T v = (T)((double)yk - (double)hb);
     const T iSf = (T)iS;
     const T vS = v * iSf;
     BN_UNROLL for (int i = 0; i < D; ++i) {
@@ -826,8 +840,22 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     }
     back((double)dt[sn], lt, Lt);
   }
+  float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;  // next (earlier) step's inputs, loaded one step ahead
+  if (L > 0) {
+    const int64_t s0 = pl.slot(j, L - 1);
+    nx_dt = dt[s0];
+    nx_y = y[s0];
+    nx_r = r[s0];
+  }
   for (int32_t k = L - 1; k >= 0; --k) {
     const int64_t s = pl.slot(j, k);
+    const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+    if (k > 0) {
+      const int64_t s1 = pl.slot(j, k - 1);
+      nx_dt = dt[s1];
+      nx_y = y[s1];
+      nx_r = r[s1];
+    }
     T m[D], P[NS];
     BN_UNROLL for (int i = 0; i < D; ++i) m[i] = (T)fin.mean[i * ns + s];
     BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = (T)fin.cov[q * ns + s];
@@ -839,9 +867,9 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     }
     T hph = T(0), hm = T(0);
     BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { hph += hP[i]; hm += m[i]; }
-    const double Sd = (double)r[s] + (double)hph;
+    const double Sd = (double)rk + (double)hph;
     const T iS = (T)(1.0 / Sd);
-    const T vS = (T)(((double)y[s] - (double)hm) / Sd);
+    const T vS = (T)(((double)yk - (double)hm) / Sd);
     BN_UNROLL for (int i = 0; i < D; ++i) kv[i] = hP[i] * iS;
     T kl = T(0);
     BN_UNROLL for (int i = 0; i < D; ++i) kl += kv[i] * lh[i];
@@ -872,7 +900,7 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     }
     out_mean[s] = (float)om;
     out_var[s] = (float)ov;
-    if (k > 0) back((double)dt[s], lt, Lt);
+    if (k > 0) back((double)dtk, lt, Lt);
   }
 }
 

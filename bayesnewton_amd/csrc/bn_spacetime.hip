@@ -525,6 +525,7 @@ __device__ double wg_chol_reg(double* a, int n, int ld, double* row) {
 }
 
 constexpr int kRC = 64;  // rows of B per chunk
+constexpr int kLC = kMaxM + 1;  // LDS row stride of a chunk: consecutive rows start in different banks
 
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
@@ -534,15 +535,16 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
   double* pan = smem;                   // 2*kPK*kPS GEMM panels
   double* sP = pan + 2 * kPK * kPS;     // M*M posterior covariance
   double* sW = sP + kMaxM * kMaxM;      // M*M projection B' diag(nat2) B, then its factor
-  double* sC = sW + kMaxM * kMaxM;      // kRC*M rows of B
-  double* sX = sC + kRC * kMaxM;        // kRC*M rows of B P, then nat2-scaled rows of B
-  double* sm = sX + kRC * kMaxM;        // M posterior mean
+  double* sC = sW + kMaxM * kMaxM;      // kRC rows of B (stride LC)
+  double* sX = sC + kRC * kLC;          // kRC rows of B P, then nat2-scaled rows of B (stride LC)
+  double* sm = sX + kRC * kLC;          // M posterior mean
   double* sv = sm + kMaxM;              // M: B' nat1 -> z
   double* sn = sv + kMaxM;              // kRC: new nat1 of the chunk
   double* row = sn + kRC;               // 2*kMaxM pivot rows
   double* red = row + 2 * kMaxM;        // 4
   const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
   const int64_t kt = (int64_t)k * dm.T + t;
+  const int LC = M + 1;
   constexpr bool UPDATE = MODE == BN_ST_UPDATE;
   constexpr bool PROJECT = MODE != BN_ST_ENERGY;
   constexpr bool SITES = MODE != BN_ST_PROJECT;
@@ -555,17 +557,20 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
   for (int i0 = 0; i0 < R; i0 += kRC) {
     const int nr = min(kRC, R - i0);
     __syncthreads();
-    for (int e = tid; e < kRC * M; e += kT) sC[e] = (e < nr * M) ? io.B[(kt * R + i0) * M + e] : 0.0;
+    for (int e = tid; e < kRC * M; e += kT) {
+      const int i = e / M, j = e - i * M;
+      sC[i * LC + j] = (e < nr * M) ? io.B[(kt * R + i0) * M + e] : 0.0;
+    }
     if (SITES) {
       // X = C P ; row i: mean_f = c_i . m, var_f = c_i . x_i + diag C
-      wg_gemm_small(kRC, M, M, 1.0, MatRef{sC, M, 1}, MatRef{sP, M, 1}, 0.0, sX, M, pan);
+      wg_gemm_small(kRC, M, M, 1.0, MatRef{sC, LC, 1}, MatRef{sP, M, 1}, 0.0, sX, LC, pan);
       if (tid < nr) {
         const int64_t s = kt * R + i0 + tid;
         double mf = 0.0, vf = io.cdiag[s];
         for (int j = 0; j < M; ++j) {
-          const double c = sC[tid * M + j];
+          const double c = sC[tid * LC + j];
           mf = fma(c, sm[j], mf);
-          vf = fma(c, sX[tid * M + j], vf);
+          vf = fma(c, sX[tid * LC + j], vf);
         }
         const double yv = (double)io.y[s];
         if (UPDATE) {
@@ -592,17 +597,17 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
       __syncthreads();
       // X = diag(nat2) C (rows of the chunk), v += C' nat1, W += C' X
       for (int e = tid; e < kRC * M; e += kT) {
-        const int i = e / M;
-        sX[e] = (i < nr) ? io.nat2[kt * R + i0 + i] * sC[e] : 0.0;
+        const int i = e / M, j = e - i * M;
+        sX[i * LC + j] = (i < nr) ? io.nat2[kt * R + i0 + i] * sC[i * LC + j] : 0.0;
       }
       if (tid < kRC) sn[tid] = (tid < nr) ? io.nat1[kt * R + i0 + tid] : 0.0;
       __syncthreads();
       if (tid < M) {
         double u = sv[tid];
-        for (int i = 0; i < nr; ++i) u = fma(sC[i * M + tid], sn[i], u);
+        for (int i = 0; i < nr; ++i) u = fma(sC[i * LC + tid], sn[i], u);
         sv[tid] = u;
       }
-      wg_gemm_small(M, M, kRC, 1.0, MatRef{sC, 1, M}, MatRef{sX, M, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
+      wg_gemm_small(M, M, kRC, 1.0, MatRef{sC, 1, LC}, MatRef{sX, LC, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
     }
   }
   if (UPDATE) {
@@ -620,20 +625,20 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
     if (tid == 0) io.ldl[kt] = ld;
   } else {
     const double e1 = wg_sum(acc0, red);
-    // whitened expected density of the pseudo observation
+    // whitened expected density of the pseudo observation:
+    // |z - L' m|^2 and tr(L' P L) = sum_ac L(a,c) (P L)(a,c), L in LDS (sW), P L in sX
     const double* L = io.Lw + kt * M * M;
     const double* z = io.zw + kt * M;
+    for (int e = tid; e < M * M; e += kT) sW[e] = L[e];
+    __syncthreads();
     double q2 = 0.0, tr = 0.0;
     if (tid < M) {
       double u = z[tid];
-      for (int a = tid; a < M; ++a) u = fma(-L[a * M + tid], sm[a], u);   // (z - L' m)_tid
+      for (int a = tid; a < M; ++a) u = fma(-sW[a * M + tid], sm[a], u);   // (z - L' m)_tid
       q2 = u * u;
-      for (int a = tid; a < M; ++a) {
-        double pa = 0.0;
-        for (int c = tid; c < M; ++c) pa = fma(sP[a * M + c], L[c * M + tid], pa);
-        tr = fma(L[a * M + tid], pa, tr);
-      }
     }
+    wg_gemm_small(M, M, M, 1.0, MatRef{sP, M, 1}, MatRef{sW, M, 1}, 0.0, sX, M, pan);
+    for (int e = tid; e < M * M; e += kT) tr = fma(sW[e], sX[e], tr);
     const double Q2 = wg_sum(q2, red), TR = wg_sum(tr, red);
     if (tid == 0) {
       io.part0[kt] = e1;
@@ -786,10 +791,18 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
     }
     wg_gemm(D, D, M, -1.0, MatRef{V, M, 1}, MatRef{U, 1, M}, 1.0, P, D, pan);
     for (int i = tid; i < D; i += kT) io.fmean[kt * D + i] = vm[i];
+    // P - V U' is symmetric only to rounding, and the asymmetry compounds through the next
+    // steps' Mm (whose inverse assumes symmetry): P := (P + P')/2, each pair (i < j) by the
+    // one lane that also writes its packed entry
     double* fc = io.fcov + kt * ((int64_t)D * (D + 1) / 2);
     for (int i = tid >> 6; i < D; i += kT / 64) {  // one wave per row
       const int64_t o = pidx(i, i, D) - i;
-      for (int j = i + (tid & 63); j < D; j += 64) fc[o + j] = P[(int64_t)i * D + j];
+      for (int j = i + (tid & 63); j < D; j += 64) {
+        const double v = 0.5 * (P[(int64_t)i * D + j] + P[(int64_t)j * D + i]);
+        P[(int64_t)i * D + j] = v;
+        P[(int64_t)j * D + i] = v;
+        fc[o + j] = v;
+      }
     }
     __syncthreads();
   }
@@ -874,8 +887,9 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
           double v = (a == c) ? sQ[i * KT + j] : 0.0;
           for (int q = 0; q < KT; ++q) v = fma(af[i][q], sA[j * KT + q], v);
           const int64_t o = (int64_t)(i * M + a) * D + j * M + c;
+          const int64_t ot = (int64_t)(j * M + c) * D + i * M + a;
           sL[o] = v;
-          Dl[o] = Ps[o] - v;
+          Dl[o] = 0.5 * (Ps[o] + Ps[ot]) - v;  // symmetric part of the next step's P_s
         }
     }
     if (tid < M) {
@@ -921,7 +935,7 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
 constexpr size_t kSitesSmem =
-    sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kMaxM + 2 * kMaxM + kRC + 2 * kMaxM + 4);
+    sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
   if (d == nullptr) return st_fail(BN_EINVAL, "dims is NULL");

@@ -152,28 +152,38 @@ class WarmStartMLP:
         nat.check(nat.lib().bn_mlp_to_bf16(nat.ptr(self.params), nat.ptr(self.wbf), nat.stream_ptr()))
 
 
-def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0, log=None):
+def run_to_convergence(model, tol=1e-7, max_iter=200, lr=1.0, log=None, record=None):
     """inference(lr) + energy() until |E_k - E_{k-1}| <= tol |E_k| for every run.
     A run whose energy is no longer finite has diverged (the plain iteration from the
     N(0, 1e2) sites can overshoot past the range of exp on large counts): it leaves the
     test and keeps iterating with the others; callers read it from the non-finite
     energy.  Returns (iterations, final energies [K], wall seconds); the GPU is
     synchronised before the clock is read.  `log(it, energies)` is called after every
-    iteration."""
+    iteration.  With a dict `record`, record["first_met"] holds per run the first
+    iteration at which the test held (0: never) and record["diverged_at"] the first
+    iteration with a non-finite energy (0: never)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     prev = None
     it = 0
+    first = torch.zeros(model.K, dtype=torch.int64, device=model.device)
+    div = torch.zeros_like(first)
     while it < max_iter:
         model.inference(lr=lr, return_terms=False)
         e = model.energy_per_run()
         it += 1
         if log is not None:
             log(it, e)
+        bad = ~torch.isfinite(e)
+        div = torch.where((div == 0) & bad, torch.full_like(div, it), div)
         if prev is not None:
-            done = (torch.abs(e - prev) <= tol * torch.abs(e)) | ~torch.isfinite(e)
-            if bool(torch.all(done)):
+            met = (torch.abs(e - prev) <= tol * torch.abs(e)) & ~bad
+            first = torch.where((first == 0) & met, torch.full_like(first, it), first)
+            if bool(torch.all(met | bad)):
                 break
         prev = e
+    if record is not None:
+        record["first_met"] = first.cpu().tolist()
+        record["diverged_at"] = div.cpu().tolist()
     torch.cuda.synchronize()
     return it, e, time.perf_counter() - t0

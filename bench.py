@@ -56,11 +56,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "spacetime", "gradient"],
+    ap.add_argument("--workload", default="batched", choices=["batched", "spacetime", "spacetime-pass", "gradient"],
                     help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
-                         "time to converge of the d=128 Cox process, learned warm start vs plain; gradient: "
+                         "time to converge of the d=128 Cox process, learned warm start vs plain; spacetime-pass: "
+                         "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; gradient: "
                          "200 Adam steps on the energy of one 500k-step series")
     ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
+    ap.add_argument("--st-runs", type=int, default=200, help="spacetime-pass: runs in the batch")
+    ap.add_argument("--st-heldout", type=int, default=1000, help="spacetime: first held-out seed")
+    ap.add_argument("--st-heldout-n", type=int, default=10, help="spacetime: number of held-out runs (one batch)")
     ap.add_argument("--st-max-iter", type=int, default=600, help="spacetime: iteration cap of every run")
     ap.add_argument("--st-save", default=None, help="spacetime: write the trained network and training record (.npz)")
     ap.add_argument("--st-train-only", action="store_true", help="spacetime: stop after training (with --st-save)")
@@ -172,8 +176,9 @@ def bench_spacetime(args):
     Matern32 in time (lengthscale 10) x Matern32 in space (lengthscale 0.2), T steps of
     R = 256 locations, Poisson(binsize 2)), MarkovVariationalGP.  Trains the warm-start
     network on the converged sites of --st-train plain runs (seeds 2000, 2001, ...) and
-    compares plain vs warm start on the held-out run of seed 1005: iterations and wall time
-    to the same convergence test (relative energy change <= --st-tol), final energies."""
+    compares plain vs warm start on --st-heldout-n held-out runs (seeds --st-heldout, ...),
+    each side as one batch: iterations and wall time to the same convergence test
+    (relative energy change <= --st-tol), final energies, diverged runs."""
     import numpy as np
     import torch
     from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, spacetime as S
@@ -273,43 +278,186 @@ def bench_spacetime(args):
                                                      "kernel_ms_per_launch": kern_ms}}), flush=True)
             return
     pass_ms = kern_ms.get("k_st_filter", 0.0) + kern_ms.get("k_st_smoother", 0.0)
-    # held-out run (seed 1005): plain vs warm start, same convergence test
-    d = workloads.c5(seed=1005, T=T, device=dev)
-    plain = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
+    # held-out runs: plain vs warm start, one batch each, same convergence test
+    seeds = list(range(args.st_heldout, args.st_heldout + args.st_heldout_n))
+    held = [workloads.c5(seed=sd, T=T, device=dev) for sd in seeds]
+
+    def batch():
+        return S.SpaceTimeVariationalGP.from_runs(kern(), lik, [d["t"] for d in held], [d["r"] for d in held],
+                                                  [d["y"] for d in held])
+
+    def ab_log(tag):
+        def f(i, e):
+            if i % 10 == 0 or i < 5:
+                ok = torch.isfinite(e)
+                log(f"[spacetime] {tag} iteration {i}: {int(ok.sum())}/{e.numel()} finite, "
+                    f"energies {[round(float(v), 4) for v in e[:4]]} ...")
+        return f
+
+    plain = batch()
+    rec_p = {}
     it_p, e_p, sec_p = W.run_to_convergence(plain, tol=args.st_tol, max_iter=args.st_max_iter, lr=args.st_lr,
-                                            log=lambda i, e: log(f"[spacetime] plain iteration {i}: {float(e[0]):.8f}"))
-    log(f"[spacetime] plain: {it_p} iterations, {sec_p:.2f}s, energy {float(e_p[0]):.6f}")
+                                            log=ab_log("plain"), record=rec_p)
+    log(f"[spacetime] plain: {it_p} iterations, {sec_p:.2f}s")
     del plain
-    warm = S.SpaceTimeVariationalGP(kern(), lik, d["t"], d["y"], d["r"])
+    warm = batch()
     torch.cuda.synchronize()
     c0 = time.perf_counter()
     net.init_sites(warm)
     torch.cuda.synchronize()
     init_s = time.perf_counter() - c0
+    rec_w = {}
     it_w, e_w, sec_w = W.run_to_convergence(warm, tol=args.st_tol, max_iter=args.st_max_iter, lr=args.st_lr,
-                                            log=lambda i, e: log(f"[spacetime] warm iteration {i}: {float(e[0]):.8f}"))
+                                            log=ab_log("warm"), record=rec_w)
     sec_w += init_s
-    log(f"[spacetime] warm: {it_w} iterations, {sec_w:.2f}s (network {init_s:.3f}s), energy {float(e_w[0]):.6f}")
+    log(f"[spacetime] warm: {it_w} iterations, {sec_w:.2f}s (network {init_s:.3f}s)")
+    ep, ew = e_p.cpu().numpy(), e_w.cpu().numpy()
+    runs = []
+    for i, sd in enumerate(seeds):
+        runs.append({"seed": sd, "plain_iterations": rec_p["first_met"][i] or None,
+                     "plain_diverged_at": rec_p["diverged_at"][i] or None,
+                     "warm_iterations": rec_w["first_met"][i] or None,
+                     "warm_diverged_at": rec_w["diverged_at"][i] or None,
+                     "plain_energy": float(ep[i]), "warm_energy": float(ew[i]),
+                     "energy_rel_diff": float(abs(ew[i] - ep[i]) / abs(ep[i])) if np.isfinite(ep[i]) else None})
+    both = [r for r in runs if r["plain_iterations"] and r["warm_iterations"]]
+    per_it_p, per_it_w = sec_p / it_p, (sec_w - init_s) / it_w
     out = {
         "metric": "time-to-converge, learned warm start vs plain (space-time Cox process, d=128)",
         "value": sec_p / sec_w,
-        "unit": "x (plain wall time / warm-start wall time)",
+        "unit": "x (plain wall time / warm-start wall time, held-out batch)",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "f64 filter/smoother/sites, bf16 weights + f32 accumulation in the network",
         "data": "synthetic (seeded prior draws of the model, Poisson counts)",
         "config": {"workload": f"space-time Cox process: T={T}, R=256 locations/step, M=64 inducing (8x8), "
                                "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), MarkovVariationalGP",
-                   "train_runs": args.st_train, "held_out_seed": 1005, "tol": args.st_tol, "lr": args.st_lr,
+                   "train_runs": args.st_train, "held_out_seeds": seeds, "tol": args.st_tol, "lr": args.st_lr,
                    "max_iter": args.st_max_iter, "network_loaded_from": args.st_load},
-        "plain": {"iterations": it_p, "seconds": sec_p, "energy": float(e_p[0])},
-        "warm": {"iterations": it_w, "seconds": sec_w, "network_seconds": init_s, "energy": float(e_w[0])},
-        "energy_rel_diff": abs(float(e_w[0]) - float(e_p[0])) / abs(float(e_p[0])),
+        "plain": {"iterations": it_p, "seconds": sec_p, "seconds_per_iteration": per_it_p,
+                  "diverged": sum(1 for r in runs if r["plain_diverged_at"])},
+        "warm": {"iterations": it_w, "seconds": sec_w, "network_seconds": init_s, "seconds_per_iteration": per_it_w,
+                 "diverged": sum(1 for r in runs if r["warm_diverged_at"])},
+        "converged_in_both": len(both),
+        "mean_iterations_plain_vs_warm": [float(np.mean([r["plain_iterations"] for r in both])) if both else None,
+                                          float(np.mean([r["warm_iterations"] for r in both])) if both else None],
+        "energy_rel_diff_max": max((r["energy_rel_diff"] for r in both), default=None),
+        "runs": runs,
         "train": {"plain_iterations": it_tr, "plain_seconds": sec_tr, "plain_diverged": n_div, "fit_seconds": fit_s,
                   "loss": hist},
         "batched_pass": {"runs": steps_per_pass // T, "filter_plus_smoother_ms": pass_ms,
                          "steps_per_s": steps_per_pass / (pass_ms / 1e3) if pass_ms else None,
                          "kernel_ms_per_launch": kern_ms},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def st_flops_per_step(D, M, R):
+    """Algorithmic FP64 flops per time step of the d = 128 kernels (mul + add = 2):
+    smoother: sweep inverse of P- (D pivots x D^2 updates) + three D^3 products;
+    filter: U = P-[:, :M] L, M x M products L'U and the sweep of Mm, V = U Mm^-1, P -= V U';
+    sites (one launch): C P (R x M x M) and B' diag(n2) B (M x M x R)."""
+    return {"k_st_smoother": 2.0 * D ** 3 + 3 * 2.0 * D ** 3,
+            "k_st_filter": 2.0 * (2 * D * M * M + 2 * M ** 3 + D * D * M),
+            "k_st_sites": 2.0 * (2 * R * M * M)}
+
+
+def bench_spacetime_pass(args):
+    """Throughput of the d = 128 space-time path: --st-runs runs of the Cox-process shape
+    (T = --st-steps, R = 256, 8x8 inducing grid, Matern32 x Matern32), one VI iteration per
+    step (site update, projection, filter, smoother, energy), lr = --st-lr.  `value` is
+    filter + smoother time-steps per second over the batch; `roofline` is the smoother's
+    algorithmic FP64 rate (st_flops_per_step) against MI355X's FP64 peak."""
+    import numpy as np
+    import torch
+    from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, spacetime as S, workloads
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    T = args.st_steps
+    g = np.linspace(0.0, 1.0, 8)
+    za, zb = np.meshgrid(g, g)
+    z = np.stack([za.reshape(-1), zb.reshape(-1)], 1)
+
+    def kern():
+        return K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=10.0),
+                                      K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
+
+    runs = [workloads.c5(seed=2000 + k, T=T, device=dev) for k in range(args.st_runs)]
+    m = S.SpaceTimeVariationalGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
+                                           [d["r"] for d in runs], [d["y"] for d in runs])
+    del runs
+    log(f"[spacetime-pass] {m.K} runs x {T} steps ready")
+
+    def step():
+        m.inference(lr=args.st_lr, return_terms=False)
+        m.energy_per_run()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    L_ = nat.lib()
+    L_.bn_timing_enable(1)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"[spacetime-pass] iteration {i + 1}/{args.steps}")
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    kern_ms = {}
+    for kid in range(L_.bn_num_kernel_ids()):
+        tot, cnt = ctypes.c_double(), ctypes.c_int32()
+        nat.check(L_.bn_timing_read(kid, ctypes.byref(tot), ctypes.byref(cnt)))
+        if cnt.value:
+            kern_ms[L_.bn_kernel_name(kid).decode()] = (tot.value / cnt.value, cnt.value)
+    L_.bn_timing_enable(0)
+    steps = m.K * m.T
+    fs_ms = kern_ms["k_st_filter"][0] + kern_ms["k_st_smoother"][0]
+    flops = st_flops_per_step(m.D, m.M, m.n_loc)
+    dom = max(("k_st_sites", "k_st_filter", "k_st_smoother"), key=lambda k: kern_ms[k][0] * kern_ms[k][1])
+    dom_ms = kern_ms[dom][0]
+    achieved = flops[dom] * steps / (dom_ms / 1e3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (FP64 NumPy) on a bounded sample: one run, the first 40 steps
+        from oracle import kernels as OK, likelihoods as OL, models as OM
+        d = workloads.c5(seed=2000, T=40)
+        ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 10.0), OK.Matern32(1.0, 0.2), z, sparse=True,
+                                     conditional="Full")
+        om = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI")
+        c0 = time.perf_counter()
+        n_it = 3
+        for _ in range(n_it):
+            om.inference(lr=args.st_lr)
+            om.energy()
+        csec = (time.perf_counter() - c0) / n_it
+        cpu = {"value": 40 / csec, "unit": "time-steps/s (whole VI iteration incl. energy)",
+               "cores": cpu_threads(), "kind": "port",
+               "sample": f"oracle/models.py MarkovGP (FP64 NumPy, BLAS threads = cores), one run of 40 steps, "
+                         f"{n_it} VI iterations + energy, {csec:.2f}s per iteration"}
+    out = {
+        "metric": "space-time VI iteration: time-steps/s over a batch of d=128 runs (filter + smoother kernels)",
+        "value": steps / (fs_ms / 1e3),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded prior draws of the model, Poisson counts)",
+        "config": {"workload": f"space-time Cox process: {m.K} runs, T={T}, R={m.n_loc}, M={m.M} (8x8), d={m.D}, "
+                               "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), one VI iteration per step",
+                   "runs": m.K, "time_steps": T, "state_dim": m.D},
+        "iteration_time_steps_per_s": steps / sec,
+        "roofline": {"bound": "mfma", "compute": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFS, "traffic": None, "kernel": dom,
+                     "alg_flops_per_step": flops[dom], "steps_per_launch": steps, "avg_launch_ms": dom_ms},
+        "kernel_ms_per_launch": {k: v[0] for k, v in kern_ms.items()},
+        "kernel_tflops": {k: flops[k] * steps / (kern_ms[k][0] / 1e3) / 1e12 for k in flops if k in kern_ms},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -347,6 +495,9 @@ def main():
         return
     if args.workload == "spacetime":
         bench_spacetime(args)
+        return
+    if args.workload == "spacetime-pass":
+        bench_spacetime_pass(args)
         return
     if args.workload == "gradient":
         bench_gradient(args)
