@@ -10,8 +10,9 @@
 // with one linear part J_k for every hyperparameter.  A lane propagates, over
 // its chunk, the NZ = D + D(D+1)/2 unit tangents (the chunk's linear map M and
 // functional f) and the NH hyperparameter tangents started at zero (sources o,
-// constants c); a per-series pass over the chunks then applies
-// z <- M_j z + o_j, grad += f_j . z + c_j.  Nothing is stored per time step.
+// constants c).  Applying the chunks in order, z <- M_j z + o_j, grad += f_j . z + c_j,
+// is a composition of affine maps, which bn_markov.hip evaluates as an associative
+// reduction (k_grad_group, k_grad_total).  Nothing is stored per time step.
 #pragma once
 #include "bn_lanes.h"
 
@@ -325,33 +326,6 @@ BN_HD void lane_grad_chunk(const PlanView& pl, const HyperView& hv, const float*
   for (int c = 0; c < NZ; ++c) out.agg[(int64_t)(f++) * st + j] = first_chunk ? 0.0 : acc[c];
   for (int h = 0; h < NH; ++h) out.agg[(int64_t)(f++) * st + j] = acc[NZ + h];
   out.ell[j] = ell;
-}
-
-// One thread per (series, hyperparameter): apply the chunk maps in order.
-template <int K1, int K2>
-BN_HD void lane_grad_series(const PlanView& pl, const double* __restrict__ agg, int64_t stride, int32_t b, int32_t h,
-                            double* __restrict__ grad) {
-  using GD = GradDims<K1, K2>;
-  constexpr int NZ = GD::NZ, NH = GD::NH;
-  double z[NZ];
-  for (int q = 0; q < NZ; ++q) z[q] = 0.0;
-  double g = 0.0;
-  const int32_t c0 = pl.series_first_chunk[b], nc = pl.series_nchunks[b];
-  for (int32_t jj = 0; jj < nc; ++jj) {
-    const int64_t j = c0 + jj;
-    const int fo = NZ * NZ + h * NZ, ff = NZ * NZ + NH * NZ, fc = ff + NZ + h;
-    double t = agg[(int64_t)fc * stride + j];
-    for (int q = 0; q < NZ; ++q) t += agg[(int64_t)(ff + q) * stride + j] * z[q];
-    g += t;
-    double nz[NZ];
-    for (int p = 0; p < NZ; ++p) {
-      double u = agg[(int64_t)(fo + p) * stride + j];
-      for (int q = 0; q < NZ; ++q) u += agg[(int64_t)(q * NZ + p) * stride + j] * z[q];
-      nz[p] = u;
-    }
-    for (int p = 0; p < NZ; ++p) z[p] = nz[p];
-  }
-  grad[(int64_t)b * NH + h] = g;
 }
 
 }  // namespace bn

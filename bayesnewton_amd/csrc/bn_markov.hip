@@ -14,6 +14,7 @@
 // A series that fits in one chunk skips both scans (the sequential filter).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -249,12 +250,111 @@ __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv
   lane_grad_chunk<K1, K2>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
 }
 
+// The chunk elements e = (M, O, f, c) of a series compose associatively (chunk order):
+//   (n o a) = (Mn Ma, Mn Oa + On, fa + Ma' fn, ca + cn + Oa' fn),
+// and the gradient is c of the whole series' composition (its incoming tangent is 0).
+// Two levels: k_grad_group composes groups of kGradGroup consecutive chunks of a series
+// in parallel (one workgroup per group, the result replaces the group's first chunk),
+// k_grad_total composes a series' group elements.  Layout of e: M column-major (NZ*NZ),
+// O (NH columns of NZ), f (NZ), c (NH); field q of chunk j at agg[q * stride + j].
+constexpr int kGradGroup = 64;
+
+template <int NZ, int NH>
+__device__ void compose_elem(const double* __restrict__ a, const double* __restrict__ n, double* __restrict__ out) {
+  constexpr int MM = NZ * NZ, FO = MM, FF = MM + NH * NZ, FC = FF + NZ, NF = FC + NH;
+  for (int e = threadIdx.x; e < NF; e += blockDim.x) {
+    double v;
+    if (e < MM) {
+      const int q = e / NZ, p = e - q * NZ;
+      v = 0.0;
+      for (int k = 0; k < NZ; ++k) v = fma(n[k * NZ + p], a[q * NZ + k], v);
+    } else if (e < FF) {
+      const int h = (e - FO) / NZ, p = (e - FO) - h * NZ;
+      v = n[e];
+      for (int k = 0; k < NZ; ++k) v = fma(n[k * NZ + p], a[FO + h * NZ + k], v);
+    } else if (e < FC) {
+      const int q = e - FF;
+      v = a[e];
+      for (int k = 0; k < NZ; ++k) v = fma(a[q * NZ + k], n[FF + k], v);
+    } else {
+      const int h = e - FC;
+      v = a[e] + n[e];
+      for (int k = 0; k < NZ; ++k) v = fma(a[FO + h * NZ + k], n[FF + k], v);
+    }
+    out[e] = v;
+  }
+}
+
+// Composes `cnt` elements at chunk indices j0, j0 + step, ... into acc (LDS, NF doubles);
+// the next element is read from memory into registers while the current one is composed.
+template <int NZ, int NH, int NT>
+__device__ const double* compose_run(const double* __restrict__ agg, int64_t stride, int64_t j0, int64_t step,
+                                     int32_t cnt, double* buf) {
+  constexpr int NF = NZ * NZ + NH * NZ + NZ + NH;
+  constexpr int NR = (NF + NT - 1) / NT;
+  double* ac = buf;
+  double* nx = buf + NF;
+  double* out = buf + 2 * NF;
+  double pre[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int e = threadIdx.x + r * NT;
+    if (e < NF) ac[e] = agg[(int64_t)e * stride + j0];
+    pre[r] = (cnt > 1 && e < NF) ? agg[(int64_t)e * stride + j0 + step] : 0.0;
+  }
+  for (int32_t i = 1; i < cnt; ++i) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int e = threadIdx.x + r * NT;
+      if (e < NF) nx[e] = pre[r];
+      pre[r] = (i + 1 < cnt && e < NF) ? agg[(int64_t)e * stride + j0 + (int64_t)(i + 1) * step] : 0.0;
+    }
+    __syncthreads();
+    compose_elem<NZ, NH>(ac, nx, out);
+    __syncthreads();
+    double* t = ac;
+    ac = out;
+    out = t;
+  }
+  __syncthreads();
+  return ac;
+}
+
 template <int K1, int K2>
-__global__ void k_grad_series(PlanView pl, const double* __restrict__ agg, int64_t stride, double* __restrict__ grad) {
-  constexpr int NH = GradDims<K1, K2>::NH;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)pl.n_series * NH) return;
-  lane_grad_series<K1, K2>(pl, agg, stride, (int32_t)(t / NH), (int32_t)(t % NH), grad);
+__global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_grad_group(PlanView pl,
+                                                                                      double* __restrict__ agg,
+                                                                                      int64_t stride) {
+  using GD = GradDims<K1, K2>;
+  constexpr int NT = GD::NF <= 128 ? 128 : 256;
+  __shared__ double buf[3 * GD::NF];
+  const int32_t b = blockIdx.x;
+  const int32_t first = blockIdx.y * kGradGroup;
+  const int32_t nc = pl.series_nchunks[b];
+  if (first >= nc) return;
+  const int64_t j0 = (int64_t)pl.series_first_chunk[b] + first;
+  const int32_t cnt = min(kGradGroup, nc - first);
+  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, j0, 1, cnt, buf);
+  for (int e = threadIdx.x; e < GD::NF; e += NT) agg[(int64_t)e * stride + j0] = ac[e];
+}
+
+template <int K1, int K2>
+__global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_grad_total(PlanView pl,
+                                                                                      const double* __restrict__ agg,
+                                                                                      int64_t stride,
+                                                                                      double* __restrict__ grad) {
+  using GD = GradDims<K1, K2>;
+  constexpr int NT = GD::NF <= 128 ? 128 : 256;
+  constexpr int FC = GD::NZ * GD::NZ + GD::NH * GD::NZ + GD::NZ;
+  __shared__ double buf[3 * GD::NF];
+  const int32_t b = blockIdx.x;
+  const int32_t nc = pl.series_nchunks[b];
+  if (nc == 0) {
+    if ((int)threadIdx.x < GD::NH) grad[(int64_t)b * GD::NH + threadIdx.x] = 0.0;
+    return;
+  }
+  const int32_t ng = (nc + kGradGroup - 1) / kGradGroup;
+  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, pl.series_first_chunk[b], kGradGroup, ng, buf);
+  if ((int)threadIdx.x < GD::NH) grad[(int64_t)b * GD::NH + threadIdx.x] = ac[FC + threadIdx.x];
 }
 
 // Deterministic per-series sum of a per-chunk FP64 array (fixed tree order).
@@ -510,8 +610,13 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   GradOut go{(double*)gws, p.n_chunks_pad, b.ell};
   hipLaunchKernelGGL((k_grad_chunk<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
                      cs(b.fagg), cs(b.ftcarry), go);
-  const int64_t nthreads = (int64_t)p.n_series * GradDims<K1, K2>::NH;
-  hipLaunchKernelGGL((k_grad_series<K1, K2>), dim3((unsigned)((nthreads + 63) / 64)), dim3(64), 0, st, plan->view,
+  constexpr int NT = GradDims<K1, K2>::NF <= 128 ? 128 : 256;
+  int32_t max_nc = 1;
+  for (int32_t v : p.series_nchunks) max_nc = std::max(max_nc, v);
+  const unsigned max_groups = (unsigned)((max_nc + kGradGroup - 1) / kGradGroup);
+  hipLaunchKernelGGL((k_grad_group<K1, K2>), dim3((unsigned)p.n_series, max_groups), dim3(NT), 0, st, plan->view,
+                     (double*)gws, (int64_t)p.n_chunks_pad);
+  hipLaunchKernelGGL((k_grad_total<K1, K2>), dim3((unsigned)p.n_series), dim3(NT), 0, st, plan->view,
                      (const double*)gws, (int64_t)p.n_chunks_pad, grad);
   hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
   BN_HIP_CHECK(hipGetLastError());

@@ -162,7 +162,7 @@ def bench_gradient(args):
         "first_gradient": {"names": names, "gpu": [float(v) for v in g0],
                            "fd_fp64_oracle": None if fd is None else [float(v) for v in fd],
                            "max_rel_err": None if fd is None else float(np.max(np.abs(g0 - fd) / np.abs(fd)))},
-        "energy_first": float(E0), "energy_last": float(E),
+        "energy_first": float(E0), "energy_last": float(E.detach()),
         "hypers_last": [float(v) for v in np.log1p(np.exp(raw.detach().cpu().numpy()))],
         "cpu_baseline": None if cpu_s is None else {
             "value": 1.0 / cpu_s, "unit": "log-likelihood evaluations/s (C FP64 filter+smoother, 1 core)",
