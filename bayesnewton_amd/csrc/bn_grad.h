@@ -188,55 +188,63 @@ struct GradOut {
   double* ell;     // per-chunk log-likelihood partial (primal)
 };
 
-// Source terms of one hyperparameter column at one step, as full D x D arrays.
-template <int K1, int K2>
-struct StepSources {
-  static constexpr int D = K1 + K2;
-  double dA[GradDims<K1, K2>::NH][D * D];
-  double dQ[GradDims<K1, K2>::NH][D * D];
-  double dPinf[GradDims<K1, K2>::NH][D * D];
-};
-
+// A, Q of one component block at one step and, for kind 0 (lengthscale) or 1 (variance),
+// that hyperparameter's source terms dA, dQ, dPinf on the block (full D x D arrays,
+// row-major; kind -1: none).  lam = c / ls, so d/dls = (-lam / ls) d/dlam; Pinf and Q are
+// proportional to var, A does not depend on it.
 template <int K, int O, int D>
-BN_HD void comp_sources(double lam, double var, double ls, double dt, const double* Pinf_full, int h0, double (*dA)[D * D],
-                        double (*dQ)[D * D], double (*dPinf)[D * D], double A[D][D], double Q[D][D]) {
-  double a[9], da[9], q[6], dq[6];
+BN_HD void comp_step_col(double lam, double var, double ls, double dt, const double* Pinf_full, int kind,
+                         double A[D][D], double Q[D][D], double* dA, double* dQ, double* dPinf) {
+  double a[9], q[6], dq[6];
   comp_transition<K>(lam, dt, a);
-  comp_dA_dlam<K>(lam, dt, da);
   comp_dQ_dlam<K>(lam, var, dt, q, dq);
-  const double dlam_dls = -lam / ls;  // lam = c / ls
+  const double dlam_dls = -lam / ls;
   int idx = 0;
   for (int i = 0; i < K; ++i)
     for (int j = i; j < K; ++j) {
       Q[O + i][O + j] = Q[O + j][O + i] = q[idx];
-      const double dl = dq[idx] * dlam_dls, dv = q[idx] / var;
-      dQ[h0][(O + i) * D + O + j] = dQ[h0][(O + j) * D + O + i] = dl;
-      dQ[h0 + 1][(O + i) * D + O + j] = dQ[h0 + 1][(O + j) * D + O + i] = dv;
+      if (kind >= 0) {
+        const double v = kind == 0 ? dq[idx] * dlam_dls : q[idx] / var;
+        dQ[(O + i) * D + O + j] = dQ[(O + j) * D + O + i] = v;
+      }
       ++idx;
     }
+  double da[9];
+  if (kind == 0) comp_dA_dlam<K>(lam, dt, da);
   for (int i = 0; i < K; ++i)
     for (int j = 0; j < K; ++j) {
       A[O + i][O + j] = a[i * K + j];
-      dA[h0][(O + i) * D + O + j] = da[i * K + j] * dlam_dls;
-      // Pinf_ij = var lam^(i+j) C_ij: d/dlam = (i+j) Pinf / lam, d/dvar = Pinf / var
-      const double p = Pinf_full[(O + i) * D + O + j];
-      dPinf[h0][(O + i) * D + O + j] = (i + j) * p / lam * dlam_dls;
-      dPinf[h0 + 1][(O + i) * D + O + j] = p / var;
+      if (kind >= 0) {
+        const double p = Pinf_full[(O + i) * D + O + j];
+        dA[(O + i) * D + O + j] = kind == 0 ? da[i * K + j] * dlam_dls : 0.0;
+        dPinf[(O + i) * D + O + j] = kind == 0 ? (i + j) * p / lam * dlam_dls : p / var;
+      }
     }
 }
 
-// One lane: the chunk's tangent aggregate (see the header comment).
+// One lane: column `col` of chunk j's tangent aggregate (see the header comment) -- a
+// unit tangent (col < NZ: column col of M and entry col of f) or a hyperparameter
+// tangent started at zero (h = col - NZ: column h of O and entry h of c).  The columns
+// of a chunk are independent given the primal trajectory, which every lane recomputes.
 template <int K1, int K2>
-BN_HD void lane_grad_chunk(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
-                           const float* __restrict__ y, const float* __restrict__ r,
-                           const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry, const GradOut& out, int32_t j) {
+BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
+                         const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
+                         CSoA fagg, CSoA fcarry, const GradOut& out, int32_t j, int col) {
   using S = Spec<K1, K2>;
   using GD = GradDims<K1, K2>;
   constexpr int D = S::D, NZ = GD::NZ, NH = GD::NH;
+  constexpr int FF = NZ * NZ + NH * NZ, FC = FF + NZ;
+  const int64_t st = out.stride;
   const int32_t b = pl.chunk_series[j];
   const int32_t L = pl.chunk_len_[j];
-  const uint8_t fl = pl.chunk_flags[j];
-  const bool first_chunk = (fl & CH_FIRST) != 0;
+  const bool first_chunk = (pl.chunk_flags[j] & CH_FIRST) != 0;
+  const int h = col - NZ;
+  if (h < 0 && first_chunk) {  // a series' first chunk has no incoming tangent: M = 0, f = 0
+    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(col * NZ + q) * st + j] = 0.0;
+    out.agg[(int64_t)(FF + col) * st + j] = 0.0;
+    return;
+  }
+  const int comp = h >= 0 ? h / 2 : -1, kind = h >= 0 ? h % 2 : -1;
   Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
   const int64_t ho = (int64_t)b * hv.stride;
   const double ls1 = hv.ls[ho], var1 = hv.var[ho];
@@ -250,28 +258,21 @@ BN_HD void lane_grad_chunk(const PlanView& pl, const HyperView& hv, const float*
   } else {
     for (int i = 0; i < D; ++i) { m[i] = 0.0; for (int c = 0; c < D; ++c) P[i][c] = 0.0; }
   }
-  // columns: NZ unit tangents (the chunk's linear map; not needed for a series'
-  // first chunk, whose incoming tangent is zero) then NH hyperparameter tangents
-  const int c0 = first_chunk ? NZ : 0;
-  double Z[NZ + NH][NZ];
-  double acc[NZ + NH];
-  for (int c = 0; c < NZ + NH; ++c) {
-    acc[c] = 0.0;
-    for (int q = 0; q < NZ; ++q) Z[c][q] = (c < NZ && c == q) ? 1.0 : 0.0;
-  }
-  StepSources<K1, K2> src;
-  double ell = 0.0;
+  double z[NZ];
+  for (int q = 0; q < NZ; ++q) z[q] = (q == col) ? 1.0 : 0.0;
+  double acc = 0.0, ell = 0.0;
   for (int32_t k = 0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
     const bool first = (k == 0) && first_chunk;
-    double A[D][D], Q[D][D];
-    for (int h = 0; h < NH; ++h)
-      for (int q = 0; q < D * D; ++q) { src.dA[h][q] = 0.0; src.dQ[h][q] = 0.0; src.dPinf[h][q] = 0.0; }
+    double A[D][D], Q[D][D], dA[D * D], dQ[D * D], dPinf[D * D];
     for (int i = 0; i < D; ++i)
       for (int c = 0; c < D; ++c) { A[i][c] = 0.0; Q[i][c] = 0.0; }
+    for (int q = 0; q < D * D; ++q) { dA[q] = 0.0; dQ[q] = 0.0; dPinf[q] = 0.0; }
     const double dts = first ? 0.0 : (double)dt[s];
-    comp_sources<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, 0, src.dA, src.dQ, src.dPinf, A, Q);
-    if (K2 > 0) comp_sources<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, 2, src.dA, src.dQ, src.dPinf, A, Q);
+    comp_step_col<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, comp == 0 ? kind : -1, A, Q, dA, dQ, dPinf);
+    if (K2 > 0)
+      comp_step_col<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, comp == 1 ? kind : -1, A, Q, dA,
+                                              dQ, dPinf);
     // primal prediction
     double mp[D], Pp[D][D];
     if (first) {
@@ -306,26 +307,21 @@ BN_HD void lane_grad_chunk(const PlanView& pl, const HyperView& hv, const float*
     const double v = (double)y[s] - hm;
     const bool masked = mask != nullptr && mask[s];
     if (!masked) ell += -0.5 * (v * v / sd + kLog2Pi + log(sd));
-    for (int c = c0; c < NZ + NH; ++c) {
-      const int h = c - NZ;
-      acc[c] += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, h >= 0 ? src.dA[h] : nullptr,
-                                     h >= 0 ? src.dQ[h] : nullptr, h >= 0 ? src.dPinf[h] : nullptr, Z[c]);
-    }
+    acc += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, h >= 0 ? dA : nullptr, h >= 0 ? dQ : nullptr,
+                                h >= 0 ? dPinf : nullptr, z);
     // primal update
     for (int i = 0; i < D; ++i) m[i] = mp[i] + bb[i] * v / sd;
     for (int i = 0; i < D; ++i)
       for (int c = i; c < D; ++c) P[i][c] = P[c][i] = Pp[i][c] - bb[i] * bb[c] / sd;
   }
-  // store: M column-major (column c = Z[c]), o, f, c
-  const int64_t st = out.stride;
-  int f = 0;
-  for (int c = 0; c < NZ; ++c)
-    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(f++) * st + j] = first_chunk ? 0.0 : Z[c][q];
-  for (int h = 0; h < NH; ++h)
-    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(f++) * st + j] = Z[NZ + h][q];
-  for (int c = 0; c < NZ; ++c) out.agg[(int64_t)(f++) * st + j] = first_chunk ? 0.0 : acc[c];
-  for (int h = 0; h < NH; ++h) out.agg[(int64_t)(f++) * st + j] = acc[NZ + h];
-  out.ell[j] = ell;
+  if (h < 0) {
+    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(col * NZ + q) * st + j] = z[q];
+    out.agg[(int64_t)(FF + col) * st + j] = acc;
+  } else {
+    for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(NZ * NZ + h * NZ + q) * st + j] = z[q];
+    out.agg[(int64_t)(FC + h) * st + j] = acc;
+    if (h == 0) out.ell[j] = ell;
+  }
 }
 
 }  // namespace bn

@@ -245,9 +245,13 @@ __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv
                                                        const float* __restrict__ y, const float* __restrict__ r,
                                                        const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry,
                                                        GradOut out) {
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= pl.n_chunks) return;
-  lane_grad_chunk<K1, K2>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
+  // lane l -> column l / stride of chunk l % stride: a wave covers consecutive chunks of one column
+  constexpr int NC = GradDims<K1, K2>::NZ + GradDims<K1, K2>::NH;
+  const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int col = (int)(l / out.stride);
+  const int32_t j = (int32_t)(l - (int64_t)col * out.stride);
+  if (col >= NC || j >= pl.n_chunks) return;
+  lane_grad_col<K1, K2>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, col);
 }
 
 // The chunk elements e = (M, O, f, c) of a series compose associatively (chunk order):
@@ -608,8 +612,9 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   Bufs b = ws_bufs(p, D, ws);
   launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
   GradOut go{(double*)gws, p.n_chunks_pad, b.ell};
-  hipLaunchKernelGGL((k_grad_chunk<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
-                     cs(b.fagg), cs(b.ftcarry), go);
+  constexpr int64_t NC = GradDims<K1, K2>::NZ + GradDims<K1, K2>::NH;
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2>), dim3((unsigned)((NC * p.n_chunks_pad + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
   constexpr int NT = GradDims<K1, K2>::NF <= 128 ? 128 : 256;
   int32_t max_nc = 1;
   for (int32_t v : p.series_nchunks) max_nc = std::max(max_nc, v);

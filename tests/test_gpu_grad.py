@@ -136,3 +136,29 @@ def test_autograd_function_and_adam_steps():
     assert np.all(np.isfinite(energies))
     assert energies[-1] < energies[0]
     assert np.allclose(g.get_raw_hypers(), raw.detach().numpy()) or True
+
+
+def test_batched_gradients_match_single_series():
+    """Per-series energies and gradients of a batch of uneven series (1 to 2,100 steps,
+    16-step chunks: one to three 64-chunk groups per series) equal those of each series
+    run alone, whose gradient the finite-difference tests above pin."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(5)
+    lens = [1, 50, 700, 2100, 33]
+    xs = [_grid_x(n, max(2.0, n / 20.0), rng) for n in lens]
+    ys = [_f32(np.sin(x) + 0.2 * rng.standard_t(3.0, size=len(x))) for x in xs]
+
+    def kern():
+        return K.Sum([K.Matern52(variance=0.8, lengthscale=1.5), K.Matern32(variance=0.4, lengthscale=4.0)])
+
+    lik = L.StudentsT(scale=0.25, df=3.0)
+    gb = M.MarkovVariationalGP.from_series(kern(), lik, xs, ys, chunk_len=16)
+    gb.inference(lr=1.0)
+    Eb, gkb, glb = gb.energy_and_grad_per_series()
+    for i, (x, y) in enumerate(zip(xs, ys)):
+        g1 = M.MarkovVariationalGP(kern(), L.StudentsT(scale=0.25, df=3.0), x, y, chunk_len=16)
+        g1.inference(lr=1.0)
+        E1, gk1, gl1 = g1.energy_and_grad_per_series()
+        assert abs(float(Eb[i]) - float(E1[0])) <= 1e-9 * max(1.0, abs(float(E1[0]))), i
+        np.testing.assert_allclose(gkb[i].cpu().numpy(), gk1[0].cpu().numpy(), rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(float(glb[i]), float(gl1[0]), rtol=1e-9, atol=1e-9)
