@@ -67,6 +67,8 @@ SIGNATURES = {
     "bn_grad_workspace_bytes": (i64, [vp, vp]),
     "bn_filter_loglik_grad": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_rts_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    "bn_predict": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]),
+    "bn_lik_predict": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bn_site_update": (i32, [vp, vp, vp, i32, i32, f64, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                               vp, vp]),
     "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

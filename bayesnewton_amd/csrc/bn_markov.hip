@@ -361,6 +361,172 @@ __global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_g
   if ((int)threadIdx.x < GD::NH) grad[(int64_t)b * GD::NH + threadIdx.x] = ac[FC + threadIdx.x];
 }
 
+// ---- prediction (MarkovGaussianProcess.predict, basemodels.py:766-816) --------------
+// One thread per test time.  temporal_conditional / predict_from_state_ /
+// compute_conditional_statistics (utils.py:99-215): the training times of the test
+// point's series are augmented with sentinels at -/+1e10 holding (0, Pinf); the left
+// neighbour is the last training time < x*, the state at x* given the two neighbours
+// u-, u+ is N(P [u-; u+], T) with
+//   Q-+ = Q_b + A_b Q_f A_b',  V = (Q-+ + 1e-8 I)^-1 A_b,  W = Q_f V',
+//   P = [A_f - W A_b A_f, W],  T = Q_f - (A_b Q_f)' V Q_f,
+// (A_f, Q_f) over x* - x-, (A_b, Q_b) over x+ - x*, and the joint smoothed moments of
+// (u-, u+) with cross covariance G- P+ (G- the smoother gain at u-).  Outputs H-projected
+// mean and variance.
+struct PredictIO {
+  const int64_t* train_off;  // [n_series + 1] CSR offsets of the training times
+  const double* x_train;     // training times (sorted within each series)
+  const double* means;       // smoothed state means, d tiled planes
+  const double* covs;        // smoothed state covariances, d*d tiled planes (row-major)
+  const double* gains;       // smoother gains, d*d tiled planes
+  const int64_t* test_off;   // [n_series + 1] CSR offsets of the test times
+  const double* x_test;
+  int64_t n_test;
+  double* out_mean;
+  double* out_var;
+};
+
+constexpr double kPredictEdge = 1e10;   // utils.py / basemodels.py:795-796 sentinel times
+constexpr double kPredictJitter = 1e-8;  // utils.py:207
+
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_predict(PlanView pl, HyperView hv, PredictIO io) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= io.n_test) return;
+  // series of test point i: the last b with test_off[b] <= i
+  int32_t lo = 0, hi = pl.n_series - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (io.test_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const int32_t b = lo;
+  const double xs = io.x_test[i];
+  const int64_t t0 = io.train_off[b];
+  const int64_t n = io.train_off[b + 1] - t0;
+  // cnt = #{training times < x*}
+  int64_t a = 0, c = n;
+  while (a < c) {
+    const int64_t mid = (a + c) >> 1;
+    if (io.x_train[t0 + mid] < xs) a = mid + 1; else c = mid;
+  }
+  const int64_t cnt = a;
+  const Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  const int32_t fc = pl.series_first_chunk[b];
+  auto slot_of = [&](int64_t k) {
+    return pl.slot(fc + (int32_t)(k / pl.chunk_len), (int32_t)(k % pl.chunk_len));
+  };
+  const int64_t ns = pl.n_slots;
+  double ml[D], mr[D], Pl[D][D], Pr[D][D], G[D][D];
+  double xl, xr;
+  if (cnt == 0) {
+    xl = -kPredictEdge;
+    for (int p = 0; p < D; ++p) { ml[p] = 0.0; for (int q = 0; q < D; ++q) { Pl[p][q] = mdl.Pinf[p][q]; G[p][q] = 0.0; } }
+  } else {
+    const int64_t s = slot_of(cnt - 1);
+    xl = io.x_train[t0 + cnt - 1];
+    for (int p = 0; p < D; ++p) {
+      ml[p] = io.means[p * ns + s];
+      for (int q = 0; q < D; ++q) {
+        Pl[p][q] = io.covs[(int64_t)(p * D + q) * ns + s];
+        G[p][q] = io.gains[(int64_t)(p * D + q) * ns + s];
+      }
+    }
+  }
+  if (cnt == n) {
+    xr = kPredictEdge;
+    for (int p = 0; p < D; ++p) { mr[p] = 0.0; for (int q = 0; q < D; ++q) Pr[p][q] = mdl.Pinf[p][q]; }
+  } else {
+    const int64_t s = slot_of(cnt);
+    xr = io.x_train[t0 + cnt];
+    for (int p = 0; p < D; ++p) {
+      mr[p] = io.means[p * ns + s];
+      for (int q = 0; q < D; ++q) Pr[p][q] = io.covs[(int64_t)(p * D + q) * ns + s];
+    }
+  }
+  double Af[D][D], Qf[D][D], Ab[D][D], Qb[D][D];
+  mdl.discretise(xs - xl, Af, Qf);
+  mdl.discretise(xr - xs, Ab, Qb);
+  // ABQ = A_b Q_f ; Qmp = Q_b + ABQ A_b' + jitter
+  double ABQ[D][D], L[D][D];
+  for (int p = 0; p < D; ++p)
+    for (int q = 0; q < D; ++q) {
+      double t = 0.0;
+      for (int k = 0; k < D; ++k) t = fma(Ab[p][k], Qf[k][q], t);
+      ABQ[p][q] = t;
+    }
+  for (int p = 0; p < D; ++p)
+    for (int q = 0; q < D; ++q) {
+      double t = Qb[p][q] + (p == q ? kPredictJitter : 0.0);
+      for (int k = 0; k < D; ++k) t = fma(ABQ[p][k], Ab[q][k], t);
+      L[p][q] = t;
+    }
+  chol_spd<D>(L);
+  // V = Qmp^-1 A_b
+  double V[D][D];
+  for (int p = 0; p < D; ++p)
+    for (int q = 0; q < D; ++q) V[p][q] = Ab[p][q];
+  lower_solve<D, D, double>(L, V);
+  upper_t_solve<D, D, double>(L, V);
+  // h W = h Q_f V' (row), h T h' = h Q_f h' - (A_b Q_f h')' (V Q_f h')
+  double hQ[D], hW[D], QfH[D];
+  for (int q = 0; q < D; ++q) {
+    double t = 0.0;
+    for (int p = 0; p < D; ++p) if (S::h(p)) t += Qf[p][q];
+    hQ[q] = t;
+    QfH[q] = t;  // Q_f symmetric: Q_f h' = (h Q_f)'
+  }
+  for (int q = 0; q < D; ++q) {
+    double t = 0.0;
+    for (int k = 0; k < D; ++k) t = fma(hQ[k], V[q][k], t);
+    hW[q] = t;
+  }
+  double hTh = 0.0;
+  for (int p = 0; p < D; ++p) if (S::h(p)) hTh += QfH[p];
+  {
+    double VQh[D], ABQh[D];  // V Q_f h' and A_b Q_f h'
+    for (int p = 0; p < D; ++p) {
+      double t = 0.0, u = 0.0;
+      for (int k = 0; k < D; ++k) {
+        t = fma(V[p][k], QfH[k], t);
+        if (S::h(k)) u += ABQ[p][k];
+      }
+      VQh[p] = t;
+      ABQh[p] = u;
+    }
+    for (int p = 0; p < D; ++p) hTh -= ABQh[p] * VQh[p];
+  }
+  // u1 = h (A_f - W A_b A_f), u2 = h W
+  double hWAb[D], u1[D];
+  for (int q = 0; q < D; ++q) {
+    double t = 0.0;
+    for (int k = 0; k < D; ++k) t = fma(hW[k], Ab[k][q], t);
+    hWAb[q] = t;
+  }
+  for (int q = 0; q < D; ++q) {
+    double t = 0.0;
+    for (int k = 0; k < D; ++k) t += ((S::h(k) ? 1.0 : 0.0) - hWAb[k]) * Af[k][q];
+    u1[q] = t;
+  }
+  // mean = u1 ml + u2 mr ; var = u1 Pl u1' + 2 u1 (G Pr) u2' + u2 Pr u2' + h T h'
+  double mean = 0.0;
+  for (int p = 0; p < D; ++p) mean += u1[p] * ml[p] + hW[p] * mr[p];
+  double Pru2[D];
+  for (int p = 0; p < D; ++p) {
+    double t = 0.0;
+    for (int q = 0; q < D; ++q) t = fma(Pr[p][q], hW[q], t);
+    Pru2[p] = t;
+  }
+  double var = hTh;
+  for (int p = 0; p < D; ++p) {
+    double t = 0.0, g = 0.0;
+    for (int q = 0; q < D; ++q) { t = fma(Pl[p][q], u1[q], t); g = fma(G[p][q], Pru2[q], g); }
+    var += u1[p] * t + 2.0 * u1[p] * g + hW[p] * Pru2[p];
+  }
+  io.out_mean[i] = mean;
+  io.out_var[i] = var;
+}
+
 // Deterministic per-series sum of a per-chunk FP64 array (fixed tree order).
 __global__ __launch_bounds__(kBlock) void k_series_sum(PlanView pl, const double* __restrict__ per_chunk,
                                                        double* __restrict__ out) {
@@ -624,6 +790,15 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   hipLaunchKernelGGL((k_grad_total<K1, K2>), dim3((unsigned)p.n_series), dim3(NT), 0, st, plan->view,
                      (const double*)gws, (int64_t)p.n_chunks_pad, grad);
   hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+template <int K1, int K2>
+int run_predict(const bn_plan* plan, HyperView hv, PredictIO io, hipStream_t st) {
+  if (io.n_test > 0)
+    hipLaunchKernelGGL((k_predict<K1, K2>), dim3((unsigned)((io.n_test + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                       plan->view, hv, io);
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -985,6 +1160,22 @@ int bn_rts_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const doubl
   HyperView hv{lengthscale, variance, hyper_stride};
   return BN_DISPATCH(kern, run_rts_smoother, plan, hv, dt, filter_mean, filter_cov, means, covs, gains, flags,
                      workspace, (hipStream_t)stream);
+}
+
+int bn_predict(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale, const double* variance,
+               int32_t hyper_stride, const int64_t* train_offsets, const double* x_train, const double* means,
+               const double* covs, const double* gains, const int64_t* test_offsets, const double* x_test,
+               int64_t n_test, double* out_mean, double* out_var, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (n_test < 0) return fail(BN_EINVAL, "n_test must be >= 0");
+  if (!lengthscale || !variance || !train_offsets || !x_train || !means || !covs || !gains || !test_offsets ||
+      (n_test > 0 && (!x_test || !out_mean || !out_var)))
+    return fail(BN_EINVAL, "NULL array argument to bn_predict");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  PredictIO io{train_offsets, x_train, means, covs, gains, test_offsets, x_test, n_test, out_mean, out_var};
+  return BN_DISPATCH(kern, run_predict, plan, hv, io, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -251,6 +251,54 @@ int make_gh_table(int n, GHTable& gh) { return make_gh(n, gh); }
 int check_lik(const bn_lik_desc* d, LikParams& L) { return make_lik_checked(d, L); }
 }  // namespace bn
 
+// Predictive moments in data space and the predictive log density at one point
+// (Likelihood.predict / predict_cubature, cubature.py:438-465; Gaussian.predict and
+// Gaussian.log_density, likelihoods.py:784-803; log_density_cubature, cubature.py:249-276
+// with Z clipped at 1e-8).  Conditional moments (likelihoods.py): Bernoulli (p, p - p^2)
+// with the jittered probit link, Poisson (binsize e^f, binsize e^f), Student-t
+// (f, scale^2 df / (df - 2)).
+__global__ void k_lik_predict(LikParams L, GHTable gh, int64_t n, const double* __restrict__ mean_f,
+                              const double* __restrict__ var_f, const double* __restrict__ y,
+                              double* __restrict__ mean_y, double* __restrict__ var_y,
+                              double* __restrict__ log_density) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mean_f[i], v = var_f[i];
+  if (L.kind == LIK_GAUSSIAN) {
+    if (mean_y) mean_y[i] = m;
+    if (var_y) var_y[i] = v + L.p0;
+    if (log_density) {
+      const double s = L.p0 + v, r = y[i] - m;
+      log_density[i] = -0.5 * (kLog2Pi + log(s) + r * r / s);
+    }
+    return;
+  }
+  const double sv = sqrt(v);
+  double Ey = 0.0, Ey2 = 0.0, Z = 0.0;
+  const double yi = log_density ? y[i] : 0.0;
+  for (int k = 0; k < gh.n; ++k) {
+    const double f = m + sv * gh.x[k];
+    double mu, s2;
+    if (L.kind == LIK_BERNOULLI_PROBIT) {
+      mu = norm_cdf(f) * (1.0 - 2.0 * kProbitEps) + kProbitEps;
+      s2 = mu - mu * mu;
+    } else if (L.kind == LIK_POISSON) {
+      mu = exp(f) * L.p0;
+      s2 = mu;
+    } else {
+      mu = f;
+      s2 = L.p0 * L.p0 * L.p1 / (L.p1 - 2.0);
+    }
+    Ey += gh.w[k] * mu;
+    Ey2 += gh.w[k] * (s2 + mu * mu);
+    if (log_density) Z += gh.w[k] * exp(log_lik(L, yi, f));
+  }
+  if (mean_y) mean_y[i] = Ey;
+  if (var_y) var_y[i] = Ey2 - Ey * Ey;
+  // np.maximum(Z, 1e-8) propagates NaN (a missing y), so nanmean skips the point
+  if (log_density) log_density[i] = (Z != Z) ? Z : log(Z > 1e-8 ? Z : 1e-8);
+}
+
 extern "C" {
 
 int bn_gauss_hermite(int32_t n, double* x, double* w) {
@@ -376,6 +424,23 @@ int bn_lik_param_grad(const bn_plan* plan, const bn_kernel_desc* kern, const bn_
   // reuse the pairwise reducer; the second output is scratch
   hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, grad, p2);
   BN_HIP_CHECK2(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_lik_predict(const bn_lik_desc* lik, int32_t gh_points, int64_t n, const double* mean_f, const double* var_f,
+                   const double* y, double* mean_y, double* var_y, double* log_density, void* stream) {
+  LikParams L;
+  if (int rc = check_lik(lik, L)) return rc;
+  GHTable gh;
+  if (int rc = make_gh_table(gh_points, gh)) return rc;
+  if (n < 0) return fail(BN_EINVAL, "n must be >= 0");
+  if (n > 0 && (!mean_f || !var_f)) return fail(BN_EINVAL, "NULL mean_f / var_f");
+  if (n > 0 && log_density && !y) return fail(BN_EINVAL, "log_density needs y");
+  if (n > 0)
+    hipLaunchKernelGGL(k_lik_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, L, gh, n,
+                       mean_f, var_f, y, mean_y, var_y, log_density);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(BN_EHIP, std::string("bn_lik_predict: ") + hipGetErrorString(e));
   return BN_OK;
 }
 

@@ -267,6 +267,107 @@ class MarkovGaussianProcess:
         return self.energy_per_series(cubature).sum()
 
 
+    # ---- prediction (basemodels.py:766-816, 165-193) ----------------------------------------
+    def _full_posterior(self):
+        """Filtered, then smoothed full state of the current sites: means [d] and
+        covariances / gains [d*d] as tiled FP64 planes (the filter and smoother of
+        predict(), basemodels.py:779-792, with the smoother's shifted dt)."""
+        key = self._state_key()
+        if getattr(self, "_full_key", None) == key:
+            return self._full
+        ls, var = self._hypers()
+        P = self.plan
+        d = self.kernel.state_dim
+        dev = self.device
+        if getattr(self, "_dt_shift", None) is None:
+            sh = np.zeros(self.num_data, dtype=np.float64)
+            for b in range(self.num_series):
+                a, e = self.offsets[b], self.offsets[b + 1]
+                sh[a:e - 1] = self.dt[a + 1:e]
+            self._dt_shift = P.gather(torch.as_tensor(sh, dtype=torch.float32, device=dev))
+            self._x_train = torch.as_tensor(self.X[:, 0], dtype=torch.float64, device=dev)
+            self._train_off = torch.as_tensor(self.offsets, dtype=torch.int64, device=dev)
+        fm = torch.empty((d, P.n_slots), dtype=torch.float64, device=dev)
+        fP = torch.empty((d * d, P.n_slots), dtype=torch.float64, device=dev)
+        ll = torch.empty(self.num_series, dtype=torch.float64, device=dev)
+        ws = P.workspace(self._desc)
+        nat.check(nat.lib().bn_kalman_filter(P.handle, ctypes.byref(self._desc), nat.ptr(ls), nat.ptr(var),
+                                             self._hyper_stride, nat.ptr(self._dt), nat.ptr(self._py),
+                                             nat.ptr(self._pr), nat.ptr(self._mask), nat.ptr(fm), nat.ptr(fP),
+                                             nat.ptr(ll), 0, nat.ptr(ws), nat.stream_ptr()))
+        sm = torch.empty_like(fm)
+        sP = torch.empty_like(fP)
+        G = torch.empty_like(fP)
+        nat.check(nat.lib().bn_rts_smoother(P.handle, ctypes.byref(self._desc), nat.ptr(ls), nat.ptr(var),
+                                            self._hyper_stride, nat.ptr(self._dt_shift), nat.ptr(fm), nat.ptr(fP),
+                                            nat.ptr(sm), nat.ptr(sP), nat.ptr(G), nat.BN_RETURN_FULL, nat.ptr(ws),
+                                            nat.stream_ptr()))
+        self._full, self._full_key = (sm, sP, G), key
+        return self._full
+
+    def _test_points(self, X):
+        """Test times as one FP64 device array in series order plus CSR offsets: X is an
+        array (single-series model, [n] or [n, 1]; only column 0, the time, is used) or a
+        list with one array per series."""
+        if self.num_series == 1 and not isinstance(X, (list, tuple)):
+            X = [X]
+        if len(X) != self.num_series:
+            raise ValueError(f"expected test inputs for {self.num_series} series")
+        xs = [np.asarray(x, dtype=np.float64).reshape(len(x), -1)[:, 0] for x in X]
+        off = np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.int64)
+        xt = np.concatenate(xs) if off[-1] > 0 else np.zeros(0)
+        return (torch.as_tensor(xt, dtype=torch.float64, device=self.device),
+                torch.as_tensor(off, dtype=torch.int64, device=self.device))
+
+    def predict(self, X=None, R=None):
+        """basemodels.py:766-816: predictive mean and variance of f at test times X
+        (default: the training inputs), FP64 device tensors [n_test] in input order
+        (series after series for a multi-series model)."""
+        if R is not None:
+            raise NotImplementedError("spatial inputs R are for the space-time models")
+        if X is None:
+            X = [self.X[self.offsets[b]:self.offsets[b + 1], 0] for b in range(self.num_series)]
+        sm, sP, G = self._full_posterior()
+        xt, off = self._test_points(X)
+        n = int(xt.shape[0])
+        mean = torch.empty(n, dtype=torch.float64, device=self.device)
+        var = torch.empty_like(mean)
+        ls, lv = self._hypers()
+        nat.check(nat.lib().bn_predict(self.plan.handle, ctypes.byref(self._desc), nat.ptr(ls), nat.ptr(lv),
+                                       self._hyper_stride, nat.ptr(self._train_off), nat.ptr(self._x_train),
+                                       nat.ptr(sm), nat.ptr(sP), nat.ptr(G), nat.ptr(off), nat.ptr(xt), n,
+                                       nat.ptr(mean), nat.ptr(var), nat.stream_ptr()))
+        return mean, var
+
+    def _lik_predict(self, mean_f, var_f, y=None, moments=True, cubature=None):
+        n = int(mean_f.shape[0])
+        my = torch.empty(n, dtype=torch.float64, device=self.device) if moments else None
+        vy = torch.empty_like(my) if moments else None
+        ld = torch.empty(n, dtype=torch.float64, device=self.device) if y is not None else None
+        lik = self.likelihood.desc()
+        nat.check(nat.lib().bn_lik_predict(ctypes.byref(lik), cub.num_points(cubature), n, nat.ptr(mean_f),
+                                           nat.ptr(var_f), nat.ptr(y), nat.ptr(my), nat.ptr(vy), nat.ptr(ld),
+                                           nat.stream_ptr()))
+        return my, vy, ld
+
+    def predict_y(self, X, R=None, cubature=None):
+        """basemodels.py:165-175: predictive mean and variance of y at X."""
+        mean_f, var_f = self.predict(X, R)
+        my, vy, _ = self._lik_predict(mean_f, var_f, cubature=cubature)
+        return my, vy
+
+    def negative_log_predictive_density(self, X, Y, R=None, cubature=None):
+        """basemodels.py:177-193: -nanmean log p(y* | data) over the test points (Y in the
+        order of X; NaN entries are skipped)."""
+        mean_f, var_f = self.predict(X, R)
+        if self.num_series > 1 or isinstance(Y, (list, tuple)):
+            Y = np.concatenate([np.asarray(v, dtype=np.float64).reshape(-1) for v in Y])
+        y = torch.as_tensor(np.asarray(Y, dtype=np.float64).reshape(-1), dtype=torch.float64, device=self.device)
+        if y.shape[0] != mean_f.shape[0]:
+            raise ValueError("Y must hold one value per test input")
+        _, _, ld = self._lik_predict(mean_f, var_f, y=y, moments=False, cubature=cubature)
+        return float(-torch.nanmean(ld))
+
     # ---- hyperparameter gradients ----------------------------------------------------
     def _trainables(self):
         """(owner, attribute) of every trainable raw (softplus-inverse) hyperparameter, in

@@ -176,6 +176,30 @@ int bn_rts_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const doubl
                     const double *filter_cov, double *means, double *covs, double *gains, uint32_t flags,
                     void *workspace, void *stream);
 
+/* --- prediction at test times: MarkovGaussianProcess.predict (basemodels.py:766-816)
+ * with temporal_conditional / predict_from_state_ / compute_conditional_statistics
+ * (utils.py:99-215).  For every test time the smoothed state is conditioned on the two
+ * neighbouring training states (sentinels (0, Pinf) at -/+1e10 outside the data) and
+ * projected by H.  train_offsets / test_offsets: device int64 [n_series + 1] CSR
+ * offsets; x_train: the training times the plan was built on (sorted within each
+ * series); x_test: test times of each series (any order); means / covs / gains: the
+ * smoother's full output (bn_rts_smoother with BN_RETURN_FULL, tiled planes, FP64).
+ * Outputs out_mean / out_var [n_test], FP64. */
+int bn_predict(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale, const double *variance,
+               int32_t hyper_stride, const int64_t *train_offsets, const double *x_train, const double *means,
+               const double *covs, const double *gains, const int64_t *test_offsets, const double *x_test,
+               int64_t n_test, double *out_mean, double *out_var, void *stream);
+
+/* --- predictive moments in data space and log predictive density per point:
+ * Likelihood.predict (predict_cubature, cubature.py:438-465; Gaussian.predict,
+ * likelihoods.py:802-803) and Likelihood.log_density (log_density_cubature,
+ * cubature.py:249-276; Gaussian.log_density, likelihoods.py:784-800), the per-point
+ * maps of predict_y and negative_log_predictive_density (basemodels.py:165-193).
+ * mean_f / var_f / y [n] FP64 (y only with log_density; NaN = missing gives NaN);
+ * mean_y / var_y / log_density [n] FP64, each may be NULL. */
+int bn_lik_predict(const bn_lik_desc *lik, int32_t gh_points, int64_t n, const double *mean_f, const double *var_f,
+                   const double *y, double *mean_y, double *var_y, double *log_density, void *stream);
+
 /* --- site update: one inference step's site map for every time step,
  * InferenceMixin.inference (inference.py:65-90) between its two
  * update_posterior calls: update_variational_params (VI 170-195, Newton

@@ -66,6 +66,15 @@ def log_density_cubature(loglik_fn, y, m, v, num=20):
     return np.log(np.maximum(Z, 1e-8))
 
 
+def predict_cubature(cond_moments, m, v, num=20):
+    """cubature.py:438-465: E[y] = sum_i w_i E[y|f_i], Var[y] = sum_i w_i (Var[y|f_i] + E[y|f_i]^2) - E[y]^2."""
+    sig, w = _cubature_points(m, v, num)
+    mu, s2 = cond_moments(sig)
+    Ey = np.sum(w[None, :] * mu, axis=1)
+    Ey2 = np.sum(w[None, :] * (s2 + mu ** 2), axis=1)
+    return Ey, Ey2 - Ey ** 2
+
+
 class Likelihood:
     """Base-class site maps of likelihoods.py:299-412 (scalar sites, NaN = missing)."""
 
@@ -112,6 +121,10 @@ class Likelihood:
     def log_density(self, y, m, v, num=20):
         return log_density_cubature(self.log_lik, y, m, v, num)
 
+    # likelihoods.py:493-505 (Likelihood.predict -> predict_cubature)
+    def predict(self, m, v, num=20):
+        return predict_cubature(self.conditional_moments, np.asarray(m, np.float64), np.asarray(v, np.float64), num)
+
 
 class Gaussian(Likelihood):
     """likelihoods.py:684-803"""
@@ -142,6 +155,10 @@ class Gaussian(Likelihood):
         """likelihoods.py:784-800"""
         var = self.variance + v
         return -0.5 * (LOG2PI + np.log(var) + (y - m) ** 2 / var)
+
+    def predict(self, m, v, num=20):
+        """likelihoods.py:802-803"""
+        return np.asarray(m, np.float64), np.asarray(v, np.float64) + self.variance
 
 
 PROBIT_JITTER = 1e-3  # likelihoods.py:828
@@ -175,6 +192,11 @@ class Bernoulli(Likelihood):
         H = -(1 - 2 * eps) * s * f * phi / p - J ** 2
         return np.log(p), J, H
 
+    def conditional_moments(self, f):
+        """likelihoods.py:854-860"""
+        p = _probit_link(f)
+        return p, p - p ** 2
+
 
 Probit = Bernoulli
 
@@ -201,6 +223,11 @@ class Poisson(Likelihood):
         E = y * np.log(self.binsize) + y * m - emc - gammaln(y + 1.0)
         return E, y - emc, -emc
 
+    def conditional_moments(self, f):
+        """likelihoods.py:952-960"""
+        mu = np.exp(f) * self.binsize
+        return mu, mu
+
 
 class StudentsT(Likelihood):
     """likelihoods.py:1011-1044"""
@@ -221,3 +248,7 @@ class StudentsT(Likelihood):
         J = (df + 1.0) * r / den
         H = (df + 1.0) * (r ** 2 - df * s2) / den ** 2
         return self.log_lik(y, f), J, H
+
+    def conditional_moments(self, f):
+        """likelihoods.py:1043-1044"""
+        return f, (self.scale ** 2) * (self.df / (self.df - 2.0)) * np.ones_like(f)

@@ -2,7 +2,8 @@
 
 Restates the reference's model/inference stack for the Markov GP:
 bayesnewton/basemodels.py:52-262 (sites, BaseModel) and 625-764
-(MarkovGaussianProcess), bayesnewton/inference.py:21-325 (newton_update,
+(MarkovGaussianProcess) with prediction 766-816 and 165-193 (utils.py:99-215),
+bayesnewton/inference.py:21-325 (newton_update,
 InferenceMixin.inference, Newton/Laplace, VariationalInference,
 ExpectationPropagation) and bayesnewton/models.py:118-132, 216-231, 297-299.
 
@@ -39,6 +40,49 @@ def _diag_mat(v):
     idx = np.arange(v.shape[-1])
     out[..., idx, idx] = v
     return out
+
+
+def compute_conditional_statistics(x_test, x, kernel, ind):
+    """utils.py:185-215: p(x_n | u-, u+) = N(P_n [u-; u+], T_n) for every test time."""
+    n = x_test.shape[0]
+    Pinf = kernel.stationary_covariance()
+    d = Pinf.shape[0]
+    P = np.empty((n, d, 2 * d))
+    Tn = np.empty((n, d, d))
+    for i in range(n):
+        A_fwd = kernel.state_transition(x_test[i, 0] - x[ind[i], 0])
+        A_back = kernel.state_transition(x[ind[i] + 1, 0] - x_test[i, 0])
+        Q_fwd = Pinf - A_fwd @ Pinf @ A_fwd.T
+        Q_back = Pinf - A_back @ Pinf @ A_back.T
+        A_back_Q_fwd = A_back @ Q_fwd
+        Q_mp = Q_back + A_back @ A_back_Q_fwd.T
+        L = np.linalg.cholesky(Q_mp + 1e-8 * np.eye(d))
+        V = np.linalg.solve(L.T, np.linalg.solve(L, A_back))
+        Tn[i] = Q_fwd - A_back_Q_fwd.T @ V @ Q_fwd
+        W = Q_fwd @ V.T
+        P[i] = np.concatenate([A_fwd - W @ A_back @ A_fwd, W], axis=-1)
+    return P, Tn
+
+
+def temporal_conditional(X, X_test, mean, cov, gain, kernel):
+    """utils.py:125-141 with predict_from_state_ (utils.py:110-122)."""
+    Pinf = kernel.stationary_covariance()[None, ...]
+    minf = np.zeros([1, Pinf.shape[1], 1])
+    mean_aug = np.concatenate([minf, mean, minf])
+    cov_aug = np.concatenate([Pinf, cov, Pinf])
+    gain = np.concatenate([np.zeros_like(gain[:1]), gain])
+    ind = np.searchsorted(X.reshape(-1), X_test.reshape(-1)) - 1
+    P, Tn = compute_conditional_statistics(X_test, X, kernel, ind)
+    d = Pinf.shape[1]
+    out_m = np.empty((X_test.shape[0], d, 1))
+    out_P = np.empty((X_test.shape[0], d, d))
+    for i, k in enumerate(ind):
+        mean_joint = np.concatenate([mean_aug[k], mean_aug[k + 1]])
+        cross = gain[k] @ cov_aug[k + 1]
+        cov_joint = np.block([[cov_aug[k], cross], [cross.T, cov_aug[k + 1]]])
+        out_m[i] = P[i] @ mean_joint
+        out_P[i] = P[i] @ cov_joint @ P[i].T + Tn[i]
+    return out_m, out_P
 
 
 class MarkovGP:
@@ -121,6 +165,39 @@ class MarkovGP:
         dt = np.concatenate([self.dt[1:], np.array([0.0])])
         sm, sP, _ = ops.rauch_tung_striebel_smoother(dt, self.kernel, fm, fP, parallel=self.parallel)
         self.posterior_mean, self.posterior_variance = sm, sP
+
+    # ---- prediction (basemodels.py:766-816, 165-193) --------------------------
+    def predict(self, X=None, R=None):
+        """basemodels.py:766-816 (temporal models): filter, full smoother with gains,
+        then the conditional of every test time on its two neighbouring states."""
+        if self.spatio_temporal:
+            raise NotImplementedError("space-time prediction is not restated here")
+        X = self.X if X is None else np.asarray(X, np.float64).reshape(len(X), -1)
+        X = X[:, :1]
+        pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
+        _, (fm, fP) = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var, mask=self._mask3(),
+                                        parallel=self.parallel)
+        dt = np.concatenate([self.dt[1:], np.array([0.0])])
+        sm, sP, gain = ops.rauch_tung_striebel_smoother(dt, self.kernel, fm, fP, return_full=True,
+                                                        parallel=self.parallel)
+        inf = 1e10 * np.ones_like(self.X[0, :1])
+        X_aug = np.block([[-inf], [self.X[:, :1]], [inf]])
+        state_mean, state_cov = temporal_conditional(X_aug, X, sm, sP, gain, self.kernel)
+        H = self.kernel.measurement_model()
+        test_mean, test_var = H @ state_mean, H @ state_cov @ T(H)
+        return np.squeeze(test_mean), np.squeeze(test_var)
+
+    def predict_y(self, X, R=None):
+        """basemodels.py:165-175"""
+        mean_f, var_f = self.predict(X, R)
+        return self.likelihood.predict(np.atleast_1d(mean_f), np.atleast_1d(var_f), self.gh_points)
+
+    def negative_log_predictive_density(self, X, Y, R=None):
+        """basemodels.py:177-193"""
+        mean_f, var_f = self.predict(X, R)
+        ld = self.likelihood.log_density(np.asarray(Y, np.float64).reshape(-1), np.atleast_1d(mean_f),
+                                         np.atleast_1d(var_f), self.gh_points)
+        return -np.nanmean(ld)
 
     def compute_log_lik(self, pseudo_y=None, pseudo_var=None):
         """basemodels.py:726-741"""
