@@ -56,10 +56,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
-    ap.add_argument("--workload", default="batched", choices=["batched", "spacetime", "spacetime-pass", "gradient"],
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "spacetime", "spacetime-pass", "gradient", "predict"],
                     help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
                          "time to converge of the d=128 Cox process, learned warm start vs plain; spacetime-pass: "
-                         "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; gradient: "
+                         "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; predict: "
+                         "predictive mean / variance at one test time per training time of the C3 batch; gradient: "
                          "200 Adam steps on the energy of one 500k-step series")
     ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
     ap.add_argument("--st-runs", type=int, default=200, help="spacetime-pass: runs in the batch")
@@ -462,6 +464,104 @@ def bench_spacetime_pass(args):
     print(json.dumps(out), flush=True)
 
 
+# Algorithmic bytes per predicted test time (d = 5): the test time and both outputs
+# (8 + 16), the two neighbouring smoothed states (2 x (5 + 25) doubles) and the left
+# gain (25 doubles): 24 + 480 + 200 = 704 B.
+PRED_BYTES = 704.0
+
+
+def bench_predict(args):
+    """Prediction over the C3 batch (MarkovGaussianProcess.predict, basemodels.py:766-816):
+    4096 series fitted with one VI step, one test time per training time (uniform over
+    each series' span, 0.5 beyond both ends).  `value` is test times per second through
+    bn_predict with the test times resident on the GPU and the smoothed full state
+    computed beforehand (that pass, once per site update, is timed beside it)."""
+    import numpy as np
+    import torch
+    from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, models as M, workloads
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    data = workloads.c3()
+    off = data["offsets"]
+    B = len(off) - 1
+    xs = [data["x"][off[b]:off[b + 1]] for b in range(B)]
+    ys = [data["y"][off[b]:off[b + 1]].astype(np.float32).astype(np.float64) for b in range(B)]
+    kern = K.Sum([K.Matern52(), K.Matern32()])
+    m = M.MarkovVariationalGP.from_series(kern, L.StudentsT(scale=0.2, df=3.0), xs, ys, lengthscale=data["ls"],
+                                          variance=data["var"], chunk_len=args.chunk)
+    m.inference(lr=1.0, return_terms=False)
+    rng = np.random.default_rng(7)
+    tests = [rng.uniform(x[0] - 0.5, x[-1] + 0.5, len(x)) for x in xs]
+    xt, toff = m._test_points(tests)
+    n = int(xt.shape[0])
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    sm, sP, G = m._full_posterior()
+    torch.cuda.synchronize()
+    full_s = time.perf_counter() - c0
+    mean = torch.empty(n, dtype=torch.float64, device=dev)
+    var = torch.empty_like(mean)
+    ls, lv = m._hypers()
+
+    def launch():
+        nat.check(nat.lib().bn_predict(m.plan.handle, ctypes.byref(m._desc), nat.ptr(ls), nat.ptr(lv),
+                                       m._hyper_stride, nat.ptr(m._train_off), nat.ptr(m._x_train), nat.ptr(sm),
+                                       nat.ptr(sP), nat.ptr(G), nat.ptr(toff), nat.ptr(xt), n, nat.ptr(mean),
+                                       nat.ptr(var), nat.stream_ptr()))
+
+    for _ in range(args.warmup):
+        launch()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        launch()
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    kern_ms = e0.elapsed_time(e1) / args.steps
+    gbs = PRED_BYTES * n / (kern_ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import kernels as OK, likelihoods as OL, models as OM
+        b = int(np.argmin(np.abs(np.diff(off) - 2000)))
+        o = OM.MarkovGP(OK.Sum([OK.Matern52(1.0, data["ls"][b, 0]), OK.Matern32(1.0, data["ls"][b, 1])]),
+                        OL.StudentsT(0.2, 3.0), xs[b], ys[b], inference="VI")
+        o.inference(lr=1.0)
+        c0 = time.perf_counter()
+        o.predict(tests[b][:, None])
+        csec = time.perf_counter() - c0
+        cpu = {"value": len(tests[b]) / csec, "unit": "test times/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/models.py predict (FP64 NumPy: filter, smoother, per-point conditional) on series "
+                         f"{b} ({len(xs[b])} steps, {len(tests[b])} test times), {csec:.2f}s"}
+    out = {
+        "metric": "prediction: test times/s (C3 batch, predictive mean and variance of f)",
+        "value": n / (kern_ms / 1e3),
+        "unit": "test times/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": kern_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (the C3 batch, one VI step of Student-t sites)",
+        "config": {"workload": f"predict at {n} test times over {B} series (one per training time), d=5",
+                   "series": B, "test_times": n},
+        "wall_ms_per_call": wall * 1e3,
+        "full_state_filter_smoother_s": full_s,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_predict", "alg_bytes_per_test_time": PRED_BYTES,
+                     "test_times_per_launch": n, "avg_launch_ms": kern_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration():
     """A plain 1 GiB device copy: FETCH_SIZE / WRITE_SIZE of its dispatch against the
     known 1 GiB read + 1 GiB written give the factor that turns those counters into bytes."""
@@ -501,6 +601,9 @@ def main():
         return
     if args.workload == "gradient":
         bench_gradient(args)
+        return
+    if args.workload == "predict":
+        bench_predict(args)
         return
     import numpy as np
     import torch
