@@ -118,6 +118,12 @@ template <int K>
 BN_HD constexpr int qpolydeg() {
   return K == 1 ? kQPolyDeg1 : (K == 2 ? kQPolyDeg2 : kQPolyDeg3);
 }
+// declared row width of the small-x series tables (rows shorter than it are zero-padded)
+template <int K>
+BN_HD constexpr int qser_width() {
+  return K == 1 ? (int)(sizeof(kQSeries1[0]) / sizeof(double))
+                : (K == 2 ? (int)(sizeof(kQSeries2[0]) / sizeof(double)) : (int)(sizeof(kQSeries3[0]) / sizeof(double)));
+}
 
 // Per-series model: hypers and Pinf, in FP64.
 template <int K1, int K2>
@@ -172,21 +178,22 @@ struct Model {
     BN_UNROLL for (int i = 0; i < K; ++i) {
       double lp = lp_i;
       BN_UNROLL for (int j = i; j < K; ++j) {
-        // Horner over a table row read from constant memory (a runtime loop, so
-        // the coefficients are not hoisted into registers across time steps).
-        // Small x: F_ij = x^(2K-1-i-j) G_ij(x); else F_ij = C_ij - e^{-2x} P_ij(x).
-        const double* tab = qtab<K>(idx, small);
-        const int nt = small ? qlen<K>(idx) : qpolydeg<K>();
-        double v = tab[nt - 1];
-#if defined(__HIPCC__)
-#pragma unroll 1
-#endif
-        for (int n = nt - 2; n >= 0; --n) v = v * x + tab[n];
-        if (small) {
-          BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) v *= x;
-        } else {
-          v = qc<K>(idx) - e2 * v;
-        }
+        // Both forms from coefficient rows every lane of the wave reads at the same
+        // address (scalar loads, independent of the lane's x), fully unrolled, then a
+        // per-lane select.  Small x: F_ij = x^(2K-1-i-j) G_ij(x); else
+        // F_ij = C_ij - e^{-2x} P_ij(x).  A per-lane choice of table turns every
+        // coefficient into a dependent vector load on the recurrence's critical path.
+        // The series rows are zero-padded to their declared width, which leaves the
+        // Horner value unchanged.
+        const double* ts = qtab<K>(idx, true);
+        const double* tp = qtab<K>(idx, false);
+        constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
+        double vs = ts[WS - 1];
+        BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = vs * x + ts[n];
+        double vp = tp[WP - 1];
+        BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = vp * x + tp[n];
+        BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) vs *= x;
+        const double v = small ? vs : qc<K>(idx) - e2 * vp;
         const T q = (T)(lp * v);
         Q[O + i][O + j] = q;
         Q[O + j][O + i] = q;

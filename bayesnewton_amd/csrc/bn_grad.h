@@ -57,24 +57,26 @@ BN_HD void comp_dQ_dlam(double lam, double var, double dt, double* Qp, double* d
   int idx = 0;
   for (int i = 0; i < K; ++i)
     for (int j = i; j < K; ++j) {
-      const double* tab = qtab<K>(idx, small);
-      const int nt = small ? qlen<K>(idx) : qpolydeg<K>();
-      double v = tab[nt - 1], dv = 0.0;
-      for (int n = nt - 2; n >= 0; --n) {
-        dv = dv * x + v;
-        v = v * x + tab[n];
+      // both forms from wave-uniform coefficient rows, selected per lane (as in
+      // Model::comp_block): no per-lane table choice, no dependent vector loads
+      const double* ts = qtab<K>(idx, true);
+      const double* tp = qtab<K>(idx, false);
+      constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
+      double vs = ts[WS - 1], dvs = 0.0;
+      for (int n = WS - 2; n >= 0; --n) {
+        dvs = dvs * x + vs;
+        vs = vs * x + ts[n];
       }
-      double F, dF;
-      if (small) {
-        const int m = 2 * K - 1 - i - j;
-        double xm1 = 1.0;
-        for (int q = 0; q < m - 1; ++q) xm1 *= x;
-        F = xm1 * x * v;
-        dF = xm1 * (m * v + x * dv);
-      } else {
-        F = qc<K>(idx) - e2 * v;
-        dF = e2 * (2.0 * v - dv);
+      double vp = tp[WP - 1], dvp = 0.0;
+      for (int n = WP - 2; n >= 0; --n) {
+        dvp = dvp * x + vp;
+        vp = vp * x + tp[n];
       }
+      const int m = 2 * K - 1 - i - j;
+      double xm1 = 1.0;
+      for (int q = 0; q < m - 1; ++q) xm1 *= x;
+      const double F = small ? xm1 * x * vs : qc<K>(idx) - e2 * vp;
+      const double dF = small ? xm1 * (m * vs + x * dvs) : e2 * (2.0 * vp - dvp);
       const double lp = pow(lam, (double)(i + j));
       Qp[idx] = var * lp * F;
       dQp[idx] = var * ((i + j) * lp / lam * F + lp * dt * dF);

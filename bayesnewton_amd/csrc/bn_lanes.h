@@ -448,6 +448,10 @@ struct FilterPassOut {
   bool predict;        // write predicted instead of filtered moments
   SoA sagg;            // smoothing element per chunk; sagg.base == nullptr: not built
   double* ell;         // per-chunk log-likelihood partial
+  // (D + 1) planes: H P- (D) and H m- at every step, all the Bryson-Frazier smoother
+  // reads per step; when set, `state` is written only at each chunk's first step (the
+  // smoother's anchor).  nullptr: `state` at every step.
+  state_t* hrow = nullptr;
 };
 
 // filter_pass: the filter over one chunk from its true incoming state
@@ -529,7 +533,8 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
         }
       }
     }
-    if (out.state.mean != nullptr && out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+    if (out.state.mean != nullptr && out.predict && (out.hrow == nullptr || k == 0))
+      write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
     // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
     T hP[D];
     BN_UNROLL for (int c = 0; c < D; ++c) {
@@ -540,6 +545,10 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     double Sd = (double)r[s];
     T hm = T(0);
     BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
+    if (out.hrow != nullptr) {
+      BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (state_t)hP[q];
+      out.hrow[D * pl.n_slots + s] = (state_t)hm;
+    }
     const double vd = (double)y[s] - (double)hm;
     const double iSd = 1.0 / Sd;
     const T vS = (T)(vd * iSd);
@@ -773,8 +782,8 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
 template <int K1, int K2, class T>
 BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                              const float* __restrict__ y, const float* __restrict__ r, CSoA sagg, CSoA scarry,
-                             const CStateIO& fin, float* __restrict__ out_mean, float* __restrict__ out_var,
-                             int32_t j) {
+                             const CStateIO& fin, const state_t* __restrict__ hrow, float* __restrict__ out_mean,
+                             float* __restrict__ out_var, int32_t j) {
   using S = Spec<K1, K2>;
   constexpr int D = S::D;
   constexpr int NS = S::NS;
@@ -856,17 +865,11 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
       nx_y = y[s1];
       nx_r = r[s1];
     }
-    T m[D], P[NS];
-    BN_UNROLL for (int i = 0; i < D; ++i) m[i] = (T)fin.mean[i * ns + s];
-    BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = (T)fin.cov[q * ns + s];
     T hP[D], kv[D];
-    BN_UNROLL for (int c = 0; c < D; ++c) {
-      T t = T(0);
-      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
-      hP[c] = t;
-    }
-    T hph = T(0), hm = T(0);
-    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) { hph += hP[i]; hm += m[i]; }
+    BN_UNROLL for (int c = 0; c < D; ++c) hP[c] = (T)hrow[c * ns + s];
+    const T hm = (T)hrow[D * ns + s];
+    T hph = T(0);
+    BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) hph += hP[i];
     const double Sd = (double)rk + (double)hph;
     const T iS = (T)(1.0 / Sd);
     const T vS = (T)(((double)yk - (double)hm) / Sd);

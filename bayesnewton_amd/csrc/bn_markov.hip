@@ -233,11 +233,12 @@ __global__ __launch_bounds__(kBlock) void k_smoother_pass(PlanView pl, HyperView
 template <int K1, int K2>
 __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                          const float* __restrict__ y, const float* __restrict__ r,
-                                                         CSoA sagg, CSoA scarry, CStateIO fin, float* pmean,
+                                                         CSoA sagg, CSoA scarry, CStateIO fin,
+                                                         const state_t* __restrict__ hrow, float* pmean,
                                                          float* pvar) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
-  lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, pmean, pvar, j);
+  lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, j);
 }
 
 template <int K1, int K2>
@@ -652,7 +653,7 @@ WsLayout ws_layout(const PlanHost& p, int D) {
   w.sagg = o;    o = align256(o + SA * p.n_chunks_pad * ea);
   w.stagg = o;   o = align256(o + SA * nt * ea);
   w.stcarry = o; o = align256(o + SA * nt * ea);
-  w.fbuf = o;    o = align256(o + (D + NS) * p.n_slots * es);
+  w.fbuf = o;    o = align256(o + (D + NS + D + 1) * p.n_slots * es);  // state planes, then H P-, H m-
   w.ell = o;     o = align256(o + p.n_chunks_pad * 8);
   w.sp1 = o;     o = align256(o + p.n_chunks_pad * 8);
   w.sp2 = o;     o = align256(o + p.n_chunks_pad * 8);
@@ -728,6 +729,7 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   // recursion runs on them without factorising a covariance per step
   fo.state = StateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
   fo.predict = true;
+  fo.hrow = b.fbuf + (int64_t)(D + D * (D + 1) / 2) * p.n_slots;
   fo.sagg = b.sagg;
   fo.ell = b.ell;
   {
@@ -739,7 +741,7 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   {
     BN_TIMED(KID_SMOOTHER_MBF, st);
     hipLaunchKernelGGL((k_smoother_mbf<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
-                       cs(b.sagg), cs(b.stcarry), CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true}, pmean,
+                       cs(b.sagg), cs(b.stcarry), CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true}, fo.hrow, pmean,
                        pvar);
   }
   {
