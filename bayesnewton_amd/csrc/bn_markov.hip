@@ -49,7 +49,7 @@ using CT = double;  // compute type of the recurrences (see bn_lanes.h)
 // Kernels
 // ---------------------------------------------------------------------------
 template <int K1, int K2>
-__global__ __launch_bounds__(kBlock) void k_filter_reduce(PlanView pl, HyperView hv, const float* __restrict__ dt,
+__global__ __launch_bounds__(kBlock, 2) void k_filter_reduce(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                           const float* __restrict__ y, const float* __restrict__ r,
                                                           SoA agg) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
