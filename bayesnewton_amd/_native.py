@@ -83,6 +83,10 @@ SIGNATURES = {
     "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_mf_workspace_bytes": (i64, [vp]),
+    "bn_mf_pseudo": (i32, [vp, vp, vp, vp, vp, vp]),
+    "bn_mf_filter": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_mf_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, vp]),
     "bn_ws_features": (i32, [vp, vp, vp, vp, f64, f64, f64, f64, vp, vp]),
     "bn_mlp_forward": (i32, [i64, vp, vp, vp, vp, vp, vp]),
     "bn_mlp_grad_workspace_bytes": (i64, [i64]),

@@ -22,6 +22,10 @@
 //                     its covariance error grows geometrically over long runs with
 //                     precise sites, so the filtered moments are stored instead.
 //   k_st_sum          per-run FP64 sums of per-step partials.
+//   k_mf_*            the mean-field forms of the filter and smoother (block-diagonal state
+//                     covariance, ops.py:429-706): per-run sequential filter with one M x M
+//                     inverse per step, the associative form as per-step M x M work plus a
+//                     scalar filter per (run, block) lane, and a per-lane RTS smoother.
 // Dense products are workgroup GEMMs on FP64 operands staged through LDS.
 #include <hip/hip_runtime.h>
 
@@ -932,8 +936,342 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
   if (threadIdx.x == 0) out[k] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Mean-field space-time filter and smoother (ops.py:429-706, the filter and smoother of
+// MarkovMeanFieldGaussianProcess, basemodels.py:1158-1175).  The state is M independent
+// temporal blocks (block i = the KT-dim state of latent i, kernels.py:526-575); each update
+// keeps only the diagonal blocks of the state covariance.  With P block diagonal and
+// H = I_M (x) e_0', the dense update of ops.py:452-461 reduces to
+//   S = R + diag(d),  d_i = P-_i[0][0],  v = y - (m-_i[0])_i,  u = S^-1 v,
+//   m_i = m-_i + P-_i e_0 u_i,  P_i = P-_i - [S^-1]_ii (P-_i e_0)(P-_i e_0)'
+// (block i of K H P- is P-_i e_0 [S^-1]_ii e_0' P-_i: only the diagonal of S^-1 reaches it).
+// The associative-scan form (ops.py:468-578) builds every element from S_t = R_t + diag(h Q_t h')
+// and projects it to the blocks; per block its elements are those of an ordinary scalar-
+// observation filter with noise r_i = 1/[S_t^-1]_ii - h Q_t,i h' and observation
+// [S_t^-1 y_t]_i / [S_t^-1]_ii, which k_mf_blocks runs; its log likelihood is then taken
+// from the predictions with the dense S (ops.py:572-576).
+// Layouts: y [K,T,M], noise R [K,T,M,M] (symmetric), block means [K,T,M,KT], block
+// covariances [K,T,M,KT,KT].
+// ---------------------------------------------------------------------------
+
+// Pseudo observation of compute_full_pseudo_lik (basemodels.py:676-687) from the whitened
+// form: R = (L L')^-1, y = R L z.  Block per (step, run).
+__global__ __launch_bounds__(kT) void k_mf_pseudo(StDims dm, const double* __restrict__ Lw,
+                                                  const double* __restrict__ zw, double* __restrict__ R,
+                                                  double* __restrict__ y) {
+  extern __shared__ double smem[];
+  double* sL = smem;                  // M*M
+  double* sW = sL + kMaxM * kMaxM;    // M*M
+  double* row = sW + kMaxM * kMaxM;   // 2*kMaxM
+  double* sv = row + 2 * kMaxM;       // M
+  const int t = blockIdx.x, k = blockIdx.y, M = dm.M, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  const double* L = Lw + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
+  __syncthreads();
+  for (int e = tid; e < M * M; e += kT) {
+    const int i = e / M, j = e % M, qe = min(i, j);
+    double s = 0.0;
+    for (int q = 0; q <= qe; ++q) s = fma(sL[i * M + q], sL[j * M + q], s);
+    sW[e] = s;
+  }
+  if (tid < M) {
+    double s = 0.0;
+    for (int q = 0; q <= tid; ++q) s = fma(sL[tid * M + q], zw[kt * M + q], s);
+    sv[tid] = s;
+  }
+  wg_sweep_inv<kMaxM / 16, false>(sW, M, M, row);  // sW = -(L L')^-1
+  double* Rk = R + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) Rk[e] = -sW[e];
+  if (tid < M) {
+    double s = 0.0;
+    for (int j = 0; j < M; ++j) s = fma(-sW[tid * M + j], sv[j], s);
+    y[kt * M + tid] = s;
+  }
+}
+
+struct MfFilterIO {
+  const double* dt;   // [K,T] (dt[k,0] = 0 for the first step of a run)
+  const double* y;    // [K,T,M]
+  const double* R;    // [K,T,M,M]
+  double* fmean;      // [K,T,M,KT]
+  double* fcov;       // [K,T,M,KT,KT]
+  double* ell;        // [K,T]
+  double* reff;       // [K,T,M] (associative form)
+  double* yeff;       // [K,T,M]
+};
+
+template <int KT>
+__device__ inline void mf_predict(const double A[KT * KT], const double Q[KT * KT], const double m[KT],
+                                  const double P[KT][KT], double mp[KT], double Pp[KT][KT]) {
+  double AP[KT][KT];
+  for (int i = 0; i < KT; ++i) {
+    double s = 0.0;
+    for (int q = 0; q < KT; ++q) s = fma(A[i * KT + q], m[q], s);
+    mp[i] = s;
+    for (int j = 0; j < KT; ++j) {
+      double a = 0.0;
+      for (int q = 0; q < KT; ++q) a = fma(A[i * KT + q], P[q][j], a);
+      AP[i][j] = a;
+    }
+  }
+  for (int i = 0; i < KT; ++i)
+    for (int j = 0; j < KT; ++j) {
+      double s = Q[i * KT + j];
+      for (int q = 0; q < KT; ++q) s = fma(AP[i][q], A[j * KT + q], s);
+      Pp[i][j] = s;
+    }
+  for (int i = 0; i < KT; ++i)
+    for (int j = i + 1; j < KT; ++j) Pp[i][j] = Pp[j][i] = 0.5 * (Pp[i][j] + Pp[j][i]);
+}
+
+template <int KT>
+__device__ inline void mf_pinf(double ls, double var, double P[KT][KT]) {
+  Model<KT, 0> mdl;
+  mdl.init(ls, var, 1.0, 1.0);
+  for (int i = 0; i < KT; ++i)
+    for (int j = 0; j < KT; ++j) P[i][j] = mdl.Pinf[i][j];
+}
+
+// Sequential form (ops.py:429-465): block per run, loop over T.  Thread i < M owns block i
+// in registers; S = R_t + diag(d) is inverted in LDS by the sweep operator (log det from
+// its pivots).
+template <int KT>
+__global__ __launch_bounds__(kT) void k_mf_filter(StDims dm, StHyp hy, MfFilterIO io) {
+  __shared__ double sS[kMaxM * kMaxM];
+  __shared__ double row[2 * kMaxM];
+  __shared__ double sv[kMaxM];
+  __shared__ double red[4];
+  const int k = blockIdx.x, M = dm.M, tid = threadIdx.x;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  double m[KT], P[KT][KT], mp[KT], Pp[KT][KT];
+  mf_pinf<KT>(ls, var, P);
+  for (int i = 0; i < KT; ++i) m[i] = 0.0;
+  for (int t = 0; t < dm.T; ++t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    const double* Rt = io.R + kt * M * M;
+    for (int e = tid; e < M * M; e += kT) sS[e] = Rt[e];
+    if (tid < M) {
+      double A[KT * KT], Q[KT * KT];
+      temporal_AQ<KT>(ls, var, io.dt[kt], A, Q);
+      mf_predict<KT>(A, Q, m, P, mp, Pp);
+      sv[tid] = io.y[kt * M + tid] - mp[0];
+    }
+    __syncthreads();
+    if (tid < M) sS[tid * M + tid] += Pp[0][0];
+    const double ld = wg_sweep_inv<kMaxM / 16, true>(sS, M, M, row);  // sS = -S^-1
+    double q = 0.0, u = 0.0, s = 0.0;
+    if (tid < M) {
+      for (int j = 0; j < M; ++j) u = fma(-sS[tid * M + j], sv[j], u);
+      s = -sS[tid * M + tid];
+      q = sv[tid] * u;
+    }
+    const double quad = wg_sum(q, red);
+    if (tid == 0) io.ell[kt] = -0.5 * (quad + M * kLog2Pi + ld);
+    if (tid < M) {
+      double* fm = io.fmean + (kt * M + tid) * KT;
+      double* fc = io.fcov + (kt * M + tid) * KT * KT;
+      for (int i = 0; i < KT; ++i) {
+        m[i] = fma(Pp[i][0], u, mp[i]);
+        fm[i] = m[i];
+        for (int j = 0; j < KT; ++j) {
+          P[i][j] = fma(-s * Pp[i][0], Pp[j][0], Pp[i][j]);
+          fc[i * KT + j] = P[i][j];
+        }
+      }
+    }
+  }
+}
+
+// Per-step dense part of the associative form, block per (step, run).
+//   ELEMENTS: d = h Q_t h' (Q_0 = Pinf), reff_i = 1/[S^-1]_ii - d_i, yeff_i = [S^-1 y]_i / [S^-1]_ii
+//   LOGLIK:   d, v from the prediction of the filtered blocks at t-1 (Pinf, 0 at t = 0),
+//             ell_t = log N(v | 0, S)
+template <int KT, bool LOGLIK>
+__global__ __launch_bounds__(kT) void k_mf_sdiag(StDims dm, StHyp hy, MfFilterIO io) {
+  __shared__ double sS[kMaxM * kMaxM];
+  __shared__ double row[2 * kMaxM];
+  __shared__ double sv[kMaxM];
+  __shared__ double red[4];
+  const int t = blockIdx.x, k = blockIdx.y, M = dm.M, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  const double* Rt = io.R + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) sS[e] = Rt[e];
+  double d = 0.0;
+  if (tid < M) {
+    double P[KT][KT];
+    mf_pinf<KT>(ls, var, P);
+    double A[KT * KT], Q[KT * KT];
+    temporal_AQ<KT>(ls, var, io.dt[kt], A, Q);
+    if (!LOGLIK) {
+      d = t == 0 ? P[0][0] : Q[0];
+      sv[tid] = io.y[kt * M + tid];
+    } else {
+      double m[KT], mp[KT], Pp[KT][KT];
+      for (int i = 0; i < KT; ++i) m[i] = 0.0;
+      if (t > 0) {
+        const double* fm = io.fmean + ((kt - 1) * M + tid) * KT;
+        const double* fc = io.fcov + ((kt - 1) * M + tid) * KT * KT;
+        for (int i = 0; i < KT; ++i) {
+          m[i] = fm[i];
+          for (int j = 0; j < KT; ++j) P[i][j] = fc[i * KT + j];
+        }
+      }
+      mf_predict<KT>(A, Q, m, P, mp, Pp);
+      d = Pp[0][0];
+      sv[tid] = io.y[kt * M + tid] - mp[0];
+    }
+  }
+  __syncthreads();
+  if (tid < M) sS[tid * M + tid] += d;
+  const double ld = wg_sweep_inv<kMaxM / 16, LOGLIK>(sS, M, M, row);  // sS = -S^-1
+  double q = 0.0, u = 0.0;
+  if (tid < M) {
+    for (int j = 0; j < M; ++j) u = fma(-sS[tid * M + j], sv[j], u);
+    q = sv[tid] * u;
+  }
+  if (LOGLIK) {
+    const double quad = wg_sum(q, red);
+    if (tid == 0) io.ell[kt] = -0.5 * (quad + M * kLog2Pi + ld);
+  } else if (tid < M) {
+    const double s = -sS[tid * M + tid];
+    io.reff[kt * M + tid] = 1.0 / s - d;
+    io.yeff[kt * M + tid] = u / s;
+  }
+}
+
+// Scalar-observation filter of every block over the per-step (reff, yeff), lane per
+// (run, block).
+template <int KT>
+__global__ __launch_bounds__(64) void k_mf_blocks(StDims dm, StHyp hy, MfFilterIO io) {
+  const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (g >= (int64_t)dm.K * dm.M) return;
+  const int k = (int)(g / dm.M), i = (int)(g % dm.M), M = dm.M;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  double m[KT], P[KT][KT], mp[KT], Pp[KT][KT];
+  mf_pinf<KT>(ls, var, P);
+  for (int a = 0; a < KT; ++a) m[a] = 0.0;
+  for (int t = 0; t < dm.T; ++t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    double A[KT * KT], Q[KT * KT];
+    temporal_AQ<KT>(ls, var, io.dt[kt], A, Q);
+    mf_predict<KT>(A, Q, m, P, mp, Pp);
+    const double S = Pp[0][0] + io.reff[kt * M + i];
+    const double v = io.yeff[kt * M + i] - mp[0];
+    double* fm = io.fmean + (kt * M + i) * KT;
+    double* fc = io.fcov + (kt * M + i) * KT * KT;
+    for (int a = 0; a < KT; ++a) {
+      const double Ka = Pp[a][0] / S;
+      m[a] = fma(Ka, v, mp[a]);
+      fm[a] = m[a];
+      for (int b = 0; b < KT; ++b) {
+        P[a][b] = fma(-Ka, Pp[b][0], Pp[a][b]);
+        fc[a * KT + b] = P[a][b];
+      }
+    }
+  }
+}
+
+struct MfSmootherIO {
+  const double* dt;     // [K,T] step vector of the smoother (already the next step's dt)
+  const double* fmean;  // [K,T,M,KT]
+  const double* fcov;   // [K,T,M,KT,KT]
+  double* hmean;        // optional [K,T,M]: h m_s
+  double* hvar;         // optional: h P_s h' at hvar[kt*var_step + i*var_elem]
+  int64_t var_step, var_elem;
+  double* smean;        // optional [K,T,M,KT]
+  double* scov;         // optional [K,T,M,KT,KT]
+  double* gain;         // optional [K,T,M,KT,KT]
+};
+
+// RTS smoother of every block (ops.py:611-641; the associative form 644-678 gives the
+// same result), lane per (run, block), from the last step back.
+template <int KT>
+__global__ __launch_bounds__(64) void k_mf_smoother(StDims dm, StHyp hy, MfSmootherIO io) {
+  const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (g >= (int64_t)dm.K * dm.M) return;
+  const int k = (int)(g / dm.M), i = (int)(g % dm.M), M = dm.M;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  double sm[KT], sP[KT][KT];
+  {
+    const int64_t kt = (int64_t)k * dm.T + dm.T - 1;
+    for (int a = 0; a < KT; ++a) {
+      sm[a] = io.fmean[(kt * M + i) * KT + a];
+      for (int b = 0; b < KT; ++b) sP[a][b] = io.fcov[((kt * M + i) * KT + a) * KT + b];
+    }
+  }
+  for (int t = dm.T - 1; t >= 0; --t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    double fm[KT], fP[KT][KT], pm[KT], pP[KT][KT], A[KT * KT], Q[KT * KT];
+    for (int a = 0; a < KT; ++a) {
+      fm[a] = io.fmean[(kt * M + i) * KT + a];
+      for (int b = 0; b < KT; ++b) fP[a][b] = io.fcov[((kt * M + i) * KT + a) * KT + b];
+    }
+    temporal_AQ<KT>(ls, var, io.dt[kt], A, Q);
+    mf_predict<KT>(A, Q, fm, fP, pm, pP);
+    // G = (A fP)' pP^-1 = fP A' pP^-1
+    double AfP[KT][KT], iP[KT][KT], G[KT][KT];
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(A[a * KT + q], fP[q][b], s);
+        AfP[a][b] = s;
+      }
+    if (KT == 1) {
+      iP[0][0] = 1.0 / pP[0][0];
+    } else {
+      const double det = pP[0][0] * pP[KT - 1][KT - 1] - pP[0][KT - 1] * pP[KT - 1][0];
+      iP[0][0] = pP[KT - 1][KT - 1] / det;
+      iP[KT - 1][KT - 1] = pP[0][0] / det;
+      iP[0][KT - 1] = -pP[0][KT - 1] / det;
+      iP[KT - 1][0] = -pP[KT - 1][0] / det;
+    }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(AfP[q][a], iP[q][b], s);
+        G[a][b] = s;
+      }
+    double dm_[KT], dP[KT][KT], GdP[KT][KT];
+    for (int a = 0; a < KT; ++a) {
+      dm_[a] = sm[a] - pm[a];
+      for (int b = 0; b < KT; ++b) dP[a][b] = sP[a][b] - pP[a][b];
+    }
+    for (int a = 0; a < KT; ++a) {
+      double s = fm[a];
+      for (int q = 0; q < KT; ++q) s = fma(G[a][q], dm_[q], s);
+      sm[a] = s;
+      for (int b = 0; b < KT; ++b) {
+        double w = 0.0;
+        for (int q = 0; q < KT; ++q) w = fma(G[a][q], dP[q][b], w);
+        GdP[a][b] = w;
+      }
+    }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = fP[a][b];
+        for (int q = 0; q < KT; ++q) s = fma(GdP[a][q], G[b][q], s);
+        sP[a][b] = s;
+      }
+    for (int a = 0; a < KT; ++a)
+      for (int b = a + 1; b < KT; ++b) sP[a][b] = sP[b][a] = 0.5 * (sP[a][b] + sP[b][a]);
+    if (io.hmean) io.hmean[kt * M + i] = sm[0];
+    if (io.hvar) io.hvar[kt * io.var_step + i * io.var_elem] = sP[0][0];
+    if (io.smean)
+      for (int a = 0; a < KT; ++a) io.smean[(kt * M + i) * KT + a] = sm[a];
+    if (io.scov)
+      for (int a = 0; a < KT; ++a)
+        for (int b = 0; b < KT; ++b) io.scov[((kt * M + i) * KT + a) * KT + b] = sP[a][b];
+    if (io.gain)
+      for (int a = 0; a < KT; ++a)
+        for (int b = 0; b < KT; ++b) io.gain[((kt * M + i) * KT + a) * KT + b] = G[a][b];
+  }
+}
+
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
+constexpr size_t kPseudoSmem = sizeof(double) * (2 * kMaxM * kMaxM + 3 * kMaxM);
 constexpr size_t kSitesSmem =
     sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
 
@@ -970,6 +1308,8 @@ int set_smem_limits() {
 #define ST_SITES_ATTR(MT, MD)                                                                              \
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_sites<MT, MD>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                    (int)kSitesSmem))
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_mf_pseudo, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kPseudoSmem));
   ST_SITES_ATTR(M_VI, BN_ST_UPDATE);
   ST_SITES_ATTR(M_VI, BN_ST_ENERGY);
   ST_SITES_ATTR(M_VI, BN_ST_PROJECT);
@@ -1116,6 +1456,95 @@ int bn_st_sum(int32_t n_runs, int32_t n_steps, const double* per_step, double* p
   hipStream_t st = (hipStream_t)stream;
   BN_TIMED(KID_ST_SUM, st);
   hipLaunchKernelGGL(k_st_sum, dim3(n_runs), dim3(kT), 0, st, n_steps, per_step, per_run);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int64_t bn_mf_workspace_bytes(const bn_st_dims* dims) {
+  StDims dm;
+  if (check_dims(dims, dm) != BN_OK) return -1;
+  return 2 * (int64_t)dm.K * dm.T * dm.M * (int64_t)sizeof(double) + 256;
+}
+
+int bn_mf_pseudo(const bn_st_dims* dims, const double* Lw, const double* zw, double* noise, double* y,
+                 void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!Lw || !zw || !noise || !y) return st_fail(BN_EINVAL, "NULL argument to bn_mf_pseudo");
+  if (int rc = set_smem_limits()) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_MF_PSEUDO, st);
+  hipLaunchKernelGGL(k_mf_pseudo, dim3(dm.T, dm.K), dim3(kT), kPseudoSmem, st, dm, Lw, zw, noise, y);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_mf_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, int32_t parallel, const double* dt,
+                 const double* y, const double* noise, double* filter_mean, double* filter_cov, double* ell_steps,
+                 double* loglik, void* workspace, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !dt || !y || !noise || !filter_mean || !filter_cov || !ell_steps || !loglik)
+    return st_fail(BN_EINVAL, "NULL argument to bn_mf_filter");
+  if (parallel && !workspace) return st_fail(BN_EINVAL, "the associative form needs bn_mf_workspace_bytes of workspace");
+  double* ws = (double*)workspace;
+  const int64_t ktm = (int64_t)dm.K * dm.T * dm.M;
+  MfFilterIO io{dt, y, noise, filter_mean, filter_cov, ell_steps, ws, ws ? ws + ktm : nullptr};
+  const StHyp hy = make_hyp(hyp);
+  hipStream_t st = (hipStream_t)stream;
+  if (!parallel) {
+    BN_TIMED(KID_MF_FILTER, st);
+    if (dm.KT == 1) hipLaunchKernelGGL(k_mf_filter<1>, dim3(dm.K), dim3(kT), 0, st, dm, hy, io);
+    else hipLaunchKernelGGL(k_mf_filter<2>, dim3(dm.K), dim3(kT), 0, st, dm, hy, io);
+    ST_HIP_CHECK(hipGetLastError());
+  } else {
+    const dim3 grid(dm.T, dm.K);
+    const int nb = (int)((dm.K * (int64_t)dm.M + 63) / 64);
+    {
+      BN_TIMED(KID_MF_SDIAG, st);
+      if (dm.KT == 1) hipLaunchKernelGGL((k_mf_sdiag<1, false>), grid, dim3(kT), 0, st, dm, hy, io);
+      else hipLaunchKernelGGL((k_mf_sdiag<2, false>), grid, dim3(kT), 0, st, dm, hy, io);
+      ST_HIP_CHECK(hipGetLastError());
+    }
+    {
+      BN_TIMED(KID_MF_BLOCKS, st);
+      if (dm.KT == 1) hipLaunchKernelGGL(k_mf_blocks<1>, dim3(nb), dim3(64), 0, st, dm, hy, io);
+      else hipLaunchKernelGGL(k_mf_blocks<2>, dim3(nb), dim3(64), 0, st, dm, hy, io);
+      ST_HIP_CHECK(hipGetLastError());
+    }
+    {
+      BN_TIMED(KID_MF_SDIAG, st);
+      if (dm.KT == 1) hipLaunchKernelGGL((k_mf_sdiag<1, true>), grid, dim3(kT), 0, st, dm, hy, io);
+      else hipLaunchKernelGGL((k_mf_sdiag<2, true>), grid, dim3(kT), 0, st, dm, hy, io);
+      ST_HIP_CHECK(hipGetLastError());
+    }
+  }
+  {
+    BN_TIMED(KID_ST_SUM, st);
+    hipLaunchKernelGGL(k_st_sum, dim3(dm.K), dim3(kT), 0, st, dm.T, ell_steps, loglik);
+  }
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_mf_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* filter_mean,
+                   const double* filter_cov, double* post_mean, double* post_var, int64_t var_step_stride,
+                   int64_t var_elem_stride, double* smooth_mean, double* smooth_cov, double* gains, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !dt || !filter_mean || !filter_cov) return st_fail(BN_EINVAL, "NULL argument to bn_mf_smoother");
+  if (!post_mean && !post_var && !smooth_mean && !smooth_cov && !gains)
+    return st_fail(BN_EINVAL, "bn_mf_smoother: no output requested");
+  if (post_var && (var_step_stride < 0 || var_elem_stride < 0 ||
+                   (dm.M - 1) * var_elem_stride >= (var_step_stride > 0 ? var_step_stride : INT64_MAX)))
+    return st_fail(BN_EINVAL, "bn_mf_smoother: post_var strides overlap");
+  MfSmootherIO io{dt, filter_mean, filter_cov, post_mean, post_var, var_step_stride, var_elem_stride,
+                  smooth_mean, smooth_cov, gains};
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = (int)((dm.K * (int64_t)dm.M + 63) / 64);
+  BN_TIMED(KID_MF_SMOOTHER, st);
+  if (dm.KT == 1) hipLaunchKernelGGL(k_mf_smoother<1>, dim3(nb), dim3(64), 0, st, dm, make_hyp(hyp), io);
+  else hipLaunchKernelGGL(k_mf_smoother<2>, dim3(nb), dim3(64), 0, st, dm, make_hyp(hyp), io);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

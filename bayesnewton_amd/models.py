@@ -465,6 +465,38 @@ class MarkovEnergy(torch.autograd.Function):
         return gout * g, None, None
 
 
+class MarkovMeanFieldGaussianProcess:
+    """basemodels.py:1158-1175: the space-time Markov GP with the mean-field filter and
+    smoother (bayesnewton_amd.spacetime.SpaceTimeMeanFieldGP)."""
+    method = None
+
+    def __new__(cls, kernel=None, likelihood=None, X=None, Y=None, R=None, parallel=None, device=None):
+        if not isinstance(kernel, SpatioTemporalKernel) or R is None:
+            raise NotImplementedError("the mean-field model is implemented for a SpatioTemporalKernel with "
+                                      "locations R [T, n_loc, 2]")
+        st_cls = {nat.BN_VI: spacetime.SpaceTimeVariationalMeanFieldGP,
+                  nat.BN_NEWTON: spacetime.SpaceTimeLaplaceMeanFieldGP}.get(cls.method)
+        if st_cls is None:
+            raise NotImplementedError("mean-field models support VI and Newton / Laplace")
+        return st_cls(kernel, likelihood, X, Y, R, parallel=parallel, device=device)
+
+
+MarkovMeanFieldGP = MarkovMeanFieldGaussianProcess
+
+
+class MarkovVariationalMeanFieldGP(MarkovMeanFieldGaussianProcess):
+    """models.py:154-156"""
+    method = nat.BN_VI
+
+
+class MarkovNewtonMeanFieldGP(MarkovMeanFieldGaussianProcess):
+    """models.py:307-309"""
+    method = nat.BN_NEWTON
+
+
+MarkovLaplaceGPMeanField = MarkovNewtonMeanFieldGP
+
+
 class MarkovVariationalGP(MarkovGaussianProcess):
     """models.py:118-132 (VariationalInference + MarkovGaussianProcess)."""
     method = nat.BN_VI

@@ -153,3 +153,91 @@ def filter_loglik(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=None,
                                          nat.ptr(mask), nat.ptr(loglik), nat.ptr(plan.workspace(kernel_desc)),
                                          nat.stream_ptr()))
     return loglik
+
+
+# ---- mean-field space-time filter / smoother (ops.py:581-608, 681-706) -------------------
+
+def _mf_setup(dt, kernel, n):
+    from .kernels import SpatioTemporalKernel
+    if not isinstance(kernel, SpatioTemporalKernel):
+        raise TypeError("the mean-field filter needs a SpatioTemporalKernel")
+    dev = _device()
+    kt = kernel.temporal_kernel
+    dims = nat.StDims(1, n, 1, kernel.M, kt.kind)
+    f64 = dict(dtype=torch.float64, device=dev)
+    ls = torch.tensor([kt.lengthscale], **f64)
+    var = torch.tensor([kt.variance], **f64)
+    hyp = nat.StHypers(nat.ptr(ls), nat.ptr(var), nat.ptr(var), 0)
+    dt = _as_tensor(dt, torch.float64, dev).reshape(-1).contiguous()
+    if dt.shape[0] != n:
+        raise ValueError("dt must have one entry per step")
+    return dev, dims, hyp, (ls, var), dt, kt.state_dim
+
+
+def kalman_filter_meanfield(dt, kernel, y, noise_cov, mask=None, parallel=False, block_index=None):
+    """ops.py:581-608.  y [N,M,1], noise_cov [N,M,M] for a SpatioTemporalKernel with M
+    inducing points.  Returns (ell, (means [N,M,KT,1], covs [N,M,KT,KT])), FP64 tensors on
+    the GPU.  block_index is implied by the kernel (kernels.py:573-580) and ignored.
+    Masks are not supported (the space-time models never mask their pseudo observations)."""
+    import ctypes
+    if mask is not None and bool(np.any(np.asarray(mask.cpu() if isinstance(mask, torch.Tensor) else mask))):
+        raise NotImplementedError("masked observations are not supported by the mean-field filter")
+    n = int(y.shape[0])
+    dev, dims, hyp, keep, dt, KT = _mf_setup(dt, kernel, n)
+    M = kernel.M
+    yv = _as_tensor(y, torch.float64, dev).reshape(n, M).contiguous()
+    R = _as_tensor(noise_cov, torch.float64, dev).reshape(n, M, M).contiguous()
+    f64 = dict(dtype=torch.float64, device=dev)
+    fm = torch.empty((n, M, KT), **f64)
+    fP = torch.empty((n, M, KT, KT), **f64)
+    ell = torch.empty(n, **f64)
+    ll = torch.empty(1, **f64)
+    ws = None
+    if parallel:
+        nb = nat.lib().bn_mf_workspace_bytes(ctypes.byref(dims))
+        nat.check(0 if nb > 0 else -1)
+        ws = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+    nat.check(nat.lib().bn_mf_filter(ctypes.byref(dims), ctypes.byref(hyp), 1 if parallel else 0, nat.ptr(dt),
+                                     nat.ptr(yv), nat.ptr(R), nat.ptr(fm), nat.ptr(fP), nat.ptr(ell), nat.ptr(ll),
+                                     nat.ptr(ws), nat.stream_ptr()))
+    return ll[0], (fm[..., None], fP)
+
+
+def rauch_tung_striebel_smoother_meanfield(dt, kernel, filter_mean, filter_cov, return_full=False, parallel=False,
+                                           block_index=None):
+    """ops.py:681-706 (dt is the smoother's step vector, as the reference takes it).
+    filter_mean [N,M,KT(,1)], filter_cov [N,M,KT,KT].  Returns (means, covs, gains) with the
+    reference's shapes: return_full -> [N,M*KT,1], [N,M*KT,M*KT] and block-diagonal gains;
+    else H m [N,M,1] and H P H' [N,M,M] (diagonal).  Both forms give the same result."""
+    import ctypes
+    n = int(filter_cov.shape[0])
+    dev, dims, hyp, keep, dt, KT = _mf_setup(dt, kernel, n)
+    M = kernel.M
+    fm = _as_tensor(filter_mean, torch.float64, dev).reshape(n, M, KT).contiguous()
+    fP = _as_tensor(filter_cov, torch.float64, dev).reshape(n, M, KT, KT).contiguous()
+    f64 = dict(dtype=torch.float64, device=dev)
+    gains = torch.empty((n, M, KT, KT), **f64)
+    if return_full:
+        sm = torch.empty((n, M, KT), **f64)
+        sP = torch.empty((n, M, KT, KT), **f64)
+        nat.check(nat.lib().bn_mf_smoother(ctypes.byref(dims), ctypes.byref(hyp), nat.ptr(dt), nat.ptr(fm),
+                                           nat.ptr(fP), None, None, 0, 0, nat.ptr(sm), nat.ptr(sP), nat.ptr(gains),
+                                           nat.stream_ptr()))
+        return sm.reshape(n, M * KT, 1), _block_diag(sP), _block_diag(gains)
+    hm = torch.empty((n, M), **f64)
+    hv = torch.zeros((n, M, M), **f64)
+    nat.check(nat.lib().bn_mf_smoother(ctypes.byref(dims), ctypes.byref(hyp), nat.ptr(dt), nat.ptr(fm), nat.ptr(fP),
+                                       nat.ptr(hm), nat.ptr(hv), M * M, M + 1, None, None, nat.ptr(gains),
+                                       nat.stream_ptr()))
+    return hm[..., None], hv, _block_diag(gains)
+
+
+def _block_diag(blocks):
+    """[N,M,d,d] -> [N,M d,M d] block-diagonal (build_block_diag, ops.py:644-647)."""
+    n, M, d = blocks.shape[0], blocks.shape[1], blocks.shape[2]
+    out = torch.zeros((n, M * d, M * d), dtype=blocks.dtype, device=blocks.device)
+    idx = torch.arange(M, device=blocks.device)
+    for a in range(d):
+        for b in range(d):
+            out[:, idx * d + a, idx * d + b] = blocks[:, :, a, b]
+    return out

@@ -45,13 +45,15 @@ class SpaceTimeMarkovGP:
     method = None
 
     def __init__(self, kernel, likelihood, X, Y, R, parallel=None, device=None):
+        self.parallel = parallel
         self._build(kernel, likelihood, [X], [R], [Y], device)
         self._single = True
 
     @classmethod
-    def from_runs(cls, kernel, likelihood, Xs, Rs, Ys, device=None):
+    def from_runs(cls, kernel, likelihood, Xs, Rs, Ys, parallel=None, device=None):
         """K runs (lists of t [T], R [T, n_loc, 2], Y [T, n_loc]) under one set of hypers."""
         self = cls.__new__(cls)
+        self.parallel = parallel
         self._build(kernel, likelihood, Xs, Rs, Ys, device)
         self._single = False
         return self
@@ -92,10 +94,9 @@ class SpaceTimeMarkovGP:
         self._Lw = torch.empty((K, T, M, M), **f64)
         self._zw = torch.empty((K, T, M), **f64)
         self._ldl = torch.empty((K, T), **f64)
-        self._fm = torch.empty((K, T, self.D), **f64)
-        self._fP = torch.empty((K, T, self.D * (self.D + 1) // 2), **f64)
         self._ell = torch.empty((K, T), **f64)
         self._loglik = torch.zeros(K, **f64)
+        self._alloc_filter()
         # posterior initialised as in basemodels.py:134-135 (zeros, identity)
         self._pm = torch.zeros((K, T, M), **f64)
         self._pP = torch.eye(M, **f64).expand(K, T, M, M).contiguous()
@@ -103,15 +104,22 @@ class SpaceTimeMarkovGP:
         self._p1 = torch.empty((K, T), **f64)
         self._s0 = torch.empty(K, **f64)
         self._s1 = torch.empty(K, **f64)
-        nb = nat.lib().bn_st_workspace_bytes(ctypes.byref(self._dims))
-        nat.check(0 if nb > 0 else -1)
-        self._ws = torch.empty(int(nb), dtype=torch.uint8, device=dev)
         self._hyp_key = None
         self._cond_key = None
         self._proj_key = None
         self._post_key = None
         self._ll_key = None
         self._sites_version = 0
+
+    def _alloc_filter(self):
+        """Filter outputs and scratch: dense state [K,T,D] and packed covariances."""
+        K, T, D = self.K, self.T, self.D
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self._fm = torch.empty((K, T, D), **f64)
+        self._fP = torch.empty((K, T, D * (D + 1) // 2), **f64)
+        nb = nat.lib().bn_st_workspace_bytes(ctypes.byref(self._dims))
+        nat.check(0 if nb > 0 else -1)
+        self._ws = torch.empty(int(nb), dtype=torch.uint8, device=self.device)
 
     # ---- hyperparameters ---------------------------------------------------------------
     def _hyp(self):
@@ -272,4 +280,79 @@ class SpaceTimeVariationalGP(SpaceTimeMarkovGP):
 
 class SpaceTimeLaplaceGP(SpaceTimeMarkovGP):
     """MarkovLaplaceGP / MarkovNewtonGP with a SpatioTemporalKernel (models.py:297-299, 328)."""
+    method = nat.BN_NEWTON
+
+
+class SpaceTimeMeanFieldGP(SpaceTimeMarkovGP):
+    """MarkovMeanFieldGaussianProcess (basemodels.py:1158-1175): the same model with the
+    mean-field filter and smoother (ops.py:581-608, 681-706), which keep only the M
+    diagonal KT x KT blocks of the state covariance.  The projected pseudo observation is
+    formed explicitly (noise = (B' diag(nat2) B + 1e-12 I)^-1, as compute_full_pseudo_lik
+    does), then filtered.  parallel=None picks the associative-scan form, as the reference
+    does on a GPU backend (basemodels.py:640-641); parallel=False the sequential form.  The
+    two are different approximations when the pseudo noise couples the blocks."""
+
+    def _alloc_filter(self):
+        K, T, M = self.K, self.T, self.M
+        KT = self.kernel.temporal_kernel.state_dim
+        self.parallel = True if self.parallel is None else bool(self.parallel)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self._noise = torch.empty((K, T, M, M), **f64)
+        self._ypse = torch.empty((K, T, M), **f64)
+        self._fm = torch.empty((K, T, M, KT), **f64)
+        self._fP = torch.empty((K, T, M, KT, KT), **f64)
+        dts = self._dt.clone()
+        dts[:, :-1] = self._dt[:, 1:]
+        dts[:, -1] = 0.0
+        self._dt_s = dts.contiguous()
+        nb = nat.lib().bn_mf_workspace_bytes(ctypes.byref(self._dims))
+        nat.check(0 if nb > 0 else -1)
+        self._ws = torch.empty(int(nb), dtype=torch.uint8, device=self.device)
+        self._pseudo_key = None
+
+    def _run_filter(self):
+        h, key = self._hyp()
+        if self._pseudo_key != self._proj_key:
+            nat.check(nat.lib().bn_mf_pseudo(
+                ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._noise),
+                nat.ptr(self._ypse), nat.stream_ptr()))
+            self._pseudo_key = self._proj_key
+        nat.check(nat.lib().bn_mf_filter(
+            ctypes.byref(self._dims), ctypes.byref(h), 1 if self.parallel else 0, nat.ptr(self._dt),
+            nat.ptr(self._ypse), nat.ptr(self._noise), nat.ptr(self._fm), nat.ptr(self._fP), nat.ptr(self._ell),
+            nat.ptr(self._loglik), nat.ptr(self._ws), nat.stream_ptr()))
+        self._ll_key = (self._proj_key, key[0], key[1])
+        return h
+
+    def update_posterior(self):
+        self._ensure_projection()
+        h = self._run_filter()
+        M = self.M
+        # h P_s h' of block i lands on the diagonal of the [K,T,M,M] posterior covariance,
+        # whose off-diagonal entries stay zero (mean-field posterior)
+        nat.check(nat.lib().bn_mf_smoother(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt_s), nat.ptr(self._fm), nat.ptr(self._fP),
+            nat.ptr(self._pm), nat.ptr(self._pP), M * M, M + 1, None, None, None, nat.stream_ptr()))
+        self._post_key = self._ll_key
+
+    @property
+    def pseudo_likelihood_full(self):
+        """(pseudo_y [T, M, 1], pseudo_var [T, M, M]) of compute_full_pseudo_lik."""
+        self._ensure_projection()
+        self._run_filter()
+        return self._out(self._ypse[..., None]), self._out(self._noise)
+
+    @property
+    def filter_state(self):
+        """Filtered block means [T, M, KT, 1] and covariances [T, M, KT, KT]."""
+        return self._out(self._fm[..., None]), self._out(self._fP)
+
+
+class SpaceTimeVariationalMeanFieldGP(SpaceTimeMeanFieldGP):
+    """MarkovVariationalMeanFieldGP (models.py:154-156) with a SpatioTemporalKernel."""
+    method = nat.BN_VI
+
+
+class SpaceTimeLaplaceMeanFieldGP(SpaceTimeMeanFieldGP):
+    """MarkovNewtonMeanFieldGP / MarkovLaplaceGPMeanField (models.py:307-309, 332)."""
     method = nat.BN_NEWTON

@@ -327,6 +327,42 @@ int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *p
 int bn_st_marginals(const bn_st_dims *dims, const double *B, const double *cdiag, const double *post_mean,
                     const double *post_cov, double *mean_f, double *var_f, void *stream);
 
+/* --- mean-field space-time filter and smoother: kalman_filter_meanfield /
+ * rauch_tung_striebel_smoother_meanfield (ops.py:581-608, 681-706) as used by
+ * MarkovMeanFieldGaussianProcess (basemodels.py:1158-1175).  The state is M independent
+ * temporal blocks of size KT (stationary_covariance_meanfield / state_transition_meanfield,
+ * kernels.py:526-561); each update keeps only the diagonal blocks of the covariance.
+ * Layouts: y [K,T,M], noise [K,T,M,M] (symmetric), block means [K,T,M,KT], block
+ * covariances [K,T,M,KT,KT].  dims->n_loc is not used (any value >= 1). */
+
+/* Device scratch of bn_mf_filter's associative form (bytes). */
+int64_t bn_mf_workspace_bytes(const bn_st_dims *dims);
+
+/* Pseudo observation of compute_full_pseudo_lik (basemodels.py:676-687) from bn_st_sites'
+ * whitened form: noise = (Lw Lw')^-1, y = noise Lw zw. */
+int bn_mf_pseudo(const bn_st_dims *dims, const double *Lw, const double *zw, double *noise, double *y,
+                 void *stream);
+
+/* Mean-field Kalman filter.  parallel = 0: the sequential form (_sequential_kf_mf,
+ * ops.py:429-465); parallel = 1: the associative-scan form (_parallel_kf_mf, ops.py:468-578),
+ * a different approximation when the noise couples the blocks.  dt [K,T] (first step of a
+ * run: its prior is the stationary state whatever dt[k,0]).  Outputs: filter_mean,
+ * filter_cov, per-step log N terms ell_steps [K,T] and their sums loglik [K].  No mask:
+ * the space-time models filter unmasked pseudo observations. */
+int bn_mf_filter(const bn_st_dims *dims, const bn_st_hypers *hyp, int32_t parallel, const double *dt,
+                 const double *y, const double *noise, double *filter_mean, double *filter_cov, double *ell_steps,
+                 double *loglik, void *workspace, void *stream);
+
+/* Mean-field RTS smoother (_sequential_rts_mf ops.py:611-641; _parallel_rts_mf 644-678 gives
+ * the same result).  dt [K,T] is the smoother's step vector (update_posterior passes the next
+ * step's dt and 0 at the end, basemodels.py:700).  Optional outputs: post_mean [K,T,M] = h m_s,
+ * h P_s h' of block i at step kt written to post_var[kt*var_step_stride + i*var_elem_stride]
+ * (stride M*M and M+1 fill the diagonal of a [K,T,M,M] array), the full smoothed blocks
+ * smooth_mean / smooth_cov and the gains [K,T,M,KT,KT] (return_full). */
+int bn_mf_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *filter_mean,
+                   const double *filter_cov, double *post_mean, double *post_var, int64_t var_step_stride,
+                   int64_t var_elem_stride, double *smooth_mean, double *smooth_cov, double *gains, void *stream);
+
 /* --- learned warm start of the space-time sites.  No counterpart in the reference:
  * it replaces the fixed site initialisation N(0, 1e2 I) (basemodels.py:130-133) via
  * update_nat_params (basemodels.py:97-100); the plain iteration then runs to the same

@@ -3,6 +3,7 @@
 Restates the reference's model/inference stack for the Markov GP:
 bayesnewton/basemodels.py:52-262 (sites, BaseModel) and 625-764
 (MarkovGaussianProcess) with prediction 766-816 and 165-193 (utils.py:99-215),
+its mean-field form 1158-1175 (space-time kernels),
 bayesnewton/inference.py:21-325 (newton_update,
 InferenceMixin.inference, Newton/Laplace, VariationalInference,
 ExpectationPropagation) and bayesnewton/models.py:118-132, 216-231, 297-299.
@@ -93,8 +94,13 @@ class MarkovGP:
     """
 
     def __init__(self, kernel, likelihood, X, Y, R=None, parallel=False, inference="VI", power=1.0,
-                 gh_points=20):
+                 gh_points=20, mean_field=False):
         X, Y, R, dt = input_admin(X, Y, R)
+        # MarkovMeanFieldGaussianProcess (basemodels.py:1158-1175) swaps in the
+        # mean-field filter and smoother (ops.py:581-608, 681-706); nothing else changes.
+        self.mean_field = bool(mean_field)
+        if self.mean_field and not isinstance(kernel, SpatioTemporalKernel):
+            raise NotImplementedError("the mean-field model is restated for space-time kernels only")
         self.X, self.Y, self.R, self.dt = X, Y, R, dt
         self.kernel = kernel
         self.likelihood = likelihood
@@ -156,14 +162,24 @@ class MarkovGP:
             return T(B) @ self.nat1, T(B) @ self.nat2 @ B
         return self.nat1, self.nat2
 
+    def _filter(self, *args, **kwargs):
+        if self.mean_field:
+            return ops.kalman_filter_meanfield(*args, **kwargs)
+        return ops.kalman_filter(*args, **kwargs)
+
+    def _smoother(self, *args, **kwargs):
+        if self.mean_field:
+            return ops.rauch_tung_striebel_smoother_meanfield(*args, **kwargs)
+        return ops.rauch_tung_striebel_smoother(*args, **kwargs)
+
     def update_posterior(self):
         """basemodels.py:689-706"""
         pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
-        ell, (fm, fP) = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var,
-                                          mask=self._mask3(), parallel=self.parallel)
+        ell, (fm, fP) = self._filter(self.dt, self.kernel, pseudo_y, pseudo_var,
+                                     mask=self._mask3(), parallel=self.parallel)
         self.last_filter_loglik = ell
         dt = np.concatenate([self.dt[1:], np.array([0.0])])
-        sm, sP, _ = ops.rauch_tung_striebel_smoother(dt, self.kernel, fm, fP, parallel=self.parallel)
+        sm, sP, _ = self._smoother(dt, self.kernel, fm, fP, parallel=self.parallel)
         self.posterior_mean, self.posterior_variance = sm, sP
 
     # ---- prediction (basemodels.py:766-816, 165-193) --------------------------
@@ -203,8 +219,8 @@ class MarkovGP:
         """basemodels.py:726-741"""
         if pseudo_y is None:
             pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
-        ell, _ = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var,
-                                   mask=self._mask3(), parallel=self.parallel)
+        ell, _ = self._filter(self.dt, self.kernel, pseudo_y, pseudo_var,
+                              mask=self._mask3(), parallel=self.parallel)
         return ell
 
     def compute_kl(self):

@@ -1,7 +1,9 @@
 """FP64 Kalman filter / RTS smoother of the oracle (test infrastructure only).
 
 Restates bayesnewton/ops.py:149-380 of the reference: the sequential
-(lax.scan) and associative-scan forms of the filter and the smoother.
+(lax.scan) and associative-scan forms of the filter and the smoother; and
+ops.py:429-706, their mean-field forms for space-time kernels (block-diagonal
+state covariance).
 Shapes follow the reference: y [N,p,1], noise_cov [N,p,p], means [N,d,1],
 covs [N,d,d].
 """
@@ -208,4 +210,158 @@ def rauch_tung_striebel_smoother(dt, kernel, filter_mean, filter_cov, return_ful
     Pinf, As, Qs = discretise(kernel, dt)
     H = kernel.measurement_model()
     f = _parallel_rts if parallel else _sequential_rts
+    return f(np.asarray(filter_mean, np.float64), np.asarray(filter_cov, np.float64), As, Qs, H, return_full)
+
+
+# ---- mean-field space-time filter / smoother (ops.py:429-706) -------------------------
+# The state is M independent temporal blocks of size d (SpatioTemporalKernel's
+# *_meanfield forms, kernels.py:526-575); after every update only the diagonal
+# blocks of the state covariance are kept.  Written with full matrices, as the
+# reference is, so that the GPU's block-reduced formulas are checked against it.
+
+def block_diag(Pb):
+    """[..., M, d, d] blocks -> [..., Md, Md] (build_block_diag, ops.py:431-433)."""
+    M, d = Pb.shape[-3], Pb.shape[-1]
+    out = np.zeros(Pb.shape[:-3] + (M * d, M * d))
+    for i in range(M):
+        out[..., i * d:(i + 1) * d, i * d:(i + 1) * d] = Pb[..., i, :, :]
+    return out
+
+
+def diag_blocks(P, M, d):
+    """[..., Md, Md] -> its diagonal blocks [..., M, d, d] (get_block_cov, ops.py:438-439)."""
+    return np.stack([P[..., i * d:(i + 1) * d, i * d:(i + 1) * d] for i in range(M)], axis=-3)
+
+
+def discretise_meanfield(kernel, dt):
+    """ops.py:599-603: Pinf [M,d,d] (stationary_covariance_meanfield, kernels.py:526-532),
+    As, Qs [N,M,d,d] (state_transition_meanfield, kernels.py:553-561)."""
+    kt, M = kernel.temporal_kernel, kernel.M
+    Pt = kt.stationary_covariance()
+    At = kt.state_transition_batch(dt)
+    Qt = process_noise_covariance(At, Pt)
+    N, d = At.shape[0], At.shape[1]
+    Pinf = np.broadcast_to(Pt, (M, d, d)).copy()
+    As = np.broadcast_to(At[:, None], (N, M, d, d)).copy()
+    Qs = np.broadcast_to(Qt[:, None], (N, M, d, d)).copy()
+    return Pinf, As, Qs
+
+
+def _sequential_kf_mf(As, Qs, H, ys, noise_covs, m0, P0, masks):
+    """ops.py:429-465"""
+    N, M, d = As.shape[0], As.shape[1], As.shape[2]
+    m, P, ell = m0, P0, 0.0
+    fms = np.empty((N, M, d, 1))
+    fPs = np.empty((N, M, d, d))
+    for n in range(N):
+        mf = (As[n] @ m).reshape(-1, 1)
+        Pf = block_diag(As[n] @ P @ T(As[n]) + Qs[n])
+        obs_mean = H @ mf
+        HP = H @ Pf
+        S = HP @ H.T + noise_covs[n]
+        ell += float(mvn_logpdf(ys[n], obs_mean, S, masks[n]))
+        K = solve(S, HP).T
+        m = (mf + K @ (ys[n] - obs_mean)).reshape(M, d, 1)
+        P = diag_blocks(Pf - K @ HP, M, d)
+        fms[n], fPs[n] = m, P
+    return ell, fms, fPs
+
+
+def parallel_filtering_element_mf(A, Q, H, noise_cov, y):
+    """ops.py:468-496 (vectorised over the leading time axis; A, Q [N,M,d,d])."""
+    M, d = A.shape[-3], A.shape[-1]
+    Qf, Af = block_diag(Q), block_diag(A)
+    HQ, HA = H @ Qf, H @ Af
+    S = HQ @ H.T + noise_cov
+    SinvH = solve(S, np.broadcast_to(H, S.shape[:-2] + H.shape))
+    K = Qf @ T(SinvH)
+    AA = diag_blocks(Af - K @ HA, M, d)
+    b = (K @ y).reshape(y.shape[:-2] + (M, d, 1))
+    C = diag_blocks(Qf - K @ HQ, M, d)
+    SinvHA = T(SinvH @ Af)
+    eta = (SinvHA @ y).reshape(y.shape[:-2] + (M, d, 1))
+    J = diag_blocks(SinvHA @ HA, M, d)
+    return AA, b, C, J, eta
+
+
+def _parallel_kf_mf(As, Qs, H, ys, noise_covs, m0, P0, masks):
+    """ops.py:523-578: elements (with Q_0 = P0 and the m0 correction of b_0), the
+    blockwise associative scan, then the log likelihood from the predictions."""
+    N, M, d = As.shape[0], As.shape[1], As.shape[2]
+    Qs0 = Qs.copy()
+    Qs0[0] = P0
+    AA, b, C, J, eta = parallel_filtering_element_mf(As, Qs0, H, noise_covs, ys)
+    Q0 = block_diag(P0)
+    S = H @ Q0 @ H.T + noise_covs[0]
+    K0 = solve(S, H @ Q0).T
+    mv = m0.reshape(-1, 1)
+    b = b.copy()
+    b[0] = b[0] + (mv - K0 @ (H @ mv)).reshape(M, d, 1)
+    final = associative_scan(parallel_filtering_operator, [AA, b, C, J, eta])
+    fms, fPs = final[1], final[2]
+    mpred = (As @ np.concatenate([m0[None], fms[:-1]])).reshape(N, M * d, 1)
+    Ppred = block_diag(As @ np.concatenate([P0[None], fPs[:-1]]) @ T(As) + Qs)
+    loglik = float(np.sum(mvn_logpdf(ys, H @ mpred, H @ Ppred @ H.T + noise_covs, masks)))
+    return loglik, fms, fPs
+
+
+def kalman_filter_meanfield(dt, kernel, y, noise_cov, mask=None, parallel=False):
+    """ops.py:581-608: returns ell, (means [N,M,d,1], covs [N,M,d,d])."""
+    y = np.asarray(y, dtype=np.float64)
+    noise_cov = np.asarray(noise_cov, dtype=np.float64)
+    if mask is None:
+        mask = np.zeros_like(y, dtype=bool)
+    mask = np.asarray(mask, dtype=bool).reshape(y.shape)
+    Pinf, As, Qs = discretise_meanfield(kernel, dt)
+    minf = np.zeros(Pinf.shape[:2] + (1,))
+    H = kernel.measurement_model()
+    f = _parallel_kf_mf if parallel else _sequential_kf_mf
+    ell, means, covs = f(As, Qs, H, y, noise_cov, minf, Pinf, mask)
+    return ell, (means, covs)
+
+
+def _sequential_rts_mf(fms, fPs, As, Qs, H, return_full):
+    """ops.py:611-641"""
+    N = fms.shape[0]
+    sm, sP = fms[-1], fPs[-1]
+    sms, sPs, gains = [None] * N, [None] * N, [None] * N
+    for n in range(N - 1, -1, -1):
+        fm, fP, A, Q = fms[n], fPs[n], As[n], Qs[n]
+        pm = A @ fm
+        AfP = A @ fP
+        pP = AfP @ T(A) + Q
+        C = T(solve(pP, AfP))
+        sm = fm + C @ (sm - pm)
+        sP = fP + C @ (sP - pP) @ T(C)
+        gains[n] = block_diag(C)
+        if return_full:
+            sms[n], sPs[n] = sm.reshape(-1, 1), block_diag(sP)
+        else:
+            sms[n], sPs[n] = H @ sm.reshape(-1, 1), H @ block_diag(sP) @ H.T
+    return np.stack(sms), np.stack(sPs), np.stack(gains)
+
+
+def _parallel_rts_mf(fms, fPs, As, Qs, H, return_full):
+    """ops.py:644-678"""
+    N, M, d = fms.shape[0], fms.shape[1], fms.shape[2]
+    E, g, L = parallel_smoothing_element(As, Qs, fms, fPs)
+    gains = block_diag(E)
+    E = E.copy(); g = g.copy(); L = L.copy()
+    E[-1] = 0.0
+    g[-1] = fms[-1]
+    L[-1] = fPs[-1]
+    final = associative_scan(parallel_smoothing_operator, [E, g, L], reverse=True)
+    sms = final[1].reshape(N, M * d, 1)
+    sPs = block_diag(final[2])
+    if return_full:
+        return sms, sPs, gains
+    return H @ sms, H @ sPs @ H.T, gains
+
+
+def rauch_tung_striebel_smoother_meanfield(dt, kernel, filter_mean, filter_cov, return_full=False,
+                                           parallel=False):
+    """ops.py:681-706 (dt is the already-shifted step vector)."""
+    Pinf, As, Qs = discretise_meanfield(kernel, dt)
+    H = kernel.measurement_model()
+    f = _parallel_rts_mf if parallel else _sequential_rts_mf
     return f(np.asarray(filter_mean, np.float64), np.asarray(filter_cov, np.float64), As, Qs, H, return_full)
