@@ -20,7 +20,8 @@ enum KernelId : int32_t {
   KID_FILTER_REDUCE = 0, KID_FSCAN_TILE, KID_FSCAN_SERIES, KID_FILTER_PASS, KID_SMOOTHER_REDUCE,
   KID_SSCAN_TILE, KID_SSCAN_SERIES, KID_SMOOTHER_PASS, KID_SERIES_SUM, KID_ST_CONDITIONAL,
   KID_ST_SITES, KID_ST_FILTER, KID_ST_SMOOTHER, KID_ST_SUM, KID_SMOOTHER_MBF, KID_MF_PSEUDO, KID_MF_FILTER,
-  KID_MF_SDIAG, KID_MF_BLOCKS, KID_MF_SMOOTHER, KID_COUNT
+  KID_MF_SDIAG, KID_MF_BLOCKS, KID_MF_SMOOTHER, KID_IH_TIED, KID_IH_DARE, KID_IH_SINV, KID_IH_FILTER,
+  KID_IH_SMOOTHER, KID_COUNT
 };
 // Brackets the launches of its scope with events on `s` when timing is enabled.
 struct TimedLaunch {

@@ -885,7 +885,8 @@ const char* const kKernelNames[KID_COUNT] = {
     "k_filter_reduce", "k_fscan_tile", "k_fscan_series", "k_filter_pass", "k_smoother_reduce",
     "k_sscan_tile", "k_sscan_series", "k_smoother_pass", "k_series_sum", "k_st_conditional",
     "k_st_sites", "k_st_filter", "k_st_smoother", "k_st_sum", "k_smoother_mbf", "k_mf_pseudo", "k_mf_filter",
-    "k_mf_sdiag", "k_mf_blocks", "k_mf_smoother"};
+    "k_mf_sdiag", "k_mf_blocks", "k_mf_smoother", "k_ih_tied", "k_ih_dare", "k_ih_sinv", "k_ih_filter",
+    "k_ih_smoother"};
 struct TimingRec { int32_t kid; hipEvent_t a, b; };
 bool g_timing = false;
 std::vector<TimingRec> g_recs;

@@ -22,6 +22,8 @@
 //                     its covariance error grows geometrically over long runs with
 //                     precise sites, so the filtered moments are stored instead.
 //   k_st_sum          per-run FP64 sums of per-step partials.
+//   k_ih_*            the infinite-horizon forms (ops.py:796-1068): per-run Riccati iterations
+//                     for a steady-state covariance and gain, means only over the data.
 //   k_mf_*            the mean-field forms of the filter and smoother (block-diagonal state
 //                     covariance, ops.py:429-706): per-run sequential filter with one M x M
 //                     inverse per step, the associative form as per-step M x M work plus a
@@ -1269,9 +1271,276 @@ __global__ __launch_bounds__(64) void k_mf_smoother(StDims dm, StHyp hy, MfSmoot
   }
 }
 
+// ---------------------------------------------------------------------------
+// Infinite-horizon space-time filter and smoother (ops.py:796-1068, the filter and
+// smoother of InfiniteHorizonGaussianProcess, basemodels.py:1257-1301).  Equispaced steps:
+// one A, Q per run (its step size).  Per run:
+//   k_ih_tied     R_tied = (mean_t B' diag(nat2_t) B)^-1 from the whitened form (L L' - 1e-12 I)
+//   k_ih_dare     `iters` Riccati iterations X <- A (X - X H' S^-1 H X) A' + Q, S = H X H' + R_tied,
+//                 from the previous solution; S_tied^-1 and cov = X - K H X (K = X H' S^-1)
+//   k_ih_sinv     heteroscedastic: S_t = H X H' + R_t inverted per (step, run), log det S_t
+//   k_ih_filter   m_t = A m_{t-1} + X H' S_t^-1 (y_t - H A m_{t-1}), log N(y_t | H A m_{t-1}, S_t)
+//   k_ih_smoother G = cov A' X^-1, Qd = cov - G X G', `iters` iterations X_s <- G X_s G' + Qd
+//                 (from the previous solution), sm_t = fm_t + G (sm_{t+1} - A fm_t) from
+//                 sm = fm_{T-1}; H X_s H' at every step.
+// State layout as in k_st_filter: index c*M + m, H picks the c = 0 block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kT) void k_ih_tied(StDims dm, const double* __restrict__ Lw, double* __restrict__ Rt) {
+  __shared__ double sL[kMaxM * kMaxM];
+  __shared__ double row[2 * kMaxM];
+  constexpr int kE = kMaxM * kMaxM / kT;  // entries per thread
+  const int k = blockIdx.x, M = dm.M, tid = threadIdx.x;
+  double acc[kE];
+#pragma unroll
+  for (int q = 0; q < kE; ++q) acc[q] = 0.0;
+  for (int t = 0; t < dm.T; ++t) {
+    const double* L = Lw + ((int64_t)k * dm.T + t) * M * M;
+    __syncthreads();
+    for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      const int e = tid + q * kT;
+      if (e < M * M) {
+        const int i = e / M, j = e % M, qe = min(i, j);
+        double s = 0.0;
+        for (int p = 0; p <= qe; ++p) s = fma(sL[i * M + p], sL[j * M + p], s);
+        acc[q] += s;
+      }
+    }
+  }
+  __syncthreads();
+  const double it = 1.0 / dm.T;
+#pragma unroll
+  for (int q = 0; q < kE; ++q) {
+    const int e = tid + q * kT;
+    if (e < M * M) sL[e] = acc[q] * it - ((e / M == e % M) ? kPseudoJitter : 0.0);
+  }
+  wg_sweep_inv<kMaxM / 16, false>(sL, M, M, row);  // -(tied precision)^-1
+  for (int e = tid; e < M * M; e += kT) Rt[(int64_t)k * M * M + e] = -sL[e];
+}
+
+struct IhIO {
+  const double* step;  // [K] step size of each run
+  const double* y;     // [K,T,M]
+  const double* R;     // [K,T,M,M] (heteroscedastic)
+  const double* Rt;    // [K,M,M] tied noise
+  double* X;           // [K,D,D] Riccati solution: start in, result out
+  double* cov;         // [K,D,D]
+  double* W1;          // [K,D,D] scratch
+  double* W2;          // [K,D,D] scratch
+  double* STn;         // [K,M,M] -S_tied^-1
+  double* ldT;         // [K]
+  double* Sn;          // [K,T,M,M] -S_t^-1 (heteroscedastic)
+  double* ldS;         // [K,T]
+  double* fmean;       // [K,T,D]
+  double* ell;         // [K,T]
+  int32_t iters, hetero;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_ih_dare(StDims dm, StHyp hy, IhIO io) {
+  extern __shared__ double smem[];
+  double* pan = smem;                   // 2*kPK*kPS
+  double* sS = pan + 2 * kPK * kPS;     // M*M
+  double* sR = sS + kMaxM * kMaxM;      // M*M
+  double* row = sR + kMaxM * kMaxM;     // 2*kMaxM
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  const int k = blockIdx.x, M = dm.M, D = dm.D, tid = threadIdx.x;
+  const int64_t DD = (int64_t)D * D;
+  double* X = io.X + k * DD;
+  double* W1 = io.W1 + k * DD;
+  double* W2 = io.W2 + k * DD;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  if (tid == 0) temporal_AQ<KT>(ls, var, io.step[k], sA, sQ);
+  for (int e = tid; e < M * M; e += kT) sR[e] = io.Rt[(int64_t)k * M * M + e];
+  __syncthreads();
+  for (int it = 0; it < io.iters; ++it) {
+    for (int e = tid; e < M * M; e += kT) sS[e] = X[(int64_t)(e / M) * D + e % M] + sR[e];
+    wg_sweep_inv<kMaxM / 16, false>(sS, M, M, row);  // sS = -S^-1
+    wg_gemm(D, M, M, -1.0, MatRef{X, D, 1}, MatRef{sS, M, 1}, 0.0, W1, M, pan);  // K = X H' S^-1
+    for (int64_t e = tid; e < DD; e += kT) W2[e] = X[e];
+    __syncthreads();
+    wg_gemm(D, D, M, -1.0, MatRef{W1, M, 1}, MatRef{X, D, 1}, 1.0, W2, D, pan);  // X - K H X
+    __syncthreads();
+    // X = A sym(W2) A' + Q over the KT x KT blocks of M x M
+    for (int e = tid; e < M * M; e += kT) {
+      const int a = e / M, c = e % M;
+      double w[KT][KT];
+      for (int p = 0; p < KT; ++p)
+        for (int q = 0; q < KT; ++q)
+          w[p][q] = 0.5 * (W2[(int64_t)(p * M + a) * D + q * M + c] + W2[(int64_t)(q * M + c) * D + p * M + a]);
+      for (int i = 0; i < KT; ++i)
+        for (int j = 0; j < KT; ++j) {
+          double v = (a == c) ? sQ[i * KT + j] : 0.0;
+          for (int p = 0; p < KT; ++p)
+            for (int q = 0; q < KT; ++q) v = fma(sA[i * KT + p] * w[p][q], sA[j * KT + q], v);
+          X[(int64_t)(i * M + a) * D + j * M + c] = v;
+        }
+    }
+    __syncthreads();
+  }
+  // tied innovation covariance, gain and the infinite-horizon filtered covariance
+  for (int e = tid; e < M * M; e += kT) sS[e] = X[(int64_t)(e / M) * D + e % M] + sR[e];
+  const double ld = wg_sweep_inv<kMaxM / 16, true>(sS, M, M, row);
+  for (int e = tid; e < M * M; e += kT) io.STn[(int64_t)k * M * M + e] = sS[e];
+  if (tid == 0) io.ldT[k] = ld;
+  wg_gemm(D, M, M, -1.0, MatRef{X, D, 1}, MatRef{sS, M, 1}, 0.0, W1, M, pan);
+  double* cv = io.cov + k * DD;
+  for (int64_t e = tid; e < DD; e += kT) cv[e] = X[e];
+  __syncthreads();
+  wg_gemm(D, D, M, -1.0, MatRef{W1, M, 1}, MatRef{X, D, 1}, 1.0, cv, D, pan);
+}
+
+__global__ __launch_bounds__(kT) void k_ih_sinv(StDims dm, IhIO io) {
+  __shared__ double sS[kMaxM * kMaxM];
+  __shared__ double row[2 * kMaxM];
+  const int t = blockIdx.x, k = blockIdx.y, M = dm.M, D = dm.D, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  const double* X = io.X + (int64_t)k * D * D;
+  const double* Rt = io.R + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) sS[e] = Rt[e] + X[(int64_t)(e / M) * D + e % M];
+  const double ld = wg_sweep_inv<kMaxM / 16, true>(sS, M, M, row);
+  for (int e = tid; e < M * M; e += kT) io.Sn[kt * M * M + e] = sS[e];
+  if (tid == 0) io.ldS[kt] = ld;
+}
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_ih_filter(StDims dm, StHyp hy, IhIO io) {
+  __shared__ double sm[kMaxD], mp[kMaxD], sv[kMaxM], su[kMaxM], red[4];
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  const int k = blockIdx.x, M = dm.M, D = dm.D, tid = threadIdx.x;
+  const double* X = io.X + (int64_t)k * D * D;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  if (tid == 0) temporal_AQ<KT>(ls, var, io.step[k], sA, sQ);
+  for (int e = tid; e < D; e += kT) sm[e] = 0.0;
+  __syncthreads();
+  for (int t = 0; t < dm.T; ++t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    if (tid < D) {
+      const int c = tid / M, a = tid % M;
+      double v = 0.0;
+      for (int p = 0; p < KT; ++p) v = fma(sA[c * KT + p], sm[p * M + a], v);
+      mp[tid] = v;
+    }
+    __syncthreads();
+    if (tid < M) sv[tid] = io.y[kt * M + tid] - mp[tid];
+    __syncthreads();
+    const double* Sn = io.hetero ? io.Sn + kt * M * M : io.STn + (int64_t)k * M * M;  // -S^-1
+    double q = 0.0;
+    if (tid < M) {
+      double u = 0.0;
+      for (int j = 0; j < M; ++j) u = fma(-Sn[j * M + tid], sv[j], u);
+      su[tid] = u;
+      q = sv[tid] * u;
+    }
+    const double quad = wg_sum(q, red);
+    if (tid == 0) io.ell[kt] = -0.5 * (quad + M * kLog2Pi + (io.hetero ? io.ldS[kt] : io.ldT[k]));
+    if (tid < D) {
+      // X H' u: column i of X H' is row i of the (symmetric) X
+      double v = mp[tid];
+      for (int i = 0; i < M; ++i) v = fma(X[(int64_t)i * D + tid], su[i], v);
+      sm[tid] = v;
+      io.fmean[kt * D + tid] = v;
+    }
+    __syncthreads();
+  }
+}
+
+struct IhSmIO {
+  const double* step;   // [K]
+  const double* fmean;  // [K,T,D]
+  const double* X;      // [K,D,D] filter Riccati solution
+  const double* cov;    // [K,D,D]
+  double* Xs;           // [K,D,D] smoother Riccati solution: start in, result out
+  double* G;            // [K,D,D] gain
+  double* W1;           // [K,D,D] scratch
+  double* W2;           // [K,D,D] scratch
+  double* pm;           // optional [K,T,M]
+  double* pP;           // optional [K,T,M,M]
+  double* smean;        // optional [K,T,D]
+  int32_t iters;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_ih_smoother(StDims dm, StHyp hy, IhSmIO io) {
+  extern __shared__ double smem[];
+  const int k = blockIdx.x, M = dm.M, D = dm.D, tid = threadIdx.x;
+  double* pan = smem;                    // 2*kPK*kPS
+  double* sL = pan + 2 * kPK * kPS;      // D*D
+  double* row = sL + kMaxD * kMaxD;      // 2 D
+  double* ms = row + 2 * kMaxD;          // D
+  double* mf = ms + kMaxD;               // D
+  double* dv = mf + kMaxD;               // D
+  double* tmp = dv + kMaxD;              // D
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  const int64_t DD = (int64_t)D * D;
+  const double* X = io.X + k * DD;
+  const double* cov = io.cov + k * DD;
+  double* Xs = io.Xs + k * DD;
+  double* G = io.G + k * DD;
+  double* W1 = io.W1 + k * DD;
+  double* W2 = io.W2 + k * DD;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  if (tid == 0) temporal_AQ<KT>(ls, var, io.step[k], sA, sQ);
+  for (int64_t e = tid; e < DD; e += kT) sL[e] = X[e];
+  (void)wg_sweep_inv<kMaxD / 16, false>(sL, D, D, row);  // sL = -X^-1
+  // W1 = A' X^-1 ; G = cov W1
+  for (int64_t e = tid; e < DD; e += kT) {
+    const int r = (int)(e / D), j = (int)(e % D), c = r / M, m = r % M;
+    double v = 0.0;
+    for (int a = 0; a < KT; ++a) v = fma(-sA[a * KT + c], sL[(int64_t)(a * M + m) * D + j], v);
+    W1[e] = v;
+  }
+  wg_gemm(D, D, D, 1.0, MatRef{cov, D, 1}, MatRef{W1, D, 1}, 0.0, G, D, pan);
+  // Qd = cov - G X G' -> W2
+  wg_gemm(D, D, D, 1.0, MatRef{G, D, 1}, MatRef{X, D, 1}, 0.0, W1, D, pan);
+  for (int64_t e = tid; e < DD; e += kT) W2[e] = cov[e];
+  wg_gemm(D, D, D, -1.0, MatRef{W1, D, 1}, MatRef{G, 1, D}, 1.0, W2, D, pan);
+  for (int it = 0; it < io.iters; ++it) {
+    wg_gemm(D, D, D, 1.0, MatRef{G, D, 1}, MatRef{Xs, D, 1}, 0.0, W1, D, pan);
+    for (int64_t e = tid; e < DD; e += kT) Xs[e] = W2[e];
+    wg_gemm(D, D, D, 1.0, MatRef{W1, D, 1}, MatRef{G, 1, D}, 1.0, Xs, D, pan);
+  }
+  // G' into LDS for the mean recursion
+  for (int64_t e = tid; e < DD; e += kT) sL[(e % D) * D + e / D] = G[e];
+  {
+    const int64_t kt = (int64_t)k * dm.T + dm.T - 1;
+    for (int e = tid; e < D; e += kT) ms[e] = io.fmean[kt * D + e];
+  }
+  __syncthreads();
+  for (int t = dm.T - 1; t >= 0; --t) {
+    const int64_t kt = (int64_t)k * dm.T + t;
+    for (int e = tid; e < D; e += kT) mf[e] = io.fmean[kt * D + e];
+    __syncthreads();
+    if (tid < D) {
+      const int c = tid / M, a = tid % M;
+      double v = 0.0;
+      for (int p = 0; p < KT; ++p) v = fma(sA[c * KT + p], mf[p * M + a], v);
+      dv[tid] = ms[tid] - v;
+    }
+    __syncthreads();
+    if (tid < D) {
+      double v = mf[tid];
+      for (int j = 0; j < D; ++j) v = fma(sL[(int64_t)j * D + tid], dv[j], v);
+      tmp[tid] = v;
+    }
+    __syncthreads();
+    if (tid < D) {
+      ms[tid] = tmp[tid];
+      if (io.smean) io.smean[kt * D + tid] = tmp[tid];
+      if (io.pm && tid < M) io.pm[kt * M + tid] = tmp[tid];
+    }
+    if (io.pP)
+      for (int e = tid; e < M * M; e += kT) io.pP[kt * M * M + e] = Xs[(int64_t)(e / M) * D + e % M];
+    __syncthreads();
+  }
+}
+
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
 constexpr size_t kPseudoSmem = sizeof(double) * (2 * kMaxM * kMaxM + 3 * kMaxM);
+constexpr size_t kIhDareSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM);
 constexpr size_t kSitesSmem =
     sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
 
@@ -1310,6 +1579,14 @@ int set_smem_limits() {
                                    (int)kSitesSmem))
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_mf_pseudo, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kPseudoSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_ih_dare<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kIhDareSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_ih_dare<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kIhDareSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_ih_smoother<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmootherSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_ih_smoother<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kSmootherSmem));
   ST_SITES_ATTR(M_VI, BN_ST_UPDATE);
   ST_SITES_ATTR(M_VI, BN_ST_ENERGY);
   ST_SITES_ATTR(M_VI, BN_ST_PROJECT);
@@ -1545,6 +1822,118 @@ int bn_mf_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double
   BN_TIMED(KID_MF_SMOOTHER, st);
   if (dm.KT == 1) hipLaunchKernelGGL(k_mf_smoother<1>, dim3(nb), dim3(64), 0, st, dm, make_hyp(hyp), io);
   else hipLaunchKernelGGL(k_mf_smoother<2>, dim3(nb), dim3(64), 0, st, dm, make_hyp(hyp), io);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int64_t bn_ih_workspace_bytes(const bn_st_dims* dims, int32_t heteroscedastic) {
+  StDims dm;
+  if (check_dims(dims, dm) != BN_OK) return -1;
+  const int64_t K = dm.K, D = dm.D, M = dm.M, T = dm.T;
+  int64_t n = 3 * K * D * D + K * M * M + K;
+  if (heteroscedastic) n += K * T * M * M + K * T;
+  return n * (int64_t)sizeof(double) + 256;
+}
+
+int bn_ih_tied(const bn_st_dims* dims, const double* Lw, double* noise_tied, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!Lw || !noise_tied) return st_fail(BN_EINVAL, "NULL argument to bn_ih_tied");
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_IH_TIED, st);
+  hipLaunchKernelGGL(k_ih_tied, dim3(dm.K), dim3(kT), 0, st, dm, Lw, noise_tied);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_ih_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* step, int32_t iters,
+                 int32_t heteroscedastic, const double* y, const double* noise, const double* noise_tied, double* dare,
+                 double* filter_cov, double* filter_mean, double* ell_steps, double* loglik, void* workspace,
+                 void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !step || !y || !noise_tied || !dare || !filter_cov || !filter_mean || !ell_steps || !loglik ||
+      !workspace || (heteroscedastic && !noise))
+    return st_fail(BN_EINVAL, "NULL argument to bn_ih_filter");
+  if (iters < 0) return st_fail(BN_EINVAL, "bn_ih_filter: iters must be >= 0");
+  if (int rc = set_smem_limits()) return rc;
+  const int64_t K = dm.K, D = dm.D, M = dm.M;
+  double* ws = (double*)workspace;
+  IhIO io{};
+  io.step = step;
+  io.y = y;
+  io.R = noise;
+  io.Rt = noise_tied;
+  io.X = dare;
+  io.cov = filter_cov;
+  io.W1 = ws;
+  io.W2 = ws + K * D * D;
+  io.STn = ws + 3 * K * D * D;
+  io.ldT = io.STn + K * M * M;
+  io.Sn = heteroscedastic ? io.ldT + K : nullptr;
+  io.ldS = heteroscedastic ? io.Sn + K * dm.T * M * M : nullptr;
+  io.fmean = filter_mean;
+  io.ell = ell_steps;
+  io.iters = iters;
+  io.hetero = heteroscedastic ? 1 : 0;
+  const StHyp hy = make_hyp(hyp);
+  hipStream_t st = (hipStream_t)stream;
+  {
+    BN_TIMED(KID_IH_DARE, st);
+    if (dm.KT == 1) hipLaunchKernelGGL(k_ih_dare<1>, dim3(dm.K), dim3(kT), kIhDareSmem, st, dm, hy, io);
+    else hipLaunchKernelGGL(k_ih_dare<2>, dim3(dm.K), dim3(kT), kIhDareSmem, st, dm, hy, io);
+    ST_HIP_CHECK(hipGetLastError());
+  }
+  if (heteroscedastic) {
+    BN_TIMED(KID_IH_SINV, st);
+    hipLaunchKernelGGL(k_ih_sinv, dim3(dm.T, dm.K), dim3(kT), 0, st, dm, io);
+    ST_HIP_CHECK(hipGetLastError());
+  }
+  {
+    BN_TIMED(KID_IH_FILTER, st);
+    if (dm.KT == 1) hipLaunchKernelGGL(k_ih_filter<1>, dim3(dm.K), dim3(kT), 0, st, dm, hy, io);
+    else hipLaunchKernelGGL(k_ih_filter<2>, dim3(dm.K), dim3(kT), 0, st, dm, hy, io);
+    ST_HIP_CHECK(hipGetLastError());
+  }
+  {
+    BN_TIMED(KID_ST_SUM, st);
+    hipLaunchKernelGGL(k_st_sum, dim3(dm.K), dim3(kT), 0, st, dm.T, ell_steps, loglik);
+  }
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_ih_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* step, int32_t iters,
+                   const double* filter_mean, const double* dare, const double* filter_cov, double* dare_smoother,
+                   double* gain, double* post_mean, double* post_cov, double* smooth_mean, void* workspace,
+                   void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !step || !filter_mean || !dare || !filter_cov || !dare_smoother || !workspace)
+    return st_fail(BN_EINVAL, "NULL argument to bn_ih_smoother");
+  if (iters < 0) return st_fail(BN_EINVAL, "bn_ih_smoother: iters must be >= 0");
+  if (int rc = set_smem_limits()) return rc;
+  const int64_t K = dm.K, D = dm.D;
+  double* ws = (double*)workspace;
+  IhSmIO io{};
+  io.step = step;
+  io.fmean = filter_mean;
+  io.X = dare;
+  io.cov = filter_cov;
+  io.Xs = dare_smoother;
+  io.G = gain ? gain : ws + 2 * K * D * D;
+  io.W1 = ws;
+  io.W2 = ws + K * D * D;
+  io.pm = post_mean;
+  io.pP = post_cov;
+  io.smean = smooth_mean;
+  io.iters = iters;
+  hipStream_t st = (hipStream_t)stream;
+  BN_TIMED(KID_IH_SMOOTHER, st);
+  if (dm.KT == 1)
+    hipLaunchKernelGGL(k_ih_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
+  else
+    hipLaunchKernelGGL(k_ih_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

@@ -497,6 +497,36 @@ class MarkovNewtonMeanFieldGP(MarkovMeanFieldGaussianProcess):
 MarkovLaplaceGPMeanField = MarkovNewtonMeanFieldGP
 
 
+class InfiniteHorizonGaussianProcess:
+    """basemodels.py:1257-1301: the infinite-horizon space-time Markov GP
+    (bayesnewton_amd.spacetime.SpaceTimeInfiniteHorizonGP)."""
+    method = None
+
+    def __new__(cls, kernel=None, likelihood=None, X=None, Y=None, R=None, dare_iters=20, parallel=None,
+                device=None):
+        if not isinstance(kernel, SpatioTemporalKernel) or R is None:
+            raise NotImplementedError("the infinite-horizon model is implemented for a SpatioTemporalKernel with "
+                                      "locations R [T, n_loc, 2]")
+        st_cls = {nat.BN_VI: spacetime.SpaceTimeVariationalInfiniteHorizonGP,
+                  nat.BN_NEWTON: spacetime.SpaceTimeLaplaceInfiniteHorizonGP}.get(cls.method)
+        if st_cls is None:
+            raise NotImplementedError("infinite-horizon models support VI and Newton / Laplace")
+        return st_cls(kernel, likelihood, X, Y, R, dare_iters=dare_iters, parallel=parallel, device=device)
+
+
+class InfiniteHorizonVariationalGP(InfiniteHorizonGaussianProcess):
+    """models.py:162-164"""
+    method = nat.BN_VI
+
+
+class InfiniteHorizonNewtonGP(InfiniteHorizonGaussianProcess):
+    """models.py:315-317"""
+    method = nat.BN_NEWTON
+
+
+InfiniteHorizonLaplaceGP = InfiniteHorizonNewtonGP
+
+
 class MarkovVariationalGP(MarkovGaussianProcess):
     """models.py:118-132 (VariationalInference + MarkovGaussianProcess)."""
     method = nat.BN_VI

@@ -241,3 +241,94 @@ def _block_diag(blocks):
         for b in range(d):
             out[:, idx * d + a, idx * d + b] = blocks[:, :, a, b]
     return out
+
+
+# ---- infinite-horizon space-time filter / smoother (ops.py:877-953, 1018-1068) ------------
+
+def _ih_setup(kernel, n, step):
+    from .kernels import SpatioTemporalKernel
+    from .spacetime import state_perm, _pinf_temporal
+    if not isinstance(kernel, SpatioTemporalKernel):
+        raise TypeError("the infinite-horizon filter here needs a SpatioTemporalKernel")
+    dev = _device()
+    kt = kernel.temporal_kernel
+    M, KT = kernel.M, kt.state_dim
+    dims = nat.StDims(1, n, 1, M, kt.kind)
+    f64 = dict(dtype=torch.float64, device=dev)
+    ls = torch.tensor([kt.lengthscale], **f64)
+    var = torch.tensor([kt.variance], **f64)
+    hyp = nat.StHypers(nat.ptr(ls), nat.ptr(var), nat.ptr(var), 0)
+    p = torch.as_tensor(state_perm(M, KT), device=dev)
+    pinf = torch.tensor(np.kron(np.eye(M), _pinf_temporal(kt)), **f64)
+    st = torch.tensor([float(step)], **f64)
+    return dev, dims, hyp, (ls, var), p, torch.argsort(p), pinf, st, M, KT
+
+
+def kalman_filter_infinite_horizon(dt, kernel, y, noise_cov, mask=None, parallel=False, heteroscedastic=False,
+                                   noise_cov_tied=None, dare_iters=20, dare_init=None):
+    """ops.py:877-953 for a SpatioTemporalKernel (state in the reference's order).
+    y [N,M,1], noise_cov [N,M,M], noise_cov_tied [M,M], dare_init [D,D] (default Pinf).
+    Returns (ell, (means [N,D,1], (Pdare, cov))), FP64 tensors on the GPU.  Masks are not
+    supported; both of the reference's forms compute the same means."""
+    import ctypes
+    if mask is not None and bool(np.any(np.asarray(mask.cpu() if isinstance(mask, torch.Tensor) else mask))):
+        raise NotImplementedError("masked observations are not supported by the infinite-horizon filter")
+    if noise_cov_tied is None:
+        raise ValueError("noise_cov_tied is required")
+    dtv = np.asarray(dt.cpu() if isinstance(dt, torch.Tensor) else dt, dtype=np.float64).reshape(-1)
+    n = dtv.shape[0]
+    dev, dims, hyp, keep, p, ip, pinf, st, M, KT = _ih_setup(kernel, n, dtv[1])
+    D = M * KT
+    f64 = dict(dtype=torch.float64, device=dev)
+    yv = _as_tensor(y, torch.float64, dev).reshape(n, M).contiguous()
+    R = _as_tensor(noise_cov, torch.float64, dev).reshape(n, M, M).contiguous() if heteroscedastic else None
+    Rt = _as_tensor(noise_cov_tied, torch.float64, dev).reshape(M, M).contiguous()
+    X0 = pinf if dare_init is None else _as_tensor(dare_init, torch.float64, dev).reshape(D, D)
+    X = X0[p][:, p].contiguous()
+    cov = torch.empty((D, D), **f64)
+    fm = torch.empty((n, D), **f64)
+    ell = torch.empty(n, **f64)
+    ll = torch.empty(1, **f64)
+    nb = nat.lib().bn_ih_workspace_bytes(ctypes.byref(dims), 1 if heteroscedastic else 0)
+    nat.check(0 if nb > 0 else -1)
+    ws = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+    nat.check(nat.lib().bn_ih_filter(ctypes.byref(dims), ctypes.byref(hyp), nat.ptr(st), int(dare_iters),
+                                     1 if heteroscedastic else 0, nat.ptr(yv), nat.ptr(R), nat.ptr(Rt), nat.ptr(X),
+                                     nat.ptr(cov), nat.ptr(fm), nat.ptr(ell), nat.ptr(ll), nat.ptr(ws),
+                                     nat.stream_ptr()))
+    back = lambda A: A[ip][:, ip]  # noqa: E731
+    return ll[0], (fm[:, ip][..., None], (back(X), back(cov)))
+
+
+def rauch_tung_striebel_smoother_infinite_horizon(dt, kernel, filter_mean, filter_cov, return_full=False,
+                                                  parallel=False, dare_iters=20, dare_init=None):
+    """ops.py:1018-1068 for a SpatioTemporalKernel (dt is the smoother's step vector; the
+    step size is its first entry).  filter_cov = (Pdare, cov) as the filter returns it.
+    Returns (means, covs, gains, dare_cov) with the reference's shapes."""
+    import ctypes
+    dtv = np.asarray(dt.cpu() if isinstance(dt, torch.Tensor) else dt, dtype=np.float64).reshape(-1)
+    n = dtv.shape[0]
+    dev, dims, hyp, keep, p, ip, pinf, st, M, KT = _ih_setup(kernel, n, dtv[0])
+    D = M * KT
+    f64 = dict(dtype=torch.float64, device=dev)
+    fm = _as_tensor(filter_mean, torch.float64, dev).reshape(n, D)[:, p].contiguous()
+    Pd = _as_tensor(filter_cov[0], torch.float64, dev).reshape(D, D)[p][:, p].contiguous()
+    fc = _as_tensor(filter_cov[1], torch.float64, dev).reshape(D, D)[p][:, p].contiguous()
+    X0 = pinf if dare_init is None else _as_tensor(dare_init, torch.float64, dev).reshape(D, D)
+    Xs = X0[p][:, p].contiguous()
+    G = torch.empty((D, D), **f64)
+    sm = torch.empty((n, D), **f64)
+    nb = nat.lib().bn_ih_workspace_bytes(ctypes.byref(dims), 0)
+    nat.check(0 if nb > 0 else -1)
+    ws = torch.empty(int(nb), dtype=torch.uint8, device=dev)
+    nat.check(nat.lib().bn_ih_smoother(ctypes.byref(dims), ctypes.byref(hyp), nat.ptr(st), int(dare_iters),
+                                       nat.ptr(fm), nat.ptr(Pd), nat.ptr(fc), nat.ptr(Xs), nat.ptr(G), None, None,
+                                       nat.ptr(sm), nat.ptr(ws), nat.stream_ptr()))
+    back = lambda A: A[ip][:, ip]  # noqa: E731
+    dare_cov = back(Xs)
+    gain = back(G)
+    if return_full:
+        means, cov = sm[:, ip][..., None], dare_cov
+    else:
+        means, cov = sm[:, :M][..., None], Xs[:M, :M]
+    return means, cov.expand(n, *cov.shape).contiguous(), gain.expand(n, D, D).contiguous(), dare_cov

@@ -356,3 +356,121 @@ class SpaceTimeVariationalMeanFieldGP(SpaceTimeMeanFieldGP):
 class SpaceTimeLaplaceMeanFieldGP(SpaceTimeMeanFieldGP):
     """MarkovNewtonMeanFieldGP / MarkovLaplaceGPMeanField (models.py:307-309, 332)."""
     method = nat.BN_NEWTON
+
+
+def _pinf_temporal(kt):
+    """Stationary state covariance of the temporal Matern12 / Matern32 kernel
+    (kernels.py:146-150, 212-216)."""
+    v, ls = float(kt.variance), float(kt.lengthscale)
+    if kt.state_dim == 1:
+        return np.array([[v]])
+    return np.array([[v, 0.0], [0.0, 3.0 * v / ls ** 2]])
+
+
+def state_perm(M, KT):
+    """Index map between the reference's state order (kron(I_M, .): entry m*KT + c) and
+    this library's (c*M + m): x_here = x_ref[perm]."""
+    return np.array([m * KT + c for c in range(KT) for m in range(M)], dtype=np.int64)
+
+
+class SpaceTimeInfiniteHorizonGP(SpaceTimeMarkovGP):
+    """InfiniteHorizonGaussianProcess (basemodels.py:1257-1301) with a SpatioTemporalKernel:
+    equispaced steps, a steady-state covariance from `dare_iters` Riccati iterations per
+    filter / smoother call, each started from the previous call's solution, as the
+    reference keeps them (dare_init_filter / dare_init_smoother).  Like the reference,
+    inference() runs the filter and smoother before and after the site update and every
+    log-likelihood evaluation (energy) runs the filter again, so the Riccati solutions
+    advance with each call.  Heteroscedastic (per-step gains) unless the likelihood is
+    Gaussian with no missing data (basemodels.py:1270).  `parallel` is accepted: both of the
+    reference's forms compute the same means."""
+
+    def __init__(self, kernel, likelihood, X, Y, R, dare_iters=20, parallel=None, device=None):
+        self.dare_iters = int(dare_iters)
+        super().__init__(kernel, likelihood, X, Y, R, parallel=parallel, device=device)
+
+    @classmethod
+    def from_runs(cls, kernel, likelihood, Xs, Rs, Ys, dare_iters=20, parallel=None, device=None):
+        self = cls.__new__(cls)
+        self.dare_iters = int(dare_iters)
+        self.parallel = parallel
+        self._build(kernel, likelihood, Xs, Rs, Ys, device)
+        self._single = False
+        return self
+
+    def _alloc_filter(self):
+        K, T, M, D = self.K, self.T, self.M, self.D
+        if T < 2:
+            raise ValueError("the infinite-horizon model needs at least two time steps")
+        dt = self._dt.cpu().numpy()
+        if np.max(np.abs(np.diff(dt[:, 1:], axis=1)), initial=0.0) >= 1e-6:
+            raise ValueError("the infinite-horizon model needs equispaced time steps")
+        self.heteroscedastic = bool(np.any(np.isnan(self.Y))) or type(self.likelihood).__name__ != "Gaussian"
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self._step = self._dt[:, 1].contiguous()
+        self._noise = torch.empty((K, T, M, M), **f64)
+        self._ypse = torch.empty((K, T, M), **f64)
+        self._tied = torch.empty((K, M, M), **f64)
+        pinf = np.kron(_pinf_temporal(self.kernel.temporal_kernel), np.eye(M))
+        self._Xf = torch.tensor(pinf, **f64).expand(K, D, D).contiguous()
+        self._Xs = self._Xf.clone()
+        self._cov = torch.empty((K, D, D), **f64)
+        self._gain = torch.empty((K, D, D), **f64)
+        self._fm = torch.empty((K, T, D), **f64)
+        nb = nat.lib().bn_ih_workspace_bytes(ctypes.byref(self._dims), 1 if self.heteroscedastic else 0)
+        nat.check(0 if nb > 0 else -1)
+        self._ws = torch.empty(int(nb), dtype=torch.uint8, device=self.device)
+        self._pseudo_key = None
+
+    def _run_filter(self):
+        h, key = self._hyp()
+        if self._pseudo_key != self._proj_key:
+            nat.check(nat.lib().bn_mf_pseudo(
+                ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._noise),
+                nat.ptr(self._ypse), nat.stream_ptr()))
+            nat.check(nat.lib().bn_ih_tied(ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._tied),
+                                           nat.stream_ptr()))
+            self._pseudo_key = self._proj_key
+        nat.check(nat.lib().bn_ih_filter(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._step), self.dare_iters,
+            1 if self.heteroscedastic else 0, nat.ptr(self._ypse), nat.ptr(self._noise), nat.ptr(self._tied),
+            nat.ptr(self._Xf), nat.ptr(self._cov), nat.ptr(self._fm), nat.ptr(self._ell), nat.ptr(self._loglik),
+            nat.ptr(self._ws), nat.stream_ptr()))
+        self._ll_key = (self._proj_key, key[0], key[1])
+        return h
+
+    def update_posterior(self):
+        self._ensure_projection()
+        h = self._run_filter()
+        nat.check(nat.lib().bn_ih_smoother(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._step), self.dare_iters, nat.ptr(self._fm),
+            nat.ptr(self._Xf), nat.ptr(self._cov), nat.ptr(self._Xs), nat.ptr(self._gain), nat.ptr(self._pm),
+            nat.ptr(self._pP), None, nat.ptr(self._ws), nat.stream_ptr()))
+        self._post_key = self._ll_key
+
+    # the reference re-runs the filter (and advances its Riccati solution) on every call
+    def _ensure_posterior(self):
+        self.update_posterior()
+
+    def _ensure_loglik(self):
+        self._ensure_projection()
+        self._run_filter()
+
+    @property
+    def dare_solutions(self):
+        """(filter Pdare, smoother covariance) in the reference's state order, [D, D] each
+        ([K, D, D] for several runs)."""
+        p = torch.as_tensor(state_perm(self.M, self.kernel.temporal_kernel.state_dim), device=self.device)
+        ip = torch.argsort(p)
+        f = lambda X: X[:, ip][:, :, ip]  # noqa: E731
+        return self._out(f(self._Xf)), self._out(f(self._Xs))
+
+
+class SpaceTimeVariationalInfiniteHorizonGP(SpaceTimeInfiniteHorizonGP):
+    """InfiniteHorizonVariationalGP (models.py:162-164) with a SpatioTemporalKernel."""
+    method = nat.BN_VI
+
+
+class SpaceTimeLaplaceInfiniteHorizonGP(SpaceTimeInfiniteHorizonGP):
+    """InfiniteHorizonNewtonGP / InfiniteHorizonLaplaceGP (models.py:315-317, 336) with a
+    SpatioTemporalKernel."""
+    method = nat.BN_NEWTON

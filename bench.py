@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "spacetime", "spacetime-pass", "gradient", "predict"],
+                    choices=["batched", "spacetime", "spacetime-pass", "meanfield-seq", "meanfield-par", "gradient",
+                             "predict"],
                     help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
                          "time to converge of the d=128 Cox process, learned warm start vs plain; spacetime-pass: "
                          "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; predict: "
@@ -364,12 +365,27 @@ def st_flops_per_step(D, M, R):
             "k_st_sites": 2.0 * (2 * R * M * M)}
 
 
-def bench_spacetime_pass(args):
+def mf_flops_per_step(M):
+    """Algorithmic FP64 flops per time step of the mean-field kernels: the sweep inverse of
+    an M x M matrix (M pivots x M^2 updates) and the M x M product S^-1 v; k_mf_pseudo also
+    forms L L' (M^3 / 3 multiply-adds).  The per-block filters and smoother are O(M KT^3)."""
+    sweep = 2.0 * M ** 3 + 2.0 * M * M
+    return {"k_mf_filter": sweep, "k_mf_sdiag": sweep, "k_mf_pseudo": sweep + 2.0 * M ** 3 / 3}
+
+
+MF_PASS_KERNELS = {"seq": ("k_mf_pseudo", "k_mf_filter", "k_mf_smoother"),
+                   "par": ("k_mf_pseudo", "k_mf_sdiag", "k_mf_blocks", "k_mf_smoother")}
+
+
+def bench_spacetime_pass(args, mean_field=None):
     """Throughput of the d = 128 space-time path: --st-runs runs of the Cox-process shape
     (T = --st-steps, R = 256, 8x8 inducing grid, Matern32 x Matern32), one VI iteration per
     step (site update, projection, filter, smoother, energy), lr = --st-lr.  `value` is
-    filter + smoother time-steps per second over the batch; `roofline` is the smoother's
-    algorithmic FP64 rate (st_flops_per_step) against MI355X's FP64 peak."""
+    filter + smoother time-steps per second over the batch; `roofline` is the dominant
+    kernel's algorithmic FP64 rate (st_flops_per_step) against MI355X's FP64 peak.
+    mean_field "seq" / "par": the same model with the mean-field filter and smoother
+    (MarkovVariationalMeanFieldGP; sequential or associative form), whose filter + smoother
+    time includes forming the pseudo noise (k_mf_pseudo)."""
     import numpy as np
     import torch
     from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, spacetime as S, workloads
@@ -386,10 +402,15 @@ def bench_spacetime_pass(args):
                                       K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
 
     runs = [workloads.c5(seed=2000 + k, T=T, device=dev) for k in range(args.st_runs)]
-    m = S.SpaceTimeVariationalGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
-                                           [d["r"] for d in runs], [d["y"] for d in runs])
+    if mean_field is None:
+        m = S.SpaceTimeVariationalGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
+                                               [d["r"] for d in runs], [d["y"] for d in runs])
+    else:
+        m = S.SpaceTimeVariationalMeanFieldGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
+                                                        [d["r"] for d in runs], [d["y"] for d in runs],
+                                                        parallel=(mean_field == "par"))
     del runs
-    log(f"[spacetime-pass] {m.K} runs x {T} steps ready")
+    log(f"[spacetime-pass] {m.K} runs x {T} steps ready (mean field: {mean_field})")
 
     def step():
         m.inference(lr=args.st_lr, return_terms=False)
@@ -414,9 +435,18 @@ def bench_spacetime_pass(args):
             kern_ms[L_.bn_kernel_name(kid).decode()] = (tot.value / cnt.value, cnt.value)
     L_.bn_timing_enable(0)
     steps = m.K * m.T
-    fs_ms = kern_ms["k_st_filter"][0] + kern_ms["k_st_smoother"][0]
-    flops = st_flops_per_step(m.D, m.M, m.n_loc)
-    dom = max(("k_st_sites", "k_st_filter", "k_st_smoother"), key=lambda k: kern_ms[k][0] * kern_ms[k][1])
+    if mean_field is None:
+        pass_kernels = ("k_st_filter", "k_st_smoother")
+        flops = st_flops_per_step(m.D, m.M, m.n_loc)
+    else:
+        # per VI iteration; k_mf_sdiag runs twice (elements, log likelihood) in the
+        # associative form
+        pass_kernels = MF_PASS_KERNELS[mean_field]
+        flops = {"k_st_sites": st_flops_per_step(m.D, m.M, m.n_loc)["k_st_sites"], **mf_flops_per_step(m.M)}
+    per_iter = {k: kern_ms[k][0] * kern_ms[k][1] / args.steps for k in kern_ms}
+    fs_ms = sum(per_iter[k] for k in pass_kernels if k in per_iter)
+    cands = [k for k in flops if k in kern_ms]
+    dom = max(cands, key=lambda k: kern_ms[k][0] * kern_ms[k][1])
     dom_ms = kern_ms[dom][0]
     achieved = flops[dom] * steps / (dom_ms / 1e3) / 1e12
     cpu = None
@@ -426,7 +456,8 @@ def bench_spacetime_pass(args):
         d = workloads.c5(seed=2000, T=40)
         ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 10.0), OK.Matern32(1.0, 0.2), z, sparse=True,
                                      conditional="Full")
-        om = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI")
+        om = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI",
+                         mean_field=mean_field is not None, parallel=(mean_field == "par"))
         c0 = time.perf_counter()
         n_it = 3
         for _ in range(n_it):
@@ -438,7 +469,8 @@ def bench_spacetime_pass(args):
                "sample": f"oracle/models.py MarkovGP (FP64 NumPy, BLAS threads = cores), one run of 40 steps, "
                          f"{n_it} VI iterations + energy, {csec:.2f}s per iteration"}
     out = {
-        "metric": "space-time VI iteration: time-steps/s over a batch of d=128 runs (filter + smoother kernels)",
+        "metric": ("space-time VI iteration: time-steps/s over a batch of d=128 runs (filter + smoother kernels)"
+                   + ("" if mean_field is None else f", mean-field {mean_field} form")),
         "value": steps / (fs_ms / 1e3),
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -451,7 +483,8 @@ def bench_spacetime_pass(args):
         "dtype": "f64",
         "data": "synthetic (seeded prior draws of the model, Poisson counts)",
         "config": {"workload": f"space-time Cox process: {m.K} runs, T={T}, R={m.n_loc}, M={m.M} (8x8), d={m.D}, "
-                               "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), one VI iteration per step",
+                               "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), one VI iteration per step"
+                               + ("" if mean_field is None else f", mean-field filter/smoother ({mean_field})"),
                    "runs": m.K, "time_steps": T, "state_dim": m.D},
         "iteration_time_steps_per_s": steps / sec,
         "roofline": {"bound": "mfma", "compute": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFS,
@@ -598,6 +631,9 @@ def main():
         return
     if args.workload == "spacetime-pass":
         bench_spacetime_pass(args)
+        return
+    if args.workload in ("meanfield-seq", "meanfield-par"):
+        bench_spacetime_pass(args, mean_field=args.workload.split("-")[1])
         return
     if args.workload == "gradient":
         bench_gradient(args)

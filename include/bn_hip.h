@@ -363,6 +363,40 @@ int bn_mf_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double
                    const double *filter_cov, double *post_mean, double *post_var, int64_t var_step_stride,
                    int64_t var_elem_stride, double *smooth_mean, double *smooth_cov, double *gains, void *stream);
 
+/* --- infinite-horizon space-time filter and smoother: kalman_filter_infinite_horizon /
+ * rauch_tung_striebel_smoother_infinite_horizon (ops.py:796-1068) as used by
+ * InfiniteHorizonGaussianProcess (basemodels.py:1257-1301).  Equispaced steps: one step
+ * size per run (step [K]).  Dense state of dimension D = KT*M in the layout of bn_st_filter
+ * (index c*M + m, H picks the first M entries); covariances and gains [K,D,D]. */
+
+/* Device scratch of bn_ih_filter (and bn_ih_smoother, which needs less), bytes. */
+int64_t bn_ih_workspace_bytes(const bn_st_dims *dims, int32_t heteroscedastic);
+
+/* Tied pseudo noise of InfiniteHorizonGaussianProcess.filter (basemodels.py:1276-1285):
+ * noise_tied [K,M,M] = (mean_t B' diag(nat2_t) B)^-1 from bn_st_sites' whitened Lw. */
+int bn_ih_tied(const bn_st_dims *dims, const double *Lw, double *noise_tied, void *stream);
+
+/* Infinite-horizon Kalman filter (ops.py:877-953).  dare [K,D,D]: the Riccati solution's
+ * start (dare_init) on entry, after `iters` iterations of dare() (ops.py:796-823) on exit
+ * (Pdare).  filter_cov [K,D,D] = Pdare - K H Pdare.  heteroscedastic = 1: per-step gains
+ * from S_t = H Pdare H' + noise_t (noise [K,T,M,M]); 0: the tied gain for every step (noise
+ * unused).  y [K,T,M]; filter_mean [K,T,D]; ell_steps [K,T] log N(y_t | H A m_{t-1}, S_t);
+ * loglik [K].  The associative form composes the same maps (ops.py:849-874). */
+int bn_ih_filter(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *step, int32_t iters,
+                 int32_t heteroscedastic, const double *y, const double *noise, const double *noise_tied, double *dare,
+                 double *filter_cov, double *filter_mean, double *ell_steps, double *loglik, void *workspace,
+                 void *stream);
+
+/* Infinite-horizon RTS smoother (ops.py:1018-1068): gain G = filter_cov A' Pdare^-1 (optional
+ * output [K,D,D]); dare_smoother [K,D,D]: start on entry, after `iters` iterations of
+ * rts_dare (ops.py:956-973) on exit; means sm_t = fm_t + G (sm_{t+1} - A fm_t) from sm =
+ * fm_{T-1}.  Optional outputs: post_mean [K,T,M] = H sm_t, post_cov [K,T,M,M] = H X_s H' at
+ * every step, smooth_mean [K,T,D]. */
+int bn_ih_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *step, int32_t iters,
+                   const double *filter_mean, const double *dare, const double *filter_cov, double *dare_smoother,
+                   double *gain, double *post_mean, double *post_cov, double *smooth_mean, void *workspace,
+                   void *stream);
+
 /* --- learned warm start of the space-time sites.  No counterpart in the reference:
  * it replaces the fixed site initialisation N(0, 1e2 I) (basemodels.py:130-133) via
  * update_nat_params (basemodels.py:97-100); the plain iteration then runs to the same

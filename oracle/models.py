@@ -3,7 +3,7 @@
 Restates the reference's model/inference stack for the Markov GP:
 bayesnewton/basemodels.py:52-262 (sites, BaseModel) and 625-764
 (MarkovGaussianProcess) with prediction 766-816 and 165-193 (utils.py:99-215),
-its mean-field form 1158-1175 (space-time kernels),
+its mean-field form 1158-1175 (space-time kernels), the infinite-horizon model 1257-1301,
 bayesnewton/inference.py:21-325 (newton_update,
 InferenceMixin.inference, Newton/Laplace, VariationalInference,
 ExpectationPropagation) and bayesnewton/models.py:118-132, 216-231, 297-299.
@@ -94,8 +94,12 @@ class MarkovGP:
     """
 
     def __init__(self, kernel, likelihood, X, Y, R=None, parallel=False, inference="VI", power=1.0,
-                 gh_points=20, mean_field=False):
+                 gh_points=20, mean_field=False, infinite_horizon=False, dare_iters=20):
         X, Y, R, dt = input_admin(X, Y, R)
+        # InfiniteHorizonGaussianProcess (basemodels.py:1257-1301): equispaced steps, the
+        # infinite-horizon filter / smoother, each Riccati solve started from the previous one
+        self.infinite_horizon = bool(infinite_horizon)
+        self.dare_iters = int(dare_iters)
         # MarkovMeanFieldGaussianProcess (basemodels.py:1158-1175) swaps in the
         # mean-field filter and smoother (ops.py:581-608, 681-706); nothing else changes.
         self.mean_field = bool(mean_field)
@@ -128,6 +132,13 @@ class MarkovGP:
         if self.func_dim != self.obs_dim and self.spatio_temporal:
             self.mask_pseudo_y = None
         self.last_filter_loglik = None
+        if self.infinite_horizon:
+            if np.max(np.abs(np.diff(self.dt[1:]))) >= 1e-6:
+                raise ValueError("the infinite-horizon model needs equispaced time steps")
+            self.heteroscedastic = bool(np.any(np.isnan(self.Y))) or type(likelihood).__name__ != "Gaussian"
+            Pinf = kernel.stationary_covariance()
+            self.dare_init_filter = Pinf
+            self.dare_init_smoother = Pinf
 
     # ---- sites ----------------------------------------------------------------
     def update_nat_params(self, nat1, nat2):
@@ -163,11 +174,29 @@ class MarkovGP:
         return self.nat1, self.nat2
 
     def _filter(self, *args, **kwargs):
+        if self.infinite_horizon:
+            # basemodels.py:1276-1290
+            if self.spatio_temporal:
+                B, C = self.spatial_conditional()
+                tied = np.mean(T(B) @ self.nat2 @ B, axis=0)
+            else:
+                tied = np.mean(self.nat2, axis=0)
+            out = ops.kalman_filter_infinite_horizon(*args, **kwargs, heteroscedastic=self.heteroscedastic,
+                                                     noise_cov_tied=inv(tied), dare_iters=self.dare_iters,
+                                                     dare_init=self.dare_init_filter)
+            self.dare_init_filter = out[1][1][0]
+            return out
         if self.mean_field:
             return ops.kalman_filter_meanfield(*args, **kwargs)
         return ops.kalman_filter(*args, **kwargs)
 
     def _smoother(self, *args, **kwargs):
+        if self.infinite_horizon:
+            # basemodels.py:1292-1298
+            means, covs, gains, dare_cov = ops.rauch_tung_striebel_smoother_infinite_horizon(
+                *args, **kwargs, dare_iters=self.dare_iters, dare_init=self.dare_init_smoother)
+            self.dare_init_smoother = dare_cov
+            return means, covs, gains
         if self.mean_field:
             return ops.rauch_tung_striebel_smoother_meanfield(*args, **kwargs)
         return ops.rauch_tung_striebel_smoother(*args, **kwargs)

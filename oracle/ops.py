@@ -365,3 +365,117 @@ def rauch_tung_striebel_smoother_meanfield(dt, kernel, filter_mean, filter_cov, 
     H = kernel.measurement_model()
     f = _parallel_rts_mf if parallel else _sequential_rts_mf
     return f(np.asarray(filter_mean, np.float64), np.asarray(filter_cov, np.float64), As, Qs, H, return_full)
+
+
+# ---- infinite-horizon filter / smoother (ops.py:796-1068) ----------------------------
+# A fixed step (dt[1]) and a steady-state covariance from a fixed number of Riccati
+# iterations started at `dare_init`; only the means run over the data.
+
+def dare(A, H, Q, R, Pinit, num_iters=20):
+    """ops.py:796-823: X <- A (X - K H X) A' + Q, K = (S^-1 H X)', S = H X H' + R."""
+    X = np.array(Pinit, dtype=np.float64)
+    for _ in range(num_iters):
+        HX = H @ X
+        S = HX @ H.T + R
+        K = solve(S, HX).T
+        X = A @ (X - K @ HX) @ A.T + Q
+    return X
+
+
+def _ih_affine_scan(Ms, vs):
+    """Prefix compositions x_n = M_n x_{n-1} + v_n from x = 0 (the maps of
+    ops.py:849-874 composed in a tree: (M1, v1) then (M2, v2) -> (M2 M1, M2 v1 + v2))."""
+    def op(e1, e2):
+        M1, v1 = e1
+        M2, v2 = e2
+        return M2 @ M1, M2 @ v1 + v2
+    return associative_scan(op, [Ms, vs])[1]
+
+
+def kalman_filter_infinite_horizon(dt, kernel, y, noise_cov, mask=None, parallel=False, heteroscedastic=False,
+                                   noise_cov_tied=None, dare_iters=20, dare_init=None):
+    """ops.py:877-953: returns ell, (means [N,d,1], (Pdare, cov)).  The associative form
+    composes the same affine maps (ops.py:849-874; its heteroscedastic elements are their
+    inverses), so it is restated by the scan of the maps themselves."""
+    y = np.asarray(y, dtype=np.float64)
+    noise_cov = np.asarray(noise_cov, dtype=np.float64)
+    if mask is None:
+        mask = np.zeros_like(y, dtype=bool)
+    mask = np.asarray(mask, dtype=bool).reshape(y.shape)
+    dt = np.asarray(dt, dtype=np.float64).reshape(-1)
+    Pinf = kernel.stationary_covariance()
+    m0 = np.zeros([Pinf.shape[0], 1])
+    A = kernel.state_transition(dt[1])
+    Q = process_noise_covariance(A, Pinf)
+    H = kernel.measurement_model()
+    Pdare = dare(A, H, Q, noise_cov_tied, Pinf if dare_init is None else dare_init, dare_iters)
+    S = H @ Pdare @ H.T + noise_cov_tied
+    K = Pdare @ solve(S, H).T
+    HA = H @ A
+    N = dt.shape[0]
+    if heteroscedastic:
+        Ss = H @ Pdare @ H.T + noise_cov
+        Ks = Pdare @ T(solve(Ss, np.broadcast_to(H, (N,) + H.shape)))
+        AKHAs = A - Ks @ HA
+    else:
+        Ss = np.broadcast_to(S, (N,) + S.shape)
+        Ks = np.broadcast_to(K, (N,) + K.shape)
+        AKHAs = np.broadcast_to(A - K @ HA, (N,) + A.shape)
+    Kys = Ks @ y
+    cov = Pdare - K @ H @ Pdare
+    if parallel:
+        means = _ih_affine_scan(np.array(AKHAs), np.array(Kys))
+    else:
+        means = np.empty((N,) + m0.shape)
+        m = m0
+        for n in range(N):
+            m = AKHAs[n] @ m + Kys[n]
+            means[n] = m
+    obs_means = HA @ np.concatenate([m0[None], means[:-1]])
+    ell = float(np.sum(mvn_logpdf(y, obs_means, np.array(Ss), mask)))
+    return ell, (means, (Pdare, cov))
+
+
+def rts_dare(A, Q, Pinf, num_iters=20):
+    """ops.py:956-973: X <- A X A' + Q."""
+    X = np.array(Pinf, dtype=np.float64)
+    for _ in range(num_iters):
+        X = A @ X @ A.T + Q
+    return X
+
+
+def rauch_tung_striebel_smoother_infinite_horizon(dt, kernel, filter_mean, filter_cov, return_full=False,
+                                                  parallel=False, dare_iters=20, dare_init=None):
+    """ops.py:1018-1068 (dt is the smoother's step vector; A from its first entry):
+    returns means, covs, gains, dare_cov.  Both forms run the same constant-gain
+    recursion, sm_n = fm_n + G (sm_{n+1} - A fm_n) from sm = fm_{N-1}."""
+    dt = np.asarray(dt, dtype=np.float64).reshape(-1)
+    Pinf = kernel.stationary_covariance()
+    A = kernel.state_transition(dt[0])
+    H = kernel.measurement_model()
+    N = dt.shape[0]
+    Pdare, fcov = filter_cov
+    gain = fcov @ solve(Pdare, A).T
+    Qdare = fcov - gain @ Pdare @ gain.T
+    dare_cov = rts_dare(gain, Qdare, Pinf if dare_init is None else dare_init, dare_iters)
+    fms = np.asarray(filter_mean, dtype=np.float64)
+    Afms = A @ fms
+    if parallel:
+        # ops.py:990-1012: elements (G, fm - G A fm) and a last (0, fm_{N-1}), reverse scan
+        Gs = np.concatenate([np.broadcast_to(gain, (N,) + gain.shape), np.zeros((1,) + gain.shape)])
+        vs = np.concatenate([fms - gain @ Afms, fms[-1:]])
+        final = associative_scan(lambda e1, e2: (e2[0] @ e1[0], e2[0] @ e1[1] + e2[1]), [Gs, vs], reverse=True)
+        sms = final[1][:-1]
+    else:
+        sms = np.empty_like(fms)
+        sm = fms[-1]
+        for n in range(N - 1, -1, -1):
+            sm = fms[n] + gain @ (sm - Afms[n])
+            sms[n] = sm
+    if return_full:
+        means, cov = sms, dare_cov
+    else:
+        means, cov = H @ sms, H @ dare_cov @ H.T
+    covs = np.broadcast_to(cov, (N,) + cov.shape).copy()
+    gains = np.broadcast_to(gain, (N,) + gain.shape).copy()
+    return means, covs, gains, dare_cov
