@@ -88,7 +88,7 @@ SIGNATURES = {
     "bn_mf_filter": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_mf_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, vp]),
     "bn_ih_workspace_bytes": (i64, [vp, i32]),
-    "bn_ih_tied": (i32, [vp, vp, vp, vp]),
+    "bn_ih_tied": (i32, [vp, vp, vp, vp, vp]),
     "bn_ih_filter": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_ih_smoother": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_ws_features": (i32, [vp, vp, vp, vp, f64, f64, f64, f64, vp, vp]),

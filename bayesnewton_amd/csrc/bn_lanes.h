@@ -71,8 +71,16 @@ BN_HD Model<K1, K2> load_model(const HyperView& hv, int32_t b) {
 #ifndef BN_STATE_STORE
 #define BN_STATE_STORE double
 #endif
+// Storage of the per-step rows the Bryson-Frazier smoother reads (H P-, H m-): FP32.
+// They enter the smoothed moments linearly and the chunk-local recursion runs at most
+// one chunk from its FP64 anchor, so their rounding stays at the 1e-7 level of the
+// FP32 outputs; FP64 rows cost 24 B more per step in each direction.
+#ifndef BN_HROW_STORE
+#define BN_HROW_STORE float
+#endif
 using agg_t = BN_AGG_STORE;
 using state_t = BN_STATE_STORE;
+using hrow_t = BN_HROW_STORE;
 
 // Field-major (structure-of-arrays) element storage: field f of item i at base[f * stride + i].
 struct SoA {
@@ -451,7 +459,7 @@ struct FilterPassOut {
   // (D + 1) planes: H P- (D) and H m- at every step, all the Bryson-Frazier smoother
   // reads per step; when set, `state` is written only at each chunk's first step (the
   // smoother's anchor).  nullptr: `state` at every step.
-  state_t* hrow = nullptr;
+  hrow_t* hrow = nullptr;
 };
 
 // filter_pass: the filter over one chunk from its true incoming state
@@ -546,8 +554,8 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     T hm = T(0);
     BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
     if (out.hrow != nullptr) {
-      BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (state_t)hP[q];
-      out.hrow[D * pl.n_slots + s] = (state_t)hm;
+      BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
+      out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
     }
     const double vd = (double)y[s] - (double)hm;
     const double iSd = 1.0 / Sd;
@@ -782,7 +790,7 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
 template <int K1, int K2, class T>
 BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                              const float* __restrict__ y, const float* __restrict__ r, CSoA sagg, CSoA scarry,
-                             const CStateIO& fin, const state_t* __restrict__ hrow, float* __restrict__ out_mean,
+                             const CStateIO& fin, const hrow_t* __restrict__ hrow, float* __restrict__ out_mean,
                              float* __restrict__ out_var, int32_t j) {
   using S = Spec<K1, K2>;
   constexpr int D = S::D;

@@ -234,7 +234,7 @@ template <int K1, int K2>
 __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                          const float* __restrict__ y, const float* __restrict__ r,
                                                          CSoA sagg, CSoA scarry, CStateIO fin,
-                                                         const state_t* __restrict__ hrow, float* pmean,
+                                                         const hrow_t* __restrict__ hrow, float* pmean,
                                                          float* pvar) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
@@ -729,7 +729,7 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   // recursion runs on them without factorising a covariance per step
   fo.state = StateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true};
   fo.predict = true;
-  fo.hrow = b.fbuf + (int64_t)(D + D * (D + 1) / 2) * p.n_slots;
+  fo.hrow = (hrow_t*)(b.fbuf + (int64_t)(D + D * (D + 1) / 2) * p.n_slots);
   fo.sagg = b.sagg;
   fo.ell = b.ell;
   {

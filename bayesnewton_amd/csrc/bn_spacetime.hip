@@ -957,29 +957,23 @@ __global__ __launch_bounds__(kT) void k_st_sum(int32_t T, const double* __restri
 // ---------------------------------------------------------------------------
 
 // Pseudo observation of compute_full_pseudo_lik (basemodels.py:676-687) from the whitened
-// form: R = (L L')^-1, y = R L z.  Block per (step, run).
+// form: R = (L L')^-1, y = R L z.  Block per (step, run); L L' on the matrix cores, the
+// inverse by the sweep operator in LDS.
 __global__ __launch_bounds__(kT) void k_mf_pseudo(StDims dm, const double* __restrict__ Lw,
                                                   const double* __restrict__ zw, double* __restrict__ R,
                                                   double* __restrict__ y) {
   extern __shared__ double smem[];
-  double* sL = smem;                  // M*M
-  double* sW = sL + kMaxM * kMaxM;    // M*M
+  double* pan = smem;                 // 2*kPK*kPS
+  double* sW = pan + 2 * kPK * kPS;   // M*M
   double* row = sW + kMaxM * kMaxM;   // 2*kMaxM
   double* sv = row + 2 * kMaxM;       // M
   const int t = blockIdx.x, k = blockIdx.y, M = dm.M, tid = threadIdx.x;
   const int64_t kt = (int64_t)k * dm.T + t;
-  const double* L = Lw + kt * M * M;
-  for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
-  __syncthreads();
-  for (int e = tid; e < M * M; e += kT) {
-    const int i = e / M, j = e % M, qe = min(i, j);
-    double s = 0.0;
-    for (int q = 0; q <= qe; ++q) s = fma(sL[i * M + q], sL[j * M + q], s);
-    sW[e] = s;
-  }
+  const double* L = Lw + kt * M * M;  // lower triangular, upper part zero
+  wg_gemm(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 0.0, sW, M, pan);
   if (tid < M) {
     double s = 0.0;
-    for (int q = 0; q <= tid; ++q) s = fma(sL[tid * M + q], zw[kt * M + q], s);
+    for (int q = 0; q <= tid; ++q) s = fma(L[tid * M + q], zw[kt * M + q], s);
     sv[tid] = s;
   }
   wg_sweep_inv<kMaxM / 16, false>(sW, M, M, row);  // sW = -(L L')^-1
@@ -1275,7 +1269,9 @@ __global__ __launch_bounds__(64) void k_mf_smoother(StDims dm, StHyp hy, MfSmoot
 // Infinite-horizon space-time filter and smoother (ops.py:796-1068, the filter and
 // smoother of InfiniteHorizonGaussianProcess, basemodels.py:1257-1301).  Equispaced steps:
 // one A, Q per run (its step size).  Per run:
-//   k_ih_tied     R_tied = (mean_t B' diag(nat2_t) B)^-1 from the whitened form (L L' - 1e-12 I)
+//   k_ih_tied_part, k_ih_tied
+//                 R_tied = (mean_t B' diag(nat2_t) B)^-1 from the whitened form (L L' - 1e-12 I),
+//                 partial sums over step ranges on the matrix cores
 //   k_ih_dare     `iters` Riccati iterations X <- A (X - X H' S^-1 H X) A' + Q, S = H X H' + R_tied,
 //                 from the previous solution; S_tied^-1 and cov = X - K H X (K = X H' S^-1)
 //   k_ih_sinv     heteroscedastic: S_t = H X H' + R_t inverted per (step, run), log det S_t
@@ -1285,36 +1281,37 @@ __global__ __launch_bounds__(64) void k_mf_smoother(StDims dm, StHyp hy, MfSmoot
 //                 sm = fm_{T-1}; H X_s H' at every step.
 // State layout as in k_st_filter: index c*M + m, H picks the c = 0 block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kT) void k_ih_tied(StDims dm, const double* __restrict__ Lw, double* __restrict__ Rt) {
-  __shared__ double sL[kMaxM * kMaxM];
-  __shared__ double row[2 * kMaxM];
-  constexpr int kE = kMaxM * kMaxM / kT;  // entries per thread
-  const int k = blockIdx.x, M = dm.M, tid = threadIdx.x;
-  double acc[kE];
-#pragma unroll
-  for (int q = 0; q < kE; ++q) acc[q] = 0.0;
-  for (int t = 0; t < dm.T; ++t) {
+constexpr int kTiedParts = 32;  // step ranges summed in parallel per run
+
+// part[k, p] = sum of L_t L_t' over the p-th of kTiedParts step ranges of run k.
+__global__ __launch_bounds__(kT) void k_ih_tied_part(StDims dm, const double* __restrict__ Lw,
+                                                     double* __restrict__ part) {
+  extern __shared__ double smem[];
+  double* pan = smem;                 // 2*kPK*kPS
+  double* acc = pan + 2 * kPK * kPS;  // M*M
+  const int p = blockIdx.x, k = blockIdx.y, M = dm.M, tid = threadIdx.x;
+  const int t0 = (int)((int64_t)dm.T * p / kTiedParts), t1 = (int)((int64_t)dm.T * (p + 1) / kTiedParts);
+  for (int e = tid; e < M * M; e += kT) acc[e] = 0.0;
+  for (int t = t0; t < t1; ++t) {
     const double* L = Lw + ((int64_t)k * dm.T + t) * M * M;
-    __syncthreads();
-    for (int e = tid; e < M * M; e += kT) sL[e] = L[e];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kE; ++q) {
-      const int e = tid + q * kT;
-      if (e < M * M) {
-        const int i = e / M, j = e % M, qe = min(i, j);
-        double s = 0.0;
-        for (int p = 0; p <= qe; ++p) s = fma(sL[i * M + p], sL[j * M + p], s);
-        acc[q] += s;
-      }
-    }
+    wg_gemm(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 1.0, acc, M, pan);
   }
   __syncthreads();
+  double* out = part + ((int64_t)k * kTiedParts + p) * M * M;
+  for (int e = tid; e < M * M; e += kT) out[e] = acc[e];
+}
+
+// R_tied = (mean_t L_t L_t' - 1e-12 I)^-1 = (mean_t B' diag(nat2_t) B)^-1, the partial
+// sums added in a fixed order.
+__global__ __launch_bounds__(kT) void k_ih_tied(StDims dm, const double* __restrict__ part, double* __restrict__ Rt) {
+  __shared__ double sL[kMaxM * kMaxM];
+  __shared__ double row[2 * kMaxM];
+  const int k = blockIdx.x, M = dm.M, tid = threadIdx.x;
   const double it = 1.0 / dm.T;
-#pragma unroll
-  for (int q = 0; q < kE; ++q) {
-    const int e = tid + q * kT;
-    if (e < M * M) sL[e] = acc[q] * it - ((e / M == e % M) ? kPseudoJitter : 0.0);
+  for (int e = tid; e < M * M; e += kT) {
+    double s = 0.0;
+    for (int p = 0; p < kTiedParts; ++p) s += part[((int64_t)k * kTiedParts + p) * M * M + e];
+    sL[e] = s * it - ((e / M == e % M) ? kPseudoJitter : 0.0);
   }
   wg_sweep_inv<kMaxM / 16, false>(sL, M, M, row);  // -(tied precision)^-1
   for (int e = tid; e < M * M; e += kT) Rt[(int64_t)k * M * M + e] = -sL[e];
@@ -1539,7 +1536,8 @@ __global__ __launch_bounds__(kT) void k_ih_smoother(StDims dm, StHyp hy, IhSmIO 
 
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
-constexpr size_t kPseudoSmem = sizeof(double) * (2 * kMaxM * kMaxM + 3 * kMaxM);
+constexpr size_t kPseudoSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 3 * kMaxM);
+constexpr size_t kTiedSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM);
 constexpr size_t kIhDareSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM);
 constexpr size_t kSitesSmem =
     sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
@@ -1830,18 +1828,23 @@ int64_t bn_ih_workspace_bytes(const bn_st_dims* dims, int32_t heteroscedastic) {
   StDims dm;
   if (check_dims(dims, dm) != BN_OK) return -1;
   const int64_t K = dm.K, D = dm.D, M = dm.M, T = dm.T;
-  int64_t n = 3 * K * D * D + K * M * M + K;
+  int64_t n = 3 * K * D * D + K * M * M + K + K * kTiedParts * M * M;
   if (heteroscedastic) n += K * T * M * M + K * T;
   return n * (int64_t)sizeof(double) + 256;
 }
 
-int bn_ih_tied(const bn_st_dims* dims, const double* Lw, double* noise_tied, void* stream) {
+int bn_ih_tied(const bn_st_dims* dims, const double* Lw, double* noise_tied, void* workspace, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
-  if (!Lw || !noise_tied) return st_fail(BN_EINVAL, "NULL argument to bn_ih_tied");
+  if (!Lw || !noise_tied || !workspace) return st_fail(BN_EINVAL, "NULL argument to bn_ih_tied");
+  // partial sums after bn_ih_filter's regions (see bn_ih_workspace_bytes)
+  const int64_t K = dm.K, D = dm.D, M = dm.M;
+  double* part = (double*)workspace + 3 * K * D * D + K * M * M + K;
   hipStream_t st = (hipStream_t)stream;
   BN_TIMED(KID_IH_TIED, st);
-  hipLaunchKernelGGL(k_ih_tied, dim3(dm.K), dim3(kT), 0, st, dm, Lw, noise_tied);
+  hipLaunchKernelGGL(k_ih_tied_part, dim3(kTiedParts, dm.K), dim3(kT), kTiedSmem, st, dm, Lw, part);
+  ST_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_ih_tied, dim3(dm.K), dim3(kT), 0, st, dm, part, noise_tied);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -1870,7 +1873,7 @@ int bn_ih_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* 
   io.W2 = ws + K * D * D;
   io.STn = ws + 3 * K * D * D;
   io.ldT = io.STn + K * M * M;
-  io.Sn = heteroscedastic ? io.ldT + K : nullptr;
+  io.Sn = heteroscedastic ? io.ldT + K + K * kTiedParts * M * M : nullptr;
   io.ldS = heteroscedastic ? io.Sn + K * dm.T * M * M : nullptr;
   io.fmean = filter_mean;
   io.ell = ell_steps;

@@ -428,7 +428,7 @@ class SpaceTimeInfiniteHorizonGP(SpaceTimeMarkovGP):
                 ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._noise),
                 nat.ptr(self._ypse), nat.stream_ptr()))
             nat.check(nat.lib().bn_ih_tied(ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._tied),
-                                           nat.stream_ptr()))
+                                           nat.ptr(self._ws), nat.stream_ptr()))
             self._pseudo_key = self._proj_key
         nat.check(nat.lib().bn_ih_filter(
             ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._step), self.dare_iters,

@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "spacetime", "spacetime-pass", "meanfield-seq", "meanfield-par", "gradient",
-                             "predict"],
+                    choices=["batched", "spacetime", "spacetime-pass", "meanfield-seq", "meanfield-par",
+                             "infinite-horizon", "gradient", "predict"],
                     help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
                          "time to converge of the d=128 Cox process, learned warm start vs plain; spacetime-pass: "
                          "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; predict: "
@@ -373,8 +373,11 @@ def mf_flops_per_step(M):
     return {"k_mf_filter": sweep, "k_mf_sdiag": sweep, "k_mf_pseudo": sweep + 2.0 * M ** 3 / 3}
 
 
-MF_PASS_KERNELS = {"seq": ("k_mf_pseudo", "k_mf_filter", "k_mf_smoother"),
-                   "par": ("k_mf_pseudo", "k_mf_sdiag", "k_mf_blocks", "k_mf_smoother")}
+# kernels of one filter + smoother pass and their launches per pass
+MF_PASS_KERNELS = {"seq": {"k_mf_pseudo": 1, "k_mf_filter": 1, "k_mf_smoother": 1},
+                   "par": {"k_mf_pseudo": 1, "k_mf_sdiag": 2, "k_mf_blocks": 1, "k_mf_smoother": 1},
+                   "ih": {"k_mf_pseudo": 1, "k_ih_tied": 1, "k_ih_dare": 1, "k_ih_sinv": 1, "k_ih_filter": 1,
+                          "k_ih_smoother": 1}}
 
 
 def bench_spacetime_pass(args, mean_field=None):
@@ -384,8 +387,10 @@ def bench_spacetime_pass(args, mean_field=None):
     filter + smoother time-steps per second over the batch; `roofline` is the dominant
     kernel's algorithmic FP64 rate (st_flops_per_step) against MI355X's FP64 peak.
     mean_field "seq" / "par": the same model with the mean-field filter and smoother
-    (MarkovVariationalMeanFieldGP; sequential or associative form), whose filter + smoother
-    time includes forming the pseudo noise (k_mf_pseudo)."""
+    (MarkovVariationalMeanFieldGP; sequential or associative form); "ih": the
+    infinite-horizon model (InfiniteHorizonVariationalGP, 20 Riccati iterations per call).
+    Their filter + smoother time includes forming the pseudo noise (k_mf_pseudo) and, for
+    "ih", the tied noise and the Riccati iterations."""
     import numpy as np
     import torch
     from bayesnewton_amd import _native as nat, kernels as K, likelihoods as L, spacetime as S, workloads
@@ -405,6 +410,9 @@ def bench_spacetime_pass(args, mean_field=None):
     if mean_field is None:
         m = S.SpaceTimeVariationalGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
                                                [d["r"] for d in runs], [d["y"] for d in runs])
+    elif mean_field == "ih":
+        m = S.SpaceTimeVariationalInfiniteHorizonGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
+                                                              [d["r"] for d in runs], [d["y"] for d in runs])
     else:
         m = S.SpaceTimeVariationalMeanFieldGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
                                                         [d["r"] for d in runs], [d["y"] for d in runs],
@@ -436,15 +444,15 @@ def bench_spacetime_pass(args, mean_field=None):
     L_.bn_timing_enable(0)
     steps = m.K * m.T
     if mean_field is None:
-        pass_kernels = ("k_st_filter", "k_st_smoother")
+        pass_kernels = {"k_st_filter": 1, "k_st_smoother": 1}
         flops = st_flops_per_step(m.D, m.M, m.n_loc)
     else:
-        # per VI iteration; k_mf_sdiag runs twice (elements, log likelihood) in the
-        # associative form
         pass_kernels = MF_PASS_KERNELS[mean_field]
         flops = {"k_st_sites": st_flops_per_step(m.D, m.M, m.n_loc)["k_st_sites"], **mf_flops_per_step(m.M)}
-    per_iter = {k: kern_ms[k][0] * kern_ms[k][1] / args.steps for k in kern_ms}
-    fs_ms = sum(per_iter[k] for k in pass_kernels if k in per_iter)
+        if mean_field == "ih":
+            flops = {"k_st_sites": flops["k_st_sites"], "k_ih_sinv": flops["k_mf_sdiag"],
+                     "k_mf_pseudo": flops["k_mf_pseudo"]}
+    fs_ms = sum(kern_ms[k][0] * n for k, n in pass_kernels.items() if k in kern_ms)
     cands = [k for k in flops if k in kern_ms]
     dom = max(cands, key=lambda k: kern_ms[k][0] * kern_ms[k][1])
     dom_ms = kern_ms[dom][0]
@@ -457,7 +465,8 @@ def bench_spacetime_pass(args, mean_field=None):
         ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 10.0), OK.Matern32(1.0, 0.2), z, sparse=True,
                                      conditional="Full")
         om = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI",
-                         mean_field=mean_field is not None, parallel=(mean_field == "par"))
+                         mean_field=mean_field in ("seq", "par"), parallel=(mean_field == "par"),
+                         infinite_horizon=(mean_field == "ih"))
         c0 = time.perf_counter()
         n_it = 3
         for _ in range(n_it):
@@ -470,7 +479,8 @@ def bench_spacetime_pass(args, mean_field=None):
                          f"{n_it} VI iterations + energy, {csec:.2f}s per iteration"}
     out = {
         "metric": ("space-time VI iteration: time-steps/s over a batch of d=128 runs (filter + smoother kernels)"
-                   + ("" if mean_field is None else f", mean-field {mean_field} form")),
+                   + {None: "", "seq": ", mean-field sequential form", "par": ", mean-field associative form",
+                      "ih": ", infinite-horizon"}[mean_field]),
         "value": steps / (fs_ms / 1e3),
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -484,7 +494,10 @@ def bench_spacetime_pass(args, mean_field=None):
         "data": "synthetic (seeded prior draws of the model, Poisson counts)",
         "config": {"workload": f"space-time Cox process: {m.K} runs, T={T}, R={m.n_loc}, M={m.M} (8x8), d={m.D}, "
                                "Matern32(l=10) x Matern32(l=0.2), Poisson(binsize=2), one VI iteration per step"
-                               + ("" if mean_field is None else f", mean-field filter/smoother ({mean_field})"),
+                               + {None: "", "seq": ", mean-field filter/smoother (sequential)",
+                                  "par": ", mean-field filter/smoother (associative)",
+                                  "ih": ", infinite-horizon filter/smoother (20 Riccati iterations per call)"}[
+                                   mean_field],
                    "runs": m.K, "time_steps": T, "state_dim": m.D},
         "iteration_time_steps_per_s": steps / sec,
         "roofline": {"bound": "mfma", "compute": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFS,
@@ -634,6 +647,9 @@ def main():
         return
     if args.workload in ("meanfield-seq", "meanfield-par"):
         bench_spacetime_pass(args, mean_field=args.workload.split("-")[1])
+        return
+    if args.workload == "infinite-horizon":
+        bench_spacetime_pass(args, mean_field="ih")
         return
     if args.workload == "gradient":
         bench_gradient(args)

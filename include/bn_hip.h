@@ -373,8 +373,9 @@ int bn_mf_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double
 int64_t bn_ih_workspace_bytes(const bn_st_dims *dims, int32_t heteroscedastic);
 
 /* Tied pseudo noise of InfiniteHorizonGaussianProcess.filter (basemodels.py:1276-1285):
- * noise_tied [K,M,M] = (mean_t B' diag(nat2_t) B)^-1 from bn_st_sites' whitened Lw. */
-int bn_ih_tied(const bn_st_dims *dims, const double *Lw, double *noise_tied, void *stream);
+ * noise_tied [K,M,M] = (mean_t B' diag(nat2_t) B)^-1 from bn_st_sites' whitened Lw.  workspace:
+ * bn_ih_workspace_bytes (the same buffer bn_ih_filter uses; its contents are not kept). */
+int bn_ih_tied(const bn_st_dims *dims, const double *Lw, double *noise_tied, void *workspace, void *stream);
 
 /* Infinite-horizon Kalman filter (ops.py:877-953).  dare [K,D,D]: the Riccati solution's
  * start (dare_init) on entry, after `iters` iterations of dare() (ops.py:796-823) on exit
