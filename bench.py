@@ -29,12 +29,14 @@ FP64_PEAK_TFS = 78.6
 D = 5
 # Algorithmic bytes per time step (FP32 words, d = 5): dt, y, r in, smoothed mean
 # and variance out, filtered mean + packed covariance written once and read
-# once: 4*(3+2) + 2*4*(d + d(d+1)/2) = 180 B per filter+smoother step.  Split by
-# kernel: the filter pass moves dt, y, r in and the filtered moments out
-# (12 + 80 = 92 B), the smoother pass reads them back and writes mean / variance
-# (80 + 8 = 88 B), the chunk reduction reads dt, y, r (12 B).
+# once: 4*(3+2) + 2*4*(d + d(d+1)/2) = 180 B per filter+smoother step (the
+# sequential filter-then-smoother's own I/O).  Per kernel of this pass: the chunk
+# reduction reads dt, y, r (12 B); the filter pass reads them and writes the
+# smoother's FP32 rows H P- (d) and H m- (12 + 24 = 36 B); the Bryson-Frazier
+# smoother reads inputs and rows and writes mean / variance (12 + 24 + 8 = 44 B);
+# the two-pass RTS form (k_smoother_pass) reads the filtered moments (80 + 8 = 88 B).
 PASS_BYTES_PER_STEP = 180.0
-ALG_BYTES = {"k_filter_reduce": 12.0, "k_filter_pass": 92.0, "k_smoother_pass": 88.0, "k_smoother_mbf": 88.0}
+ALG_BYTES = {"k_filter_reduce": 12.0, "k_filter_pass": 36.0, "k_smoother_pass": 88.0, "k_smoother_mbf": 44.0}
 # Algorithmic FP64 flops per time step (dense d x d algebra, H a 1 x d row,
 # mul+add = 2), counted on the recurrence each kernel runs:
 #  k_filter_reduce (chunk filtering element, ops.py:183-219 folded step by step):
