@@ -831,6 +831,9 @@ struct StSmootherIO {
   double* T1;
   double* pm;           // [K,T,M]
   double* pP;           // [K,T,M,M]
+  double* smean;        // optional [K,T,D] smoothed state (return_full)
+  double* scov;         // optional [K,T,D,D]
+  double* scross;       // optional [K,T,D,D]: G_t P_s,t+1 = Cov(x_t, x_t+1 | y); Pinf at the last step
 };
 
 template <int KT>
@@ -861,6 +864,10 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
   auto emit = [&](int64_t kt) {
     for (int e = tid; e < M; e += kT) io.pm[kt * M + e] = ms[e];
     for (int e = tid; e < M * M; e += kT) io.pP[kt * M * M + e] = Ps[(int64_t)(e / M) * D + e % M];
+    if (io.smean) {
+      for (int e = tid; e < D; e += kT) io.smean[kt * D + e] = ms[e];
+      for (int64_t e = tid; e < DD; e += kT) io.scov[kt * DD + e] = Ps[e];
+    }
   };
   {
     const int64_t kt = (int64_t)k * dm.T + dm.T - 1;
@@ -868,6 +875,14 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
     for (int e = tid; e < D; e += kT) ms[e] = io.fmean[kt * D + e];
     __syncthreads();
     emit(kt);
+    if (io.scross) {
+      Model<KT, 0> mdl;
+      mdl.init(ls, var, 1.0, 1.0);
+      for (int64_t e = tid; e < DD; e += kT) {
+        const int i = (int)(e / D), j = (int)(e % D);
+        io.scross[kt * DD + e] = (i % M == j % M) ? mdl.Pinf[i / M][j / M] : 0.0;
+      }
+    }
   }
   for (int t = dm.T - 2; t >= 0; --t) {
     const int64_t kt = (int64_t)k * dm.T + t;
@@ -910,6 +925,7 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
     (void)wg_sweep_inv<kMaxD / 16, false>(sL, D, D, row);  // sL = -(P-)^-1
     // T1 = G' = (P-)^-1 A P ; GT = G (Ps' - P-) ; Ps = P + GT G'
     wg_gemm(D, D, D, -1.0, MatRef{sL, D, 1}, MatRef{GT, D, 1}, 0.0, T1, D, pan);
+    if (io.scross) wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Ps, D, 1}, 0.0, io.scross + kt * DD, D, pan);
     wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Dl, D, 1}, 0.0, GT, D, pan);
     for (int e = tid; e < D; e += kT) dv[e] = ms[e] - mp[e];
     wg_gemm_128(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{T1, D, 1}, 1.0, F, D, Ps, D, pan);
@@ -1534,6 +1550,173 @@ __global__ __launch_bounds__(kT) void k_ih_smoother(StDims dm, StHyp hy, IhSmIO 
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_st_predict: MarkovGaussianProcess.predict on the space-time branch
+// (basemodels.py:766-816 with temporal_conditional, utils.py:125-141, 185-215), block per
+// (test time, run).  The prior is Kronecker in (temporal state) x (latent), so the
+// conditional of x(t*) on the two neighbouring smoothed states is P_t (x) I_M, T_t (x) I_M
+// with the KT-dim statistics P_t [KT x 2KT], T_t of the temporal kernel (the reference's
+// 1e-8 jitter is per diagonal entry either way).  Only H x(t*) is needed:
+//   H m* = sum_u P_t[0][u] m_j[u],  S = sum_uv P_t[0][u] P_t[0][v] Sig_j[u][v] + T_t[0][0] I
+// over the 2KT M x M blocks of the joint (m_j, Sig_j) of the neighbours (cross block
+// G_t P_s,t+1 from the smoother); then f(t*, r) has mean b_r' H m* and variance
+// b_r' S b_r + c_r (the diagonal of W P W' + C).  Sentinels at -+1e10 carry (0, Pinf).
+// ---------------------------------------------------------------------------
+struct StPredIO {
+  const double* x;       // [K,T] training times
+  const double* xs;      // [K,NS] test times
+  const double* B;       // [K,NS,RS,M] spatial conditional rows of the test locations
+  const double* cdiag;   // [K,NS,RS]
+  const double* smean;   // [K,T,D]
+  const double* scov;    // [K,T,D,D]
+  const double* scross;  // [K,T,D,D]
+  double* mean;          // [K,NS,RS]
+  double* var;           // [K,NS,RS]
+  int32_t ns, rs;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_st_predict(StDims dm, StHyp hy, StPredIO io) {
+  __shared__ double sS[kMaxM * kMaxM];
+  __shared__ double hm[kMaxM];
+  __shared__ double pc[2 * KT + 1];  // P_t[0][u], then T_t[0][0]
+  __shared__ int sidx;
+  const int i = blockIdx.x, k = blockIdx.y, M = dm.M, D = dm.D, T = dm.T, tid = threadIdx.x;
+  const int64_t DD = (int64_t)D * D;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  const double* xk = io.x + (int64_t)k * T;
+  const double ts = io.xs[(int64_t)k * io.ns + i];
+  Model<KT, 0> mdl;
+  mdl.init(ls, var, 1.0, 1.0);
+  if (tid == 0) {
+    // augmented grid [-1e10, x_0 .. x_T-1, 1e10]; ind = searchsorted(left) - 1
+    int lo = 0, hi = T;  // first training index with x >= ts
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (xk[mid] < ts) lo = mid + 1;
+      else hi = mid;
+    }
+    const int ind = lo;  // interval (aug[ind], aug[ind + 1]]
+    sidx = ind;
+    const double xl = ind == 0 ? -1e10 : xk[ind - 1];
+    const double xr = ind == T ? 1e10 : xk[ind];
+    double Af[KT][KT], Qf[KT][KT], Ab[KT][KT], Qb[KT][KT];
+    mdl.discretise(ts - xl, Af, Qf);
+    mdl.discretise(xr - ts, Ab, Qb);
+    // Q-+ = Qb + Ab Qf Ab' + 1e-8 I ; V = Q-+^-1 Ab ; T = Qf - (Ab Qf)' V Qf ; W = Qf V'
+    double AQ[KT][KT], Qm[KT][KT], V[KT][KT], W[KT][KT], Pt[KT][2 * KT];
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(Ab[a][q], Qf[q][b], s);
+        AQ[a][b] = s;
+      }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = Qb[a][b] + (a == b ? 1e-8 : 0.0);
+        for (int q = 0; q < KT; ++q) s = fma(AQ[a][q], Ab[b][q], s);
+        Qm[a][b] = s;
+      }
+    double iQ[KT][KT];
+    if (KT == 1) {
+      iQ[0][0] = 1.0 / Qm[0][0];
+    } else {
+      const double det = Qm[0][0] * Qm[KT - 1][KT - 1] - Qm[0][KT - 1] * Qm[KT - 1][0];
+      iQ[0][0] = Qm[KT - 1][KT - 1] / det;
+      iQ[KT - 1][KT - 1] = Qm[0][0] / det;
+      iQ[0][KT - 1] = -Qm[0][KT - 1] / det;
+      iQ[KT - 1][0] = -Qm[KT - 1][0] / det;
+    }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(iQ[a][q], Ab[q][b], s);
+        V[a][b] = s;
+      }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(Qf[a][q], V[b][q], s);
+        W[a][b] = s;
+      }
+    double WAb[KT][KT];
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = 0.0;
+        for (int q = 0; q < KT; ++q) s = fma(W[a][q], Ab[q][b], s);
+        WAb[a][b] = s;
+      }
+    for (int a = 0; a < KT; ++a)
+      for (int b = 0; b < KT; ++b) {
+        double s = Af[a][b];
+        for (int q = 0; q < KT; ++q) s = fma(-WAb[a][q], Af[q][b], s);
+        Pt[a][b] = s;
+        Pt[a][KT + b] = W[a][b];
+      }
+    // T[0][0] = Qf[0][0] - (Ab Qf)[:,0]' V Qf[:,0]
+    double VQ[KT];
+    for (int a = 0; a < KT; ++a) {
+      double s = 0.0;
+      for (int q = 0; q < KT; ++q) s = fma(V[a][q], Qf[q][0], s);
+      VQ[a] = s;
+    }
+    double t00 = Qf[0][0];
+    for (int a = 0; a < KT; ++a) t00 = fma(-AQ[a][0], VQ[a], t00);
+    for (int u = 0; u < 2 * KT; ++u) pc[u] = Pt[0][u];
+    pc[2 * KT] = t00;
+  }
+  __syncthreads();
+  const int ind = sidx;
+  const bool lsent = ind == 0, rsent = ind == T;
+  const int64_t tl = (int64_t)k * T + (lsent ? 0 : ind - 1), tr = (int64_t)k * T + (rsent ? T - 1 : ind);
+  const double* mL = io.smean + tl * D;
+  const double* mR = io.smean + tr * D;
+  const double* PL = io.scov + tl * DD;
+  const double* PR = io.scov + tr * DD;
+  const double* Cx = io.scross + tl * DD;  // Cov(left, right); unused (0) after the left sentinel
+  for (int a = tid; a < M; a += kT) {
+    double s = 0.0;
+    for (int c = 0; c < KT; ++c) {
+      if (!lsent) s = fma(pc[c], mL[c * M + a], s);
+      if (!rsent) s = fma(pc[KT + c], mR[c * M + a], s);
+    }
+    hm[a] = s;
+  }
+  for (int e = tid; e < M * M; e += kT) {
+    const int a = e / M, b = e % M;
+    double s = (a == b) ? pc[2 * KT] : 0.0;
+    for (int c = 0; c < KT; ++c)
+      for (int c2 = 0; c2 < KT; ++c2) {
+        const int64_t ia = (int64_t)(c * M + a) * D, ib = (int64_t)(c2 * M + b) * D;
+        const double wl = pc[c], wr = pc[KT + c], vl = pc[c2], vr = pc[KT + c2];
+        const double pl = lsent ? (a == b ? mdl.Pinf[c][c2] : 0.0) : PL[ia + c2 * M + b];
+        const double pr = rsent ? (a == b ? mdl.Pinf[c][c2] : 0.0) : PR[ia + c2 * M + b];
+        const double xlr = lsent ? 0.0 : Cx[ia + c2 * M + b];   // Cov(left_a, right_b)
+        const double xrl = lsent ? 0.0 : Cx[ib + c * M + a];    // Cov(right_a, left_b)
+        s = fma(wl * vl, pl, s);
+        s = fma(wr * vr, pr, s);
+        s = fma(wl * vr, xlr, s);
+        s = fma(wr * vl, xrl, s);
+      }
+    sS[e] = s;
+  }
+  __syncthreads();
+  const int64_t base = ((int64_t)k * io.ns + i) * io.rs;
+  for (int r = tid; r < io.rs; r += kT) {
+    const double* b = io.B + (base + r) * M;
+    double mu = 0.0, v = io.cdiag[base + r];
+    for (int a = 0; a < M; ++a) {
+      const double ba = b[a];
+      mu = fma(ba, hm[a], mu);
+      double u = 0.0;
+      for (int c = 0; c < M; ++c) u = fma(sS[a * M + c], b[c], u);
+      v = fma(ba, u, v);
+    }
+    io.mean[base + r] = mu;
+    io.var[base + r] = v;
+  }
+}
+
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
 constexpr size_t kPseudoSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 3 * kMaxM);
@@ -1694,15 +1877,24 @@ int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* 
 
 int bn_st_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* filter_mean,
                    const double* filter_cov, double* post_mean, double* post_cov, void* workspace, void* stream) {
+  return bn_st_smoother_full(dims, hyp, dt, filter_mean, filter_cov, post_mean, post_cov, nullptr, nullptr, nullptr,
+                             workspace, stream);
+}
+
+int bn_st_smoother_full(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* filter_mean,
+                        const double* filter_cov, double* post_mean, double* post_cov, double* smooth_mean,
+                        double* smooth_cov, double* smooth_cross, void* workspace, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
   if (!hyp || !dt || !filter_mean || !filter_cov || !post_mean || !post_cov || !workspace)
     return st_fail(BN_EINVAL, "NULL argument to bn_st_smoother");
+  if ((smooth_mean == nullptr) != (smooth_cov == nullptr) || (smooth_mean == nullptr) != (smooth_cross == nullptr))
+    return st_fail(BN_EINVAL, "bn_st_smoother_full: smooth_mean, smooth_cov and smooth_cross go together");
   if (int rc = set_smem_limits()) return rc;
   double* ws = (double*)workspace;
   const int64_t D = dm.D, K = dm.K, DD = D * D;
   StSmootherIO io{dt, filter_mean, filter_cov, ws, ws + K * DD, ws + 2 * K * DD, ws + 3 * K * DD, ws + 4 * K * DD,
-                  post_mean, post_cov};
+                  post_mean, post_cov, smooth_mean, smooth_cov, smooth_cross};
   hipStream_t st = (hipStream_t)stream;
   BN_TIMED(KID_ST_SMOOTHER, st);
   if (dm.KT == 1)
@@ -1937,6 +2129,26 @@ int bn_ih_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double
     hipLaunchKernelGGL(k_ih_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
   else
     hipLaunchKernelGGL(k_ih_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_st_predict(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* t_train, int32_t n_test,
+                  int32_t n_loc_test, const double* t_test, const double* B_test, const double* cdiag_test,
+                  const double* smooth_mean, const double* smooth_cov, const double* smooth_cross, double* mean,
+                  double* var, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!hyp || !t_train || !t_test || !B_test || !cdiag_test || !smooth_mean || !smooth_cov || !smooth_cross || !mean ||
+      !var)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_predict");
+  if (n_test < 1 || n_loc_test < 1) return st_fail(BN_EINVAL, "bn_st_predict: empty test set");
+  StPredIO io{t_train, t_test, B_test, cdiag_test, smooth_mean, smooth_cov, smooth_cross, mean, var, n_test, n_loc_test};
+  hipStream_t st = (hipStream_t)stream;
+  if (dm.KT == 1)
+    hipLaunchKernelGGL(k_st_predict<1>, dim3(n_test, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), io);
+  else
+    hipLaunchKernelGGL(k_st_predict<2>, dim3(n_test, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), io);
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

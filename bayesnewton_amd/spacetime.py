@@ -233,6 +233,16 @@ class SpaceTimeMarkovGP:
         self._nat2.copy_(torch.as_tensor(nat2, dtype=torch.float64).reshape(self._nat2.shape))
         self._sites_version += 1
 
+    # ---- prediction (basemodels.py:766-816, 165-193) ------------------------------------------
+    def predict(self, X=None, R=None):
+        """Predictive mean and variance of f at test times X [n_test] and locations
+        R [n_test, n_loc, 2] (default: the training inputs), from the filter and a
+        full-state smoother with the current sites and hypers; [n_test, n_loc] FP64 device
+        tensors ([K, n_test, n_loc] for several runs).  The spatial covariance is reduced to
+        its diagonal, as the reference does."""
+        self._ensure_projection()
+        return _predict_spacetime(self, X, R)
+
     # ---- inference (inference.py:65-90) --------------------------------------------------
     def inference(self, lr=1.0, batch_ind=None, cubature=None, ensure_psd=True, return_terms=True, **kwargs):
         """One site update + posterior update.  Returns ((mean, jacobian, hessian), (diff1,
@@ -271,6 +281,50 @@ class SpaceTimeMarkovGP:
             raise NotImplementedError("mini-batched energies are not implemented on the HIP path")
         e = self.energy_per_run(cubature)
         return e[0] if self._single else e.sum()
+
+
+def _predict_spacetime(model, X=None, R=None):
+    """MarkovGaussianProcess.predict on the space-time branch (basemodels.py:766-816)."""
+    h = model._run_filter()
+    K, T, M, D = model.K, model.T, model.M, model.D
+    dev = model.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    if X is None:
+        Xt, Rt = model.X, model.R
+    else:
+        Xt = np.broadcast_to(np.asarray(X, dtype=np.float64).reshape(-1), (K, np.size(X)))
+        Rt = np.asarray(R, dtype=np.float64)
+        if Rt.ndim != 3 or Rt.shape[0] != Xt.shape[1] or Rt.shape[2] != 2:
+            raise ValueError("R must be [n_test, n_loc, 2] for X [n_test]")
+        Rt = np.broadcast_to(Rt, (K,) + Rt.shape)
+    ns, rs = Xt.shape[1], Rt.shape[2]
+    sm = torch.empty((K, T, D), **f64)
+    sP = torch.empty((K, T, D, D), **f64)
+    sx = torch.empty((K, T, D, D), **f64)
+    pm = torch.empty((K, T, M), **f64)
+    pP = torch.empty((K, T, M, M), **f64)
+    nat.check(nat.lib().bn_st_smoother_full(
+        ctypes.byref(model._dims), ctypes.byref(h), nat.ptr(model._dt), nat.ptr(model._fm), nat.ptr(model._fP),
+        nat.ptr(pm), nat.ptr(pP), nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(model._ws), nat.stream_ptr()))
+    tdims = nat.StDims(K, ns, rs, M, model.kernel.temporal_kernel.kind)
+    rt = torch.tensor(np.ascontiguousarray(Rt), dtype=torch.float32, device=dev)
+    Bt = torch.empty((K, ns, rs, M), **f64)
+    ct = torch.empty((K, ns, rs), **f64)
+    model._ensure_conditional()
+    nat.check(nat.lib().bn_st_conditional(
+        ctypes.byref(tdims), ctypes.byref(h), model.kernel.spatial_kernel.kind,
+        float(model.kernel.spatial_kernel.lengthscale), nat.ptr(model._z), nat.ptr(model._Lzz), nat.ptr(rt),
+        nat.ptr(Bt), nat.ptr(ct), nat.stream_ptr()))
+    if model.kernel.conditional == "DTC":
+        ct.zero_()
+    xtr = torch.tensor(np.ascontiguousarray(model.X), **f64)
+    xte = torch.tensor(np.ascontiguousarray(Xt), **f64)
+    mean = torch.empty((K, ns, rs), **f64)
+    var = torch.empty((K, ns, rs), **f64)
+    nat.check(nat.lib().bn_st_predict(
+        ctypes.byref(model._dims), ctypes.byref(h), nat.ptr(xtr), ns, rs, nat.ptr(xte), nat.ptr(Bt), nat.ptr(ct),
+        nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(mean), nat.ptr(var), nat.stream_ptr()))
+    return model._out(mean), model._out(var)
 
 
 class SpaceTimeVariationalGP(SpaceTimeMarkovGP):
@@ -323,6 +377,9 @@ class SpaceTimeMeanFieldGP(SpaceTimeMarkovGP):
             nat.ptr(self._loglik), nat.ptr(self._ws), nat.stream_ptr()))
         self._ll_key = (self._proj_key, key[0], key[1])
         return h
+
+    def predict(self, X=None, R=None):
+        raise NotImplementedError("prediction is provided for the dense space-time model only")
 
     def update_posterior(self):
         self._ensure_projection()
@@ -446,6 +503,9 @@ class SpaceTimeInfiniteHorizonGP(SpaceTimeMarkovGP):
             nat.ptr(self._Xf), nat.ptr(self._cov), nat.ptr(self._Xs), nat.ptr(self._gain), nat.ptr(self._pm),
             nat.ptr(self._pP), None, nat.ptr(self._ws), nat.stream_ptr()))
         self._post_key = self._ll_key
+
+    def predict(self, X=None, R=None):
+        raise NotImplementedError("prediction is provided for the dense space-time model only")
 
     # the reference re-runs the filter (and advances its Riccati solution) on every call
     def _ensure_posterior(self):

@@ -319,6 +319,27 @@ int bn_st_filter(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *
 int bn_st_smoother(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *filter_mean,
                    const double *filter_cov, double *post_mean, double *post_cov, void *workspace, void *stream);
 
+/* bn_st_smoother with the full smoothed state as well (return_full, ops.py:294-305 as used by
+ * MarkovGaussianProcess.predict, basemodels.py:787-793): smooth_mean [K,T,D], smooth_cov
+ * [K,T,D,D] and smooth_cross [K,T,D,D] = G_t P_s,t+1 = Cov(x_t, x_t+1 | y) (gain times the
+ * next smoothed covariance, as temporal_conditional joins neighbours, utils.py:132-137);
+ * at the last step G = I (the reference's gain for a zero step) and smooth_cross = Pinf.
+ * The three are given together or not at all. */
+int bn_st_smoother_full(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *dt, const double *filter_mean,
+                        const double *filter_cov, double *post_mean, double *post_cov, double *smooth_mean,
+                        double *smooth_cov, double *smooth_cross, void *workspace, void *stream);
+
+/* Space-time prediction (MarkovGaussianProcess.predict, basemodels.py:766-816): for every
+ * test time t_test [K,n_test] the state conditional on its two neighbouring smoothed states
+ * (temporal_conditional, utils.py:125-141 with compute_conditional_statistics 185-215;
+ * sentinels (0, Pinf) at -+1e10), then f(t*, r) = B H x(t*) + N(0, C) at n_loc_test
+ * locations: B_test [K,n_test,n_loc_test,M] and cdiag_test from bn_st_conditional.
+ * t_train [K,T].  Outputs mean, var [K,n_test,n_loc_test] (the diagonal of W P W' + C). */
+int bn_st_predict(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *t_train, int32_t n_test,
+                  int32_t n_loc_test, const double *t_test, const double *B_test, const double *cdiag_test,
+                  const double *smooth_mean, const double *smooth_cov, const double *smooth_cross, double *mean,
+                  double *var, void *stream);
+
 /* per_run[k] = sum_t per_step[k, t] (FP64, fixed order). */
 int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *per_run, void *stream);
 

@@ -215,9 +215,11 @@ class MarkovGP:
     def predict(self, X=None, R=None):
         """basemodels.py:766-816 (temporal models): filter, full smoother with gains,
         then the conditional of every test time on its two neighbouring states."""
-        if self.spatio_temporal:
-            raise NotImplementedError("space-time prediction is not restated here")
+        if self.mean_field or self.infinite_horizon:
+            raise NotImplementedError("prediction is restated for the dense filter and smoother")
         X = self.X if X is None else np.asarray(X, np.float64).reshape(len(X), -1)
+        if self.spatio_temporal and R is None:
+            R = self.R
         X = X[:, :1]
         pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
         _, (fm, fP) = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var, mask=self._mask3(),
@@ -229,6 +231,14 @@ class MarkovGP:
         X_aug = np.block([[-inf], [self.X[:, :1]], [inf]])
         state_mean, state_cov = temporal_conditional(X_aug, X, sm, sP, gain, self.kernel)
         H = self.kernel.measurement_model()
+        if self.spatio_temporal:
+            # basemodels.py:805-812: f(t, R) = B H x(t) + N(0, C); the spatial covariance
+            # is reduced to its diagonal
+            B, C = self.kernel.spatial_conditional(X, R, predict=True)
+            W = B @ H
+            test_mean = W @ state_mean
+            test_var = np.diagonal(W @ state_cov @ T(W) + C, axis1=1, axis2=2)
+            return np.squeeze(test_mean), np.squeeze(test_var)
         test_mean, test_var = H @ state_mean, H @ state_cov @ T(H)
         return np.squeeze(test_mean), np.squeeze(test_var)
 

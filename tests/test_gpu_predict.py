@@ -106,3 +106,46 @@ def test_predict_multi_series_matches_per_series_oracle():
         om, ov = o.predict(t[:, None])
         np.testing.assert_allclose(gm[off[b]:off[b + 1]], om, rtol=TOL, atol=TOL * np.max(np.abs(om)))
         np.testing.assert_allclose(gv[off[b]:off[b + 1]], ov, rtol=TOL, atol=TOL * np.max(ov))
+
+
+@pytest.mark.parametrize("g,T,nr,kind", [(3, 20, 16, "VI"), (2, 15, 9, "Newton"), (8, 12, 256, "VI")])
+def test_spacetime_predict_matches_oracle(g, T, nr, kind):
+    """Space-time prediction (basemodels.py:766-816): filter, full smoother with the
+    neighbours' cross covariances, Kronecker-structured state conditional and the spatial
+    conditional of the test locations, against the oracle's dense restatement.  With 64
+    inducing points the steps carry 256 locations: with as many locations as inducing points
+    B' diag(nat2) B is numerically singular (condition ~1e17), and the reference's explicit
+    inverse of it (basemodels.py:683), which the oracle restates, is then rounding noise."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as GM
+    from oracle import kernels as OK, likelihoods as OL, models as OM
+    rng = np.random.default_rng(40 + g)
+    t = np.arange(T, dtype=np.float64)
+    r = rng.uniform(size=(T, nr, 2)).astype(np.float32).astype(np.float64)
+    f = np.sin(0.3 * t)[:, None] + np.cos(3.0 * r[..., 0]) * np.sin(2.0 * r[..., 1])
+    y = rng.poisson(2.0 * np.exp(f)).astype(np.float64)
+    gz = np.linspace(0, 1, g)
+    a, b = np.meshgrid(gz, gz)
+    z = np.stack([a.ravel(), b.ravel()], 1)
+    gk = K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=5.0), K.Matern32(variance=1.0, lengthscale=0.3),
+                                z=z, sparse=True, conditional="Full")
+    ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 5.0), OK.Matern32(1.0, 0.3), z)
+    cls = {"VI": GM.MarkovVariationalGP, "Newton": GM.MarkovLaplaceGP}[kind]
+    gm = cls(gk, L.Poisson(binsize=2.0), t, y, R=r)
+    om = OM.MarkovGP(ok, OL.Poisson(2.0), t[:, None], y, R=r, inference=kind)
+    for _ in range(2):
+        gm.inference(lr=1.0)
+        om.inference(lr=1.0)
+    ts = np.array([-2.0, 0.0, 3.0, 4.25, 7.5, T - 1.0, T + 1.5])
+    rs = rng.uniform(size=(ts.shape[0], 5, 2)).astype(np.float32).astype(np.float64)
+    print(f"[st predict] g={g} T={T} R={nr} kind={kind}", flush=True)
+    mu, var = gm.predict(ts, rs)
+    omu, ovar = om.predict(ts[:, None], rs)
+    mu, var = mu.cpu().numpy(), var.cpu().numpy()
+    assert mu.shape == omu.shape and var.shape == ovar.shape
+    assert np.max(np.abs(mu - omu)) <= 1e-4 * np.max(np.abs(omu))
+    assert np.max(np.abs(var - ovar) / ovar) <= 1e-4
+    # default test inputs: the training times and locations
+    mu0, var0 = gm.predict()
+    omu0, ovar0 = om.predict(t[:, None], r)
+    assert np.max(np.abs(mu0.cpu().numpy() - omu0)) <= 1e-4 * np.max(np.abs(omu0))
+    assert np.max(np.abs(var0.cpu().numpy() - ovar0) / ovar0) <= 1e-4

@@ -89,3 +89,43 @@ def test_log_density_matches_quadrature():
             f = lambda z, i=i: math.exp(float(lik.log_lik(y[i], z))) * math.exp(-0.5 * (z - m[i]) ** 2 / v[i]) / math.sqrt(2 * math.pi * v[i])  # noqa: E731
             Z, _ = integrate.quad(f, m[i] - 12 * math.sqrt(v[i]), m[i] + 12 * math.sqrt(v[i]), epsabs=1e-14, epsrel=1e-12, limit=200)
             assert abs(ld[i] - math.log(Z)) < 1e-6, (type(lik).__name__, i, ld[i], math.log(Z))
+
+
+def test_spacetime_predict_matches_dense_posterior():
+    """Space-time prediction (basemodels.py:766-816 on the spatio-temporal branch): with a
+    Gaussian likelihood one VI step gives the exact posterior of the inducing processes
+    u(t) (a GP with covariance k_t(t, t') I_M) under y_t = B_t u(t) + noise, so the
+    predictive mean and variance at (t*, r*) are b*' E[u(t*) | y] and
+    b*' Cov(u(t*) | y) b* + c*, computed here from the dense joint covariance."""
+    from oracle import kernels as OK, likelihoods as OL, models as OM
+    rng = np.random.default_rng(8)
+    g, T_, R_, s2 = 2, 10, 5, 0.2
+    gz = np.linspace(0, 1, g)
+    a, b = np.meshgrid(gz, gz)
+    z = np.stack([a.ravel(), b.ravel()], 1)
+    M = z.shape[0]
+    t = np.sort(rng.uniform(0, 8, T_))
+    r = rng.uniform(size=(T_, R_, 2))
+    y = rng.standard_normal((T_, R_))
+    kt, ks = OK.Matern32(1.3, 2.0), OK.Matern32(0.8, 0.4)
+    kern = OK.SpatioTemporalKernel(kt, ks, z)
+    o = OM.MarkovGP(kern, OL.Gaussian(s2), t[:, None], y, R=r, inference="VI")
+    o.inference(lr=1.0)
+    ts = np.array([-1.0, t[0], 0.5 * (t[2] + t[3]), t[6], 9.5])
+    rs = rng.uniform(size=(ts.shape[0], 3, 2))
+    pm, pv = o.predict(ts[:, None], rs)
+    # dense: u at the training times and one test time
+    B, _ = kern.spatial_conditional(t[:, None], r)
+    Bs, Cs = kern.spatial_conditional(ts[:, None], rs, predict=True)
+    Bbig = np.zeros((T_ * R_, T_ * M))
+    for k in range(T_):
+        Bbig[k * R_:(k + 1) * R_, k * M:(k + 1) * M] = B[k]
+    Kt = kt(t[:, None], t[:, None])
+    Sy = Bbig @ np.kron(Kt, np.eye(M)) @ Bbig.T + s2 * np.eye(T_ * R_)
+    for i, tt in enumerate(ts):
+        kx = kt(np.array([[tt]]), t[:, None])
+        Cuy = np.kron(kx, np.eye(M)) @ Bbig.T
+        mu = Cuy @ np.linalg.solve(Sy, y.reshape(-1))
+        Su = kt(np.array([[tt]]), np.array([[tt]]))[0, 0] * np.eye(M) - Cuy @ np.linalg.solve(Sy, Cuy.T)
+        np.testing.assert_allclose(pm[i], Bs[i] @ mu, rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(pv[i], np.diagonal(Bs[i] @ Su @ Bs[i].T + Cs[i]), rtol=1e-6, atol=1e-7)
