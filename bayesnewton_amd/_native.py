@@ -82,7 +82,7 @@ SIGNATURES = {
     "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_smoother_full": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "bn_st_predict": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_st_predict": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_mf_workspace_bytes": (i64, [vp]),

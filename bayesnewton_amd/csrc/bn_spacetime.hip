@@ -1570,6 +1570,7 @@ struct StPredIO {
   const double* smean;   // [K,T,D]
   const double* scov;    // [K,T,D,D]
   const double* scross;  // [K,T,D,D]
+  const double* steady;  // [K,D,D] or nullptr: the infinite-horizon state covariance at every test time
   double* mean;          // [K,NS,RS]
   double* var;           // [K,NS,RS]
   int32_t ns, rs;
@@ -1671,9 +1672,9 @@ __global__ __launch_bounds__(kT) void k_st_predict(StDims dm, StHyp hy, StPredIO
   const int64_t tl = (int64_t)k * T + (lsent ? 0 : ind - 1), tr = (int64_t)k * T + (rsent ? T - 1 : ind);
   const double* mL = io.smean + tl * D;
   const double* mR = io.smean + tr * D;
-  const double* PL = io.scov + tl * DD;
-  const double* PR = io.scov + tr * DD;
-  const double* Cx = io.scross + tl * DD;  // Cov(left, right); unused (0) after the left sentinel
+  const double* PL = io.steady ? nullptr : io.scov + tl * DD;
+  const double* PR = io.steady ? nullptr : io.scov + tr * DD;
+  const double* Cx = io.steady ? nullptr : io.scross + tl * DD;  // Cov(left, right); 0 after the left sentinel
   for (int a = tid; a < M; a += kT) {
     double s = 0.0;
     for (int c = 0; c < KT; ++c) {
@@ -1682,6 +1683,11 @@ __global__ __launch_bounds__(kT) void k_st_predict(StDims dm, StHyp hy, StPredIO
     }
     hm[a] = s;
   }
+  if (io.steady) {
+    // temporal_conditional_infinite_horizon (utils.py:163-177): the smoother's steady
+    // covariance replaces the conditional one
+    for (int e = tid; e < M * M; e += kT) sS[e] = io.steady[k * DD + (int64_t)(e / M) * D + e % M];
+  } else
   for (int e = tid; e < M * M; e += kT) {
     const int a = e / M, b = e % M;
     double s = (a == b) ? pc[2 * KT] : 0.0;
@@ -2135,15 +2141,16 @@ int bn_ih_smoother(const bn_st_dims* dims, const bn_st_hypers* hyp, const double
 
 int bn_st_predict(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* t_train, int32_t n_test,
                   int32_t n_loc_test, const double* t_test, const double* B_test, const double* cdiag_test,
-                  const double* smooth_mean, const double* smooth_cov, const double* smooth_cross, double* mean,
-                  double* var, void* stream) {
+                  const double* smooth_mean, const double* smooth_cov, const double* smooth_cross,
+                  const double* steady_cov, double* mean, double* var, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
-  if (!hyp || !t_train || !t_test || !B_test || !cdiag_test || !smooth_mean || !smooth_cov || !smooth_cross || !mean ||
-      !var)
+  if (!hyp || !t_train || !t_test || !B_test || !cdiag_test || !smooth_mean || !mean || !var ||
+      (!steady_cov && (!smooth_cov || !smooth_cross)))
     return st_fail(BN_EINVAL, "NULL argument to bn_st_predict");
   if (n_test < 1 || n_loc_test < 1) return st_fail(BN_EINVAL, "bn_st_predict: empty test set");
-  StPredIO io{t_train, t_test, B_test, cdiag_test, smooth_mean, smooth_cov, smooth_cross, mean, var, n_test, n_loc_test};
+  StPredIO io{t_train, t_test, B_test, cdiag_test, smooth_mean, smooth_cov, smooth_cross, steady_cov, mean, var, n_test,
+              n_loc_test};
   hipStream_t st = (hipStream_t)stream;
   if (dm.KT == 1)
     hipLaunchKernelGGL(k_st_predict<1>, dim3(n_test, dm.K), dim3(kT), 0, st, dm, make_hyp(hyp), io);

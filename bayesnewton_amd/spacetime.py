@@ -283,29 +283,26 @@ class SpaceTimeMarkovGP:
         return e[0] if self._single else e.sum()
 
 
-def _predict_spacetime(model, X=None, R=None):
-    """MarkovGaussianProcess.predict on the space-time branch (basemodels.py:766-816)."""
-    h = model._run_filter()
-    K, T, M, D = model.K, model.T, model.M, model.D
+def _test_inputs(model, X, R):
+    K = model.K
+    if X is None:
+        return model.X, model.R
+    Xt = np.broadcast_to(np.asarray(X, dtype=np.float64).reshape(-1), (K, np.size(X)))
+    Rt = np.asarray(R, dtype=np.float64)
+    if Rt.ndim != 3 or Rt.shape[0] != Xt.shape[1] or Rt.shape[2] != 2:
+        raise ValueError("R must be [n_test, n_loc, 2] for X [n_test]")
+    return Xt, np.broadcast_to(Rt, (K,) + Rt.shape)
+
+
+def _predict_from_state(model, h, X, R, sm, sP=None, sx=None, steady=None):
+    """f at test times / locations from the smoothed full state (layout c*M + m):
+    temporal conditional on the neighbours (k_st_predict) and the spatial conditional of
+    the test locations (basemodels.py:787-812)."""
+    K, M = model.K, model.M
     dev = model.device
     f64 = dict(dtype=torch.float64, device=dev)
-    if X is None:
-        Xt, Rt = model.X, model.R
-    else:
-        Xt = np.broadcast_to(np.asarray(X, dtype=np.float64).reshape(-1), (K, np.size(X)))
-        Rt = np.asarray(R, dtype=np.float64)
-        if Rt.ndim != 3 or Rt.shape[0] != Xt.shape[1] or Rt.shape[2] != 2:
-            raise ValueError("R must be [n_test, n_loc, 2] for X [n_test]")
-        Rt = np.broadcast_to(Rt, (K,) + Rt.shape)
+    Xt, Rt = _test_inputs(model, X, R)
     ns, rs = Xt.shape[1], Rt.shape[2]
-    sm = torch.empty((K, T, D), **f64)
-    sP = torch.empty((K, T, D, D), **f64)
-    sx = torch.empty((K, T, D, D), **f64)
-    pm = torch.empty((K, T, M), **f64)
-    pP = torch.empty((K, T, M, M), **f64)
-    nat.check(nat.lib().bn_st_smoother_full(
-        ctypes.byref(model._dims), ctypes.byref(h), nat.ptr(model._dt), nat.ptr(model._fm), nat.ptr(model._fP),
-        nat.ptr(pm), nat.ptr(pP), nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(model._ws), nat.stream_ptr()))
     tdims = nat.StDims(K, ns, rs, M, model.kernel.temporal_kernel.kind)
     rt = torch.tensor(np.ascontiguousarray(Rt), dtype=torch.float32, device=dev)
     Bt = torch.empty((K, ns, rs, M), **f64)
@@ -323,8 +320,24 @@ def _predict_spacetime(model, X=None, R=None):
     var = torch.empty((K, ns, rs), **f64)
     nat.check(nat.lib().bn_st_predict(
         ctypes.byref(model._dims), ctypes.byref(h), nat.ptr(xtr), ns, rs, nat.ptr(xte), nat.ptr(Bt), nat.ptr(ct),
-        nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(mean), nat.ptr(var), nat.stream_ptr()))
+        nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(steady), nat.ptr(mean), nat.ptr(var), nat.stream_ptr()))
     return model._out(mean), model._out(var)
+
+
+def _predict_spacetime(model, X=None, R=None):
+    """MarkovGaussianProcess.predict on the space-time branch (basemodels.py:766-816)."""
+    h = model._run_filter()
+    K, T, M, D = model.K, model.T, model.M, model.D
+    f64 = dict(dtype=torch.float64, device=model.device)
+    sm = torch.empty((K, T, D), **f64)
+    sP = torch.empty((K, T, D, D), **f64)
+    sx = torch.empty((K, T, D, D), **f64)
+    pm = torch.empty((K, T, M), **f64)
+    pP = torch.empty((K, T, M, M), **f64)
+    nat.check(nat.lib().bn_st_smoother_full(
+        ctypes.byref(model._dims), ctypes.byref(h), nat.ptr(model._dt), nat.ptr(model._fm), nat.ptr(model._fP),
+        nat.ptr(pm), nat.ptr(pP), nat.ptr(sm), nat.ptr(sP), nat.ptr(sx), nat.ptr(model._ws), nat.stream_ptr()))
+    return _predict_from_state(model, h, X, R, sm, sP, sx)
 
 
 class SpaceTimeVariationalGP(SpaceTimeMarkovGP):
@@ -379,7 +392,48 @@ class SpaceTimeMeanFieldGP(SpaceTimeMarkovGP):
         return h
 
     def predict(self, X=None, R=None):
-        raise NotImplementedError("prediction is provided for the dense space-time model only")
+        """MarkovMeanFieldGaussianProcess.predict (basemodels.py:1177-1228): the sequential
+        mean-field filter whatever `parallel` is (as the reference), the smoother's full
+        blocks, then the dense temporal conditional of the block-diagonal state (layout
+        expanded to c*M + m) and the spatial conditional.  Returns [n_test, n_loc] mean
+        and variance."""
+        self._ensure_projection()
+        h, key = self._hyp()
+        if self._pseudo_key != self._proj_key:
+            nat.check(nat.lib().bn_mf_pseudo(
+                ctypes.byref(self._dims), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._noise),
+                nat.ptr(self._ypse), nat.stream_ptr()))
+            self._pseudo_key = self._proj_key
+        K, T, M, D = self.K, self.T, self.M, self.D
+        KT = self.kernel.temporal_kernel.state_dim
+        f64 = dict(dtype=torch.float64, device=self.device)
+        fm = torch.empty((K, T, M, KT), **f64)
+        fP = torch.empty((K, T, M, KT, KT), **f64)
+        ell = torch.empty((K, T), **f64)
+        ll = torch.empty(K, **f64)
+        nat.check(nat.lib().bn_mf_filter(
+            ctypes.byref(self._dims), ctypes.byref(h), 0, nat.ptr(self._dt), nat.ptr(self._ypse), nat.ptr(self._noise),
+            nat.ptr(fm), nat.ptr(fP), nat.ptr(ell), nat.ptr(ll), None, nat.stream_ptr()))
+        bm = torch.empty((K, T, M, KT), **f64)
+        bP = torch.empty((K, T, M, KT, KT), **f64)
+        G = torch.empty((K, T, M, KT, KT), **f64)
+        nat.check(nat.lib().bn_mf_smoother(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._dt_s), nat.ptr(fm), nat.ptr(fP), None, None, 0, 0,
+            nat.ptr(bm), nat.ptr(bP), nat.ptr(G), nat.stream_ptr()))
+        # cross covariance of neighbours per block: G_t P_s,t+1; at the end G = I and the
+        # right neighbour is the sentinel (Pinf)
+        Cx = torch.empty_like(bP)
+        Cx[:, :-1] = G[:, :-1] @ bP[:, 1:]
+        Cx[:, -1] = torch.tensor(_pinf_temporal(self.kernel.temporal_kernel), **f64)
+        idx = torch.arange(M, device=self.device)
+        sm = bm.permute(0, 1, 3, 2).reshape(K, T, D).contiguous()
+        sP = torch.zeros((K, T, D, D), **f64)
+        sx = torch.zeros((K, T, D, D), **f64)
+        for c in range(KT):
+            for c2 in range(KT):
+                sP[:, :, c * M + idx, c2 * M + idx] = bP[:, :, :, c, c2]
+                sx[:, :, c * M + idx, c2 * M + idx] = Cx[:, :, :, c, c2]
+        return _predict_from_state(self, h, X, R, sm, sP, sx)
 
     def update_posterior(self):
         self._ensure_projection()
@@ -505,7 +559,20 @@ class SpaceTimeInfiniteHorizonGP(SpaceTimeMarkovGP):
         self._post_key = self._ll_key
 
     def predict(self, X=None, R=None):
-        raise NotImplementedError("prediction is provided for the dense space-time model only")
+        """MarkovGaussianProcess.predict with temporal_conditional_infinite_horizon
+        (basemodels.py:766-816, 1303-1308; utils.py:142-177): the filter and the full
+        smoother (both advance their Riccati solutions, as in the reference), the mean by
+        the conditional on the neighbouring smoothed means, the smoother's steady covariance
+        as the state covariance.  Returns [n_test, n_loc] mean and variance."""
+        self._ensure_projection()
+        h = self._run_filter()
+        f64 = dict(dtype=torch.float64, device=self.device)
+        sm = torch.empty((self.K, self.T, self.D), **f64)
+        nat.check(nat.lib().bn_ih_smoother(
+            ctypes.byref(self._dims), ctypes.byref(h), nat.ptr(self._step), self.dare_iters, nat.ptr(self._fm),
+            nat.ptr(self._Xf), nat.ptr(self._cov), nat.ptr(self._Xs), nat.ptr(self._gain), None, None, nat.ptr(sm),
+            nat.ptr(self._ws), nat.stream_ptr()))
+        return _predict_from_state(self, h, X, R, sm, steady=self._Xs)
 
     # the reference re-runs the filter (and advances its Riccati solution) on every call
     def _ensure_posterior(self):

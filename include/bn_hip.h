@@ -334,11 +334,14 @@ int bn_st_smoother_full(const bn_st_dims *dims, const bn_st_hypers *hyp, const d
  * (temporal_conditional, utils.py:125-141 with compute_conditional_statistics 185-215;
  * sentinels (0, Pinf) at -+1e10), then f(t*, r) = B H x(t*) + N(0, C) at n_loc_test
  * locations: B_test [K,n_test,n_loc_test,M] and cdiag_test from bn_st_conditional.
- * t_train [K,T].  Outputs mean, var [K,n_test,n_loc_test] (the diagonal of W P W' + C). */
+ * t_train [K,T].  Outputs mean, var [K,n_test,n_loc_test] (the diagonal of W P W' + C).
+ * steady_cov [K,D,D] (may be NULL): the infinite-horizon form
+ * (temporal_conditional_infinite_horizon, utils.py:163-177), whose state covariance is
+ * the smoother's steady one at every test time; smooth_cov / smooth_cross are then unused. */
 int bn_st_predict(const bn_st_dims *dims, const bn_st_hypers *hyp, const double *t_train, int32_t n_test,
                   int32_t n_loc_test, const double *t_test, const double *B_test, const double *cdiag_test,
-                  const double *smooth_mean, const double *smooth_cov, const double *smooth_cross, double *mean,
-                  double *var, void *stream);
+                  const double *smooth_mean, const double *smooth_cov, const double *smooth_cross,
+                  const double *steady_cov, double *mean, double *var, void *stream);
 
 /* per_run[k] = sum_t per_step[k, t] (FP64, fixed order). */
 int bn_st_sum(int32_t n_runs, int32_t n_steps, const double *per_step, double *per_run, void *stream);

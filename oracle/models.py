@@ -213,23 +213,31 @@ class MarkovGP:
 
     # ---- prediction (basemodels.py:766-816, 165-193) --------------------------
     def predict(self, X=None, R=None):
-        """basemodels.py:766-816 (temporal models): filter, full smoother with gains,
-        then the conditional of every test time on its two neighbouring states."""
-        if self.mean_field or self.infinite_horizon:
-            raise NotImplementedError("prediction is restated for the dense filter and smoother")
+        """basemodels.py:766-816: filter, full smoother with gains, then the conditional of
+        every test time on its two neighbouring states.  The mean-field model filters with
+        the sequential form whatever `parallel` is (basemodels.py:1177-1228); the
+        infinite-horizon model uses the smoother's steady covariance at every test time
+        (temporal_conditional_infinite_horizon, utils.py:142-177) and advances both Riccati
+        solutions, as its filter and smoother calls do."""
         X = self.X if X is None else np.asarray(X, np.float64).reshape(len(X), -1)
         if self.spatio_temporal and R is None:
             R = self.R
         X = X[:, :1]
         pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
-        _, (fm, fP) = ops.kalman_filter(self.dt, self.kernel, pseudo_y, pseudo_var, mask=self._mask3(),
-                                        parallel=self.parallel)
+        _, (fm, fP) = self._filter(self.dt, self.kernel, pseudo_y, pseudo_var, mask=self._mask3(),
+                                   parallel=False if self.mean_field else self.parallel)
         dt = np.concatenate([self.dt[1:], np.array([0.0])])
-        sm, sP, gain = ops.rauch_tung_striebel_smoother(dt, self.kernel, fm, fP, return_full=True,
-                                                        parallel=self.parallel)
+        sm, sP, gain = self._smoother(dt, self.kernel, fm, fP, return_full=True, parallel=self.parallel)
         inf = 1e10 * np.ones_like(self.X[0, :1])
         X_aug = np.block([[-inf], [self.X[:, :1]], [inf]])
-        state_mean, state_cov = temporal_conditional(X_aug, X, sm, sP, gain, self.kernel)
+        if self.infinite_horizon:
+            mean_aug = np.concatenate([np.zeros((1,) + sm.shape[1:]), sm, np.zeros((1,) + sm.shape[1:])])
+            ind = np.searchsorted(X_aug.reshape(-1), X.reshape(-1)) - 1
+            P, _ = compute_conditional_statistics(X, X_aug, self.kernel, ind)
+            state_mean = np.stack([P[i] @ np.concatenate([mean_aug[k], mean_aug[k + 1]]) for i, k in enumerate(ind)])
+            state_cov = np.broadcast_to(sP[0], (X.shape[0],) + sP.shape[1:])
+        else:
+            state_mean, state_cov = temporal_conditional(X_aug, X, sm, sP, gain, self.kernel)
         H = self.kernel.measurement_model()
         if self.spatio_temporal:
             # basemodels.py:805-812: f(t, R) = B H x(t) + N(0, C); the spatial covariance

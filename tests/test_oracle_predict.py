@@ -129,3 +129,61 @@ def test_spacetime_predict_matches_dense_posterior():
         Su = kt(np.array([[tt]]), np.array([[tt]]))[0, 0] * np.eye(M) - Cuy @ np.linalg.solve(Sy, Cuy.T)
         np.testing.assert_allclose(pm[i], Bs[i] @ mu, rtol=1e-6, atol=1e-7)
         np.testing.assert_allclose(pv[i], np.diagonal(Bs[i] @ Su @ Bs[i].T + Cs[i]), rtol=1e-6, atol=1e-7)
+
+
+def _decoupled_spacetime(seed, T_):
+    from oracle import kernels as OK, likelihoods as OL
+    rng = np.random.default_rng(seed)
+    gz = np.linspace(0, 1, 3)
+    a, b = np.meshgrid(gz, gz)
+    z = np.stack([a.ravel(), b.ravel()], 1)
+    t = np.arange(T_, dtype=np.float64)
+    r = np.broadcast_to(z, (T_,) + z.shape).copy()
+    y = rng.poisson(2.0, size=(T_, z.shape[0])).astype(np.float64)
+    kt, ks = OK.Matern32(1.2, 4.0), OK.Matern32(0.7, 1e-3)
+    return OK.SpatioTemporalKernel(kt, ks, z), OL.Poisson(), t, r, y, z
+
+
+def test_meanfield_predict_equals_dense_when_space_decouples():
+    """Mean-field prediction (basemodels.py:1177-1228): with a spatial kernel that decouples
+    the inducing points and data at them, the mean-field model is exact, so its predictions
+    equal the dense model's (both forms of the filter)."""
+    from oracle import models as OM
+    kern, lik, t, r, y, z = _decoupled_spacetime(9, 12)
+    ts = np.array([-1.0, 0.0, 2.5, 7.0, 13.0])
+    rs = np.broadcast_to(z[:4], (ts.shape[0], 4, 2)).copy()
+    for par in (False, True):
+        dense = OM.MarkovGP(kern, lik, t[:, None], y, R=r, inference="VI")
+        mf = OM.MarkovGP(kern, lik, t[:, None], y, R=r, inference="VI", mean_field=True, parallel=par)
+        for o in (dense, mf):
+            o.inference(lr=1.0)
+        dm, dv = dense.predict(ts[:, None], rs)
+        mm, mv = mf.predict(ts[:, None], rs)
+        np.testing.assert_allclose(mm, dm, rtol=1e-8, atol=1e-10)
+        np.testing.assert_allclose(mv, dv, rtol=1e-8, atol=1e-10)
+
+
+def test_infinite_horizon_predict_reaches_the_dense_steady_state():
+    """Infinite-horizon prediction (temporal_conditional_infinite_horizon, utils.py:142-177):
+    with many Riccati iterations, in the middle of a long equispaced series and with the
+    sites fixed, its predictive mean and variance equal the dense model's."""
+    from oracle import kernels as OK, likelihoods as OL, models as OM
+    rng = np.random.default_rng(10)
+    T_ = 300
+    t = np.arange(T_, dtype=np.float64)
+    z = np.array([[0.2, 0.3], [0.7, 0.6]])
+    r = np.broadcast_to(z, (T_,) + z.shape).copy()
+    y = np.sin(0.1 * t)[:, None] + 0.3 * rng.standard_normal((T_, 2))
+    kern = OK.SpatioTemporalKernel(OK.Matern32(1.0, 3.0), OK.Matern32(1.0, 0.5), z)
+    dense = OM.MarkovGP(kern, OL.Gaussian(0.1), t[:, None], y, R=r, inference="VI")
+    ih = OM.MarkovGP(kern, OL.Gaussian(0.1), t[:, None], y, R=r, inference="VI", infinite_horizon=True,
+                     dare_iters=400)
+    nat1, nat2 = dense.nat1, dense.nat2
+    for o in (dense, ih):
+        o.update_nat_params(np.full_like(nat1, 0.0) + y[..., None] / 0.1, np.broadcast_to(np.eye(2) / 0.1, nat2.shape).copy())
+    ts = np.array([140.0, 150.5, 160.25])
+    rs = np.broadcast_to(z, (3, 2, 2)).copy()
+    dm, dv = dense.predict(ts[:, None], rs)
+    im, iv = ih.predict(ts[:, None], rs)
+    np.testing.assert_allclose(im, dm, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(iv[0], dv[0], rtol=1e-7)
