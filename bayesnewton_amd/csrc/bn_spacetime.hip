@@ -530,19 +530,22 @@ __device__ double wg_chol_reg(double* a, int n, int ld, double* row) {
   return ldl;
 }
 
-constexpr int kRC = 64;  // rows of B per chunk
+constexpr int kRC = 32;  // rows of B per chunk
 constexpr int kLC = kMaxM + 1;  // LDS row stride of a chunk: consecutive rows start in different banks
+static_assert(2 * kRC * kLC >= kMaxM * kMaxM, "the two chunk buffers hold an M x M matrix");
 
 template <int METHOD, int MODE>
-__global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
+__global__ __launch_bounds__(kT, 2) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
                                                  int ensure_psd) {
   extern __shared__ double smem[];
   const int M = dm.M, R = dm.R;
+  // LDS is kept near 53 KB (three workgroups per CU): the posterior covariance is read
+  // from global memory by the GEMM, the projection W accumulates in its output slot Lw
+  // (global, L2-resident) and comes into LDS (the chunk buffers) for the factorisation.
   double* pan = smem;                   // 2*kPK*kPS GEMM panels
-  double* sP = pan + 2 * kPK * kPS;     // M*M posterior covariance
-  double* sW = sP + kMaxM * kMaxM;      // M*M projection B' diag(nat2) B, then its factor
-  double* sC = sW + kMaxM * kMaxM;      // kRC rows of B (stride LC)
+  double* sC = pan + 2 * kPK * kPS;     // kRC rows of B (stride LC)
   double* sX = sC + kRC * kLC;          // kRC rows of B P, then nat2-scaled rows of B (stride LC)
+  double* sW = sC;                      // M*M after the chunk loop (sC and sX together hold it)
   double* sm = sX + kRC * kLC;          // M posterior mean
   double* sv = sm + kMaxM;              // M: B' nat1 -> z
   double* sn = sv + kMaxM;              // kRC: new nat1 of the chunk
@@ -554,8 +557,9 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
   constexpr bool UPDATE = MODE == BN_ST_UPDATE;
   constexpr bool PROJECT = MODE != BN_ST_ENERGY;
   constexpr bool SITES = MODE != BN_ST_PROJECT;
+  const double* gP = io.pP + kt * M * M;
+  double* gW = io.Lw + kt * M * M;
   if (SITES) {
-    for (int e = tid; e < M * M; e += kT) sP[e] = io.pP[kt * M * M + e];
     for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
   }
   for (int e = tid; e < M; e += kT) sv[e] = 0.0;
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
     }
     if (SITES) {
       // X = C P ; row i: mean_f = c_i . m, var_f = c_i . x_i + diag C
-      wg_gemm_small(kRC, M, M, 1.0, MatRef{sC, LC, 1}, MatRef{sP, M, 1}, 0.0, sX, LC, pan);
+      wg_gemm_small(kRC, M, M, 1.0, MatRef{sC, LC, 1}, MatRef{gP, M, 1}, 0.0, sX, LC, pan);
       if (tid < nr) {
         const int64_t s = kt * R + i0 + tid;
         double mf = 0.0, vf = io.cdiag[s];
@@ -613,7 +617,7 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
         for (int i = 0; i < nr; ++i) u = fma(sC[i * LC + tid], sn[i], u);
         sv[tid] = u;
       }
-      wg_gemm_small(M, M, kRC, 1.0, MatRef{sC, 1, LC}, MatRef{sX, LC, 1}, i0 == 0 ? 0.0 : 1.0, sW, M, pan);
+      wg_gemm_small(M, M, kRC, 1.0, MatRef{sC, 1, LC}, MatRef{sX, LC, 1}, i0 == 0 ? 0.0 : 1.0, gW, M, pan);
     }
   }
   if (UPDATE) {
@@ -622,7 +626,7 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
   }
   if (PROJECT) {
     __syncthreads();
-    for (int e = tid; e < M; e += kT) sW[e * M + e] += kPseudoJitter;
+    for (int e = tid; e < M * M; e += kT) sW[e] = gW[e] + ((e / M == e % M) ? kPseudoJitter : 0.0);
     const double ld = wg_chol_reg<kMaxM / 16>(sW, M, M, row);
     wg_lower_solve(sW, M, M, sv, 1, 0, 1);
     double* Lo = io.Lw + kt * M * M;
@@ -635,16 +639,14 @@ __global__ __launch_bounds__(kT) void k_st_sites(StDims dm, LikParams lik, GHTab
     // |z - L' m|^2 and tr(L' P L) = sum_ac L(a,c) (P L)(a,c), L in LDS (sW), P L in sX
     const double* L = io.Lw + kt * M * M;
     const double* z = io.zw + kt * M;
-    for (int e = tid; e < M * M; e += kT) sW[e] = L[e];
-    __syncthreads();
     double q2 = 0.0, tr = 0.0;
     if (tid < M) {
       double u = z[tid];
-      for (int a = tid; a < M; ++a) u = fma(-sW[a * M + tid], sm[a], u);   // (z - L' m)_tid
+      for (int a = tid; a < M; ++a) u = fma(-L[a * M + tid], sm[a], u);   // (z - L' m)_tid
       q2 = u * u;
     }
-    wg_gemm_small(M, M, M, 1.0, MatRef{sP, M, 1}, MatRef{sW, M, 1}, 0.0, sX, M, pan);
-    for (int e = tid; e < M * M; e += kT) tr = fma(sW[e], sX[e], tr);
+    wg_gemm_small(M, M, M, 1.0, MatRef{gP, M, 1}, MatRef{L, M, 1}, 0.0, sW, M, pan);  // P L
+    for (int e = tid; e < M * M; e += kT) tr = fma(L[e], sW[e], tr);
     const double Q2 = wg_sum(q2, red), TR = wg_sum(tr, red);
     if (tid == 0) {
       io.part0[kt] = e1;
@@ -1729,7 +1731,7 @@ constexpr size_t kPseudoSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM +
 constexpr size_t kTiedSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM);
 constexpr size_t kIhDareSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM);
 constexpr size_t kSitesSmem =
-    sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
+    sizeof(double) * (2 * kPK * kPS + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
   if (d == nullptr) return st_fail(BN_EINVAL, "dims is NULL");
