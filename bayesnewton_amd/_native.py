@@ -85,6 +85,7 @@ SIGNATURES = {
     "bn_st_predict": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_pairs_filter": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_mf_workspace_bytes": (i64, [vp]),
     "bn_mf_pseudo": (i32, [vp, vp, vp, vp, vp, vp]),
     "bn_mf_filter": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -839,6 +839,148 @@ int run_discretise(HyperView hv, int64_t n, const float* dt, float* A, float* Q,
   return BN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// k_pairs_filter: Kalman filter over pairs of neighbouring states (ops.py:383-426, the
+// filter of the sparse Markov model): pair (a, b) -> (b, A b), P- = [[Pbb + 1e-32 I,
+// Pbb A'], [A Pbb, A Pbb A' + Q]], observed whole with a full 2d x 2d noise per step.  One
+// 64-thread block per series, the 2d x 2d algebra in LDS (2d <= 12), FP64.  Outputs the
+// pair moments after every step and the per-series log likelihood.
+// ---------------------------------------------------------------------------
+template <int K1, int K2>
+__global__ __launch_bounds__(64) void k_pairs_filter(HyperView hv, const int64_t* __restrict__ off,
+                                                     const double* __restrict__ dt, const double* __restrict__ y,
+                                                     const double* __restrict__ R, double* __restrict__ mo,
+                                                     double* __restrict__ Po, double* __restrict__ ll) {
+  constexpr int D = K1 + K2, E = 2 * D;
+  __shared__ double P[E][E], Pm[E][E], L[E][E], X[E][E + 1];
+  __shared__ double m[E], mm[E], sv[E], A[D][D], Q[D][D];
+  __shared__ double s_ld;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t o0 = off[b], n_steps = off[b + 1] - o0;
+  const Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
+  for (int e = tid; e < E * E; e += 64) {
+    const int i = e / E, j = e % E;
+    double v = 0.0;
+    if (i < D && j < D) v = mdl.Pinf[i][j];
+    if (i >= D && j >= D) v = mdl.Pinf[i - D][j - D];
+    P[i][j] = v;
+  }
+  for (int e = tid; e < E; e += 64) m[e] = 0.0;
+  double ell = 0.0;
+  __syncthreads();
+  for (int64_t n = 0; n < n_steps; ++n) {
+    const int64_t s = o0 + n;
+    if (tid == 0) {
+      double Aa[D][D], Qq[D][D];
+      mdl.discretise(dt[s], Aa, Qq);
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) { A[i][j] = Aa[i][j]; Q[i][j] = Qq[i][j]; }
+    }
+    __syncthreads();
+    for (int i = tid; i < E; i += 64) {
+      double v = 0.0;
+      if (i < D) v = m[D + i];
+      else
+        for (int q = 0; q < D; ++q) v = fma(A[i - D][q], m[D + q], v);
+      mm[i] = v;
+    }
+    for (int e = tid; e < E * E; e += 64) {
+      const int i = e / E, j = e % E;
+      double v;
+      if (i < D && j < D) {
+        v = P[D + i][D + j] + (i == j ? 1e-32 : 0.0);
+      } else if (i < D) {
+        v = 0.0;
+        for (int q = 0; q < D; ++q) v = fma(P[D + i][D + q], A[j - D][q], v);
+      } else if (j < D) {
+        v = 0.0;
+        for (int q = 0; q < D; ++q) v = fma(A[i - D][q], P[D + q][D + j], v);
+      } else {
+        v = Q[i - D][j - D];
+        for (int q = 0; q < D; ++q) {
+          double t = 0.0;
+          for (int r = 0; r < D; ++r) t = fma(P[D + q][D + r], A[j - D][r], t);
+          v = fma(A[i - D][q], t, v);
+        }
+      }
+      Pm[i][j] = v;
+    }
+    __syncthreads();
+    // S = P- + R_n ; right-hand sides [P- | y - m-]
+    for (int e = tid; e < E * E; e += 64) {
+      const int i = e / E, j = e % E;
+      L[i][j] = Pm[i][j] + R[s * E * E + e];
+      X[i][j] = Pm[i][j];
+    }
+    for (int i = tid; i < E; i += 64) {
+      const double v = y[s * E + i] - mm[i];
+      sv[i] = v;
+      X[i][E] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {  // Cholesky (lower) of S; a non-positive pivot gives NaN, as the reference
+      double ld = 0.0;
+      for (int j = 0; j < E; ++j) {
+        double d = L[j][j];
+        for (int q = 0; q < j; ++q) d -= L[j][q] * L[j][q];
+        d = sqrt(d);
+        L[j][j] = d;
+        ld += 2.0 * log(d);
+        for (int i = j + 1; i < E; ++i) {
+          double v = L[i][j];
+          for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
+          L[i][j] = v / d;
+        }
+      }
+      s_ld = ld;
+    }
+    __syncthreads();
+    if (tid <= E) {  // X := S^-1 X, column tid
+      const int c = tid;
+      for (int i = 0; i < E; ++i) {
+        double v = X[i][c];
+        for (int q = 0; q < i; ++q) v -= L[i][q] * X[q][c];
+        X[i][c] = v / L[i][i];
+      }
+      for (int i = E - 1; i >= 0; --i) {
+        double v = X[i][c];
+        for (int q = i + 1; q < E; ++q) v -= L[q][i] * X[q][c];
+        X[i][c] = v / L[i][i];
+      }
+    }
+    __syncthreads();
+    // m = m- + P- S^-1 v ; P = P- - P- S^-1 P-
+    for (int i = tid; i < E; i += 64) {
+      double v = mm[i];
+      for (int q = 0; q < E; ++q) v = fma(Pm[i][q], X[q][E], v);
+      m[i] = v;
+      mo[s * E + i] = v;
+    }
+    for (int e = tid; e < E * E; e += 64) {
+      const int i = e / E, j = e % E;
+      double v = Pm[i][j];
+      for (int q = 0; q < E; ++q) v = fma(-Pm[i][q], X[q][j], v);
+      P[i][j] = v;
+      Po[s * E * E + e] = v;
+    }
+    if (tid == 0) {
+      double quad = 0.0;
+      for (int i = 0; i < E; ++i) quad = fma(sv[i], X[i][E], quad);
+      ell += -0.5 * (quad + s_ld + E * kLog2Pi);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) ll[b] = ell;
+}
+
+template <int K1, int K2>
+int run_pairs_filter(HyperView hv, int32_t n_series, const int64_t* off, const double* dt, const double* y,
+                     const double* R, double* mo, double* Po, double* ll, hipStream_t st) {
+  hipLaunchKernelGGL((k_pairs_filter<K1, K2>), dim3(n_series), dim3(64), 0, st, hv, off, dt, y, R, mo, Po, ll);
+  BN_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
 // Calls F<K1, K2>::run(args...) for the descriptor's component kinds.
 #define BN_DISPATCH(desc, FN, ...)                                              \
   [&]() -> int {                                                                \
@@ -1081,6 +1223,18 @@ int bn_plan_gather_u8(const bn_plan* plan, const uint8_t* src, uint8_t* dst, uin
                      src, dst, pad);
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
+}
+
+int bn_pairs_filter(const bn_kernel_desc* kern, const double* lengthscale, const double* variance,
+                    int32_t hyper_stride, int32_t n_series, const int64_t* offsets, const double* dt, const double* y,
+                    const double* noise, double* pair_means, double* pair_covs, double* loglik, void* stream) {
+  if (int rc = check_desc(kern)) return rc;
+  if (n_series < 1 || !offsets || !dt || !y || !noise || !pair_means || !pair_covs || !loglik || !lengthscale ||
+      !variance)
+    return fail(BN_EINVAL, "bad argument to bn_pairs_filter");
+  HyperView hv{lengthscale, variance, hyper_stride};
+  return BN_DISPATCH(kern, run_pairs_filter, hv, n_series, offsets, dt, y, noise, pair_means, pair_covs, loglik,
+                     (hipStream_t)stream);
 }
 
 int bn_discretise(const bn_kernel_desc* kern, const double* lengthscale, const double* variance, int64_t n,

@@ -332,3 +332,32 @@ def rauch_tung_striebel_smoother_infinite_horizon(dt, kernel, filter_mean, filte
     else:
         means, cov = sm[:, :M][..., None], Xs[:M, :M]
     return means, cov.expand(n, *cov.shape).contiguous(), gain.expand(n, D, D).contiguous(), dare_cov
+
+
+# ---- Kalman filter over pairs of states (ops.py:383-426) ----------------------------------
+
+def kalman_filter_pairs(dt, kernel, y, noise_cov, mask=None, parallel=False):
+    """ops.py:383-426 (the filter of the sparse Markov model): y [N,2d,1], noise_cov
+    [N,2d,2d].  Returns (ell, (means [N-1,d,1], covs [N-1,d,d])), FP64 tensors on the GPU:
+    the first state of each pair after the first step.  Both of the reference's forms give
+    this result; masks are not supported."""
+    import ctypes
+    if mask is not None and bool(np.any(np.asarray(mask.cpu() if isinstance(mask, torch.Tensor) else mask))):
+        raise NotImplementedError("masked observations are not supported by the pairs filter")
+    dev = _device()
+    d = kernel.state_dim
+    dtv = _as_tensor(dt, torch.float64, dev).reshape(-1).contiguous()
+    n = dtv.shape[0]
+    yv = _as_tensor(y, torch.float64, dev).reshape(n, 2 * d).contiguous()
+    R = _as_tensor(noise_cov, torch.float64, dev).reshape(n, 2 * d, 2 * d).contiguous()
+    f64 = dict(dtype=torch.float64, device=dev)
+    off = torch.tensor([0, n], dtype=torch.int64, device=dev)
+    mo = torch.empty((n, 2 * d), **f64)
+    Po = torch.empty((n, 2 * d, 2 * d), **f64)
+    ll = torch.empty(1, **f64)
+    desc = kernel.desc()
+    ls, var = kernel.hypers(dev)
+    nat.check(nat.lib().bn_pairs_filter(ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), 0, 1, nat.ptr(off),
+                                        nat.ptr(dtv), nat.ptr(yv), nat.ptr(R), nat.ptr(mo), nat.ptr(Po), nat.ptr(ll),
+                                        nat.stream_ptr()))
+    return ll[0], (mo[1:, :d, None], Po[1:, :d, :d])

@@ -249,6 +249,17 @@ int bn_site_reparametrise(const bn_plan *plan, const double *nat1, const double 
  * times the Hermite roots, w = weights / sqrt(pi), by Golub-Welsch on the host. */
 int bn_gauss_hermite(int32_t n, double *x, double *w);
 
+/* Kalman filter over pairs of neighbouring states, kalman_filter_pairs (ops.py:383-426), the
+ * filter of SparseMarkovGaussianProcess (basemodels.py:928-1011): the pair (x_{k-1}, x_k)
+ * with A_pair = [[0, I], [0, A]], Q_pair = blkdiag(1e-32 I, Q), observed whole (H = I)
+ * with a full 2d x 2d noise per step.  n_series series of steps offsets[s] ..
+ * offsets[s+1] (int64, device); dt [N], y [N,2d], noise [N,2d,2d] (FP64).  Outputs the pair
+ * means [N,2d] and covariances [N,2d,2d] after every step (the reference returns the first
+ * state of the pairs after each series' first step) and loglik [n_series].  No mask. */
+int bn_pairs_filter(const bn_kernel_desc *kern, const double *lengthscale, const double *variance,
+                    int32_t hyper_stride, int32_t n_series, const int64_t *offsets, const double *dt, const double *y,
+                    const double *noise, double *pair_means, double *pair_covs, double *loglik, void *stream);
+
 /* --- space-time models: SpatioTemporalKernel (kernels.py:385-586, sparse, 'Full'
  * conditional) driven by MarkovGaussianProcess (basemodels.py:625-764) --------
  * K independent runs ("runs"), each T steps with R observation locations in 2-D

@@ -479,3 +479,33 @@ def rauch_tung_striebel_smoother_infinite_horizon(dt, kernel, filter_mean, filte
     covs = np.broadcast_to(cov, (N,) + cov.shape).copy()
     gains = np.broadcast_to(gain, (N,) + gain.shape).copy()
     return means, covs, gains, dare_cov
+
+
+# ---- Kalman filter over pairs of states (ops.py:383-426), the sparse Markov model's filter --
+
+def kalman_filter_pairs(dt, kernel, y, noise_cov, mask=None, parallel=False):
+    """ops.py:383-426: the state is the pair (x_{k-1}, x_k) of neighbouring inducing states,
+    A_pair = [[0, I], [0, A]], Q_pair = blkdiag(1e-32 I, Q), observed whole (H = I).
+    y [N,2d,1], noise_cov [N,2d,2d].  Returns ell, (means [N-1,d,1], covs [N-1,d,d]): the
+    first state of each pair after the first step."""
+    y = np.asarray(y, dtype=np.float64)
+    noise_cov = np.asarray(noise_cov, dtype=np.float64)
+    if mask is None:
+        mask = np.zeros_like(y, dtype=bool)
+    mask = np.asarray(mask, dtype=bool).reshape(y.shape)
+    Pinf, As, Qs = discretise(kernel, dt)
+    d = Pinf.shape[0]
+    Z = np.zeros((d, d))
+    Pinfpair = np.block([[Pinf, Z], [Z, Pinf]])
+    minfpair = np.zeros((2 * d, 1))
+    N = As.shape[0]
+    Apairs = np.zeros((N, 2 * d, 2 * d))
+    Qpairs = np.zeros((N, 2 * d, 2 * d))
+    Apairs[:, :d, d:] = np.eye(d)
+    Apairs[:, d:, d:] = As
+    Qpairs[:, :d, :d] = 1e-32 * np.eye(d)
+    Qpairs[:, d:, d:] = Qs
+    H = np.eye(2 * d)
+    f = _parallel_kf if parallel else _sequential_kf
+    ell, means, covs = f(Apairs, Qpairs, H, y, noise_cov, minfpair, Pinfpair, mask)
+    return ell, (means[1:, :d], covs[1:, :d, :d])
