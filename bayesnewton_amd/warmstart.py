@@ -3,7 +3,7 @@
 The reference initialises every site as N(0, 1e2 I) (basemodels.py:130-133) and has no
 convergence loop.  Here a small network, trained on converged sites of plain runs,
 proposes initial sites; the plain iteration (inference.py:65-90) then runs from that
-start to the same convergence test, so it stays both the fallback and the judge of the
+start to the same convergence test, so it stays both the fallback and the check on the
 network's error.
 
 Network: features (BN_WS_FEATURES per site: the site's count, kernel-smoothed log

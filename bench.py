@@ -64,7 +64,7 @@ def parse():
                     help="batched: filter+smoother steps/s over 4096 uneven series (default); spacetime: "
                          "time to converge of the d=128 Cox process, learned warm start vs plain; spacetime-pass: "
                          "VI iterations (sites + filter + smoother + energy) over --st-runs d=128 runs; predict: "
-                         "predictive mean / variance at one test time per training time of the C3 batch; gradient: "
+                         "predictive mean / variance at one test time per training time of the 4096-series batch; gradient: "
                          "200 Adam steps on the energy of one 500k-step series")
     ap.add_argument("--st-train", type=int, default=200, help="spacetime: number of training runs")
     ap.add_argument("--st-runs", type=int, default=200, help="spacetime-pass: runs in the batch")
@@ -519,7 +519,7 @@ PRED_BYTES = 704.0
 
 
 def bench_predict(args):
-    """Prediction over the C3 batch (MarkovGaussianProcess.predict, basemodels.py:766-816):
+    """Prediction over the 4096-series batch (MarkovGaussianProcess.predict, basemodels.py:766-816):
     4096 series fitted with one VI step, one test time per training time (uniform over
     each series' span, 0.5 beyond both ends).  `value` is test times per second through
     bn_predict with the test times resident on the GPU and the smoothed full state
@@ -586,7 +586,7 @@ def bench_predict(args):
                "sample": f"oracle/models.py predict (FP64 NumPy: filter, smoother, per-point conditional) on series "
                          f"{b} ({len(xs[b])} steps, {len(tests[b])} test times), {csec:.2f}s"}
     out = {
-        "metric": "prediction: test times/s (C3 batch, predictive mean and variance of f)",
+        "metric": "prediction: test times/s (4096-series batch, predictive mean and variance of f)",
         "value": n / (kern_ms / 1e3),
         "unit": "test times/s",
         "n_gpus": 1,
@@ -597,7 +597,7 @@ def bench_predict(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (the C3 batch, one VI step of Student-t sites)",
+        "data": "synthetic (the 4096-series batch, one VI step of Student-t sites)",
         "config": {"workload": f"predict at {n} test times over {B} series (one per training time), d=5",
                    "series": B, "test_times": n},
         "wall_ms_per_call": wall * 1e3,
@@ -672,7 +672,7 @@ def main():
     dt = data["dt"].astype(np.float32)
     y = data["y"].astype(np.float32)
     r = np.full(N, data["scale"] ** 2, dtype=np.float32)
-    print(f"[bench] C3 generated in {time.time() - t_gen:.1f}s: B={B} N={N}", file=sys.stderr, flush=True)
+    print(f"[bench] batch generated in {time.time() - t_gen:.1f}s: B={B} N={N}", file=sys.stderr, flush=True)
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -762,7 +762,7 @@ def main():
         del scale
 
     out = {
-        "metric": "filter+smoother time-steps/s (batched uneven series, config 3)",
+        "metric": "filter+smoother steps/s vs CPU",
         "value": value,
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -774,7 +774,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 state recurrences, f32 per-step I/O",
         "data": "synthetic (seeded GP draws + Student-t noise)",
-        "config": {"workload": "C3: B=4096 Markov-GP series, lengths logU[256,16384], Matern52+Matern32 (d=5), "
+        "config": {"workload": "batched uneven series: B=4096 Markov-GP series, lengths logU[256,16384], Matern52+Matern32 (d=5), "
                                "one fused filter+RTS-smoother pass per step",
                    "series": B, "time_steps": N, "chunk_len": args.chunk, "state_dim": 5},
         "roofline": roof,

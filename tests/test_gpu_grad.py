@@ -6,7 +6,7 @@ Checked against central finite differences of the FP64 oracle energy
 (h = 1e-5 on the natural scale, chained through softplus) and, for a Gaussian
 likelihood at the exact sites, against the dense-GP closed form
 d/dtheta [-log N(y | 0, K + s2 I)] = -1/2 tr((a a' - Ky^-1) dKy/dtheta).
-Tolerance: 1e-4 relative (north star).
+Tolerance: 1e-4 relative (the stated error bound).
 """
 import numpy as np
 import pytest

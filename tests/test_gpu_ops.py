@@ -1,7 +1,7 @@
 """GPU parity of the ops.kalman_filter / ops.rauch_tung_striebel_smoother
 drop-ins (bayesnewton/ops.py:256-285, 357-380) against the FP64 oracle.
 
-Tolerances (north star): <= 1e-4 relative on posterior means and variances,
+Tolerances (the stated error bound): <= 1e-4 relative on posterior means and variances,
 <= 1e-5 relative on the log-likelihood.  Means are compared relative to the
 series' scale (max |mean|), variances and log-likelihoods pointwise.  The GPU
 path takes FP32 per-step inputs, so both sides get the same FP32-rounded
