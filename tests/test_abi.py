@@ -62,3 +62,39 @@ def test_gauss_hermite_golub_welsch_matches_numpy():
         np.testing.assert_allclose(x, np.sqrt(2.0) * hx, rtol=1e-12, atol=1e-12)
         np.testing.assert_allclose(w, hw / np.sqrt(np.pi), rtol=1e-9, atol=1e-300)
     assert L.bn_gauss_hermite(0, None, None) != 0
+
+
+def test_space_time_entries_reject_bad_arguments_before_any_device_work():
+    """Argument checks of the mean-field, infinite-horizon, prediction and pairs-filter
+    entries run on the host and return BN_EINVAL / BN_EUNSUPPORTED with a message (no
+    device pointer is touched, so this runs without a GPU)."""
+    from bayesnewton_amd import _native as nat
+    if not os.path.exists(nat.LIB_PATH):
+        pytest.skip("libbn_hip.so not built")
+    L = nat.lib()
+    ok = nat.StDims(2, 10, 4, 9, 2)
+    too_many = nat.StDims(2, 10, 4, 65, 2)
+    bad_kind = nat.StDims(2, 10, 4, 9, 3)
+    fake = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    hyp = nat.StHypers(fake, fake, fake, 0)
+    r = ctypes.byref
+    assert L.bn_mf_filter(r(too_many), r(hyp), 0, fake, fake, fake, fake, fake, fake, fake, None, None) != 0
+    assert L.bn_mf_filter(r(bad_kind), r(hyp), 0, fake, fake, fake, fake, fake, fake, fake, None, None) != 0
+    assert L.bn_mf_filter(r(ok), r(hyp), 1, fake, fake, fake, fake, fake, fake, fake, None, None) == -1
+    assert b"workspace" in L.bn_last_error()
+    assert L.bn_mf_filter(r(ok), None, 0, fake, fake, fake, fake, fake, fake, fake, None, None) == -1
+    assert L.bn_mf_smoother(r(ok), r(hyp), fake, fake, fake, None, None, 0, 0, None, None, None, None) == -1
+    assert L.bn_mf_smoother(r(ok), r(hyp), fake, fake, fake, None, fake, 1, 1, None, None, None, None) == -1
+    assert b"overlap" in L.bn_last_error()
+    assert L.bn_ih_filter(r(ok), r(hyp), fake, -1, 1, fake, fake, fake, fake, fake, fake, fake, fake, fake, None) == -1
+    assert L.bn_ih_filter(r(ok), r(hyp), fake, 20, 1, fake, None, fake, fake, fake, fake, fake, fake, fake, None) == -1
+    assert L.bn_ih_smoother(r(ok), r(hyp), fake, -1, fake, fake, fake, fake, None, None, None, None, fake, None) == -1
+    assert L.bn_ih_tied(r(ok), fake, fake, None, None) == -1
+    assert L.bn_st_predict(r(ok), r(hyp), fake, 0, 5, fake, fake, fake, fake, fake, fake, None, fake, fake, None) == -1
+    assert L.bn_st_predict(r(ok), r(hyp), fake, 3, 5, fake, fake, fake, fake, None, fake, None, fake, fake, None) == -1
+    assert L.bn_st_smoother_full(r(ok), r(hyp), fake, fake, fake, fake, fake, fake, None, None, fake, None) == -1
+    assert b"together" in L.bn_last_error()
+    desc = nat.KernelDesc() if hasattr(nat, "KernelDesc") else None
+    if desc is not None:
+        desc.n_comp, desc.kind[0] = 1, 2
+        assert L.bn_pairs_filter(r(desc), fake, fake, 0, 0, fake, fake, fake, fake, fake, fake, fake, None) == -1
