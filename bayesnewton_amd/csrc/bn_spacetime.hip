@@ -988,7 +988,7 @@ __global__ __launch_bounds__(kT) void k_mf_pseudo(StDims dm, const double* __res
   const int t = blockIdx.x, k = blockIdx.y, M = dm.M, tid = threadIdx.x;
   const int64_t kt = (int64_t)k * dm.T + t;
   const double* L = Lw + kt * M * M;  // lower triangular, upper part zero
-  wg_gemm(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 0.0, sW, M, pan);
+  wg_gemm_small(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 0.0, sW, M, pan);
   if (tid < M) {
     double s = 0.0;
     for (int q = 0; q <= tid; ++q) s = fma(L[tid * M + q], zw[kt * M + q], s);
@@ -1312,7 +1312,7 @@ __global__ __launch_bounds__(kT) void k_ih_tied_part(StDims dm, const double* __
   for (int e = tid; e < M * M; e += kT) acc[e] = 0.0;
   for (int t = t0; t < t1; ++t) {
     const double* L = Lw + ((int64_t)k * dm.T + t) * M * M;
-    wg_gemm(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 1.0, acc, M, pan);
+    wg_gemm_small(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 1.0, acc, M, pan);
   }
   __syncthreads();
   double* out = part + ((int64_t)k * kTiedParts + p) * M * M;
