@@ -535,7 +535,7 @@ constexpr int kLC = kMaxM + 1;  // LDS row stride of a chunk: consecutive rows s
 static_assert(2 * kRC * kLC >= kMaxM * kMaxM, "the two chunk buffers hold an M x M matrix");
 
 template <int METHOD, int MODE>
-__global__ __launch_bounds__(kT, 2) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
+__global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
                                                  int ensure_psd) {
   extern __shared__ double smem[];
   const int M = dm.M, R = dm.R;
