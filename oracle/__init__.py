@@ -14,12 +14,19 @@ the oracle is pinned by closed forms instead (see ``tests/test_oracle_*.py``):
                     H expm(F|tau|) Pinf H' vs the covariance function K(tau).
 * ``ops``         - filter log-likelihood vs the dense log N(y | 0, K + R);
                     smoother moments vs the dense GP posterior; sequential vs
-                    associative-scan forms.
+                    associative-scan forms.  Mean-field forms vs the dense filter
+                    (diagonal noise), the exact first-step conditional and the
+                    decoupled scalar-filter identity; infinite-horizon forms vs
+                    scipy's DARE and the exact steady state; the pairs filter vs
+                    dense Gaussian conditioning of the state chain.
 * ``likelihoods`` - Gauss-Hermite vs ``scipy.integrate.quad``; closed forms
                     (Gaussian, Poisson, probit moment matching); analytic
                     derivatives vs central finite differences.
 * ``models``      - Gaussian + VI + lr=1 gives the exact marginal likelihood;
-                    Markov GP == dense GP for every inference scheme.
+                    Markov GP == dense GP for every inference scheme; prediction
+                    vs the dense GP predictive (temporal and space-time); the
+                    mean-field model == the dense one when space decouples; the
+                    infinite-horizon prediction == the dense one in steady state.
 * ``kfrts.c``     - the FP64 C sequential filter/smoother (CPU baseline) vs the
                     NumPy restatement.
 

@@ -1,9 +1,10 @@
 """FP64 Kalman filter / RTS smoother of the oracle (test infrastructure only).
 
 Restates bayesnewton/ops.py:149-380 of the reference: the sequential
-(lax.scan) and associative-scan forms of the filter and the smoother; and
+(lax.scan) and associative-scan forms of the filter and the smoother;
 ops.py:429-706, their mean-field forms for space-time kernels (block-diagonal
-state covariance).
+state covariance); ops.py:796-1068, the infinite-horizon (Riccati steady-state)
+forms; and ops.py:383-426, the filter over pairs of states.
 Shapes follow the reference: y [N,p,1], noise_cov [N,p,p], means [N,d,1],
 covs [N,d,d].
 """
