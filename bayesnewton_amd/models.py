@@ -121,7 +121,6 @@ class MarkovGaussianProcess:
         self.device = dev
         if chunk_len is None:
             chunk_len = default_chunk_len(lens, self.parallel)
-        self.plan = SeriesPlan(off, chunk_len=chunk_len, device=dev)
         self._desc = kernel.desc()
         n_comp = self._desc.n_comp
         if ls is not None or var is not None:
@@ -138,6 +137,23 @@ class MarkovGaussianProcess:
             self._hyper_stride = 0
             self._per_series = False
             self._hyper_key = None
+        self.autotune = None
+        if isinstance(chunk_len, str):
+            if chunk_len != "auto" or not self.parallel:
+                raise ValueError("chunk_len must be an int, None or 'auto' (with the associative form)")
+            # time the fused pass on this batch's own steps at the initial sites
+            # (bayesnewton_amd.autotune); the values do not change the pass's cost
+            from . import autotune
+            if self._hyper_stride:
+                hl, hv = self._ls, self._var
+            else:
+                hl, hv = kernel.hypers(dev)
+            f32 = dict(dtype=torch.float32, device=dev)
+            chunk_len, tune_ms = autotune.tune_chunk_len(
+                off, self._desc, hl, hv, torch.as_tensor(self.dt, **f32), torch.zeros(self.num_data, **f32),
+                torch.full((self.num_data,), 1e2, **f32), hyper_stride=self._hyper_stride, device=dev)
+            self.autotune = {"chosen": chunk_len, "ms_per_pass": tune_ms}
+        self.plan = SeriesPlan(off, chunk_len=chunk_len, device=dev)
         P = self.plan
         self._dt = P.gather(torch.as_tensor(self.dt, dtype=torch.float32, device=dev))
         y32 = torch.as_tensor(self.Y[:, 0], dtype=torch.float32, device=dev)

@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--chunk", default="auto",
+                    help="batched workload: chunk length, or 'auto' (bayesnewton_amd.autotune, before the timed passes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference dispatch for HBM counter scaling)")
@@ -676,14 +677,25 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    plan = SeriesPlan(off, chunk_len=args.chunk, device=dev)
     kern = K.Sum([K.Matern52(), K.Matern32()])
     desc = kern.desc()
+    ls = torch.tensor(data["ls"], dtype=torch.float64, device=dev)
+    var = torch.tensor(data["var"], dtype=torch.float64, device=dev)
+    tune = None
+    if str(args.chunk) == "auto":
+        from bayesnewton_amd import autotune
+        t_tune = time.time()
+        chunk, tune_ms = autotune.tune_chunk_len(off, desc, ls, var, torch.from_numpy(dt).to(dev),
+                                                 torch.from_numpy(y).to(dev), torch.from_numpy(r).to(dev),
+                                                 hyper_stride=2, device=dev)
+        tune = {"chosen": chunk, "ms_per_pass": tune_ms, "seconds": time.time() - t_tune}
+        print(f"[bench] autotuned chunk_len {chunk}: {tune_ms}", file=sys.stderr, flush=True)
+    else:
+        chunk = int(args.chunk)
+    plan = SeriesPlan(off, chunk_len=chunk, device=dev)
     dt_t = plan.gather(torch.from_numpy(dt).to(dev))
     y_t = plan.gather(torch.from_numpy(y).to(dev))
     r_t = plan.gather(torch.from_numpy(r).to(dev), pad=1.0)
-    ls = torch.tensor(data["ls"], dtype=torch.float64, device=dev)
-    var = torch.tensor(data["var"], dtype=torch.float64, device=dev)
     pm = torch.empty(plan.n_slots, dtype=torch.float32, device=dev)
     pv = torch.empty(plan.n_slots, dtype=torch.float32, device=dev)
     ll = torch.empty(B, dtype=torch.float64, device=dev)
@@ -776,7 +788,8 @@ def main():
         "data": "synthetic (seeded GP draws + Student-t noise)",
         "config": {"workload": "batched uneven series: B=4096 Markov-GP series, lengths logU[256,16384], Matern52+Matern32 (d=5), "
                                "one fused filter+RTS-smoother pass per step",
-                   "series": B, "time_steps": N, "chunk_len": args.chunk, "state_dim": 5},
+                   "series": B, "time_steps": N, "chunk_len": chunk, "state_dim": 5},
+        "autotune": tune,
         "roofline": roof,
         "pass_roofline": {"alg_bytes_per_step": PASS_BYTES_PER_STEP,
                           "hbm_GBs": PASS_BYTES_PER_STEP * N / (ms_per_step / 1e3) / 1e9,

@@ -171,3 +171,27 @@ def test_energy_keeps_posterior_fixed_under_new_hypers():
     o.kernel.lengthscale = 1.3
     assert abs(float(g.energy()) - o.energy()) <= E_TOL * abs(o.energy())
     _check(g, o)  # the posterior was not recomputed
+
+
+def test_autotuned_chunk_length_gives_the_same_posterior():
+    """chunk_len='auto' (bayesnewton_amd.autotune: the fused pass timed on the batch for a
+    few chunk lengths) changes only the schedule: the posterior and energy equal those of a
+    fixed chunk length."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(21)
+    lens = rng.integers(40, 900, 30)
+    xs = [np.sort(rng.uniform(0, n / 10.0, n)) for n in lens]
+    ys = [(np.sin(x) + 0.3 * rng.standard_normal(x.size)).astype(np.float32).astype(np.float64) for x in xs]
+    mk = lambda c: M.MarkovVariationalGP.from_series(  # noqa: E731
+        K.Sum([K.Matern52(variance=1.0, lengthscale=2.0), K.Matern32(variance=0.5, lengthscale=0.7)]),
+        L.StudentsT(scale=0.2, df=3.0), xs, ys, chunk_len=c)
+    ma, m32 = mk("auto"), mk(32)
+    assert ma.autotune["chosen"] in ma.autotune["ms_per_pass"]
+    print(f"[autotune] {ma.autotune}", flush=True)
+    for m in (ma, m32):
+        m.inference(lr=1.0)
+    a = ma.posterior_mean.cpu().numpy().reshape(-1)
+    b = m32.posterior_mean.cpu().numpy().reshape(-1)
+    assert np.max(np.abs(a - b)) <= 1e-5 * max(np.max(np.abs(b)), 1.0)
+    ea, eb = float(ma.energy()), float(m32.energy())
+    assert abs(ea - eb) <= 1e-9 * abs(eb)
