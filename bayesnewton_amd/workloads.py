@@ -1,15 +1,19 @@
-"""Seeded synthetic workloads of the benchmark configurations (FP64, host).
+"""Seeded synthetic benchmark workloads (FP64, host).
 
-Each generator returns plain NumPy arrays; values the configurations leave
-open are fixed here once:
-  C1  Matern52 (l=1, s2=1), Gaussian(0.01); N=10,000 sorted U[0,100];
-      y = sin(0.3x) + 0.5 sin(1.7x) + 0.1 eps.                         seed 1001
-  C2  Matern32 + Matern52 (l=10, s2=0.5 each); N=200,000 sorted U[0,1000];
-      f = 2 sin(0.05x) + GP draw; y = 1[u < Phi(f)].                     seed 1002
-  C3  B=4096 series, n_b = floor(exp(U[ln 256, ln 16384])), x_b sorted
-      U[0, 0.1 n_b]; Matern52 + Matern32 with one lengthscale per series
-      l_b ~ logU[0.5, 20], s2 = 1; y = GP draw + 0.2 t_3.                seed 1003
-  C4  Matern52 (l=1, s2=1); N=500,000 sorted U[0,5000]; y = GP draw + 0.1 eps.  seed 1004
+Each generator returns plain NumPy arrays; values a workload leaves open are
+fixed here once:
+  single_series_10k    Matern52 (l=1, s2=1), Gaussian(0.01); N=10,000 sorted
+                       U[0,100]; y = sin(0.3x) + 0.5 sin(1.7x) + 0.1 eps.    seed 1001
+  binary_200k          Matern32 + Matern52 (l=10, s2=0.5 each); N=200,000 sorted
+                       U[0,1000]; f = 2 sin(0.05x) + GP draw; y = 1[u < Phi(f)].
+                                                                              seed 1002
+  batched_4096_series  B=4096 series, n_b = floor(exp(U[ln 256, ln 16384])), x_b
+                       sorted U[0, 0.1 n_b]; Matern52 + Matern32 with one lengthscale
+                       per series l_b ~ logU[0.5, 20], s2 = 1; y = GP draw + 0.2 t_3.
+                                                                              seed 1003
+  single_series_500k   Matern52 (l=1, s2=1); N=500,000 sorted U[0,5000];
+                       y = GP draw + 0.1 eps.                                 seed 1004
+  spacetime_cox        space-time log-Gaussian Cox process (docstring below). seed 1005
 GP draws use the reference's state-space sampler (basemodels.py:833-879):
 m0 = chol(Pinf) e, m <- A m + chol(Q + 1e-8 I) e, f = H m.
 """
@@ -125,14 +129,14 @@ def _dt(x):
     return np.concatenate([[0.0], np.diff(x)])
 
 
-def c1():
+def single_series_10k():
     rng = np.random.default_rng(1001)
     x = np.sort(rng.uniform(0, 100, 10_000))
     y = np.sin(0.3 * x) + 0.5 * np.sin(1.7 * x) + 0.1 * rng.standard_normal(x.size)
     return dict(x=x, y=y, kinds=["m52"], ls=[1.0], var=[1.0], noise=0.01)
 
 
-def c2():
+def binary_200k():
     rng = np.random.default_rng(1002)
     x = np.sort(rng.uniform(0, 1000, 200_000))
     f = 2 * np.sin(0.05 * x) + gp_draw(["m32", "m52"], [[10.0, 10.0]], [[0.5, 0.5]], [_dt(x)], rng)[0]
@@ -141,7 +145,7 @@ def c2():
     return dict(x=x, y=y, f=f, kinds=["m32", "m52"], ls=[10.0, 10.0], var=[0.5, 0.5])
 
 
-def c3(B=4096, seed=1003):
+def batched_4096_series(B=4096, seed=1003):
     rng = np.random.default_rng(seed)
     n = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), B))).astype(np.int64)
     xs = [np.sort(rng.uniform(0, 0.1 * nb, nb)) for nb in n]
@@ -156,7 +160,7 @@ def c3(B=4096, seed=1003):
                 kinds=["m52", "m32"], ls=ls, var=var, df=3.0, scale=0.2)
 
 
-def c4():
+def single_series_500k():
     rng = np.random.default_rng(1004)
     x = np.sort(rng.uniform(0, 5000, 500_000))
     f = gp_draw(["m52"], [[1.0]], [[1.0]], [_dt(x)], rng)[0]
@@ -176,7 +180,7 @@ def _sqdist(a, b):
     return np.sum(d * d, axis=-1)
 
 
-def c5(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize=2.0, device=None):
+def spacetime_cox(seed=1005, T=2000, R=256, grid=8, ls_t=10.0, ls_s=0.2, var_t=1.0, binsize=2.0, device=None):
     """Space-time log-Gaussian Cox process on [0,1]^2 x {0..T-1}.
 
     Inducing inputs z: grid x grid points of linspace(0,1,grid)^2.  Each step draws R

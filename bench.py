@@ -101,7 +101,7 @@ def bench_gradient(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t0 = time.time()
-    d = workloads.c4()
+    d = workloads.single_series_500k()
     log(f"[gradient] data generated in {time.time() - t0:.1f}s")
     x, y = d["x"], d["y"].astype(np.float32).astype(np.float64)
     model = M.MarkovVariationalGP(K.Matern52(variance=1.0, lengthscale=1.0), L.Gaussian(variance=0.01), x, y,
@@ -215,7 +215,7 @@ def bench_spacetime(args):
         t0 = time.perf_counter()
         train = []
         for k in range(args.st_train):
-            train.append(workloads.c5(seed=2000 + k, T=T, device=dev))
+            train.append(workloads.spacetime_cox(seed=2000 + k, T=T, device=dev))
             if (k + 1) % 20 == 0:
                 log(f"[spacetime] generated {k + 1}/{args.st_train} training runs ({time.perf_counter() - t0:.1f}s)")
         log(f"[spacetime] {args.st_train} training runs generated in {time.perf_counter() - t0:.1f}s")
@@ -286,7 +286,7 @@ def bench_spacetime(args):
     pass_ms = kern_ms.get("k_st_filter", 0.0) + kern_ms.get("k_st_smoother", 0.0)
     # held-out runs: plain vs warm start, one batch each, same convergence test
     seeds = list(range(args.st_heldout, args.st_heldout + args.st_heldout_n))
-    held = [workloads.c5(seed=sd, T=T, device=dev) for sd in seeds]
+    held = [workloads.spacetime_cox(seed=sd, T=T, device=dev) for sd in seeds]
 
     def batch():
         return S.SpaceTimeVariationalGP.from_runs(kern(), lik, [d["t"] for d in held], [d["r"] for d in held],
@@ -409,7 +409,7 @@ def bench_spacetime_pass(args, mean_field=None):
         return K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=10.0),
                                       K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
 
-    runs = [workloads.c5(seed=2000 + k, T=T, device=dev) for k in range(args.st_runs)]
+    runs = [workloads.spacetime_cox(seed=2000 + k, T=T, device=dev) for k in range(args.st_runs)]
     if mean_field is None:
         m = S.SpaceTimeVariationalGP.from_runs(kern(), L.Poisson(binsize=2.0), [d["t"] for d in runs],
                                                [d["r"] for d in runs], [d["y"] for d in runs])
@@ -464,7 +464,7 @@ def bench_spacetime_pass(args, mean_field=None):
     if not args.no_cpu:
         # the oracle (FP64 NumPy) on a bounded sample: one run, the first 40 steps
         from oracle import kernels as OK, likelihoods as OL, models as OM
-        d = workloads.c5(seed=2000, T=40)
+        d = workloads.spacetime_cox(seed=2000, T=40)
         ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 10.0), OK.Matern32(1.0, 0.2), z, sparse=True,
                                      conditional="Full")
         om = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI",
@@ -531,7 +531,7 @@ def bench_predict(args):
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    data = workloads.c3()
+    data = workloads.batched_4096_series()
     off = data["offsets"]
     B = len(off) - 1
     xs = [data["x"][off[b]:off[b + 1]] for b in range(B)]
@@ -666,7 +666,7 @@ def main():
     from bayesnewton_amd.plan import SeriesPlan
 
     t_gen = time.time()
-    data = workloads.c3()
+    data = workloads.batched_4096_series()
     off = data["offsets"]
     N = int(off[-1])
     B = len(off) - 1

@@ -157,7 +157,7 @@ def test_spacetime_precise_sites_stay_stable():
     site precisions to ~1e4 within two VI steps; the smoothed covariance must still follow
     the oracle's RTS recursion (an adjoint backward pass loses it geometrically here)."""
     from bayesnewton_amd import models as M, workloads
-    d = workloads.c5(seed=505, T=60, R=64, grid=4)
+    d = workloads.spacetime_cox(seed=505, T=60, R=64, grid=4)
     r = d["r"].astype(np.float32).astype(np.float64)
     gk, ok, gl, ol = _pair("VI", "poisson", 4, ls_t=10.0, ls_s=0.2)
     gm = M.MarkovVariationalGP(gk, gl, d["t"], d["y"], R=r)

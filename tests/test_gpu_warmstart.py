@@ -57,7 +57,7 @@ def _runs(seed0, n, T_, R_, z):
     from bayesnewton_amd import workloads
     out = []
     for k in range(n):
-        d = workloads.c5(seed=seed0 + k, T=T_, R=R_, grid=int(round(np.sqrt(len(z)))), ls_t=10.0, ls_s=0.2)
+        d = workloads.spacetime_cox(seed=seed0 + k, T=T_, R=R_, grid=int(round(np.sqrt(len(z)))), ls_t=10.0, ls_s=0.2)
         out.append((d["t"], d["r"].astype(np.float32).astype(np.float64), d["y"]))
     return out
 
