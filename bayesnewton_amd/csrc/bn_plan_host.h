@@ -69,13 +69,15 @@ inline bool build_plan(const int64_t* offsets, int32_t n_series, int32_t chunk_l
   }
   p.n_chunks = (int32_t)p.chunk_series.size();
   p.n_tiles = (int32_t)p.tile_first.size();
+  // Empty series inside a batch are fine (no chunk, log-likelihood 0); a batch
+  // with no step at all would launch empty grids, so it is refused here.
+  if (p.n_chunks == 0) { err = "offsets describe no steps (every series is empty)"; return false; }
   // Lanes are grouped 64 to a wavefront; with fewer than 64 chunks the group
   // shrinks to the next power of two so the tiled layout does not pad 64x.
   p.gshift = 6;
   while (p.gshift > 0 && (1 << (p.gshift - 1)) >= p.n_chunks) --p.gshift;
   const int32_t G = 1 << p.gshift;
   p.n_chunks_pad = ((p.n_chunks + G - 1) / G) * G;
-  if (p.n_chunks_pad == 0) p.n_chunks_pad = G;
   p.n_slots = (int64_t)(p.n_chunks_pad / G) * chunk_len * G;
   p.n_steps = offsets[n_series];
   return true;

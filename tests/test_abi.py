@@ -98,3 +98,27 @@ def test_space_time_entries_reject_bad_arguments_before_any_device_work():
     if desc is not None:
         desc.n_comp, desc.kind[0] = 1, 2
         assert L.bn_pairs_filter(r(desc), fake, fake, 0, 0, fake, fake, fake, fake, fake, fake, fake, None) == -1
+
+
+def test_plan_create_rejects_bad_offsets_before_any_device_call():
+    """bn_plan_create validates the CSR offsets on the host and returns -1 with a
+    message before it touches the device (so this runs without a GPU)."""
+    import numpy as np
+    from bayesnewton_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("libbn_hip.so not built")
+    L = _native.lib()
+    cases = [
+        ([0, 0, 0], 2, 16, "no steps"),
+        ([0, 5, 3], 2, 16, "non-decreasing"),
+        ([1, 5], 1, 16, "offsets[0]"),
+        ([0, 5], 0, 16, "n_series"),
+        ([0, 5], 1, 0, "chunk_len"),
+        ([0, 128 * 128 * 4 + 1], 1, 4, "chunks"),
+    ]
+    for off, ns, chunk, msg in cases:
+        o = np.asarray(off, dtype=np.int64)
+        h = ctypes.c_void_p()
+        rc = L.bn_plan_create(o.ctypes.data_as(ctypes.c_void_p), ns, chunk, ctypes.byref(h))
+        assert rc != 0 and h.value is None, off
+        assert msg in L.bn_last_error().decode(), (off, L.bn_last_error())

@@ -97,6 +97,39 @@ def test_discretise_matches_closed_form():
     assert np.max(np.abs(Q.cpu().numpy() - oQ)) < 1e-6 * np.max(np.abs(Pinf))
 
 
+
+@pytest.mark.parametrize("chunk", [1, 16])
+def test_batched_with_empty_series_vs_c_oracle(chunk):
+    """Empty series inside a batch (first, middle and last) own no chunk: the other
+    series' moments match the oracle and an empty series' log-likelihood is 0."""
+    import torch
+    from bayesnewton_amd import kernels as K, ops
+    from bayesnewton_amd.plan import SeriesPlan
+    rng = np.random.default_rng(700 + chunk)
+    lens = np.array([0, 7, 0, 0, 300, 1, 0, 1000, 0])
+    off = np.concatenate([[0], np.cumsum(lens)])
+    N = int(off[-1])
+    dt, y, r, _ = _data(N, 701)
+    for b in range(len(lens)):
+        if lens[b]:
+            dt[off[b]] = 0.0
+    kern = K.Matern52(variance=1.0, lengthscale=2.0)
+    dev = torch.device("cuda")
+    plan = SeriesPlan(off, chunk_len=chunk, device=dev)
+    t = lambda a, dtp=torch.float32: torch.as_tensor(a, dtype=dtp, device=dev)
+    dt_t, y_t, r_t = plan.gather(t(dt)), plan.gather(t(y)), plan.gather(t(r), pad=1.0)
+    ls, var = t([2.0], torch.float64), t([1.0], torch.float64)
+    pm, pv, ll = ops.filter_smoother(plan, kern.desc(), ls, var, dt_t, y_t, r_t, None)
+    pm = plan.scatter(pm).cpu().numpy().reshape(-1)
+    pv = plan.scatter(pv).cpu().numpy().reshape(-1)
+    ll = ll.cpu().numpy()
+    om, ov, oll = cfast.filter_smoother(["matern52"], [2.0], [1.0], off, dt, y, r)
+    assert np.max(np.abs(pm - om)) / np.max(np.abs(om)) < MEAN_TOL
+    assert np.max(np.abs(pv - ov) / np.abs(ov)) < VAR_TOL
+    empty = lens == 0
+    assert np.all(ll[empty] == 0.0) and np.all(oll[empty] == 0.0)
+    assert np.max(np.abs(ll[~empty] - oll[~empty]) / np.abs(oll[~empty])) < LL_TOL
+
 @pytest.mark.parametrize("chunk", [1, 3, 16, 64])
 def test_batched_uneven_fused_vs_c_oracle(chunk):
     """Fused update_posterior path on many ragged series (plan + tiled layout)."""
