@@ -637,7 +637,7 @@ def cpu_threads():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus N with N > 1 is not supported", file=sys.stderr)
+        print("bench.py: bayesnewton_amd runs on one GPU; --gpus N with N > 1 is not supported", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()

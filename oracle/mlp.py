@@ -1,7 +1,8 @@
 """FP64 restatement of the warm-start network and its features (test infrastructure only).
 
 The warm start has no counterpart in the reference (it replaces the site initialisation
-of basemodels.py:130-133), so this module restates the build's own definitions:
+of basemodels.py:130-133), so this module restates the definitions in
+bayesnewton_amd.warmstart and csrc/bn_warmstart.hip:
 the flat parameter layout, the 3 x 128 SiLU network with BF16-rounded weights and FP32
 biases, the standardised mean-squared-error loss and its gradient, and the site
 features.  Pinned by central finite differences of the loss (tests/test_oracle_mlp.py).
