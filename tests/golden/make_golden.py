@@ -1,0 +1,93 @@
+"""Write the committed golden fixtures of this directory from the FP64 oracle.
+
+The reference's own tests hold no golden vectors (they compare models against each
+other on unseeded data), so these fixtures are the oracle's outputs on seeded,
+FP32-representable inputs, written once and committed.  tests/test_golden.py checks
+that the oracle still reproduces them (a regression pin of the oracle) and
+tests/test_gpu_golden.py checks the HIP path against the stored arrays without
+running the oracle.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import kernels as OK, likelihoods as OL, models as OM, ops as OO  # noqa: E402
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def kf_rts_inputs():
+    """Sum of Matern32 (0.5, 3.0) and Matern52 (1.0, 1.0), 400 steps, 5 % masked."""
+    rng = np.random.default_rng(4242)
+    n = 400
+    x = np.sort(rng.uniform(0, 60.0, n))
+    dt = f32(np.concatenate([[0.0], np.diff(x)]))
+    y = f32(np.sin(0.3 * x) + 0.5 * np.sin(1.7 * x) + 0.1 * rng.standard_normal(n))
+    r = f32(0.01 + 0.05 * rng.uniform(size=n))
+    mask = rng.uniform(size=n) < 0.05
+    return dict(dt=dt, y=y, r=r, mask=mask)
+
+
+def kf_rts_kernel():
+    return OK.Sum([OK.Matern32(0.5, 3.0), OK.Matern52(1.0, 1.0)])
+
+
+def kf_rts_outputs(d):
+    k = kf_rts_kernel()
+    ell, (fm, fP) = OO.kalman_filter(d["dt"], k, d["y"][:, None, None], d["r"][:, None, None],
+                                     mask=d["mask"][:, None, None])
+    dts = np.concatenate([d["dt"][1:], [0.0]])
+    sm, sP, _ = OO.rauch_tung_striebel_smoother(dts, k, fm, fP)
+    return dict(loglik=np.asarray(ell, dtype=np.float64), filter_mean=fm, filter_cov=fP,
+                smooth_mean=sm, smooth_var=sP)
+
+
+def vi_probit_inputs():
+    """Bernoulli-probit MarkovVariationalGP, 300 points on a 2^-10 grid."""
+    rng = np.random.default_rng(4343)
+    n = 300
+    x = np.sort(rng.choice(15 * 1024, size=n, replace=False)) / 1024.0
+    f = np.sin(0.3 * x) + 0.5 * np.sin(1.7 * x)
+    y = (rng.uniform(size=n) < 0.5 * (1 + np.tanh(f))).astype(np.float64)
+    return dict(x=x, y=y)
+
+
+VI_LRS = (1.0, 0.7, 0.7)
+
+
+def vi_probit_outputs(d):
+    o = OM.MarkovGP(OK.Sum([OK.Matern32(0.5, 10.0), OK.Matern52(0.5, 10.0)]), OL.Bernoulli(), d["x"], d["y"],
+                    inference="VI")
+    means, variances, energies = [], [], []
+    for lr in VI_LRS:
+        o.inference(lr=lr)
+        means.append(o.posterior_mean.reshape(-1).copy())
+        variances.append(o.posterior_variance.reshape(-1).copy())
+        energies.append(float(o.energy()))
+    return dict(post_mean=np.stack(means), post_var=np.stack(variances), energy=np.asarray(energies))
+
+
+FIXTURES = {
+    "kf_rts_sum_m32_m52_n400.npz": (kf_rts_inputs, kf_rts_outputs),
+    "vi_probit_n300.npz": (vi_probit_inputs, vi_probit_outputs),
+}
+
+
+def main():
+    for name, (inp, out) in FIXTURES.items():
+        d = inp()
+        res = out(d)
+        np.savez_compressed(os.path.join(HERE, name), **d, **res)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

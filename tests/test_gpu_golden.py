@@ -1,0 +1,55 @@
+"""GPU path against the committed golden fixtures (tests/golden/), without running
+the oracle: ops.kalman_filter / ops.rauch_tung_striebel_smoother (ops.py:256-285,
+357-380) and three MarkovVariationalGP inference steps (inference.py:65-90) with a
+Bernoulli-probit likelihood.  Tolerances as in the other GPU parity tests: 1e-4
+relative on means (to the series' scale) and variances, 1e-5 on log-likelihoods and
+energies."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MEAN_TOL, VAR_TOL, LL_TOL = 1e-4, 1e-4, 1e-5
+
+
+def _load(name):
+    z = np.load(os.path.join(GOLDEN, name))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("parallel", [False, True])
+def test_kf_rts_matches_golden(parallel):
+    from bayesnewton_amd import kernels as K, ops
+    z = _load("kf_rts_sum_m32_m52_n400.npz")
+    kern = K.Sum([K.Matern32(variance=0.5, lengthscale=3.0), K.Matern52(variance=1.0, lengthscale=1.0)])
+    dt, y, r, mask = z["dt"], z["y"], z["r"], z["mask"]
+    ell, (fm, fP) = ops.kalman_filter(dt, kern, y[:, None, None], r[:, None, None], mask=mask[:, None, None],
+                                      parallel=parallel)
+    assert abs(float(ell) - float(z["loglik"])) < LL_TOL * abs(float(z["loglik"]))
+    ofm, ofP = z["filter_mean"], z["filter_cov"]
+    assert np.max(np.abs(fm.cpu().numpy() - ofm)) / np.max(np.abs(ofm)) < MEAN_TOL
+    assert np.max(np.abs(fP.cpu().numpy() - ofP)) / np.max(np.abs(ofP)) < VAR_TOL
+    dts = np.concatenate([dt[1:], [0.0]])
+    sm, sP, _ = ops.rauch_tung_striebel_smoother(dts, kern, ofm, ofP, parallel=parallel)
+    osm, osv = z["smooth_mean"].reshape(-1), z["smooth_var"].reshape(-1)
+    assert np.max(np.abs(sm.cpu().numpy().reshape(-1) - osm)) / np.max(np.abs(osm)) < MEAN_TOL
+    assert np.max(np.abs(sP.cpu().numpy().reshape(-1) - osv) / osv) < VAR_TOL
+
+
+def test_vi_probit_matches_golden():
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    z = _load("vi_probit_n300.npz")
+    kern = K.Sum([K.Matern32(variance=0.5, lengthscale=10.0), K.Matern52(variance=0.5, lengthscale=10.0)])
+    g = M.MarkovVariationalGP(kern, L.Bernoulli(), z["x"], z["y"], parallel=True)
+    for it, lr in enumerate((1.0, 0.7, 0.7)):
+        g.inference(lr=lr)
+        pm = g.posterior_mean.cpu().numpy().reshape(-1)
+        pv = g.posterior_variance.cpu().numpy().reshape(-1)
+        om, ov = z["post_mean"][it], z["post_var"][it]
+        assert np.max(np.abs(pm - om)) / np.max(np.abs(om)) < MEAN_TOL, it
+        assert np.max(np.abs(pv - ov) / ov) < VAR_TOL, it
+        e = float(g.energy())
+        assert abs(e - z["energy"][it]) <= LL_TOL * abs(z["energy"][it]), (it, e, z["energy"][it])
