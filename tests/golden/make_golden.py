@@ -75,9 +75,45 @@ def vi_probit_outputs(d):
     return dict(post_mean=np.stack(means), post_var=np.stack(variances), energy=np.asarray(energies))
 
 
+def spacetime_inputs():
+    """Space-time Poisson (binsize 2) data: 24 steps, 20 locations per step in [0,1]^2,
+    FP32-representable locations, inducing grid 3 x 3."""
+    rng = np.random.default_rng(4444)
+    T, nr = 24, 20
+    t = np.arange(T, dtype=np.float64)
+    r = f32(rng.uniform(size=(T, nr, 2)))
+    f = np.sin(0.3 * t)[:, None] + np.cos(3.0 * r[..., 0]) * np.sin(2.0 * r[..., 1])
+    y = rng.poisson(2.0 * np.exp(f)).astype(np.float64)
+    g = np.linspace(0, 1, 3)
+    a, b = np.meshgrid(g, g)
+    z = np.stack([a.ravel(), b.ravel()], 1)
+    return dict(t=t, r=r, y=y, z=z)
+
+
+def spacetime_outputs(d):
+    """Three VI steps of the dense, mean-field (associative form) and infinite-horizon
+    space-time models with temporal Matern32 (1, 5) and spatial Matern32 (1, 0.3)."""
+    out = {}
+    for name, kw in (("dense", {}), ("meanfield", dict(mean_field=True, parallel=True)),
+                     ("ih", dict(infinite_horizon=True))):
+        ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 5.0), OK.Matern32(1.0, 0.3), d["z"])
+        o = OM.MarkovGP(ok, OL.Poisson(2.0), d["t"][:, None], d["y"], R=d["r"], inference="VI", **kw)
+        means, variances, energies = [], [], []
+        for lr in VI_LRS:
+            o.inference(lr=lr)
+            means.append(np.asarray(o.posterior_mean).copy())
+            variances.append(np.diagonal(o.posterior_variance, axis1=1, axis2=2).copy())
+            energies.append(float(o.energy()))
+        out[f"{name}_post_mean"] = np.stack(means)
+        out[f"{name}_post_var"] = np.stack(variances)
+        out[f"{name}_energy"] = np.asarray(energies)
+    return out
+
+
 FIXTURES = {
     "kf_rts_sum_m32_m52_n400.npz": (kf_rts_inputs, kf_rts_outputs),
     "vi_probit_n300.npz": (vi_probit_inputs, vi_probit_outputs),
+    "spacetime_poisson_vi_g3_t24.npz": (spacetime_inputs, spacetime_outputs),
 }
 
 

@@ -1,7 +1,8 @@
 """GPU path against the committed golden fixtures (tests/golden/), without running
 the oracle: ops.kalman_filter / ops.rauch_tung_striebel_smoother (ops.py:256-285,
 357-380) and three MarkovVariationalGP inference steps (inference.py:65-90) with a
-Bernoulli-probit likelihood.  Tolerances as in the other GPU parity tests: 1e-4
+Bernoulli-probit likelihood, and three VI steps of the dense, mean-field and
+infinite-horizon space-time models with a Poisson likelihood.  Tolerances as in the other GPU parity tests: 1e-4
 relative on means (to the series' scale) and variances, 1e-5 on log-likelihoods and
 energies."""
 import os
@@ -53,3 +54,28 @@ def test_vi_probit_matches_golden():
         assert np.max(np.abs(pv - ov) / ov) < VAR_TOL, it
         e = float(g.energy())
         assert abs(e - z["energy"][it]) <= LL_TOL * abs(z["energy"][it]), (it, e, z["energy"][it])
+
+
+@pytest.mark.parametrize("variant", ["dense", "meanfield", "ih"])
+def test_spacetime_vi_matches_golden(variant):
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    z = _load("spacetime_poisson_vi_g3_t24.npz")
+    gk = K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=5.0), K.Matern32(variance=1.0, lengthscale=0.3),
+                                z=z["z"], sparse=True, conditional="Full")
+    lik = L.Poisson(binsize=2.0)
+    if variant == "dense":
+        g = M.MarkovVariationalGP(gk, lik, z["t"], z["y"], R=z["r"])
+    elif variant == "meanfield":
+        g = M.MarkovVariationalMeanFieldGP(gk, lik, z["t"], z["y"], R=z["r"], parallel=True)
+    else:
+        g = M.InfiniteHorizonVariationalGP(gk, lik, z["t"], z["y"], R=z["r"])
+    print(f"[golden spacetime] variant={variant}", flush=True)
+    for it, lr in enumerate((1.0, 0.7, 0.7)):
+        g.inference(lr=lr)
+        pm = g.posterior_mean.cpu().numpy()
+        pv = np.diagonal(g.posterior_variance.cpu().numpy(), axis1=1, axis2=2)
+        om, ov = z[f"{variant}_post_mean"][it], z[f"{variant}_post_var"][it]
+        assert np.max(np.abs(pm - om)) / np.max(np.abs(om)) < MEAN_TOL, it
+        assert np.max(np.abs(pv - ov) / ov) < VAR_TOL, it
+        e, oe = float(g.energy()), z[f"{variant}_energy"][it]
+        assert abs(e - oe) <= LL_TOL * abs(oe), (it, e, oe)
