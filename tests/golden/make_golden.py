@@ -110,10 +110,29 @@ def spacetime_outputs(d):
     return out
 
 
+def pairs_inputs():
+    """Filter over pairs of states: Matern52 (1.0, 3.0) + Matern32 (0.5, 0.7), 60
+    transitions with the sentinel step sizes at both ends, full 2d x 2d pseudo noise."""
+    rng = np.random.default_rng(4545)
+    n, d = 60, 5
+    dt = np.concatenate([[2e10], rng.uniform(0.2, 1.5, n - 2), [2e10]])
+    y = rng.standard_normal((n, 2 * d, 1))
+    G = rng.standard_normal((n, 2 * d, 2 * d)) * 0.3
+    noise = G @ np.swapaxes(G, 1, 2) + 0.2 * np.eye(2 * d)
+    return dict(dt=dt, y=y, noise=noise)
+
+
+def pairs_outputs(d):
+    k = OK.Sum([OK.Matern52(1.0, 3.0), OK.Matern32(0.5, 0.7)])
+    ell, (fm, fP) = OO.kalman_filter_pairs(d["dt"], k, d["y"], d["noise"], parallel=False)
+    return dict(loglik=np.asarray(ell, dtype=np.float64), pair_mean=fm, pair_cov=fP)
+
+
 FIXTURES = {
     "kf_rts_sum_m32_m52_n400.npz": (kf_rts_inputs, kf_rts_outputs),
     "vi_probit_n300.npz": (vi_probit_inputs, vi_probit_outputs),
     "spacetime_poisson_vi_g3_t24.npz": (spacetime_inputs, spacetime_outputs),
+    "pairs_sum_m52_m32_n60.npz": (pairs_inputs, pairs_outputs),
 }
 
 

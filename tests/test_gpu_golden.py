@@ -2,7 +2,9 @@
 the oracle: ops.kalman_filter / ops.rauch_tung_striebel_smoother (ops.py:256-285,
 357-380) and three MarkovVariationalGP inference steps (inference.py:65-90) with a
 Bernoulli-probit likelihood, and three VI steps of the dense, mean-field and
-infinite-horizon space-time models with a Poisson likelihood.  Tolerances as in the other GPU parity tests: 1e-4
+infinite-horizon space-time models with a Poisson likelihood, and the filter over pairs
+of states (ops.py:383-426, FP64 throughout: 1e-9 on the log likelihood, 1e-8 on the
+moments).  Tolerances as in the other GPU parity tests: 1e-4
 relative on means (to the series' scale) and variances, 1e-5 on log-likelihoods and
 energies."""
 import os
@@ -79,3 +81,17 @@ def test_spacetime_vi_matches_golden(variant):
         assert np.max(np.abs(pv - ov) / ov) < VAR_TOL, it
         e, oe = float(g.energy()), z[f"{variant}_energy"][it]
         assert abs(e - oe) <= LL_TOL * abs(oe), (it, e, oe)
+
+
+@pytest.mark.parametrize("parallel", [False, True])
+def test_pairs_filter_matches_golden(parallel):
+    from bayesnewton_amd import kernels as K, ops
+    z = _load("pairs_sum_m52_m32_n60.npz")
+    kern = K.Sum([K.Matern52(variance=1.0, lengthscale=3.0), K.Matern32(variance=0.5, lengthscale=0.7)])
+    ell, (fm, fP) = ops.kalman_filter_pairs(z["dt"], kern, z["y"], z["noise"], parallel=parallel)
+    oll = float(z["loglik"])
+    assert abs(float(ell) - oll) <= 1e-9 * abs(oll)
+    fm, fP = fm.cpu().numpy(), fP.cpu().numpy()
+    assert fm.shape == z["pair_mean"].shape and fP.shape == z["pair_cov"].shape
+    assert np.max(np.abs(fm - z["pair_mean"])) <= 1e-8 * np.max(np.abs(z["pair_mean"]))
+    assert np.max(np.abs(fP - z["pair_cov"])) <= 1e-8 * np.max(np.abs(z["pair_cov"]))
