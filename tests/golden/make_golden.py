@@ -128,11 +128,46 @@ def pairs_outputs(d):
     return dict(loglik=np.asarray(ell, dtype=np.float64), pair_mean=fm, pair_cov=fP)
 
 
+def predict_inputs():
+    """Student-t (scale 0.25, df 3) MarkovVariationalGP with Matern52 (0.8, 1.5) +
+    Matern32 (0.4, 4.0), 300 points on a 2^-8 grid, three missing; test times inside,
+    before and after the data, on training times and repeated."""
+    rng = np.random.default_rng(4646)
+    n = 300
+    x = np.sort(rng.choice(40 * 256, size=n, replace=False) / 256.0)
+    f = np.sin(0.4 * x) + 0.5 * np.cos(1.3 * x)
+    y = f32(f + 0.25 * rng.standard_t(3.0, size=n))
+    y[[10, 11, 200]] = np.nan
+    lo, hi = x.min(), x.max()
+    xs = np.concatenate([[lo - 3.0, lo - 0.01], rng.uniform(lo, hi, 40), x[[0, 3, n - 1]], [x[5], x[5]],
+                         [hi + 0.5, hi + 7.0]])
+    ok = ~np.isnan(y)
+    ys = np.where(rng.uniform(size=xs.shape[0]) < 0.1, np.nan, np.interp(xs, x[ok], y[ok]))
+    return dict(x=x, y=y, x_test=xs, y_test=ys)
+
+
+PREDICT_LRS = (1.0, 0.7)
+
+
+def predict_outputs(d):
+    o = OM.MarkovGP(OK.Sum([OK.Matern52(0.8, 1.5), OK.Matern32(0.4, 4.0)]), OL.StudentsT(0.25, 3.0), d["x"],
+                    d["y"], inference="VI")
+    for lr in PREDICT_LRS:
+        o.inference(lr=lr)
+    m, v = o.predict(d["x_test"][:, None])
+    my, vy = o.predict_y(d["x_test"][:, None])
+    nlpd = o.negative_log_predictive_density(d["x_test"][:, None], d["y_test"])
+    return dict(pred_mean=np.asarray(m, np.float64).reshape(-1), pred_var=np.asarray(v, np.float64).reshape(-1),
+                pred_y_mean=np.asarray(my, np.float64).reshape(-1), pred_y_var=np.asarray(vy, np.float64).reshape(-1),
+                nlpd=np.asarray(nlpd, dtype=np.float64))
+
+
 FIXTURES = {
     "kf_rts_sum_m32_m52_n400.npz": (kf_rts_inputs, kf_rts_outputs),
     "vi_probit_n300.npz": (vi_probit_inputs, vi_probit_outputs),
     "spacetime_poisson_vi_g3_t24.npz": (spacetime_inputs, spacetime_outputs),
     "pairs_sum_m52_m32_n60.npz": (pairs_inputs, pairs_outputs),
+    "predict_studentt_vi_n300.npz": (predict_inputs, predict_outputs),
 }
 
 

@@ -16,7 +16,8 @@ def _load(name):
 
 
 @pytest.mark.parametrize("name", ["kf_rts_sum_m32_m52_n400.npz", "vi_probit_n300.npz",
-                                  "spacetime_poisson_vi_g3_t24.npz", "pairs_sum_m52_m32_n60.npz"])
+                                  "spacetime_poisson_vi_g3_t24.npz", "pairs_sum_m52_m32_n60.npz",
+                                  "predict_studentt_vi_n300.npz"])
 def test_oracle_reproduces_golden(name):
     import importlib.util
     spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))

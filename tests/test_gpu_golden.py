@@ -4,7 +4,8 @@ the oracle: ops.kalman_filter / ops.rauch_tung_striebel_smoother (ops.py:256-285
 Bernoulli-probit likelihood, and three VI steps of the dense, mean-field and
 infinite-horizon space-time models with a Poisson likelihood, and the filter over pairs
 of states (ops.py:383-426, FP64 throughout: 1e-9 on the log likelihood, 1e-8 on the
-moments).  Tolerances as in the other GPU parity tests: 1e-4
+moments), and prediction / predict_y / NLPD (basemodels.py:165-193, 766-816) after two
+Student-t VI steps.  Tolerances as in the other GPU parity tests: 1e-4
 relative on means (to the series' scale) and variances, 1e-5 on log-likelihoods and
 energies."""
 import os
@@ -95,3 +96,26 @@ def test_pairs_filter_matches_golden(parallel):
     assert fm.shape == z["pair_mean"].shape and fP.shape == z["pair_cov"].shape
     assert np.max(np.abs(fm - z["pair_mean"])) <= 1e-8 * np.max(np.abs(z["pair_mean"]))
     assert np.max(np.abs(fP - z["pair_cov"])) <= 1e-8 * np.max(np.abs(z["pair_cov"]))
+
+
+def test_predict_matches_golden():
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    z = _load("predict_studentt_vi_n300.npz")
+    kern = K.Sum([K.Matern52(variance=0.8, lengthscale=1.5), K.Matern32(variance=0.4, lengthscale=4.0)])
+    g = M.MarkovVariationalGP(kern, L.StudentsT(scale=0.25, df=3.0), z["x"], z["y"])
+    for lr in (1.0, 0.7):
+        g.inference(lr=lr)
+
+    def close(a, key):
+        a = np.asarray(a.cpu().numpy() if hasattr(a, "cpu") else a, np.float64).reshape(-1)
+        o = z[key]
+        np.testing.assert_allclose(a, o, rtol=MEAN_TOL, atol=MEAN_TOL * np.max(np.abs(o)), err_msg=key)
+
+    m, v = g.predict(z["x_test"])
+    close(m, "pred_mean")
+    close(v, "pred_var")
+    my, vy = g.predict_y(z["x_test"])
+    close(my, "pred_y_mean")
+    close(vy, "pred_y_var")
+    nlpd = float(g.negative_log_predictive_density(z["x_test"], z["y_test"]))
+    assert abs(nlpd - float(z["nlpd"])) <= MEAN_TOL * max(1.0, abs(float(z["nlpd"])))
