@@ -730,7 +730,7 @@ def main():
     dom = max(kern_ms, key=lambda k: kern_ms[k][0])
     dom_avg_ms = kern_ms[dom][0] / kern_ms[dom][1]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "batched_4096_series_pmc_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
