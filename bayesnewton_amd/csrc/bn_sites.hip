@@ -103,7 +103,6 @@ __global__ __launch_bounds__(kBlock) void k_lik_grad(PlanView pl, LikParams L, G
     s1 += dterm1_dparam<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s]);
   }
   io.part1[j] = s1;
-  io.part2[j] = 0.0;
 }
 
 __global__ void k_reparametrise(PlanView pl, const double* __restrict__ nat1, const double* __restrict__ nat2,
@@ -112,6 +111,23 @@ __global__ void k_reparametrise(PlanView pl, const double* __restrict__ nat1, co
   if (s >= pl.n_slots) return;
   pr[s] = (float)(1.0 / nat2[s]);
   py[s] = (float)(nat1[s] / nat2[s]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_series_sum1(PlanView pl, const double* __restrict__ a,
+                                                        double* __restrict__ oa) {
+  __shared__ double ra[kBlock];
+  const int32_t s = blockIdx.x;
+  const int32_t c0 = pl.series_first_chunk[s];
+  const int32_t nc = pl.series_nchunks[s];
+  double x = 0.0;
+  for (int32_t i = threadIdx.x; i < nc; i += kBlock) x += a[c0 + i];
+  ra[threadIdx.x] = x;
+  __syncthreads();
+  for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int32_t)threadIdx.x < w) ra[threadIdx.x] += ra[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) oa[s] = ra[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_series_sum2(PlanView pl, const double* __restrict__ a,
@@ -421,8 +437,7 @@ int bn_lik_param_grad(const bn_plan* plan, const bn_kernel_desc* kern, const bn_
     hipLaunchKernelGGL(k_lik_grad<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io);
   else
     return fail(BN_EINVAL, "method must be BN_VI or BN_NEWTON");
-  // reuse the pairwise reducer; the second output is scratch
-  hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, grad, p2);
+  hipLaunchKernelGGL(k_series_sum1, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, grad);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;
 }

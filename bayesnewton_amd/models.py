@@ -183,13 +183,19 @@ class MarkovGaussianProcess:
         if key != self._hyper_key:
             self._ls, self._var = self.kernel.hypers(self.device)
             self._hyper_key = key
+            self._hyper_version += 1
         return self._ls, self._var
 
     def _state_key(self):
-        ls, var = self._hypers()
-        return (id(ls), id(var), self._sites_version)
+        """Identity of the (kernel hypers, sites) state a stored posterior or
+        log-likelihood belongs to: two counters bumped whenever the hyper tensors
+        are replaced or the sites change (tensor ids are reused by the allocator,
+        so they cannot serve as the key)."""
+        self._hypers()
+        return (self._hyper_version, self._sites_version)
 
     _sites_version = 0
+    _hyper_version = 0
 
     # ---- posterior -----------------------------------------------------------------
     def update_posterior(self):
@@ -466,7 +472,18 @@ MarkovGP = MarkovGaussianProcess
 class MarkovEnergy(torch.autograd.Function):
     """energy(raw hypers) as a differentiable torch function: forward sets the raw
     hyperparameters and returns the energy, backward returns the hand-written
-    gradient (forward-mode tangents through the chunked filter scan)."""
+    gradient (forward-mode tangents through the chunked filter scan).
+
+    The reference's training step (README.md:56-76) runs inference() on the
+    optimiser's current hypers, then GradValues(energy), then Adam.  The optimiser
+    updates `raw` in place, so call ``model.set_raw_hypers(raw)`` (or
+    ``MarkovEnergy.sync(raw, model)``) after ``opt.step()`` and before the next
+    ``inference()``; otherwise that inference runs one step behind."""
+
+    @staticmethod
+    def sync(raw, model):
+        """Push the optimiser's raw hypers into the model."""
+        model.set_raw_hypers(raw.detach().cpu().numpy())
 
     @staticmethod
     def forward(ctx, raw, model, cubature=None):

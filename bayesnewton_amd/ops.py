@@ -340,7 +340,9 @@ def kalman_filter_pairs(dt, kernel, y, noise_cov, mask=None, parallel=False):
     """ops.py:383-426 (the filter of the sparse Markov model): y [N,2d,1], noise_cov
     [N,2d,2d].  Returns (ell, (means [N-1,d,1], covs [N-1,d,d])), FP64 tensors on the GPU:
     the first state of each pair after the first step.  Both of the reference's forms give
-    this result; masks are not supported."""
+    this result, so `parallel` is accepted for the signature and both run the one
+    sequential kernel (k_pairs_filter); the associative pairs form with its 1e-32
+    jitter (ops.py:405-426) is not built.  Masks are not supported."""
     import ctypes
     if mask is not None and bool(np.any(np.asarray(mask.cpu() if isinstance(mask, torch.Tensor) else mask))):
         raise NotImplementedError("masked observations are not supported by the pairs filter")

@@ -15,6 +15,7 @@ training is mini-batch Adam on FP32 master weights, all in libbn_hip.so
 """
 import ctypes
 import math
+import os
 import time
 
 import numpy as np
@@ -24,6 +25,10 @@ from . import _native as nat
 
 F = nat.BN_WS_FEATURES
 P = nat.BN_WS_PARAMS
+# network trained on converged sites of 200 plain runs of the space-time Cox process
+# (M = 64 inducing points on an 8 x 8 grid, T = 2000, 256 locations per step, Poisson
+# counts with binsize 2, Matern32 (lengthscale 10) x Matern32 (lengthscale 0.2))
+PRETRAINED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pretrained", "spacetime_cox_warmstart.npz")
 H = 128
 # flat parameter layout of include/bn_hip.h
 OFF = {"W1": (0, (F, H)), "b1": (F * H, (H,)), "W2": (F * H + H, (H, H)), "b2": (F * H + H + H * H, (H,)),
@@ -142,6 +147,25 @@ class WarmStartMLP:
         nat.check(nat.lib().bn_ws_sites(X.shape[0], ctypes.byref(lik), nat.ptr(model._y), nat.ptr(pred),
                                         nat.ptr(model._nat1), nat.ptr(model._nat2), nat.stream_ptr()))
         model._sites_version += 1
+
+    @classmethod
+    def pretrained(cls, path=PRETRAINED, device=None):
+        """The shipped network (or another saved by save()); the file is read with
+        numpy's loader without pickles.  Returns (net, record): record holds the
+        training figures stored beside the weights."""
+        with np.load(path, allow_pickle=False) as z:
+            params = np.asarray(z["params"], np.float32)
+            norm = np.asarray(z["norm"], np.float32)
+            record = {k: z[k] for k in z.files if k not in ("params", "norm")}
+        if params.shape != (P,) or norm.shape != (2 * F + 4,):
+            raise ValueError(f"{path}: parameter vector {params.shape} / normalisation {norm.shape} do not match "
+                             f"the network layout ({P},) / ({2 * F + 4},)")
+        net = cls(device=device)
+        net.load_state_dict({"params": params, "norm": norm})
+        return net, record
+
+    def save(self, path, **record):
+        np.savez(path, **self.state_dict(), **record)
 
     def state_dict(self):
         return {"params": self.params.cpu().numpy(), "norm": self.norm.cpu().numpy()}

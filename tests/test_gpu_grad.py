@@ -127,15 +127,29 @@ def test_autograd_function_and_adam_steps():
     opt = torch.optim.Adam([raw], lr=0.1)
     energies = []
     for _ in range(15):
+        M.MarkovEnergy.sync(raw, g)
+        assert np.array_equal(g.get_raw_hypers(), raw.detach().numpy())
         g.inference(lr=1.0)
         E = M.MarkovEnergy.apply(raw, g)
         opt.zero_grad()
         E.backward()
         opt.step()
-        energies.append(float(E))
+        energies.append(float(E.detach()))
     assert np.all(np.isfinite(energies))
     assert energies[-1] < energies[0]
-    assert np.allclose(g.get_raw_hypers(), raw.detach().numpy()) or True
+    # after a step the optimiser holds new hypers and the model the ones the step used
+    assert not np.allclose(g.get_raw_hypers(), raw.detach().numpy())
+    M.MarkovEnergy.sync(raw, g)
+    np.testing.assert_array_equal(g.get_raw_hypers(), raw.detach().numpy())
+    # the last energy is the model's energy at the hypers it was evaluated at, after
+    # inference at those same hypers: a fresh model at the same sites agrees
+    g.inference(lr=1.0)
+    E_model = float(g.energy())
+    g2 = M.MarkovVariationalGP(K.Matern52(variance=0.5, lengthscale=3.0), L.Gaussian(variance=0.2), x, y)
+    g2.set_raw_hypers(raw.detach().numpy())
+    for _ in range(2):
+        g2.inference(lr=1.0)
+    assert abs(float(g2.energy()) - E_model) <= 1e-6 * abs(E_model)
 
 
 def test_batched_gradients_match_single_series():
