@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libbn_hip.so")
 BN_MATERN12, BN_MATERN32, BN_MATERN52 = 1, 2, 3
 BN_RETURN_PREDICT = 0x1
 BN_RETURN_FULL = 0x2
+BN_NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
 
 vp = ctypes.c_void_p
 i32 = ctypes.c_int32
@@ -62,6 +63,7 @@ SIGNATURES = {
     "bn_plan_gather_u8": (i32, [vp, vp, vp, u8, vp]),
     "bn_discretise": (i32, [vp, vp, vp, i64, vp, vp, vp, vp]),
     "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_filter_smoother_noalloc": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
     "bn_filter_loglik": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "bn_grad_workspace_bytes": (i64, [vp, vp]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1042,8 +1043,10 @@ hipEvent_t pool_event() {
 }
 }  // namespace
 
+thread_local bool g_timing_suppressed = false;  // set while a no-allocation entry point runs
+
 TimedLaunch::TimedLaunch(int32_t k, hipStream_t s) : kid(k), st(s) {
-  if (g_timing && (a = pool_event()) != nullptr) (void)hipEventRecord(a, st);
+  if (g_timing && !g_timing_suppressed && (a = pool_event()) != nullptr) (void)hipEventRecord(a, st);
 }
 
 TimedLaunch::~TimedLaunch() {
@@ -1054,6 +1057,41 @@ TimedLaunch::~TimedLaunch() {
   g_recs.push_back(TimingRec{kid, a, b});
 }
 }  // namespace bn
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation fused pass (bn_filter_smoother_noalloc): the
+// workspace that bn_filter_smoother takes from its caller lives here instead, in
+// a file-scope device array of BN_NOALLOC_SCRATCH_BYTES that every GPU the
+// library runs on holds until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: kernels get the address that
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[BN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (NULL and an error message on failure).
+void* scratch_ptr() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { fail(BN_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)); return nullptr; }
+  if (dev < 0 || dev >= kMaxDevices) { fail(BN_EUNSUPPORTED, "device ordinal beyond 64"); return nullptr; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      fail(BN_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1256,6 +1294,30 @@ int bn_filter_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const do
   HyperView hv{lengthscale, variance, hyper_stride};
   return BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, workspace,
                      (hipStream_t)stream);
+}
+
+int bn_filter_smoother_noalloc(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
+                               const double* variance, int32_t hyper_stride, const float* dt, const float* y,
+                               const float* r, const uint8_t* mask, float* post_mean, float* post_var,
+                               double* loglik, void* stream) {
+  if (check_plan(plan)) return BN_EINVAL;
+  if (int rc = check_desc(kern)) return rc;
+  if (!dt || !y || !r || !post_mean || !post_var || !loglik || !lengthscale || !variance)
+    return fail(BN_EINVAL, "NULL array argument to bn_filter_smoother_noalloc");
+  if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
+  const int64_t need = ws_layout(plan->host, state_dim(kern)).total;
+  if (need > (int64_t)BN_NOALLOC_SCRATCH_BYTES)
+    return fail(BN_EINVAL, "bn_filter_smoother_noalloc: this plan needs " + std::to_string(need) +
+                               " bytes of scratch, more than the " + std::to_string((int64_t)BN_NOALLOC_SCRATCH_BYTES) +
+                               " held by the library; use bn_filter_smoother with a workspace");
+  void* ws = scratch_ptr();
+  if (ws == nullptr) return BN_EHIP;
+  HyperView hv{lengthscale, variance, hyper_stride};
+  g_timing_suppressed = true;
+  const int rc = BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, ws,
+                             (hipStream_t)stream);
+  g_timing_suppressed = false;
+  return rc;
 }
 
 int bn_kalman_filter(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,

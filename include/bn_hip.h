@@ -135,6 +135,23 @@ int bn_filter_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const do
                        const uint8_t *mask, float *post_mean, float *post_var, double *loglik, void *workspace,
                        void *stream);
 
+/* --- bn_filter_smoother without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as bn_filter_smoother minus `workspace`:
+ * the scratch lives in a device array of BN_NOALLOC_SCRATCH_BYTES inside the
+ * library (one per GPU, held until the process ends; its address is looked up on
+ * the first call on each device).  A plan whose bn_workspace_bytes() exceeds
+ * that size is rejected with BN_EINVAL.  The call allocates nothing (the plan is
+ * created beforehand with bn_plan_create) and records no timing events.  Calls
+ * on one GPU share the array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be serialised by the caller. */
+#define BN_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int bn_filter_smoother_noalloc(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
+                               const double *variance, int32_t hyper_stride, const float *dt, const float *y,
+                               const float *r, const uint8_t *mask, float *post_mean, float *post_var,
+                               double *loglik, void *stream);
+
 /* --- ops.kalman_filter, ops.py:256-285.  Writes the filtered (or, with
  * BN_RETURN_PREDICT, predicted) means [d tiled planes] and covariances [d*d
  * tiled planes, row-major], FP64, and loglik[n_series]. */
