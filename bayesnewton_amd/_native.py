@@ -13,6 +13,8 @@ BN_MATERN12, BN_MATERN32, BN_MATERN52 = 1, 2, 3
 BN_RETURN_PREDICT = 0x1
 BN_RETURN_FULL = 0x2
 BN_NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
+BN_STATUS_INNOVATION_NONPD, BN_STATUS_NONFINITE, BN_STATUS_SMOOTHER_NONPD = 1, 2, 4
+BN_SPARSE_UPDATE, BN_SPARSE_ENERGY = 0, 1
 
 vp = ctypes.c_void_p
 i32 = ctypes.c_int32
@@ -62,9 +64,9 @@ SIGNATURES = {
     "bn_plan_scatter_f64": (i32, [vp, vp, vp, i32, vp]),
     "bn_plan_gather_u8": (i32, [vp, vp, vp, u8, vp]),
     "bn_discretise": (i32, [vp, vp, vp, i64, vp, vp, vp, vp]),
-    "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "bn_filter_smoother_noalloc": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
+    "bn_filter_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_filter_smoother_noalloc": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_kalman_filter": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp]),
     "bn_filter_loglik": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
     "bn_grad_workspace_bytes": (i64, [vp, vp]),
     "bn_filter_loglik_grad": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -75,7 +77,7 @@ SIGNATURES = {
                               vp, vp]),
     "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_site_reparametrise": (i32, [vp, vp, vp, vp, vp, vp]),
-    "bn_lik_param_grad": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp]),
+    "bn_lik_param_grad": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_gauss_hermite": (i32, [i32, vp, vp]),
     "bn_st_conditional": (i32, [vp, vp, i32, f64, vp, vp, vp, vp, vp, vp]),
     "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -88,6 +90,8 @@ SIGNATURES = {
     "bn_st_sum": (i32, [i32, i32, vp, vp, vp]),
     "bn_st_marginals": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_pairs_filter": (i32, [vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_sparse_sites": (i32, [vp, vp, vp, vp, i32, i32, i32, f64, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                              vp, vp, vp, vp, vp]),
     "bn_mf_workspace_bytes": (i64, [vp]),
     "bn_mf_pseudo": (i32, [vp, vp, vp, vp, vp, vp]),
     "bn_mf_filter": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -1,4 +1,4 @@
-"""Chunk-length autotuner for the fused filter + smoother pass over a batch of series.
+"""Schedule autotuner for the fused filter + smoother pass over a batch of series.
 
 The pass (update_posterior, bayesnewton/basemodels.py:689-706, as one launch sequence
 per call) cuts every series into chunks of `chunk_len` steps, one lane per chunk, and
@@ -6,14 +6,25 @@ scans the chunks' elements.  Short chunks give more lanes but more scan work and
 chunk-boundary state; long chunks the reverse, and where the balance lies depends on
 the batch (series count and lengths) and on the kernel's state dimension.  The tuner
 times a few candidate lengths on the caller's own inputs with device events and keeps
-the fastest; results are identical across chunk lengths up to rounding.
+the fastest; results are identical across chunk lengths up to rounding.  The
+sequential schedule (one lane per whole series, no scans) competes where it can win
+(tune_schedule), and tuned choices are kept in a table shipped with the package
+(tuned/schedules.json) so a known batch shape needs no timing.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import torch
 
 from . import _native as nat
 from . import ops
 from .plan import SeriesPlan
+
+# Tuned choices committed with the package, keyed by batch_signature(); written by
+# record_schedule() from a tuning run on the device they name.
+SCHEDULE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "schedules.json")
 
 CANDIDATES = (32, 64, 128, 256)
 # Waves per SIMD that the chunk-walking kernels (k_filter_reduce, k_filter_pass) reach
@@ -93,3 +104,69 @@ def tune_chunk_len(offsets, kernel_desc, lengthscale, variance, dt, y, r, hyper_
         raise nat.NativeError("no candidate chunk length fits this batch")
     best = min(times, key=times.get)
     return best, times
+
+
+def batch_signature(lengths, kernel_desc, device=None):
+    """Key of a batch for the schedule table: device architecture and compute-unit
+    count, the kernel's component kinds, the series count, the total and longest
+    length and a hash of the log2 length histogram (batches that share it cost the same
+    to within the tuner's resolution)."""
+    n = np.asarray(lengths, dtype=np.int64)
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    props = torch.cuda.get_device_properties(dev)
+    arch = getattr(props, "gcnArchName", props.name).split(":")[0]
+    kinds = "-".join(str(kernel_desc.kind[i]) for i in range(kernel_desc.n_comp))
+    hist = np.bincount(np.floor(np.log2(np.maximum(n, 1))).astype(np.int64), minlength=32)
+    h = hashlib.sha1(hist.tobytes()).hexdigest()[:12]
+    return f"{arch}/cu{props.multi_processor_count}/kinds{kinds}/B{n.size}/N{int(n.sum())}/max{int(n.max(initial=0))}/h{h}"
+
+
+def load_table(path=SCHEDULE_TABLE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def record_schedule(signature, choice, path=SCHEDULE_TABLE):
+    """Add (or replace) a tuned choice in the schedule table."""
+    table = load_table(path)
+    table[signature] = choice
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def tune_schedule(offsets, kernel_desc, lengthscale, variance, dt, y, r, hyper_stride=0, device=None, table=True,
+                  reps=10):
+    """Schedule of the fused pass for this batch: 'sequential' (one lane walks a whole
+    series: the sequential filter and smoother, no scans) or 'chunked' (series cut into
+    chunks of chunk_len steps, one lane per chunk, associative scans over the chunks'
+    elements), with the chunk length.  With table=True a choice recorded for this
+    batch's signature (schedule table shipped with the package) is used as is;
+    otherwise the candidates are timed on the caller's inputs (tune_chunk_len) and the
+    sequential schedule beside them when every lane-per-series wavefront can be
+    resident at once and the longest series is at most 8 x the longest chunk length
+    timed.  Returns a dict {"schedule", "chunk_len", "ms_per_pass", "source"}."""
+    offsets = np.asarray(offsets, dtype=np.int64)
+    lengths = np.diff(offsets)
+    sig = batch_signature(lengths, kernel_desc, device)
+    if table:
+        hit = load_table().get(sig)
+        if hit is not None:
+            return dict(hit, source="table", signature=sig)
+    best, times = tune_chunk_len(offsets, kernel_desc, lengthscale, variance, dt, y, r, hyper_stride=hyper_stride,
+                                 reps=reps, device=device)
+    out = {"schedule": "chunked", "chunk_len": int(best), "ms_per_pass": {str(k): v for k, v in times.items()}}
+    dev = device if device is not None else dt.device
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    nmax = int(lengths.max(initial=0))
+    if lengths.size <= n_cu * 4 * 2 * 64 and nmax <= 8 * max(times):
+        seq, seq_t = tune_chunk_len(offsets, kernel_desc, lengthscale, variance, dt, y, r, hyper_stride=hyper_stride,
+                                    candidates=(max(nmax, 1),), reps=max(1, reps // 5), device=device)
+        out["ms_per_pass"]["sequential"] = seq_t[seq]
+        if seq_t[seq] < times[best]:
+            out.update(schedule="sequential", chunk_len=int(seq))
+    out["source"] = "timed"
+    out["signature"] = sig
+    return out

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libbn_hip.so")
-SOURCES = ["bn_markov.hip", "bn_sites.hip", "bn_spacetime.hip", "bn_warmstart.hip"]
+SOURCES = ["bn_markov.hip", "bn_sites.hip", "bn_spacetime.hip", "bn_warmstart.hip", "bn_sparse.hip"]
 ARCH = os.environ.get("BN_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions stay disabled (-packed-fp32-ops): this code is

@@ -1,7 +1,7 @@
 // Per-lane bodies of the chunked filter / smoother kernels.
 //
 // Decomposition (one series = one reference model, bayesnewton/basemodels.py:625-706):
-//   series -> chunks of <= chunk_len consecutive steps -> tiles of <= 256 chunks.
+//   series -> chunks of <= chunk_len consecutive steps -> tiles of <= 128 chunks.
 // One lane walks one chunk sequentially; chunk summaries are combined with the
 // associative operators of ops.py:203-219 (filter) and 328-335 (smoother) by a
 // two-level scan (within a tile, then across the tiles of a series).
@@ -22,6 +22,11 @@ namespace bn {
 
 enum : uint8_t { CH_FIRST = 1, CH_LAST = 2 };
 enum : uint8_t { TL_FIRST = 1, TL_LAST = 2 };
+// Per-chunk (then per-series, OR-reduced) numerical status bits.
+enum : int32_t { ST_INNOV_NONPD = 1, ST_NONFINITE = 2, ST_SMOOTHER_NONPD = 4 };
+
+// false for NaN and +-Inf (no reliance on the math library's classification)
+BN_HD bool finite_d(double x) { return x - x == 0.0; }
 
 struct PlanView {
   int32_t n_series, n_chunks, n_chunks_pad, n_tiles, chunk_len;
@@ -460,6 +465,10 @@ struct FilterPassOut {
   // reads per step; when set, `state` is written only at each chunk's first step (the
   // smoother's anchor).  nullptr: `state` at every step.
   hrow_t* hrow = nullptr;
+  // per-chunk status bits (ST_*), or nullptr: ST_INNOV_NONPD when some step's
+  // innovation variance H P- H' + r is not > 0 (or NaN), ST_NONFINITE when the
+  // chunk's log-likelihood or final filtered mean is NaN / infinite
+  int32_t* status = nullptr;
 };
 
 // filter_pass: the filter over one chunk from its true incoming state
@@ -496,6 +505,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
   const bool want_s = out.sagg.base != nullptr;
   T mu[D], V[NS], C[D][D];  // fixed-point smoother of x_s
   double ell = 0.0;
+  int32_t st = 0;
   auto pk = [](int i, int c) { return i <= c ? S::sidx(i, c) : S::sidx(c, i); };
   for (int32_t k = 0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
@@ -553,6 +563,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     double Sd = (double)r[s];
     T hm = T(0);
     BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
+    if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
     if (out.hrow != nullptr) {
       BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
       out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
@@ -634,6 +645,12 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     soa_store(acc, out.sagg, j);
   }
   out.ell[j] = ell;
+  if (out.status != nullptr) {
+    bool fin = finite_d(ell);
+    BN_UNROLL for (int i = 0; i < D; ++i) fin = fin && finite_d((double)m[i]);
+    if (!fin) st |= ST_NONFINITE;
+    out.status[j] = st;
+  }
 }
 
 // smoother_reduce (standalone smoother API): the chunk's smoothing element
@@ -791,7 +808,7 @@ template <int K1, int K2, class T>
 BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                              const float* __restrict__ y, const float* __restrict__ r, CSoA sagg, CSoA scarry,
                              const CStateIO& fin, const hrow_t* __restrict__ hrow, float* __restrict__ out_mean,
-                             float* __restrict__ out_var, int32_t j) {
+                             float* __restrict__ out_var, int32_t* __restrict__ status, int32_t j) {
   using S = Spec<K1, K2>;
   constexpr int D = S::D;
   constexpr int NS = S::NS;
@@ -857,6 +874,7 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     }
     back((double)dt[sn], lt, Lt);
   }
+  bool smooth_ok = true;
   float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;  // next (earlier) step's inputs, loaded one step ahead
   if (L > 0) {
     const int64_t s0 = pl.slot(j, L - 1);
@@ -911,8 +929,11 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     }
     out_mean[s] = (float)om;
     out_var[s] = (float)ov;
+    smooth_ok = smooth_ok && (double)ov > 0.0 && finite_d((double)om);
     if (k > 0) back((double)dtk, lt, Lt);
   }
+  // the filter pass wrote this chunk's entry before this kernel ran
+  if (status != nullptr && !smooth_ok) status[j] |= ST_SMOOTHER_NONPD;
 }
 
 }  // namespace bn

@@ -236,10 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView 
                                                          const float* __restrict__ y, const float* __restrict__ r,
                                                          CSoA sagg, CSoA scarry, CStateIO fin,
                                                          const hrow_t* __restrict__ hrow, float* pmean,
-                                                         float* pvar) {
+                                                         float* pvar, int32_t* chst) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
-  lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, j);
+  lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, chst, j);
 }
 
 template <int K1, int K2>
@@ -530,21 +530,35 @@ __global__ __launch_bounds__(kBlock) void k_predict(PlanView pl, HyperView hv, P
 }
 
 // Deterministic per-series sum of a per-chunk FP64 array (fixed tree order).
+// With chst / status, also the OR of the series' per-chunk status bits.
 __global__ __launch_bounds__(kBlock) void k_series_sum(PlanView pl, const double* __restrict__ per_chunk,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out, const int32_t* __restrict__ chst,
+                                                       int32_t* __restrict__ status) {
   __shared__ double red[kBlock];
+  __shared__ int32_t ored[kBlock];
   const int32_t b = blockIdx.x;
   const int32_t c0 = pl.series_first_chunk[b];
   const int32_t nc = pl.series_nchunks[b];
   double s = 0.0;
-  for (int32_t i = threadIdx.x; i < nc; i += kBlock) s += per_chunk[c0 + i];
+  int32_t f = 0;
+  for (int32_t i = threadIdx.x; i < nc; i += kBlock) {
+    s += per_chunk[c0 + i];
+    if (chst != nullptr) f |= chst[c0 + i];
+  }
   red[threadIdx.x] = s;
+  ored[threadIdx.x] = f;
   __syncthreads();
   for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int32_t)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int32_t)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      ored[threadIdx.x] |= ored[threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[b] = red[0];
+  if (threadIdx.x == 0) {
+    out[b] = red[0];
+    if (status != nullptr) status[b] = ored[0];
+  }
 }
 
 __device__ __forceinline__ bool slot_to_step(const PlanView& pl, int64_t slot, int64_t& csr) {
@@ -635,7 +649,7 @@ int check_desc(const bn_kernel_desc* k) {
 }
 
 struct WsLayout {
-  int64_t fagg, ftagg, ftcarry, sagg, stagg, stcarry, fbuf, ell, sp1, sp2, total;
+  int64_t fagg, ftagg, ftcarry, sagg, stagg, stcarry, fbuf, ell, sp1, sp2, chst, total;
 };
 
 int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -658,6 +672,7 @@ WsLayout ws_layout(const PlanHost& p, int D) {
   w.ell = o;     o = align256(o + p.n_chunks_pad * 8);
   w.sp1 = o;     o = align256(o + p.n_chunks_pad * 8);
   w.sp2 = o;     o = align256(o + p.n_chunks_pad * 8);
+  w.chst = o;    o = align256(o + p.n_chunks_pad * 4);
   w.total = o;
   return w;
 }
@@ -666,6 +681,7 @@ struct Bufs {
   SoA fagg, ftagg, ftcarry, sagg, stagg, stcarry;
   state_t* fbuf;
   double* ell;
+  int32_t* chst;
 };
 
 Bufs ws_bufs(const PlanHost& p, int D, void* ws) {
@@ -681,6 +697,7 @@ Bufs ws_bufs(const PlanHost& p, int D, void* ws) {
   b.stcarry = SoA{(agg_t*)(base + w.stcarry), nt};
   b.fbuf = (state_t*)(base + w.fbuf);
   b.ell = (double*)(base + w.ell);
+  b.chst = (int32_t*)(base + w.chst);
   return b;
 }
 
@@ -720,7 +737,8 @@ void launch_smoother_scan(const bn_plan* plan, const Bufs& b, hipStream_t st) {
 
 template <int K1, int K2>
 int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
-                        const uint8_t* mask, float* pmean, float* pvar, double* loglik, void* ws, hipStream_t st) {
+                        const uint8_t* mask, float* pmean, float* pvar, double* loglik, int32_t* status, void* ws,
+                        hipStream_t st) {
   constexpr int D = K1 + K2;
   const PlanHost& p = plan->host;
   Bufs b = ws_bufs(p, D, ws);
@@ -733,6 +751,7 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   fo.hrow = (hrow_t*)(b.fbuf + (int64_t)(D + D * (D + 1) / 2) * p.n_slots);
   fo.sagg = b.sagg;
   fo.ell = b.ell;
+  fo.status = status != nullptr ? b.chst : nullptr;
   {
     BN_TIMED(KID_FILTER_PASS, st);
     hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
@@ -743,11 +762,12 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
     BN_TIMED(KID_SMOOTHER_MBF, st);
     hipLaunchKernelGGL((k_smoother_mbf<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
                        cs(b.sagg), cs(b.stcarry), CStateIO{b.fbuf, b.fbuf + (int64_t)D * p.n_slots, true}, fo.hrow, pmean,
-                       pvar);
+                       pvar, fo.status);
   }
   {
     BN_TIMED(KID_SERIES_SUM, st);
-    hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+    hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik,
+                       (const int32_t*)fo.status, status);
   }
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
@@ -755,8 +775,8 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
 
 template <int K1, int K2>
 int run_kalman_filter(const bn_plan* plan, HyperView hv, const float* dt, const float* y, const float* r,
-                      const uint8_t* mask, double* means, double* covs, double* loglik, uint32_t flags, void* ws,
-                      hipStream_t st) {
+                      const uint8_t* mask, double* means, double* covs, double* loglik, uint32_t flags,
+                      int32_t* status, void* ws, hipStream_t st) {
   constexpr int D = K1 + K2;
   const PlanHost& p = plan->host;
   Bufs b = ws_bufs(p, D, ws);
@@ -766,9 +786,11 @@ int run_kalman_filter(const bn_plan* plan, HyperView hv, const float* dt, const 
   fo.predict = (flags & BN_RETURN_PREDICT) != 0;
   fo.sagg = SoA{nullptr, 0};
   fo.ell = b.ell;
+  fo.status = status != nullptr ? b.chst : nullptr;
   hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
                      cs(b.fagg), cs(b.ftcarry), fo);
-  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik,
+                     (const int32_t*)fo.status, status);
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -792,7 +814,8 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
                      (double*)gws, (int64_t)p.n_chunks_pad);
   hipLaunchKernelGGL((k_grad_total<K1, K2>), dim3((unsigned)p.n_series), dim3(NT), 0, st, plan->view,
                      (const double*)gws, (int64_t)p.n_chunks_pad, grad);
-  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik);
+  hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik, nullptr,
+                       (int32_t*)nullptr);
   BN_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -1058,6 +1081,10 @@ TimedLaunch::~TimedLaunch() {
 }
 }  // namespace bn
 
+static_assert(ST_INNOV_NONPD == BN_STATUS_INNOVATION_NONPD && ST_NONFINITE == BN_STATUS_NONFINITE &&
+                  ST_SMOOTHER_NONPD == BN_STATUS_SMOOTHER_NONPD,
+              "lane status bits match the C ABI's");
+
 // ---------------------------------------------------------------------------
 // Scratch of the no-allocation fused pass (bn_filter_smoother_noalloc): the
 // workspace that bn_filter_smoother takes from its caller lives here instead, in
@@ -1285,21 +1312,21 @@ int bn_discretise(const bn_kernel_desc* kern, const double* lengthscale, const d
 int bn_filter_smoother(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
                            const double* variance, int32_t hyper_stride, const float* dt, const float* y,
                            const float* r, const uint8_t* mask, float* post_mean, float* post_var, double* loglik,
-                           void* workspace, void* stream) {
+                           int32_t* status, void* workspace, void* stream) {
   if (check_plan(plan)) return BN_EINVAL;
   if (int rc = check_desc(kern)) return rc;
   if (!dt || !y || !r || !post_mean || !post_var || !loglik || !workspace || !lengthscale || !variance)
     return fail(BN_EINVAL, "NULL array argument to bn_filter_smoother");
   if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
   HyperView hv{lengthscale, variance, hyper_stride};
-  return BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, workspace,
-                     (hipStream_t)stream);
+  return BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, status,
+                     workspace, (hipStream_t)stream);
 }
 
 int bn_filter_smoother_noalloc(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
                                const double* variance, int32_t hyper_stride, const float* dt, const float* y,
                                const float* r, const uint8_t* mask, float* post_mean, float* post_var,
-                               double* loglik, void* stream) {
+                               double* loglik, int32_t* status, void* stream) {
   if (check_plan(plan)) return BN_EINVAL;
   if (int rc = check_desc(kern)) return rc;
   if (!dt || !y || !r || !post_mean || !post_var || !loglik || !lengthscale || !variance)
@@ -1314,8 +1341,8 @@ int bn_filter_smoother_noalloc(const bn_plan* plan, const bn_kernel_desc* kern, 
   if (ws == nullptr) return BN_EHIP;
   HyperView hv{lengthscale, variance, hyper_stride};
   g_timing_suppressed = true;
-  const int rc = BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik, ws,
-                             (hipStream_t)stream);
+  const int rc = BN_DISPATCH(kern, run_filter_smoother, plan, hv, dt, y, r, mask, post_mean, post_var, loglik,
+                             status, ws, (hipStream_t)stream);
   g_timing_suppressed = false;
   return rc;
 }
@@ -1323,15 +1350,15 @@ int bn_filter_smoother_noalloc(const bn_plan* plan, const bn_kernel_desc* kern, 
 int bn_kalman_filter(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
                          const double* variance, int32_t hyper_stride, const float* dt, const float* y,
                          const float* r, const uint8_t* mask, double* means, double* covs, double* loglik,
-                         uint32_t flags, void* workspace, void* stream) {
+                         uint32_t flags, int32_t* status, void* workspace, void* stream) {
   if (check_plan(plan)) return BN_EINVAL;
   if (int rc = check_desc(kern)) return rc;
   if (!dt || !y || !r || !means || !covs || !loglik || !workspace || !lengthscale || !variance)
     return fail(BN_EINVAL, "NULL array argument to bn_kalman_filter");
   if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
   HyperView hv{lengthscale, variance, hyper_stride};
-  return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, means, covs, loglik, flags, workspace,
-                     (hipStream_t)stream);
+  return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, means, covs, loglik, flags, status,
+                     workspace, (hipStream_t)stream);
 }
 
 int bn_filter_loglik(const bn_plan* plan, const bn_kernel_desc* kern, const double* lengthscale,
@@ -1344,7 +1371,7 @@ int bn_filter_loglik(const bn_plan* plan, const bn_kernel_desc* kern, const doub
   if (hyper_stride != 0 && hyper_stride != kern->n_comp) return fail(BN_EINVAL, "hyper_stride must be 0 or n_comp");
   HyperView hv{lengthscale, variance, hyper_stride};
   return BN_DISPATCH(kern, run_kalman_filter, plan, hv, dt, y, r, mask, (double*)nullptr, (double*)nullptr, loglik,
-                     0u, workspace, (hipStream_t)stream);
+                     0u, (int32_t*)nullptr, workspace, (hipStream_t)stream);
 }
 
 int64_t bn_grad_workspace_bytes(const bn_plan* plan, const bn_kernel_desc* kern) {

@@ -174,13 +174,43 @@ BN_HD double dlog_lik_dparam(const LikParams& L, double y, double f) {
   return 0.0;
 }
 
+// d/d(likelihood hyperparameter) of the moment-matched log Z = log E_cav[p(y | f)^power]
+// (moment_match above) at the cavity (m, v): Gaussian closed form, else by the same
+// cubature, d log Z = sum w_i p_i^a a d log p_i / Z (0 where Z is clipped at 1e-8).
+BN_HD double dmoment_match_dparam(const LikParams& L, const GHTable& gh, double y, double m, double v, double power) {
+  if (L.kind == LIK_GAUSSIAN) {
+    const double var = L.p0 / power + v;
+    const double r = y - m;
+    return -0.5 * (1.0 / var - r * r / (var * var)) / power + 0.5 * (1.0 - power) / L.p0;
+  }
+  const double sv = sqrt(v);
+  double Z = 0.0, dZ = 0.0;
+  for (int i = 0; i < gh.n; ++i) {
+    const double f = m + sv * gh.x[i];
+    const double wl = gh.w[i] * exp(power * log_lik(L, y, f));
+    Z += wl;
+    dZ += wl * power * dlog_lik_dparam(L, y, f);
+  }
+  return Z > 1e-8 ? dZ / Z : 0.0;
+}
+
 // d term1 / d(likelihood hyperparameter) of one site (see energy_terms), with the
-// posterior held fixed: VI E_q[d log p / d phi] (closed form for the Gaussian),
-// Newton d log p(y | m) / d phi.  Masked sites contribute 0.
+// posterior and the sites held fixed: VI E_q[d log p / d phi] (closed form for the
+// Gaussian), Newton d log p(y | m) / d phi, EP (1 / power) d log Z / d phi at the
+// site's cavity (the lel term of inference.py:286-325; the pseudo terms carry no
+// likelihood hyperparameter).  Masked sites contribute 0.
 template <int METHOD>
-BN_HD double dterm1_dparam(const LikParams& L, const GHTable& gh, double y, double pm, double pv) {
+BN_HD double dterm1_dparam(const LikParams& L, const GHTable& gh, double y, double pm, double pv, double n1,
+                           double n2, double power) {
   if (y != y) return 0.0;
   if (METHOD == M_NEWTON) return dlog_lik_dparam(L, y, pm);
+  if (METHOD == M_EP) {
+    // cavity of utils.py:534-541, as site_step forms it
+    const double pn2 = 1.0 / (pv + 1e-8);
+    const double cv = 1.0 / (pn2 - power * n2);
+    const double cm = cv * (pn2 * pm - power * n1);
+    return dmoment_match_dparam(L, gh, y, cm, cv, power) / power;
+  }
   if (L.kind == LIK_GAUSSIAN) return -0.5 / L.p0 + 0.5 * ((y - pm) * (y - pm) + pv) / (L.p0 * L.p0);
   const double sv = sqrt(pv);
   double e = 0.0;

@@ -93,14 +93,15 @@ __global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams 
 }
 
 template <int METHOD>
-__global__ __launch_bounds__(kBlock) void k_lik_grad(PlanView pl, LikParams L, GHTable gh, SiteIO io) {
+__global__ __launch_bounds__(kBlock) void k_lik_grad(PlanView pl, LikParams L, GHTable gh, SiteIO io, double power) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
   const int32_t len = pl.chunk_len_[j];
   double s1 = 0.0;
   for (int32_t k = 0; k < len; ++k) {
     const int64_t s = pl.slot(j, k);
-    s1 += dterm1_dparam<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s]);
+    const double n1 = METHOD == M_EP ? io.nat1[s] : 0.0, n2 = METHOD == M_EP ? io.nat2[s] : 0.0;
+    s1 += dterm1_dparam<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], n1, n2, power);
   }
   io.part1[j] = s1;
 }
@@ -416,27 +417,30 @@ int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_li
 }
 
 int bn_lik_param_grad(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
-                      int32_t gh_points, const float* y, const float* post_mean, const float* post_var, double* grad,
-                      void* workspace, void* stream) {
+                      int32_t gh_points, double power, const float* y, const float* post_mean, const float* post_var,
+                      const double* nat1, const double* nat2, double* grad, void* workspace, void* stream) {
   if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
   if (!y || !post_mean || !post_var || !grad || !workspace) return fail(BN_EINVAL, "NULL array argument");
-  if (method == BN_EP) return fail(BN_EUNSUPPORTED, "likelihood-hyperparameter gradients are implemented for VI and Newton");
+  if (method == BN_EP && (!nat1 || !nat2)) return fail(BN_EINVAL, "EP needs the sites nat1 / nat2");
+  if (method == BN_EP && !(power > 0.0 && power <= 1.0)) return fail(BN_EINVAL, "EP power must be in (0, 1]");
   LikParams L;
   if (int rc = make_lik_checked(lik, L)) return rc;
   GHTable gh;
   if (int rc = make_gh(gh_points, gh)) return rc;
   double* p1 = site_partials(plan, kern, workspace, 0);
-  double* p2 = site_partials(plan, kern, workspace, 1);
-  if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
-  SiteIO io{y, post_mean, post_var, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, p1, p2};
+  if (!p1) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
+  SiteIO io{y, post_mean, post_var, const_cast<double*>(nat1), const_cast<double*>(nat2), nullptr, nullptr,
+            nullptr, nullptr, nullptr, p1, nullptr};
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
   if (method == BN_VI)
-    hipLaunchKernelGGL(k_lik_grad<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io);
+    hipLaunchKernelGGL(k_lik_grad<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
   else if (method == BN_NEWTON)
-    hipLaunchKernelGGL(k_lik_grad<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io);
+    hipLaunchKernelGGL(k_lik_grad<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+  else if (method == BN_EP)
+    hipLaunchKernelGGL(k_lik_grad<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
   else
-    return fail(BN_EINVAL, "method must be BN_VI or BN_NEWTON");
+    return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
   hipLaunchKernelGGL(k_series_sum1, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, grad);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;

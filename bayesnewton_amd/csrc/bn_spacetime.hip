@@ -817,20 +817,28 @@ __global__ __launch_bounds__(kT) void k_st_filter(StDims dm, StHyp hy, StFilterI
 }
 
 // ---------------------------------------------------------------------------
-// k_st_smoother: RTS backward pass (ops.py:288-311), one workgroup per run.
-//   P- = A P A' + Q (A, Q of the next step's dt, basemodels.py:700), L L' = P-,
-//   G' = L^-T L^-1 (A P), m_s = m + G (m_s' - A m), P_s = P + G (P_s' - P-) G'.
-// Outputs H m_s and H P_s H' (the first M states).
+// RTS backward pass (ops.py:288-311) in two kernels.  The gain of step t,
+//   G_t' = (P-_{t+1})^-1 A P_t,  P-_{t+1} = A P_t A' + Q  (A, Q of dt[t+1], basemodels.py:700),
+// depends on the filtered covariances only, so k_st_gains computes the gains of a block
+// of steps for every run at once, one workgroup per (step, run) (sweep inverse of P- in
+// LDS, one D^3 product); k_st_smoother then walks the block backwards, one workgroup per
+// run, with the two D^3 products left on the recursion:
+//   m_s = m + G (m_s' - A m),  P_s = P + G (P_s' - P-) G'.
+// Outputs H m_s and H P_s H' (the first M states), and with return_full the state, its
+// covariance and the neighbours' cross covariance G_t P_s,t+1.
+// Blocks of steps run from the end of the series; a block's gains and unpacked filtered
+// covariances sit in a workspace buffer of gain_block_steps() steps per run.
 // ---------------------------------------------------------------------------
 struct StSmootherIO {
   const double* dt;
   const double* fmean;  // [K,T,D]
   const double* fcov;   // [K,T,D(D+1)/2]
-  double* Ps;           // per-run scratch [K,D,D] each
-  double* F;
-  double* GT;
-  double* Dl;
-  double* T1;
+  double* Ps;           // per-run scratch [K,D,D] each: P_s of the step after the current one
+  double* Dl;           //   P_s' - P-
+  double* GT;           //   G (P_s' - P-)
+  double* ms;           // per-run [K,D]: m_s of the step after the current one
+  double* buf;          // [K, tb, 2, D, D]: unpacked P_t and G_t' of the block's steps
+  int tb;               // steps per block in buf
   double* pm;           // [K,T,M]
   double* pP;           // [K,T,M,M]
   double* smean;        // optional [K,T,D] smoothed state (return_full)
@@ -838,31 +846,93 @@ struct StSmootherIO {
   double* scross;       // optional [K,T,D,D]: G_t P_s,t+1 = Cov(x_t, x_t+1 | y); Pinf at the last step
 };
 
+// bytes of the gain block buffer: 1953 steps of one d = 128 run, 9 steps each of 200 runs
+constexpr int64_t kGainBudget = int64_t(512) << 20;
+
+int gain_block_steps(const StDims& dm) {
+  const int64_t per = (int64_t)dm.K * 2 * dm.D * dm.D * (int64_t)sizeof(double);
+  int64_t tb = kGainBudget / (per > 0 ? per : 1);
+  const int64_t need = dm.T > 1 ? dm.T - 1 : 1;
+  if (tb > need) tb = need;
+  if (tb < 1) tb = 1;
+  return (int)tb;
+}
+
 template <int KT>
-__global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmootherIO io) {
+__global__ __launch_bounds__(kT) void k_st_gains(StDims dm, StHyp hy, StSmootherIO io, int t0) {
+  extern __shared__ double smem[];
+  const int M = dm.M, D = dm.D, tid = threadIdx.x;
+  const int j = blockIdx.x, k = blockIdx.y, t = t0 + j;
+  if (t >= dm.T - 1) return;  // uniform over the workgroup
+  double* pan = smem;                        // 2*kPK*kPS
+  double* sL = pan + 2 * kPK * kPS;          // D*D
+  double* row = sL + kMaxD * kMaxD;          // 2 D (pivot rows)
+  __shared__ double sA[KT * KT], sQ[KT * KT];
+  const int64_t DD = (int64_t)D * D, NSD = (int64_t)D * (D + 1) / 2;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  double* F = io.buf + ((int64_t)k * io.tb + j) * 2 * DD;
+  double* G = F + DD;
+  const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
+  {
+    const double* pk = io.fcov + kt * NSD;
+    for (int i = tid >> 6; i < D; i += kT / 64)
+      for (int c = tid & 63; c < D; c += 64) F[(int64_t)i * D + c] = pk[i <= c ? pidx(i, c, D) : pidx(c, i, D)];
+  }
+  if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt + 1], sA, sQ);
+  __syncthreads();
+  // P- = (A (x) I) F (A (x) I)' + Q (x) I, block (a, c) of M x M position pairs at a time
+  for (int e = tid; e < M * M; e += kT) {
+    const int a = e / M, c = e % M;
+    double f[KT][KT], af[KT][KT];
+    for (int p = 0; p < KT; ++p)
+      for (int q = 0; q < KT; ++q) f[p][q] = F[(int64_t)(p * M + a) * D + q * M + c];
+    for (int i = 0; i < KT; ++i)
+      for (int q = 0; q < KT; ++q) {
+        double v = 0.0;
+        for (int p = 0; p < KT; ++p) v = fma(sA[i * KT + p], f[p][q], v);
+        af[i][q] = v;
+      }
+    for (int i = 0; i < KT; ++i)
+      for (int jj = 0; jj < KT; ++jj) {
+        double v = (a == c) ? sQ[i * KT + jj] : 0.0;
+        for (int q = 0; q < KT; ++q) v = fma(af[i][q], sA[jj * KT + q], v);
+        sL[(int64_t)(i * M + a) * D + jj * M + c] = v;
+      }
+  }
+  (void)wg_sweep_inv<kMaxD / 16, false>(sL, D, D, row);  // sL = -(P-)^-1
+  // X = sL (A (x) I): columns {p M + a} of a row mix by A' (in place, one (row, a) per thread)
+  for (int e = tid; e < D * M; e += kT) {
+    const int r = e / M, a = e % M;
+    double x[KT];
+    for (int p = 0; p < KT; ++p) x[p] = sL[(int64_t)r * D + p * M + a];
+    for (int i = 0; i < KT; ++i) {
+      double v = 0.0;
+      for (int p = 0; p < KT; ++p) v = fma(x[p], sA[p * KT + i], v);
+      sL[(int64_t)r * D + i * M + a] = v;
+    }
+  }
+  // G' = (P-)^-1 (A (x) I) F = -X F
+  wg_gemm(D, D, D, -1.0, MatRef{sL, D, 1}, MatRef{F, D, 1}, 0.0, G, D, pan);
+}
+
+template <int KT>
+__global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmootherIO io, int t1, int t0,
+                                                    int terminal) {
   extern __shared__ double smem[];
   const int M = dm.M, D = dm.D, tid = threadIdx.x;
   const int k = blockIdx.x;
   double* pan = smem;                        // 2*kPK*kPS
-  double* sL = pan + 2 * kPK * kPS;          // D*D
-  double* row = sL + kMaxD * kMaxD;          // 2 D (pivot rows)
-  double* ms = row + 2 * kMaxD;              // D
-  double* mf = ms + kMaxD;                   // D
+  double* mf = pan + 2 * kPK * kPS;          // D
   double* mp = mf + kMaxD;                   // D
   double* dv = mp + kMaxD;                   // D
+  double* ms = dv + kMaxD;                   // D
   __shared__ double sA[KT * KT], sQ[KT * KT];
   const int64_t DD = (int64_t)D * D, NSD = (int64_t)D * (D + 1) / 2;
   double* Ps = io.Ps + k * DD;
-  double* F = io.F + k * DD;
-  double* GT = io.GT + k * DD;
   double* Dl = io.Dl + k * DD;
-  double* T1 = io.T1 + k * DD;
+  double* GT = io.GT + k * DD;
+  double* msg = io.ms + (int64_t)k * D;
   const double ls = hy.ls_t[k * hy.stride], var = hy.var_t[k * hy.stride];
-  auto unpack = [&](const double* pk, double* out) {
-    for (int i = tid >> 6; i < D; i += kT / 64) {  // one wave per row
-      for (int j = tid & 63; j < D; j += 64) out[(int64_t)i * D + j] = pk[i <= j ? pidx(i, j, D) : pidx(j, i, D)];
-    }
-  };
   auto emit = [&](int64_t kt) {
     for (int e = tid; e < M; e += kT) io.pm[kt * M + e] = ms[e];
     for (int e = tid; e < M * M; e += kT) io.pP[kt * M * M + e] = Ps[(int64_t)(e / M) * D + e % M];
@@ -871,9 +941,12 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
       for (int64_t e = tid; e < DD; e += kT) io.scov[kt * DD + e] = Ps[e];
     }
   };
-  {
+  if (terminal) {
+    // the last step: the smoothed state is the filtered one (ops.py:314-315)
     const int64_t kt = (int64_t)k * dm.T + dm.T - 1;
-    unpack(io.fcov + kt * NSD, Ps);
+    const double* pk = io.fcov + kt * NSD;
+    for (int i = tid >> 6; i < D; i += kT / 64)
+      for (int c = tid & 63; c < D; c += 64) Ps[(int64_t)i * D + c] = pk[i <= c ? pidx(i, c, D) : pidx(c, i, D)];
     for (int e = tid; e < D; e += kT) ms[e] = io.fmean[kt * D + e];
     __syncthreads();
     emit(kt);
@@ -885,14 +958,18 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
         io.scross[kt * DD + e] = (i % M == j % M) ? mdl.Pinf[i / M][j / M] : 0.0;
       }
     }
+  } else {
+    for (int e = tid; e < D; e += kT) ms[e] = msg[e];
   }
-  for (int t = dm.T - 2; t >= 0; --t) {
+  __syncthreads();
+  for (int t = t1; t >= t0; --t) {
     const int64_t kt = (int64_t)k * dm.T + t;
-    unpack(io.fcov + kt * NSD, F);
+    const double* F = io.buf + ((int64_t)k * io.tb + (t - t0)) * 2 * DD;
+    const double* G = F + DD;  // G_t' (row-major)
     for (int e = tid; e < D; e += kT) mf[e] = io.fmean[kt * D + e];
     if (tid == 0) temporal_AQ<KT>(ls, var, io.dt[kt + 1], sA, sQ);
     __syncthreads();
-    // AF = (A (x) I) F -> GT ;  P- = AF (A (x) I)' + Q (x) I -> sL ;  Dl = Ps' - P-
+    // Dl = sym(P_s') - P-, P- = (A (x) I) F (A (x) I)' + Q (x) I
     for (int e = tid; e < M * M; e += kT) {
       const int a = e / M, c = e % M;
       double f[KT][KT], af[KT][KT];
@@ -903,15 +980,13 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
           double v = 0.0;
           for (int p = 0; p < KT; ++p) v = fma(sA[i * KT + p], f[p][q], v);
           af[i][q] = v;
-          GT[(int64_t)(i * M + a) * D + q * M + c] = v;
         }
       for (int i = 0; i < KT; ++i)
-        for (int j = 0; j < KT; ++j) {
-          double v = (a == c) ? sQ[i * KT + j] : 0.0;
-          for (int q = 0; q < KT; ++q) v = fma(af[i][q], sA[j * KT + q], v);
-          const int64_t o = (int64_t)(i * M + a) * D + j * M + c;
-          const int64_t ot = (int64_t)(j * M + c) * D + i * M + a;
-          sL[o] = v;
+        for (int jj = 0; jj < KT; ++jj) {
+          double v = (a == c) ? sQ[i * KT + jj] : 0.0;
+          for (int q = 0; q < KT; ++q) v = fma(af[i][q], sA[jj * KT + q], v);
+          const int64_t o = (int64_t)(i * M + a) * D + jj * M + c;
+          const int64_t ot = (int64_t)(jj * M + c) * D + i * M + a;
           Dl[o] = 0.5 * (Ps[o] + Ps[ot]) - v;  // symmetric part of the next step's P_s
         }
     }
@@ -924,18 +999,16 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
         mp[i * M + tid] = v;
       }
     }
-    (void)wg_sweep_inv<kMaxD / 16, false>(sL, D, D, row);  // sL = -(P-)^-1
-    // T1 = G' = (P-)^-1 A P ; GT = G (Ps' - P-) ; Ps = P + GT G'
-    wg_gemm(D, D, D, -1.0, MatRef{sL, D, 1}, MatRef{GT, D, 1}, 0.0, T1, D, pan);
-    if (io.scross) wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Ps, D, 1}, 0.0, io.scross + kt * DD, D, pan);
-    wg_gemm(D, D, D, 1.0, MatRef{T1, 1, D}, MatRef{Dl, D, 1}, 0.0, GT, D, pan);
+    // scross = G P_s' ; GT = G (P_s' - P-) ; P_s = P + GT G'
+    if (io.scross) wg_gemm(D, D, D, 1.0, MatRef{G, 1, D}, MatRef{Ps, D, 1}, 0.0, io.scross + kt * DD, D, pan);
+    wg_gemm(D, D, D, 1.0, MatRef{G, 1, D}, MatRef{Dl, D, 1}, 0.0, GT, D, pan);
     for (int e = tid; e < D; e += kT) dv[e] = ms[e] - mp[e];
-    wg_gemm_128(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{T1, D, 1}, 1.0, F, D, Ps, D, pan);
+    wg_gemm_128(D, D, D, 1.0, MatRef{GT, D, 1}, MatRef{G, D, 1}, 1.0, F, D, Ps, D, pan);
     {  // m_s = m + G dv, the two halves of the workgroup split q; mp is free again
       const int i = tid & (kMaxD - 1), h = tid / kMaxD;
       double v = 0.0;
       if (i < D)
-        for (int q = h; q < D; q += 2) v = fma(T1[(int64_t)q * D + i], dv[q], v);
+        for (int q = h; q < D; q += 2) v = fma(G[(int64_t)q * D + i], dv[q], v);
       if (h == 1 && i < D) mp[i] = v;
       __syncthreads();
       if (h == 0 && i < D) ms[i] = mf[i] + v + mp[i];
@@ -944,6 +1017,7 @@ __global__ __launch_bounds__(kT) void k_st_smoother(StDims dm, StHyp hy, StSmoot
     emit(kt);
     __syncthreads();
   }
+  for (int e = tid; e < D; e += kT) msg[e] = ms[e];
 }
 
 // per-run sums of per-step partials: out[k] = sum_t a[k*T + t]
@@ -1727,6 +1801,8 @@ __global__ __launch_bounds__(kT) void k_st_predict(StDims dm, StHyp hy, StPredIO
 
 constexpr size_t kFilterSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM + kMaxD + kMaxM + 4);
 constexpr size_t kSmootherSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 6 * kMaxD);
+constexpr size_t kGainsSmem = sizeof(double) * (2 * kPK * kPS + kMaxD * kMaxD + 2 * kMaxD);
+constexpr size_t kSmootherSeqSmem = sizeof(double) * (2 * kPK * kPS + 4 * kMaxD);
 constexpr size_t kPseudoSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM + 3 * kMaxM);
 constexpr size_t kTiedSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM);
 constexpr size_t kIhDareSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM);
@@ -1759,10 +1835,10 @@ int set_smem_limits() {
                                    (int)kFilterSmem));
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_filter<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kFilterSmem));
-  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_smoother<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)kSmootherSmem));
-  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_smoother<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)kSmootherSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_gains<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kGainsSmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_gains<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kGainsSmem));
 #define ST_SITES_ATTR(MT, MD)                                                                              \
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_sites<MT, MD>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                    (int)kSitesSmem))
@@ -1849,9 +1925,10 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
 int64_t bn_st_workspace_bytes(const bn_st_dims* dims) {
   StDims dm;
   if (check_dims(dims, dm) != BN_OK) return -1;
-  // per run: filter P (D x D) and V (D x M); smoother Ps, F, GT, Dl, T1 (5 x D x D)
+  // per run: filter P (D x D) and V (D x M); smoother Ps, Dl, GT (3 x D x D) and m_s (D),
+  // then the smoother's block of gains and unpacked covariances (gain_block_steps)
   const int64_t D = dm.D;
-  return (int64_t)dm.K * 5 * D * D * (int64_t)sizeof(double) + 4096;
+  return (int64_t)dm.K * (5 * D * D + 2 * D * D * gain_block_steps(dm)) * (int64_t)sizeof(double) + 4096;
 }
 
 int bn_st_filter(const bn_st_dims* dims, const bn_st_hypers* hyp, const double* dt, const double* Lw,
@@ -1901,15 +1978,34 @@ int bn_st_smoother_full(const bn_st_dims* dims, const bn_st_hypers* hyp, const d
   if (int rc = set_smem_limits()) return rc;
   double* ws = (double*)workspace;
   const int64_t D = dm.D, K = dm.K, DD = D * D;
-  StSmootherIO io{dt, filter_mean, filter_cov, ws, ws + K * DD, ws + 2 * K * DD, ws + 3 * K * DD, ws + 4 * K * DD,
-                  post_mean, post_cov, smooth_mean, smooth_cov, smooth_cross};
+  const int tb = gain_block_steps(dm);
+  StSmootherIO io{dt, filter_mean, filter_cov, ws, ws + K * DD, ws + 2 * K * DD, ws + 3 * K * DD,
+                  ws + 5 * K * DD, tb, post_mean, post_cov, smooth_mean, smooth_cov, smooth_cross};
   hipStream_t st = (hipStream_t)stream;
+  const StHyp hy = make_hyp(hyp);
   BN_TIMED(KID_ST_SMOOTHER, st);
-  if (dm.KT == 1)
-    hipLaunchKernelGGL(k_st_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
-  else
-    hipLaunchKernelGGL(k_st_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSmem, st, dm, make_hyp(hyp), io);
-  ST_HIP_CHECK(hipGetLastError());
+  // blocks of steps from the end: gains of the block in parallel, then the recursion over it
+  int t1 = dm.T - 2;
+  bool terminal = true;
+  do {
+    const int t0 = t1 - tb + 1 > 0 ? t1 - tb + 1 : 0;
+    if (t1 >= 0) {
+      const dim3 g((unsigned)(t1 - t0 + 1), (unsigned)dm.K);
+      if (dm.KT == 1)
+        hipLaunchKernelGGL(k_st_gains<1>, g, dim3(kT), kGainsSmem, st, dm, hy, io, t0);
+      else
+        hipLaunchKernelGGL(k_st_gains<2>, g, dim3(kT), kGainsSmem, st, dm, hy, io, t0);
+    }
+    if (dm.KT == 1)
+      hipLaunchKernelGGL(k_st_smoother<1>, dim3(dm.K), dim3(kT), kSmootherSeqSmem, st, dm, hy, io, t1, t0,
+                         terminal ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_st_smoother<2>, dim3(dm.K), dim3(kT), kSmootherSeqSmem, st, dm, hy, io, t1, t0,
+                         terminal ? 1 : 0);
+    ST_HIP_CHECK(hipGetLastError());
+    terminal = false;
+    t1 = t0 - 1;
+  } while (t1 >= 0);
   return BN_OK;
 }
 

@@ -149,10 +149,11 @@ class MarkovGaussianProcess:
             else:
                 hl, hv = kernel.hypers(dev)
             f32 = dict(dtype=torch.float32, device=dev)
-            chunk_len, tune_ms = autotune.tune_chunk_len(
+            choice = autotune.tune_schedule(
                 off, self._desc, hl, hv, torch.as_tensor(self.dt, **f32), torch.zeros(self.num_data, **f32),
                 torch.full((self.num_data,), 1e2, **f32), hyper_stride=self._hyper_stride, device=dev)
-            self.autotune = {"chosen": chunk_len, "ms_per_pass": tune_ms}
+            chunk_len = int(choice["chunk_len"])
+            self.autotune = choice
         self.plan = SeriesPlan(off, chunk_len=chunk_len, device=dev)
         P = self.plan
         self._dt = P.gather(torch.as_tensor(self.dt, dtype=torch.float32, device=dev))
@@ -169,6 +170,7 @@ class MarkovGaussianProcess:
         self._pm = torch.zeros(P.n_slots, dtype=torch.float32, device=dev)
         self._pv = torch.ones(P.n_slots, dtype=torch.float32, device=dev)
         self._loglik = torch.zeros(self.num_series, dtype=torch.float64, device=dev)
+        self._status = torch.zeros(self.num_series, dtype=torch.int32, device=dev)
         self._post_key = None  # state (hypers, sites) the stored posterior was computed from
         self._ll_key = None    # state the stored log-likelihood was computed from
         self._d1 = torch.empty(self.num_series, dtype=torch.float64, device=dev)
@@ -198,12 +200,39 @@ class MarkovGaussianProcess:
     _hyper_version = 0
 
     # ---- posterior -----------------------------------------------------------------
-    def update_posterior(self):
-        """basemodels.py:689-706: filter + smoother of the current sites."""
+    # Replay the fused pass from a hipGraph instead of launching it eagerly: the pass's
+    # launches (8 with the associative scan) become one graph launch.  Every buffer it
+    # touches lives as long as the model; the graph is recorded again when the hyper
+    # tensors are replaced.
+    use_graph = False
+    _graph = None
+    _graph_key = None
+
+    def _fused_pass(self):
         ls, var = self._hypers()
         ops.filter_smoother(self.plan, self._desc, ls, var, self._dt, self._py, self._pr, self._mask,
                             hyper_stride=self._hyper_stride, post_mean=self._pm, post_var=self._pv,
-                            loglik=self._loglik)
+                            loglik=self._loglik, status=self._status)
+
+    @property
+    def status(self):
+        """BN_STATUS_* bits of every series from the last posterior update (numpy int32
+        [num_series]; 0 = no numerical failure): a non-PD innovation, a non-finite
+        log-likelihood or filtered mean, a non-positive smoothed variance.  The affected
+        series' outputs carry NaN as in the reference; the others are unaffected."""
+        return self._status.cpu().numpy()
+
+    def update_posterior(self):
+        """basemodels.py:689-706: filter + smoother of the current sites."""
+        self._hypers()
+        if self.use_graph:
+            if self._graph is None or self._graph_key != self._hyper_version:
+                self.plan.workspace(self._desc)  # allocated before capture
+                self._graph = ops.capture_graph(self._fused_pass)
+                self._graph_key = self._hyper_version
+            self._graph.replay()
+        else:
+            self._fused_pass()
         self._post_key = self._ll_key = self._state_key()
 
     def _ensure_posterior(self):
@@ -316,7 +345,7 @@ class MarkovGaussianProcess:
         nat.check(nat.lib().bn_kalman_filter(P.handle, ctypes.byref(self._desc), nat.ptr(ls), nat.ptr(var),
                                              self._hyper_stride, nat.ptr(self._dt), nat.ptr(self._py),
                                              nat.ptr(self._pr), nat.ptr(self._mask), nat.ptr(fm), nat.ptr(fP),
-                                             nat.ptr(ll), 0, nat.ptr(ws), nat.stream_ptr()))
+                                             nat.ptr(ll), 0, None, nat.ptr(ws), nat.stream_ptr()))
         sm = torch.empty_like(fm)
         sP = torch.empty_like(fP)
         G = torch.empty_like(fP)
@@ -422,9 +451,7 @@ class MarkovGaussianProcess:
     def energy_and_grad_per_series(self, cubature=None):
         """Energy of every series and its gradient w.r.t. the natural-scale hypers
         with the sites and posterior held fixed (the gradient objax.GradValues(energy)
-        takes, CS3 of the reference): (E [B], dE/d(ls_c, var_c) [B, 2 n_comp], dE/dphi [B])."""
-        if self.method == nat.BN_EP:
-            raise NotImplementedError("hyperparameter gradients are implemented for VI and Newton energies")
+        takes, README.md:56-76): (E [B], dE/d(ls_c, var_c) [B, 2 n_comp], dE/dphi [B])."""
         P = self.plan
         ls, var = self._hypers()
         nh = 2 * self._desc.n_comp
@@ -438,13 +465,15 @@ class MarkovGaussianProcess:
             nat.ptr(self._py), nat.ptr(self._pr), nat.ptr(self._mask), nat.ptr(self._loglik), nat.ptr(gk),
             nat.ptr(P.workspace(self._desc)), nat.ptr(gws), nat.stream_ptr()))
         self._ll_key = self._state_key()
-        E = self.energy_per_series()
+        E = self.energy_per_series(cubature)
         gl = torch.empty(self.num_series, dtype=torch.float64, device=self.device)
         lik = self.likelihood.desc()
         nat.check(nat.lib().bn_lik_param_grad(
-            P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, 20, nat.ptr(self._y),
-            nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(gl), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
-        # energy = -(term1 - term2 + log Z): d/d kernel = -dlogZ, d/d likelihood = -dterm1
+            P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, cub.num_points(cubature),
+            self.power, nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(self._nat1),
+            nat.ptr(self._nat2), nat.ptr(gl), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
+        # energy = -(term1 - term2 + log Z): d/d kernel = -dlogZ, d/d likelihood = -dterm1 (for EP
+        # term1 is the moment-matched log Z over the power, inference.py:286-325)
         return E, -gk, -gl
 
     def energy_and_grad(self, cubature=None):

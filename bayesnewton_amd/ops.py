@@ -53,10 +53,15 @@ def _single_plan(n, parallel, dev):
     return SeriesPlan(np.array([0, n], dtype=np.int64), parallel=parallel, device=dev)
 
 
-def kalman_filter(dt, kernel, y, noise_cov, mask=None, parallel=False, return_predict=False, plan=None):
+def kalman_filter(dt, kernel, y, noise_cov, mask=None, parallel=False, return_predict=False, plan=None,
+                  status=None):
     """ops.py:256-285.  dt [N(,1)], y [N,1,1], noise_cov [N,1,1], mask [N,1,1] bool.
 
     Returns (ell, (means [N,d,1], covs [N,d,d])), FP64 tensors on the GPU.
+    
+    status: optional int32 device tensor [n_series] that receives the BN_STATUS_* bits
+    (a non-PD innovation or a non-finite log-likelihood; the outputs carry NaN as in
+    the reference).
     """
     import ctypes
     dev = _device()
@@ -79,8 +84,8 @@ def kalman_filter(dt, kernel, y, noise_cov, mask=None, parallel=False, return_pr
     flags = nat.BN_RETURN_PREDICT if return_predict else 0
     nat.check(nat.lib().bn_kalman_filter(plan.handle, ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), 0,
                                          nat.ptr(dt_t), nat.ptr(y_t), nat.ptr(r_t), nat.ptr(mk), nat.ptr(means),
-                                         nat.ptr(covs), nat.ptr(ll), flags, nat.ptr(plan.workspace(desc)),
-                                         nat.stream_ptr()))
+                                         nat.ptr(covs), nat.ptr(ll), flags, nat.ptr(status),
+                                         nat.ptr(plan.workspace(desc)), nat.stream_ptr()))
     m = plan.scatter(means).reshape(N, d, 1)
     P = plan.scatter(covs).reshape(N, d, d)
     return ll.sum(), (m, P)
@@ -122,11 +127,13 @@ def rauch_tung_striebel_smoother(dt, kernel, filter_mean, filter_cov, return_ful
 
 
 def filter_smoother(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=None, hyper_stride=0,
-                    post_mean=None, post_var=None, loglik=None):
+                    post_mean=None, post_var=None, loglik=None, status=None, noalloc=False):
     """Fused filter + smoother on tiled FP32 inputs (update_posterior, basemodels.py:689-706).
 
     lengthscale / variance: FP64 device tensors [n_comp] (hyper_stride 0) or
-    [B, n_comp] (hyper_stride n_comp).  Returns (post_mean, post_var, loglik).
+    [B, n_comp] (hyper_stride n_comp).  status: optional int32 [n_series] tensor for the
+    BN_STATUS_* bits.  noalloc: run bn_filter_smoother_noalloc (scratch inside the library;
+    outputs must then be given to allocate nothing).  Returns (post_mean, post_var, loglik).
     """
     import ctypes
     dev = plan.device
@@ -136,11 +143,32 @@ def filter_smoother(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=Non
         post_var = torch.empty(plan.n_slots, dtype=torch.float32, device=dev)
     if loglik is None:
         loglik = torch.empty(plan.n_series, dtype=torch.float64, device=dev)
-    nat.check(nat.lib().bn_filter_smoother(plan.handle, ctypes.byref(kernel_desc), nat.ptr(lengthscale),
-                                           nat.ptr(variance), hyper_stride, nat.ptr(dt), nat.ptr(y), nat.ptr(r),
-                                           nat.ptr(mask), nat.ptr(post_mean), nat.ptr(post_var), nat.ptr(loglik),
-                                           nat.ptr(plan.workspace(kernel_desc)), nat.stream_ptr()))
+    args = (plan.handle, ctypes.byref(kernel_desc), nat.ptr(lengthscale), nat.ptr(variance), hyper_stride, nat.ptr(dt),
+            nat.ptr(y), nat.ptr(r), nat.ptr(mask), nat.ptr(post_mean), nat.ptr(post_var), nat.ptr(loglik),
+            nat.ptr(status))
+    if noalloc:
+        nat.check(nat.lib().bn_filter_smoother_noalloc(*args, nat.stream_ptr()))
+    else:
+        nat.check(nat.lib().bn_filter_smoother(*args, nat.ptr(plan.workspace(kernel_desc)), nat.stream_ptr()))
     return post_mean, post_var, loglik
+
+
+def capture_graph(fn):
+    """Record the launches of fn() (library calls on torch's current stream) as one
+    hipGraph: fn runs once on a side stream first, then once under capture.  Returns
+    the torch.cuda.CUDAGraph; g.replay() re-runs every launch on the current stream
+    with the same pointers, so fn must only touch tensors that outlive the graph
+    (the library's entry points allocate nothing on the device and record no
+    memset, so a fused pass is capturable as it stands)."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    return g
 
 
 def filter_loglik(plan, kernel_desc, lengthscale, variance, dt, y, r, mask=None, hyper_stride=0, loglik=None):

@@ -282,6 +282,70 @@ class SpaceTimeMarkovGP:
         e = self.energy_per_run(cubature)
         return e[0] if self._single else e.sum()
 
+    # ---- hyperparameter gradients ------------------------------------------------------
+    def _trainables(self):
+        """(owner, attribute) of every trainable raw (softplus-inverse) hyperparameter:
+        the temporal kernel's, the spatial kernel's (kernels.py:396-444 holds both as
+        sub-kernels), then the likelihood's."""
+        out = []
+        for k in (self.kernel.temporal_kernel, self.kernel.spatial_kernel):
+            if not k.fix_lengthscale:
+                out.append((k, "transformed_lengthscale"))
+            if not k.fix_variance:
+                out.append((k, "transformed_variance"))
+        lik = self.likelihood
+        for attr, fix in (("transformed_variance", "fix_variance"), ("transformed_scale", "fix_scale")):
+            if hasattr(lik, attr) and not getattr(lik, fix, False):
+                out.append((lik, attr))
+        return out
+
+    def trainable_names(self):
+        names = {id(self.kernel.temporal_kernel): "temporal", id(self.kernel.spatial_kernel): "spatial"}
+        return [f"{names.get(id(o), type(o).__name__)}.{a}" for o, a in self._trainables()]
+
+    def get_raw_hypers(self):
+        return np.array([getattr(o, a) for o, a in self._trainables()], dtype=np.float64)
+
+    def set_raw_hypers(self, raw):
+        for (o, a), v in zip(self._trainables(), np.asarray(raw, dtype=np.float64).reshape(-1)):
+            setattr(o, a, float(v))
+
+    def energy_and_grad_per_run(self, rel_step=1e-5, cubature=None):
+        """Energy of every run and its gradient w.r.t. the raw hypers (trainable_names()
+        order) with the sites and the posterior held fixed, as objax.GradValues(energy)
+        differentiates the reference's energy (README.md:56-76; the gradient reaches the
+        spatial hypers through B and C, kernels.py:486-515, and the temporal ones through
+        C and the filter's log-likelihood).  Central differences of the FP64 energy on the
+        device, step rel_step * |natural value| on the natural scale, chained through
+        softplus; every energy evaluation recomputes B, C, the site projection and the
+        filter at the shifted hypers and leaves the stored posterior alone.
+        Returns (E [K], dE/draw [K, n_trainable])."""
+        E0 = self.energy_per_run(cubature).clone()
+        raw = self.get_raw_hypers()
+        grads = []
+        for i, (o, a) in enumerate(self._trainables()):
+            nat_attr = a.replace("transformed_", "")
+            v0 = getattr(o, nat_attr)
+            h = rel_step * max(abs(v0), 1e-12)
+            setattr(o, nat_attr, v0 + h)
+            ep = self.energy_per_run(cubature).clone()
+            setattr(o, nat_attr, v0 - h)
+            em = self.energy_per_run(cubature).clone()
+            self.set_raw_hypers(raw)  # exact restore of the stored raw value
+            sig = 1.0 / (1.0 + np.exp(-raw[i]))  # d softplus / d raw
+            grads.append((ep - em) / (2.0 * h) * sig)
+        # leave the projection and log-likelihood at the original hypers
+        self.energy_per_run(cubature)
+        return E0, torch.stack(grads, 1) if grads else torch.zeros((self.K, 0), dtype=torch.float64,
+                                                                      device=self.device)
+
+    def energy_and_grad(self, rel_step=1e-5, cubature=None):
+        """(energy, dE/draw) summed over runs, numpy gradient in trainable_names() order."""
+        E, g = self.energy_and_grad_per_run(rel_step, cubature)
+        if self._single:
+            return float(E[0]), g[0].cpu().numpy()
+        return float(E.sum()), g.sum(0).cpu().numpy()
+
 
 def _test_inputs(model, X, R):
     K = model.K
@@ -581,6 +645,10 @@ class SpaceTimeInfiniteHorizonGP(SpaceTimeMarkovGP):
     def _ensure_loglik(self):
         self._ensure_projection()
         self._run_filter()
+
+    def energy_and_grad_per_run(self, rel_step=1e-5, cubature=None):
+        raise NotImplementedError("hyperparameter gradients of the infinite-horizon models are not implemented: "
+                                  "every energy evaluation advances the Riccati solutions (basemodels.py:1276-1298)")
 
     @property
     def dare_solutions(self):

@@ -56,6 +56,10 @@ def parse():
     ap.add_argument("--chunk", default="auto",
                     help="batched workload: chunk length, or 'auto' (bayesnewton_amd.autotune, before the timed passes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--retune", action="store_true",
+                    help="batched workload: time the schedules even when the shipped table has this batch")
+    ap.add_argument("--record-schedule", default=None,
+                    help="batched workload: write the tuned schedule into this table (JSON)")
     ap.add_argument("--ab-runs", type=int, default=2,
                     help="default workload: held-out space-time runs (seeds --st-heldout, ...) of the warm-start "
                          "A/B reported beside the pass figures (0 = skip)")
@@ -705,11 +709,16 @@ def main():
     if str(args.chunk) == "auto":
         from bayesnewton_amd import autotune
         t_tune = time.time()
-        chunk, tune_ms = autotune.tune_chunk_len(off, desc, ls, var, torch.from_numpy(dt).to(dev),
-                                                 torch.from_numpy(y).to(dev), torch.from_numpy(r).to(dev),
-                                                 hyper_stride=2, device=dev)
-        tune = {"chosen": chunk, "ms_per_pass": tune_ms, "seconds": time.time() - t_tune}
-        print(f"[bench] autotuned chunk_len {chunk}: {tune_ms}", file=sys.stderr, flush=True)
+        tune = autotune.tune_schedule(off, desc, ls, var, torch.from_numpy(dt).to(dev), torch.from_numpy(y).to(dev),
+                                      torch.from_numpy(r).to(dev), hyper_stride=2, device=dev,
+                                      table=not args.retune)
+        tune["seconds"] = time.time() - t_tune
+        chunk = int(tune["chunk_len"])
+        print(f"[bench] schedule {tune['schedule']} chunk_len {chunk} ({tune['source']}): "
+              f"{tune.get('ms_per_pass')}", file=sys.stderr, flush=True)
+        if args.record_schedule:
+            rec = {k: v for k, v in tune.items() if k not in ("source", "signature", "seconds")}
+            autotune.record_schedule(tune["signature"], rec, path=args.record_schedule)
     else:
         chunk = int(args.chunk)
     plan = SeriesPlan(off, chunk_len=chunk, device=dev)

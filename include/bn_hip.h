@@ -42,7 +42,8 @@
  *  - Return value: 0 on success, a negative BN_E* code otherwise; the message is
  *    available from bn_last_error() (thread-local).  Numerical failure (a
  *    non-PD matrix) is not an error: as in the reference, NaN propagates into the
- *    affected series' outputs.
+ *    affected series' outputs, and the fused pass and the filter also report it per
+ *    series in an optional status[n_series] array of BN_STATUS_* bits.
  */
 #ifndef BN_HIP_H
 #define BN_HIP_H
@@ -62,6 +63,11 @@ extern "C" {
 #define BN_MATERN12 1
 #define BN_MATERN32 2
 #define BN_MATERN52 3
+
+/* per-series status bits (status[b], OR over the series' steps; 0 = none) */
+#define BN_STATUS_INNOVATION_NONPD 1 /* some step's innovation variance H P- H' + r was not > 0 (or NaN) */
+#define BN_STATUS_NONFINITE 2        /* the series' log-likelihood or final filtered mean is NaN / infinite */
+#define BN_STATUS_SMOOTHER_NONPD 4   /* some smoothed variance was not > 0 or a smoothed mean not finite */
 
 /* flags */
 #define BN_RETURN_PREDICT 0x1u /* kalman_filter(return_predict=True), ops.py:172-175 */
@@ -129,11 +135,12 @@ int bn_discretise(const bn_kernel_desc *kern, const double *lengthscale, const d
  * mask (1 = missing, may be NULL) are tiled.  Outputs: post_mean / post_var
  * (tiled, H m and H P H') and loglik[n_series] (FP64, the filter's log p(y),
  * i.e. compute_log_lik, basemodels.py:726-741).  With chunk_len >= the longest
- * series this is the sequential scan; otherwise the associative scan. */
+ * series this is the sequential scan; otherwise the associative scan.  status[n_series]
+ * (may be NULL) receives the BN_STATUS_* bits of every series. */
 int bn_filter_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
                        const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
-                       const uint8_t *mask, float *post_mean, float *post_var, double *loglik, void *workspace,
-                       void *stream);
+                       const uint8_t *mask, float *post_mean, float *post_var, double *loglik, int32_t *status,
+                       void *workspace, void *stream);
 
 /* --- bn_filter_smoother without allocation.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory during the call (code under
@@ -150,14 +157,15 @@ int bn_filter_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const do
 int bn_filter_smoother_noalloc(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
                                const double *variance, int32_t hyper_stride, const float *dt, const float *y,
                                const float *r, const uint8_t *mask, float *post_mean, float *post_var,
-                               double *loglik, void *stream);
+                               double *loglik, int32_t *status, void *stream);
 
 /* --- ops.kalman_filter, ops.py:256-285.  Writes the filtered (or, with
  * BN_RETURN_PREDICT, predicted) means [d tiled planes] and covariances [d*d
- * tiled planes, row-major], FP64, and loglik[n_series]. */
+ * tiled planes, row-major], FP64, loglik[n_series] and (if not NULL) status[n_series]
+ * (BN_STATUS_INNOVATION_NONPD / BN_STATUS_NONFINITE). */
 int bn_kalman_filter(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale,
                      const double *variance, int32_t hyper_stride, const float *dt, const float *y, const float *r,
-                     const uint8_t *mask, double *means, double *covs, double *loglik, uint32_t flags,
+                     const uint8_t *mask, double *means, double *covs, double *loglik, uint32_t flags, int32_t *status,
                      void *workspace, void *stream);
 
 /* --- log-likelihood only: MarkovGaussianProcess.compute_log_lik
@@ -250,13 +258,15 @@ int bn_energy_terms(const bn_plan *plan, const bn_kernel_desc *kern, const bn_li
                     const double *nat1, const double *nat2, double *term1, double *term2, void *workspace,
                     void *stream);
 
-/* d term1 / d(likelihood hyperparameter) per series with the posterior fixed
+/* d term1 / d(likelihood hyperparameter) per series with the posterior and sites fixed
  * (the likelihood part of objax.GradValues(model.energy); d energy = -this):
  * Gaussian variance or Student-t scale, natural scale; 0 for likelihoods
- * without one.  VI and Newton. */
+ * without one.  VI, Newton and EP (term1 of EP is (1/power) sum log Z_n, the
+ * moment-matched log partition at each site's cavity, which needs the sites
+ * nat1 / nat2 and power; VI and Newton ignore them and nat1 / nat2 may be NULL). */
 int bn_lik_param_grad(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
-                      int32_t gh_points, const float *y, const float *post_mean, const float *post_var, double *grad,
-                      void *workspace, void *stream);
+                      int32_t gh_points, double power, const float *y, const float *post_mean, const float *post_var,
+                      const double *nat1, const double *nat2, double *grad, void *workspace, void *stream);
 
 /* pseudo_y = nat1 / nat2, pseudo_var = 1 / nat2 on every slot (basemodels.py:85-90). */
 int bn_site_reparametrise(const bn_plan *plan, const double *nat1, const double *nat2, float *pseudo_y,
@@ -276,6 +286,31 @@ int bn_gauss_hermite(int32_t n, double *x, double *w);
 int bn_pairs_filter(const bn_kernel_desc *kern, const double *lengthscale, const double *variance,
                     int32_t hyper_stride, int32_t n_series, const int64_t *offsets, const double *dt, const double *y,
                     const double *noise, double *pair_means, double *pair_covs, double *loglik, void *stream);
+
+/* --- the sparse Markov GP's per-transition work (SparseMarkovGaussianProcess,
+ * basemodels.py:928-1152; VI / Newton mixins inference.py:65-222).  The n_z inducing
+ * states sit at z_aug[1..n_z] (sorted) with sentinels z_aug[0] = -1e10, z_aug[n_z+1] =
+ * 1e10; transition i in [0, n_z] joins u_i, u_{i+1} and carries a 2d-dim site nat1
+ * [n_z+1, 2d], nat2 [n_z+1, 2d, 2d] with its mean / covariance site_mean, site_cov.
+ * sm [n_z, d], sP [n_z, d, d], gain [n_z, d, d]: the RTS smoother's state, covariance and
+ * gain over the inducing states (contiguous).  tr_off [n_z+2]: CSR offsets of the data
+ * points per transition (t, y [N] sorted by time; y NaN = missing).  One lane per
+ * transition builds the pair posterior (written to pair_mean / pair_cov when not NULL),
+ * the conditional of every data point given its pair (utils.py:185-215) and then
+ *   BN_SPARSE_UPDATE: the site terms, their pair-space map, newton_update, the grouping
+ *     of group_natural_params (basemodels.py:1115-1140) and the lr damping, in place in
+ *     nat1 / nat2 / site_mean / site_cov; part1 / part2 [n_z+1] = |d nat1|, |d nat2| sums;
+ *   BN_SPARSE_ENERGY: part1 = the data terms (VI E_q log p, Newton log p at the mean),
+ *     part2 = E_q log N(site_mean | pair, site_cov) (basemodels.py:214-222), per transition.
+ * Temporal kernels (1 or 2 Matern components), VI and Newton, FP64. */
+#define BN_SPARSE_UPDATE 0
+#define BN_SPARSE_ENERGY 1
+int bn_sparse_sites(const bn_kernel_desc *kern, const double *lengthscale, const double *variance,
+                    const bn_lik_desc *lik, int32_t method, int32_t mode, int32_t gh_points, double lr,
+                    int32_t ensure_psd, int32_t n_z, const double *z_aug, const double *sm, const double *sP,
+                    const double *gain, const int64_t *tr_off, const double *t, const double *y, double *nat1,
+                    double *nat2, double *site_mean, double *site_cov, double *pair_mean, double *pair_cov,
+                    double *part1, double *part2, void *stream);
 
 /* --- space-time models: SpatioTemporalKernel (kernels.py:385-586, sparse, 'Full'
  * conditional) driven by MarkovGaussianProcess (basemodels.py:625-764) --------

@@ -7,7 +7,7 @@ that the oracle still reproduces them (a regression pin of the oracle) and
 tests/test_gpu_golden.py checks the HIP path against the stored arrays without
 running the oracle.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture.npz ...]
 """
 import os
 import sys
@@ -162,6 +162,68 @@ def predict_outputs(d):
                 nlpd=np.asarray(nlpd, dtype=np.float64))
 
 
+def spacetime_cox_full_inputs():
+    """The full-size space-time Cox process (T = 2000 steps, 256 locations per step, 64
+    inducing points on an 8 x 8 grid) of seed 1005: the inputs are regenerated from the
+    seed by bayesnewton_amd.workloads.spacetime_cox (host path), so only the seed, the
+    step sizes of the VI iterations and the stride of the stored time steps are kept.
+    Small steps (lr = 0.02) keep the pseudo precision B' diag(nat2) B well conditioned
+    (condition below 1e6 after two steps); at lr = 0.2 the second step's sites span five
+    orders of magnitude and the condition reaches 4e7."""
+    return dict(seed=np.asarray(1005), T=np.asarray(2000), lrs=np.asarray([0.02, 0.02]), stride=np.asarray(8))
+
+
+def _cox_model(d):
+    from bayesnewton_amd import workloads
+    w = workloads.spacetime_cox(seed=int(d["seed"]), T=int(d["T"]))
+    g = np.linspace(0, 1, 8)
+    a, b = np.meshgrid(g, g)
+    z = np.stack([a.ravel(), b.ravel()], 1)
+    ok = OK.SpatioTemporalKernel(OK.Matern32(1.0, 10.0), OK.Matern32(1.0, 0.2), z, sparse=True, conditional="Full")
+    return OM.MarkovGP(ok, OL.Poisson(2.0), w["t"][:, None], w["y"], R=f32(w["r"]), inference="VI")
+
+
+def spacetime_cox_full_outputs(d):
+    """Energies after each VI step and the posterior of the inducing processes (means and
+    marginal variances) after the last one, at every `stride`-th time step."""
+    o = _cox_model(d)
+    energies = []
+    for lr in d["lrs"]:
+        o.inference(lr=float(lr))
+        energies.append(float(o.energy()))
+    k = int(d["stride"])
+    return dict(energy=np.asarray(energies), post_mean=np.asarray(o.posterior_mean)[::k, :, 0].copy(),
+                post_var=np.diagonal(o.posterior_variance, axis1=1, axis2=2)[::k].copy())
+
+
+def spacetime_cox_diverging_inputs():
+    """Seeds 1004 and 1006 of the same process, whose plain VI iteration at lr = 0.2
+    overshoots on its first step."""
+    return dict(seeds=np.asarray([1004, 1006]), T=np.asarray(2000), lr=np.asarray(0.2))
+
+
+def spacetime_cox_diverging_outputs(d):
+    """The energy after the first VI step, and whether the second step's pseudo precision
+    B' diag(nat2) B + 1e-12 I could still be factorised (basemodels.py:683)."""
+    e1, ok2 = [], []
+    for sd in d["seeds"]:
+        o = _cox_model(dict(seed=sd, T=d["T"]))
+        o.inference(lr=float(d["lr"]))
+        e1.append(float(o.energy()))
+        try:
+            o.inference(lr=float(d["lr"]))
+            ok2.append(bool(np.isfinite(o.energy())))
+        except np.linalg.LinAlgError:
+            ok2.append(False)
+    return dict(energy_step1=np.asarray(e1), step2_finite=np.asarray(ok2))
+
+
+# full-size fixtures: minutes of oracle time each, so tests/test_golden.py does not rerun them
+LARGE_FIXTURES = {
+    "spacetime_cox_full_seed1005.npz": (spacetime_cox_full_inputs, spacetime_cox_full_outputs),
+    "spacetime_cox_diverging_seeds.npz": (spacetime_cox_diverging_inputs, spacetime_cox_diverging_outputs),
+}
+
 FIXTURES = {
     "kf_rts_sum_m32_m52_n400.npz": (kf_rts_inputs, kf_rts_outputs),
     "vi_probit_n300.npz": (vi_probit_inputs, vi_probit_outputs),
@@ -171,8 +233,10 @@ FIXTURES = {
 }
 
 
-def main():
-    for name, (inp, out) in FIXTURES.items():
+def main(names=None):
+    for name, (inp, out) in {**FIXTURES, **LARGE_FIXTURES}.items():
+        if names and name not in names:
+            continue
         d = inp()
         res = out(d)
         np.savez_compressed(os.path.join(HERE, name), **d, **res)
@@ -180,4 +244,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -45,3 +45,20 @@ def test_golden_kf_rts_invariants():
     assert np.allclose(P, np.transpose(P, (0, 2, 1)), atol=1e-12)
     assert np.all(np.linalg.eigvalsh(P) > -1e-12)
     assert np.isfinite(z["loglik"]) and abs(float(z["loglik"])) < 10 * len(z["dt"])
+
+
+@pytest.mark.parametrize("name", ["spacetime_cox_full_seed1005.npz", "spacetime_cox_diverging_seeds.npz"])
+def test_large_fixtures_hold_their_generator_inputs(name):
+    """The full-size space-time fixtures take minutes of oracle time, so only their
+    inputs (seeds, sizes, step sizes) are checked here against the generator; their
+    outputs are pinned by the GPU parity tests that read them."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    make_inputs, _ = mg.LARGE_FIXTURES[name]
+    z = _load(name)
+    for k, v in make_inputs().items():
+        np.testing.assert_array_equal(z[k], v, err_msg=k)
+    for k in z:
+        assert np.all(np.isfinite(z[k]) | (z[k].dtype == bool)), k

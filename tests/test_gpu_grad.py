@@ -48,7 +48,8 @@ def _oracle_fd(o, params, h=1e-5):
     return np.array(out)
 
 
-@pytest.mark.parametrize("case", ["m52-gauss", "sum-studentt", "m32-gauss-parallel"])
+@pytest.mark.parametrize("case", ["m52-gauss", "sum-studentt", "m32-gauss-parallel", "sum-studentt-newton",
+                                  "m52-probit-newton", "sum-studentt-ep", "m52-gauss-ep-half"])
 def test_energy_gradient_vs_oracle_fd(case):
     from bayesnewton_amd import kernels as K, likelihoods as L, models as M
     rng = np.random.default_rng(11)
@@ -68,14 +69,42 @@ def test_energy_gradient_vs_oracle_fd(case):
         oparams = [(ok.kernels[0], "lengthscale"), (ok.kernels[0], "variance"), (ok.kernels[1], "lengthscale"),
                    (ok.kernels[1], "variance"), (ol, "scale")]
         chunk = None
+    elif case in ("sum-studentt-newton", "sum-studentt-ep"):
+        y = _f32(np.sin(x) + 0.2 * rng.standard_t(3.0, size=n))
+        gk = K.Sum([K.Matern52(variance=0.8, lengthscale=1.5), K.Matern32(variance=0.4, lengthscale=4.0)])
+        ok = OK.Sum([OK.Matern52(0.8, 1.5), OK.Matern32(0.4, 4.0)])
+        gl, ol = L.StudentsT(scale=0.25, df=3.0), OL.StudentsT(0.25, 3.0)
+        oparams = [(ok.kernels[0], "lengthscale"), (ok.kernels[0], "variance"), (ok.kernels[1], "lengthscale"),
+                   (ok.kernels[1], "variance"), (ol, "scale")]
+        chunk = 16
+    elif case == "m52-probit-newton":
+        y = (rng.uniform(size=n) < 0.5 * (1 + np.tanh(np.sin(x)))).astype(np.float64)
+        gk, ok = K.Matern52(variance=1.0, lengthscale=2.0), OK.Matern52(1.0, 2.0)
+        gl, ol = L.Bernoulli(), OL.Bernoulli()
+        oparams = [(ok, "lengthscale"), (ok, "variance")]
+        chunk = None
+    elif case == "m52-gauss-ep-half":
+        y = _f32(np.sin(x) + 0.1 * rng.standard_normal(n))
+        gk, ok = K.Matern52(variance=1.0, lengthscale=1.0), OK.Matern52(1.0, 1.0)
+        gl, ol = L.Gaussian(variance=0.05), OL.Gaussian(0.05)
+        oparams = [(ok, "lengthscale"), (ok, "variance"), (ol, "variance")]
+        chunk = 64
     else:
         y = _f32(np.cos(0.5 * x) + 0.1 * rng.standard_normal(n))
         gk, ok = K.Matern32(variance=1.2, lengthscale=0.7), OK.Matern32(1.2, 0.7)
         gl, ol = L.Gaussian(variance=0.02), OL.Gaussian(0.02)
         oparams = [(ok, "lengthscale"), (ok, "variance"), (ol, "variance")]
         chunk = 16
-    g = M.MarkovVariationalGP(gk, gl, x, y, parallel=True, chunk_len=chunk)
-    o = OM.MarkovGP(ok, ol, x, y, inference="VI")
+    if case.endswith("-newton"):
+        g = M.MarkovLaplaceGP(gk, gl, x, y, parallel=True, chunk_len=chunk)
+        o = OM.MarkovGP(ok, ol, x, y, inference="Newton")
+    elif "-ep" in case:
+        power = 0.5 if case.endswith("-half") else 1.0
+        g = M.MarkovExpectationPropagationGP(gk, gl, x, y, power=power, parallel=True, chunk_len=chunk)
+        o = OM.MarkovGP(ok, ol, x, y, inference="EP", power=power)
+    else:
+        g = M.MarkovVariationalGP(gk, gl, x, y, parallel=True, chunk_len=chunk)
+        o = OM.MarkovGP(ok, ol, x, y, inference="VI")
     for _ in range(2):
         g.inference(lr=1.0)
         o.inference(lr=1.0)

@@ -186,7 +186,8 @@ def test_autotuned_chunk_length_gives_the_same_posterior():
         K.Sum([K.Matern52(variance=1.0, lengthscale=2.0), K.Matern32(variance=0.5, lengthscale=0.7)]),
         L.StudentsT(scale=0.2, df=3.0), xs, ys, chunk_len=c)
     ma, m32 = mk("auto"), mk(32)
-    assert ma.autotune["chosen"] in ma.autotune["ms_per_pass"]
+    assert ma.autotune["schedule"] in ("chunked", "sequential")
+    assert ma.autotune["schedule"] == "sequential" or str(ma.autotune["chunk_len"]) in ma.autotune["ms_per_pass"]
     print(f"[autotune] {ma.autotune}", flush=True)
     for m in (ma, m32):
         m.inference(lr=1.0)

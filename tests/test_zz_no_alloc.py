@@ -51,8 +51,9 @@ def _run(lens, seed, chunk_len):
         pm = torch.full((plan.n_slots,), float("nan"), **f32)
         pv = torch.full((plan.n_slots,), float("nan"), **f32)
         ll = torch.full((len(lens),), float("nan"), dtype=torch.float64, device=dev)
+        st = torch.full((len(lens),), -1, dtype=torch.int32, device=dev)
         args = (plan.handle, ctypes.byref(desc), nat.ptr(ls), nat.ptr(var), 2, nat.ptr(dt_t), nat.ptr(y_t),
-                nat.ptr(r_t), nat.ptr(mask), nat.ptr(pm), nat.ptr(pv), nat.ptr(ll))
+                nat.ptr(r_t), nat.ptr(mask), nat.ptr(pm), nat.ptr(pv), nat.ptr(ll), nat.ptr(st))
         print(f"[no_alloc] lens={list(lens)} chunk_len={chunk_len} noalloc={noalloc}", flush=True)
         if noalloc:
             nat.check(L.bn_filter_smoother_noalloc(*args, nat.stream_ptr()))
@@ -60,6 +61,7 @@ def _run(lens, seed, chunk_len):
             nat.check(L.bn_filter_smoother(*args, nat.ptr(plan.workspace(desc)), nat.stream_ptr()))
         torch.cuda.synchronize()
         outs.append((plan.scatter(pm).cpu(), plan.scatter(pv).cpu(), ll.cpu()))
+        assert torch.equal(st.cpu(), torch.zeros(len(lens), dtype=torch.int32)), st
     return outs
 
 
@@ -88,5 +90,5 @@ def test_noalloc_refuses_a_plan_beyond_its_scratch():
     ll = torch.zeros(1, dtype=torch.float64, device=dev)
     rc = nat.lib().bn_filter_smoother_noalloc(plan.handle, ctypes.byref(desc), nat.ptr(hyp), nat.ptr(hyp), 0,
                                               nat.ptr(buf), nat.ptr(buf), nat.ptr(buf), None, nat.ptr(buf),
-                                              nat.ptr(buf), nat.ptr(ll), nat.stream_ptr())
+                                              nat.ptr(buf), nat.ptr(ll), None, nat.stream_ptr())
     assert rc == -1 and b"scratch" in nat.lib().bn_last_error()
