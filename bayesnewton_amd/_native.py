@@ -82,6 +82,7 @@ SIGNATURES = {
     "bn_st_conditional": (i32, [vp, vp, i32, f64, vp, vp, vp, vp, vp, vp]),
     "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                           vp, vp]),
+    "bn_st_ep_sites": (i32, [vp, vp, i32, i32, f64, i32, f64] + [vp] * 18),
     "bn_st_workspace_bytes": (i64, [vp]),
     "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_st_smoother": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),

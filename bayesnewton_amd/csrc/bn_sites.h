@@ -263,6 +263,37 @@ BN_HD SiteResult site_step(const LikParams& L, const GHTable& gh, double y, doub
   return SiteResult{mean, jac, hess, J - H * mean, -H};
 }
 
+// EP site step from a cavity already formed (cm, cv): the space-time models remove
+// the projected sites from the posterior of the inducing states and carry the cavity
+// to the data (basemodels.py:230-245, 743-764) before moment matching
+// (inference.py:238-284).  mask_pseudo: missing outputs give NaN terms (the model
+// masks its pseudo observations); otherwise the moment match runs at y = cm, as
+// likelihoods.py:401-412 substitutes, and its terms are kept.
+BN_HD SiteResult ep_site_from_cavity(const LikParams& L, const GHTable& gh, double y, double cm, double cv,
+                                     double power, bool mask_pseudo, bool ensure_psd) {
+  const bool miss = y != y;
+  double lZ, dlZ, d2lZ;
+  moment_match(L, gh, miss ? cm : y, cm, cv, power, lZ, dlZ, d2lZ);
+  const double cp = 1.0 / cv;
+  const double sc = cp / (d2lZ + cp) / power;
+  double jac = sc * dlZ, hess = sc * d2lZ;
+  if (miss && mask_pseudo) { jac = NAN; hess = NAN; }
+  if (ensure_psd && -hess < 0.0) hess = -1e-2;
+  const double H = (hess != hess) ? -1e-6 : hess;
+  const double J = (jac != jac) ? H * cm : jac;
+  return SiteResult{cm, jac, hess, J - H * cm, -H};
+}
+
+// log Z of the moment match at a formed cavity (the lel term of inference.py:286-325);
+// a missing output contributes 0.
+BN_HD double ep_lel_from_cavity(const LikParams& L, const GHTable& gh, double y, double cm, double cv,
+                                double power) {
+  if (y != y) return 0.0;
+  double lZ, dlZ, d2lZ;
+  moment_match(L, gh, y, cm, cv, power, lZ, dlZ, d2lZ);
+  return lZ;
+}
+
 // Energy terms of one site (summed per series by the caller):
 //   VI      t1 = E_q[log p(y|f)]             t2 = E_q[log N(py | f, pr)]
 //   Newton  t1 = log p(y | m)                t2 = E_q[log N(py | f, pr)]

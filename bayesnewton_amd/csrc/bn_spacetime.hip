@@ -536,7 +536,7 @@ static_assert(2 * kRC * kLC >= kMaxM * kMaxM, "the two chunk buffers hold an M x
 
 template <int METHOD, int MODE>
 __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
-                                                 int ensure_psd) {
+                                                 int ensure_psd, double power, int mask_pseudo) {
   extern __shared__ double smem[];
   const int M = dm.M, R = dm.R;
   // LDS is kept near 53 KB (three workgroups per CU): the posterior covariance is read
@@ -585,7 +585,10 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
         const double yv = (double)io.y[s];
         if (UPDATE) {
           const double n1 = io.nat1[s], n2 = io.nat2[s];
-          SiteResult sr = site_step<METHOD>(lik, gh, yv, mf, vf, n1, n2, 1.0, ensure_psd != 0);
+          // EP: (pm, pP) is the cavity of the inducing states, so (mf, vf) is the cavity at the data
+          SiteResult sr = METHOD == M_EP ? ep_site_from_cavity(lik, gh, yv, mf, vf, power, mask_pseudo != 0,
+                                                               ensure_psd != 0)
+                                         : site_step<METHOD>(lik, gh, yv, mf, vf, n1, n2, 1.0, ensure_psd != 0);
           acc0 += fabs(sr.nat1 - n1);
           acc1 += fabs(sr.nat2 - n2);
           const double a1 = (1.0 - lr) * n1 + lr * sr.nat1, a2 = (1.0 - lr) * n2 + lr * sr.nat2;
@@ -596,6 +599,8 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
             io.o_jac[s] = (float)sr.jac;
             io.o_hess[s] = (float)sr.hess;
           }
+        } else if (METHOD == M_EP) {
+          acc0 += ep_lel_from_cavity(lik, gh, yv, mf, vf, power);
         } else {
           double t1, t2;
           energy_terms<METHOD>(lik, gh, yv, mf, vf, 1.0, 1.0, 1.0, t1, t2);
@@ -633,6 +638,10 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
     for (int e = tid; e < M * M; e += kT) Lo[e] = sW[e];
     for (int e = tid; e < M; e += kT) io.zw[kt * M + e] = sv[e];
     if (tid == 0) io.ldl[kt] = ld;
+  } else if (METHOD == M_EP) {
+    // the pseudo term (lel_pseudo) comes from k_st_ep_cavity
+    const double e1 = wg_sum(acc0, red);
+    if (tid == 0) io.part0[kt] = e1;
   } else {
     const double e1 = wg_sum(acc0, red);
     // whitened expected density of the pseudo observation:
@@ -652,6 +661,97 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
       io.part0[kt] = e1;
       io.part1[kt] = -0.5 * (Q2 + M * kLog2Pi - 2.0 * io.ldl[kt]) - 0.5 * TR;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_st_ep_cavity: per (step, run), the power-EP cavity of the inducing states
+// (compute_cavity, utils.py:534-541, on the projected sites of basemodels.py:667-674):
+//   Q = P + 1e-8 I,  W = B' diag(nat2) B = L L' - 1e-12 I  (L, z from the projection),
+//   cav_cov = (Q^-1 - a W)^-1,  cav_mean = cav_cov (Q^-1 m - a B' nat1),  B' nat1 = L z.
+// ENERGY also forms the pseudo term of the EP energy (basemodels.py:247-262),
+//   lel_pseudo = log N(py | cav_mean, pv / a + cav_cov) + pep_constant(pv, a),
+// with pv = (W + 1e-12 I)^-1 and py = pv B' nat1, in whitened form: w = z - L' cav_mean,
+// G = I / a + L' cav_cov L, log N = -1/2 (w' G^-1 w + M log 2pi - 2 log|L| + log|G|),
+// pep_constant = M/2 ((1 - a) log 2pi - log a) - (1 - a) log|L|; so pv, whose
+// eigenvalues reach 1e12, is never formed.  mask_pseudo with a missing output at a
+// one-location step: 0 (mvn_logpdf and pep_constant of a fully masked observation).
+// ---------------------------------------------------------------------------
+struct StCavityIO {
+  const float* y;      // [K,T,R]
+  const double* pm;    // [K,T,M]
+  const double* pP;    // [K,T,M,M]
+  const double* Lw;    // [K,T,M,M]
+  const double* zw;    // [K,T,M]
+  const double* ldl;   // [K,T]
+  double* cm;          // [K,T,M]
+  double* cP;          // [K,T,M,M]
+  double* lel_pseudo;  // [K,T] (ENERGY)
+};
+
+template <bool ENERGY>
+__global__ __launch_bounds__(kT) void k_st_ep_cavity(StDims dm, StCavityIO io, double power, int mask_pseudo) {
+  extern __shared__ double smem[];
+  const int M = dm.M;
+  double* pan = smem;                    // 2*kPK*kPS
+  double* sQ = pan + 2 * kPK * kPS;      // M*M
+  double* sA = sQ + kMaxM * kMaxM;       // M*M
+  double* sm = sA + kMaxM * kMaxM;       // M: posterior mean, then cav_mean
+  double* sr = sm + kMaxM;               // M: right-hand side, then w
+  double* row = sr + kMaxM;              // 2*kMaxM pivot rows
+  double* red = row + 2 * kMaxM;         // 4
+  const int t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int64_t kt = (int64_t)k * dm.T + t;
+  const double* P = io.pP + kt * M * M;
+  const double* L = io.Lw + kt * M * M;
+  const double* z = io.zw + kt * M;
+  for (int e = tid; e < M * M; e += kT) sQ[e] = P[e] + ((e / M == e % M) ? 1e-8 : 0.0);
+  for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
+  wg_sweep_inv<kMaxM / 16, false>(sQ, M, M, row);                          // sQ = -Q^-1
+  wg_gemm_small(M, M, M, 1.0, MatRef{L, M, 1}, MatRef{L, 1, M}, 0.0, sA, M, pan);  // L L'
+  for (int e = tid; e < M * M; e += kT)
+    sA[e] = -sQ[e] - power * sA[e] + ((e / M == e % M) ? power * kPseudoJitter : 0.0);
+  if (tid < M) {
+    double u = 0.0;
+    for (int j = 0; j < M; ++j) u = fma(-sQ[tid * M + j], sm[j], u);
+    for (int j = 0; j <= tid; ++j) u = fma(-power * L[tid * M + j], z[j], u);
+    sr[tid] = u;
+  }
+  wg_sweep_inv<kMaxM / 16, false>(sA, M, M, row);                          // sA = -cav_cov
+  double cmv = 0.0;
+  if (tid < M) {
+    for (int j = 0; j < M; ++j) cmv = fma(-sA[tid * M + j], sr[j], cmv);
+    io.cm[kt * M + tid] = cmv;
+  }
+  double* gC = io.cP + kt * M * M;
+  for (int e = tid; e < M * M; e += kT) gC[e] = -sA[e];
+  if (!ENERGY) return;
+  __syncthreads();
+  if (tid < M) sm[tid] = cmv;
+  __syncthreads();
+  if (tid < M) {
+    double u = z[tid];
+    for (int a = tid; a < M; ++a) u = fma(-L[a * M + tid], sm[a], u);
+    sr[tid] = u;                                                             // w = z - L' cav_mean
+  }
+  wg_gemm_small(M, M, M, -1.0, MatRef{sA, M, 1}, MatRef{L, M, 1}, 0.0, sQ, M, pan);  // cav_cov L
+  wg_gemm_small(M, M, M, 1.0, MatRef{L, 1, M}, MatRef{sQ, M, 1}, 0.0, sA, M, pan);   // L' cav_cov L
+  for (int e = tid; e < M * M; e += kT)
+    if (e / M == e % M) sA[e] += 1.0 / power;
+  const double ldg = wg_sweep_inv<kMaxM / 16, true>(sA, M, M, row);       // sA = -G^-1
+  double q = 0.0;
+  if (tid < M) {
+    double u = 0.0;
+    for (int j = 0; j < M; ++j) u = fma(-sA[tid * M + j], sr[j], u);
+    q = sr[tid] * u;
+  }
+  const double Q2 = wg_sum(q, red);
+  if (tid == 0) {
+    const double ldl = io.ldl[kt];
+    const double lp = -0.5 * (Q2 + M * kLog2Pi - 2.0 * ldl + ldg);
+    const double pep = 0.5 * M * ((1.0 - power) * kLog2Pi - log(power)) - (1.0 - power) * ldl;
+    const bool masked = mask_pseudo != 0 && dm.R == 1 && io.y[kt] != io.y[kt];
+    io.lel_pseudo[kt] = masked ? 0.0 : lp + pep;
   }
 }
 
@@ -1808,6 +1908,7 @@ constexpr size_t kTiedSmem = sizeof(double) * (2 * kPK * kPS + kMaxM * kMaxM);
 constexpr size_t kIhDareSmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 2 * kMaxM);
 constexpr size_t kSitesSmem =
     sizeof(double) * (2 * kPK * kPS + 2 * kRC * kLC + 2 * kMaxM + kRC + 2 * kMaxM + 4);
+constexpr size_t kCavitySmem = sizeof(double) * (2 * kPK * kPS + 2 * kMaxM * kMaxM + 4 * kMaxM + 4);
 
 int check_dims(const bn_st_dims* d, StDims& dm) {
   if (d == nullptr) return st_fail(BN_EINVAL, "dims is NULL");
@@ -1858,7 +1959,13 @@ int set_smem_limits() {
   ST_SITES_ATTR(M_NEWTON, BN_ST_UPDATE);
   ST_SITES_ATTR(M_NEWTON, BN_ST_ENERGY);
   ST_SITES_ATTR(M_NEWTON, BN_ST_PROJECT);
+  ST_SITES_ATTR(M_EP, BN_ST_UPDATE);
+  ST_SITES_ATTR(M_EP, BN_ST_ENERGY);
 #undef ST_SITES_ATTR
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_ep_cavity<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kCavitySmem));
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_ep_cavity<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)kCavitySmem));
   done = 1;
   return BN_OK;
 }
@@ -1907,7 +2014,7 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
   const dim3 grid(dm.T, dm.K);
   BN_TIMED(KID_ST_SITES, st);
 #define ST_SITES(MT, MD) \
-  hipLaunchKernelGGL((k_st_sites<MT, MD>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd)
+  hipLaunchKernelGGL((k_st_sites<MT, MD>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0)
   if (method == BN_VI) {
     if (mode == BN_ST_UPDATE) ST_SITES(M_VI, BN_ST_UPDATE);
     else if (mode == BN_ST_ENERGY) ST_SITES(M_VI, BN_ST_ENERGY);
@@ -1918,6 +2025,51 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
     else ST_SITES(M_NEWTON, BN_ST_PROJECT);
   }
 #undef ST_SITES
+  ST_HIP_CHECK(hipGetLastError());
+  return BN_OK;
+}
+
+int bn_st_ep_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t mode, int32_t gh_points, double lr,
+                   int32_t ensure_psd, double power, const float* y, const double* B, const double* cdiag,
+                   const double* post_mean, const double* post_cov, double* nat1, double* nat2, double* Lw,
+                   double* zw, double* ldl, double* cav_mean, double* cav_cov, double* part0, double* part1,
+                   float* out_mean, float* out_jac, float* out_hess, void* stream) {
+  StDims dm;
+  if (int rc = check_dims(dims, dm)) return rc;
+  if (!lik || !y || !B || !cdiag || !post_mean || !post_cov || !nat1 || !nat2 || !Lw || !zw || !ldl || !cav_mean ||
+      !cav_cov || !part0 || !part1)
+    return st_fail(BN_EINVAL, "NULL argument to bn_st_ep_sites");
+  if (mode != BN_ST_UPDATE && mode != BN_ST_ENERGY)
+    return st_fail(BN_EINVAL, "mode must be BN_ST_UPDATE or BN_ST_ENERGY (the projection is method-free: bn_st_sites)");
+  if (!(power > 0.0 && power <= 1.0)) return st_fail(BN_EINVAL, "EP power must be in (0, 1]");
+  LikParams L;
+  if (int rc = check_lik(lik, L)) return rc;
+  if (dm.R > 1 && L.kind != LIK_GAUSSIAN)
+    return st_fail(BN_EUNSUPPORTED,
+                   "EP with several locations per step moment-matches all of them jointly; only the Gaussian "
+                   "likelihood has that in closed form (one location per step takes any likelihood)");
+  if (dm.R > 1 && mode == BN_ST_ENERGY)
+    return st_fail(BN_EUNSUPPORTED, "masking in spatio-temporal models not implemented for EP");
+  GHTable gh;
+  if (int rc = make_gh_table(gh_points, gh)) return rc;
+  if (int rc = set_smem_limits()) return rc;
+  // basemodels.py:138-144, 653: the pseudo observations are masked only when their
+  // dimension (M) equals the data's (R)
+  const int mask_pseudo = dm.M == dm.R ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid(dm.T, dm.K);
+  StCavityIO cio{y, post_mean, post_cov, Lw, zw, ldl, cav_mean, cav_cov, part1};
+  StSiteIO io{y, B, cdiag, cav_mean, cav_cov, nat1, nat2, Lw, zw, ldl, part0, part1, out_mean, out_jac, out_hess};
+  BN_TIMED(KID_ST_SITES, st);
+  if (mode == BN_ST_UPDATE) {
+    hipLaunchKernelGGL((k_st_ep_cavity<false>), grid, dim3(kT), kCavitySmem, st, dm, cio, power, mask_pseudo);
+    hipLaunchKernelGGL((k_st_sites<M_EP, BN_ST_UPDATE>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                       ensure_psd, power, mask_pseudo);
+  } else {
+    hipLaunchKernelGGL((k_st_ep_cavity<true>), grid, dim3(kT), kCavitySmem, st, dm, cio, power, mask_pseudo);
+    hipLaunchKernelGGL((k_st_sites<M_EP, BN_ST_ENERGY>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                       ensure_psd, power, mask_pseudo);
+  }
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }

@@ -71,10 +71,14 @@ class MarkovGaussianProcess:
         if isinstance(kernel, SpatioTemporalKernel):
             if R is None:
                 raise ValueError("a SpatioTemporalKernel needs the locations R [T, n_loc, 2]")
+            if cls.method == nat.BN_EP:
+                power = kwargs.get("power", args[0] if args else 1.0)
+                return spacetime.SpaceTimeExpectationPropagationGP(kernel, likelihood, X, Y, R, power=power,
+                                                                   device=kwargs.get("device"))
             st_cls = {nat.BN_VI: spacetime.SpaceTimeVariationalGP,
                       nat.BN_NEWTON: spacetime.SpaceTimeLaplaceGP}.get(cls.method)
             if st_cls is None:
-                raise NotImplementedError("space-time models support VI and Newton / Laplace")
+                raise NotImplementedError("space-time models support VI, Newton / Laplace and EP")
             return st_cls(kernel, likelihood, X, Y, R, device=kwargs.get("device"))
         return super().__new__(cls)
 

@@ -414,6 +414,85 @@ class SpaceTimeLaplaceGP(SpaceTimeMarkovGP):
     method = nat.BN_NEWTON
 
 
+class SpaceTimeExpectationPropagationGP(SpaceTimeMarkovGP):
+    """MarkovExpectationPropagationGP with a SpatioTemporalKernel (models.py:216-231): power
+    EP whose cavity removes the projected sites B' diag(nat2) B, B' nat1 from the posterior
+    of the inducing states (basemodels.py:230-262 with compute_full_pseudo_nat, 667-674)
+    and is carried to the data before moment matching (inference.py:238-284).
+
+    Several locations per step: the reference moment-matches a step's locations jointly;
+    the Gaussian likelihood's closed form separates per location, which is what runs here,
+    and other likelihoods are refused.  energy() follows the reference, whose EP energy
+    refuses several locations per step (inference.py:313-317); with one location per step
+    it is -(log Z + (sum log Z_tilted - sum lel_pseudo) / power) (inference.py:286-325)."""
+    method = nat.BN_EP
+
+    def __init__(self, kernel, likelihood, X, Y, R, power=1.0, parallel=None, device=None):
+        self.power = float(power)
+        if not 0.0 < self.power <= 1.0:
+            raise ValueError("EP power must be in (0, 1]")
+        super().__init__(kernel, likelihood, X, Y, R, parallel=parallel, device=device)
+        self._check_lik()
+
+    @classmethod
+    def from_runs(cls, kernel, likelihood, Xs, Rs, Ys, power=1.0, parallel=None, device=None):
+        self = cls.__new__(cls)
+        self.power = float(power)
+        self.parallel = parallel
+        self._build(kernel, likelihood, Xs, Rs, Ys, device)
+        self._single = False
+        self._check_lik()
+        return self
+
+    def _check_lik(self):
+        if self.n_loc > 1 and type(self.likelihood).__name__ != "Gaussian":
+            raise NotImplementedError(
+                "EP with several locations per step moment-matches them jointly (a product Gauss-Hermite rule over "
+                "all of them in the reference); only the Gaussian likelihood has that in closed form")
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self._cm = torch.empty((self.K, self.T, self.M), **f64)
+        self._cP = torch.empty((self.K, self.T, self.M, self.M), **f64)
+
+    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None):
+        lik = self.likelihood.desc()
+        om, oj, oh = terms if terms is not None else (None, None, None)
+        if mode == nat.BN_ST_PROJECT:
+            # the projected pseudo observation does not depend on the inference method
+            nat.check(nat.lib().bn_st_sites(
+                ctypes.byref(self._dims), ctypes.byref(lik), nat.BN_VI, mode, cub.num_points(cubature), 1.0, 1,
+                nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag), nat.ptr(self._pm), nat.ptr(self._pP),
+                nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._ldl),
+                nat.ptr(self._p0), nat.ptr(self._p1), None, None, None, nat.stream_ptr()))
+            return
+        nat.check(nat.lib().bn_st_ep_sites(
+            ctypes.byref(self._dims), ctypes.byref(lik), mode, cub.num_points(cubature), float(lr),
+            1 if ensure_psd else 0, self.power, nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag),
+            nat.ptr(self._pm), nat.ptr(self._pP), nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw),
+            nat.ptr(self._zw), nat.ptr(self._ldl), nat.ptr(self._cm), nat.ptr(self._cP), nat.ptr(self._p0),
+            nat.ptr(self._p1), nat.ptr(om), nat.ptr(oj), nat.ptr(oh), nat.stream_ptr()))
+
+    @property
+    def cavity(self):
+        """Cavity of the inducing states from the last site update or energy, [T, M] and
+        [T, M, M] ([K, ...] for several runs)."""
+        return self._out(self._cm), self._out(self._cP)
+
+    def energy_per_run(self, cubature=None):
+        if self.n_loc > 1:
+            raise NotImplementedError("masking in spatio-temporal models not implemented for EP")
+        self._ensure_loglik()
+        self._sites(nat.BN_ST_ENERGY, cubature=cubature)
+        lel = self._sum(self._p0, self._s0).clone()
+        lel_pseudo = self._sum(self._p1, self._s1).clone()
+        logz = self._loglik.clone()
+        if self.M == self.n_loc:
+            # the filter masks the log-likelihood of a missing pseudo observation
+            # (mask_pseudo_y, basemodels.py:138-144, 698)
+            miss = torch.isnan(self._y[..., 0])
+            logz = logz - torch.where(miss, self._ell, torch.zeros_like(self._ell)).sum(1)
+        return -(logz + (lel - lel_pseudo) / self.power)
+
+
 class SpaceTimeMeanFieldGP(SpaceTimeMarkovGP):
     """MarkovMeanFieldGaussianProcess (basemodels.py:1158-1175): the same model with the
     mean-field filter and smoother (ops.py:581-608, 681-706), which keep only the M

@@ -363,6 +363,33 @@ int bn_st_sites(const bn_st_dims *dims, const bn_lik_desc *lik, int32_t method, 
                 double *ldl, double *part0, double *part1, float *out_mean, float *out_jac, float *out_hess,
                 void *stream);
 
+/* Power EP on the space-time models (ExpectationPropagation, inference.py:238-325, with
+ * the cavity of the projected sites, basemodels.py:230-262 and 667-674).  Per (run,
+ * step): the cavity of the inducing states, cav_cov = ((post_cov + 1e-8 I)^-1 - power
+ * B' diag(nat2) B)^-1 and cav_mean = cav_cov ((post_cov + 1e-8 I)^-1 post_mean - power
+ * B' nat1), written to cav_mean [K,T,M] / cav_cov [K,T,M,M] (B' diag(nat2) B and B'
+ * nat1 are taken from the current projection Lw, zw, which must match nat1 / nat2);
+ * then, at every location, the cavity at the data (B cav_mean, diag(B cav_cov B') +
+ * diag C) and the moment match of the likelihood.
+ * mode BN_ST_UPDATE: new sites damped by lr, part0/part1 = per-step sums of |dnat1|,
+ * |dnat2|, and the new projection Lw, zw, ldl (as bn_st_sites).  The reference
+ * moment-matches all R locations of a step jointly; for the Gaussian likelihood that
+ * closed form separates per location (the joint scale factor times the joint
+ * derivatives is -I / sigma^2), so R > 1 takes the Gaussian likelihood only, R = 1
+ * any likelihood.  Missing outputs: masked pseudo observations when M == R
+ * (basemodels.py:138-144), otherwise the moment match at y = the cavity mean.
+ * mode BN_ST_ENERGY (R = 1 only: the reference's EP energy refuses several locations
+ * per step, inference.py:313-317): part0 = per-step log Z of the moment match
+ * (0 where y is missing), part1 = per-step lel_pseudo = log N(pseudo_y | cav_mean,
+ * pseudo_var / power + cav_cov) + pep_constant(pseudo_var, power) in whitened form.
+ * The energy is -(log Z_filter + (sum part0 - sum part1) / power).
+ * power in (0, 1].  out_* optional [K,T,R]: cavity mean at the data, jacobian, hessian. */
+int bn_st_ep_sites(const bn_st_dims *dims, const bn_lik_desc *lik, int32_t mode, int32_t gh_points, double lr,
+                   int32_t ensure_psd, double power, const float *y, const double *B, const double *cdiag,
+                   const double *post_mean, const double *post_cov, double *nat1, double *nat2, double *Lw,
+                   double *zw, double *ldl, double *cav_mean, double *cav_cov, double *part0, double *part1,
+                   float *out_mean, float *out_jac, float *out_hess, void *stream);
+
 /* Device scratch of bn_st_filter / bn_st_smoother (bytes). */
 int64_t bn_st_workspace_bytes(const bn_st_dims *dims);
 

@@ -14,7 +14,7 @@ ops.py:52-80) and is used only to pin the Markov oracle.
 import numpy as np
 
 from . import ops
-from .utils import (inv, solve, diag, input_admin, mvn_logpdf, gaussian_expected_log_lik,
+from .utils import (inv, solve, diag, input_admin, mvn_logpdf, mvn_logpdf_and_derivs, gaussian_expected_log_lik,
                     compute_cavity, pep_constant, ensure_diagonal_positive_precision, LOG2PI)
 from .kernels import SpatioTemporalKernel
 
@@ -311,9 +311,25 @@ class MarkovGP:
         return compute_cavity(self.posterior_mean, self.posterior_variance, nat1f, nat2f, power)
 
     def _moment_match(self, cav_mean_f, cav_cov_f):
-        """likelihoods.py:401-412 vmapped over time (scalar sites only)."""
+        """likelihoods.py:401-412 vmapped over time.  Several locations per step (space-time
+        kernels): the multivariate closed form of the Gaussian likelihood
+        (likelihoods.py:755-782, mvn_logpdf_and_derivs with sigma^2 I / power + cavity
+        covariance, plus pep_constant(sigma^2 I, power)), missing outputs replaced by the
+        cavity mean (likelihoods.py:401-412); other likelihoods integrate over all the
+        step's locations at once with a product Gauss-Hermite rule (cubature.py:310-370),
+        which is not restated."""
         if cav_mean_f.shape[1] != 1:
-            raise NotImplementedError("EP moment matching is implemented for scalar sites")
+            if type(self.likelihood).__name__ != "Gaussian":
+                raise NotImplementedError("multivariate EP moment matching is restated for the Gaussian "
+                                          "likelihood only")
+            Nn, p = cav_mean_f.shape[0], cav_mean_f.shape[1]
+            y = self.Y.reshape(Nn, p, 1)
+            y = np.where(np.isnan(y), cav_mean_f, y)
+            s2 = self.likelihood.variance
+            lik_cov = s2 * np.eye(p)
+            lZ, dlZ, d2lZ = mvn_logpdf_and_derivs(y, cav_mean_f, lik_cov / self.power + cav_cov_f)
+            lZ = lZ + pep_constant(np.broadcast_to(lik_cov, (Nn, p, p)), self.power)
+            return lZ, dlZ, d2lZ
         y = self.Y.reshape(-1)
         m = cav_mean_f.reshape(-1)
         v = cav_cov_f.reshape(-1)
@@ -384,6 +400,9 @@ class MarkovGP:
             return -(np.nansum(ll) - self.compute_kl())
         if kind == "EP":
             # inference.py:286-325 with basemodels.py:247-262
+            if self.mask_y.ndim == 2 and self.mask_y.shape[1] > 1:
+                # inference.py:313-317: mask_y [N, n_loc] against one lel per step
+                raise NotImplementedError("masking in spatio-temporal models not implemented for EP")
             power = self.power
             cav_mean, cav_cov = self.cavity_distribution(power)
             pseudo_y, pseudo_var = self.compute_full_pseudo_lik()
