@@ -73,15 +73,13 @@ SIGNATURES = {
     "bn_rts_smoother": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, u32, vp, vp]),
     "bn_predict": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]),
     "bn_lik_predict": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, vp, vp]),
-    "bn_site_update": (i32, [vp, vp, vp, i32, i32, f64, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                              vp, vp]),
-    "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bn_site_update": (i32, [vp, vp, vp, i32, i32, f64, f64, i32] + [vp] * 15),
+    "bn_energy_terms": (i32, [vp, vp, vp, i32, i32, f64] + [vp] * 10),
     "bn_site_reparametrise": (i32, [vp, vp, vp, vp, vp, vp]),
     "bn_lik_param_grad": (i32, [vp, vp, vp, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bn_gauss_hermite": (i32, [i32, vp, vp]),
     "bn_st_conditional": (i32, [vp, vp, i32, f64, vp, vp, vp, vp, vp, vp]),
-    "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                          vp, vp]),
+    "bn_st_sites": (i32, [vp, vp, i32, i32, i32, f64, i32] + [vp] * 17),
     "bn_st_ep_sites": (i32, [vp, vp, i32, i32, f64, i32, f64] + [vp] * 18),
     "bn_st_workspace_bytes": (i64, [vp]),
     "bn_st_filter": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -40,6 +40,7 @@ struct SiteIO {
   float* o_hess;
   double* part1;  // per chunk: sum |nat1_new - nat1| (update) or t1 (energy)
   double* part2;
+  const float* batch;  // optional per slot: nonzero = in the mini-batch (NULL: every site)
 };
 
 template <int METHOD>
@@ -51,6 +52,8 @@ __global__ __launch_bounds__(kBlock) void k_site_update(PlanView pl, LikParams L
   double d1 = 0.0, d2 = 0.0;
   for (int32_t k = 0; k < len; ++k) {
     const int64_t s = pl.slot(j, k);
+    // group_natural_params (basemodels.py:195-199): sites outside the batch keep their values
+    if (io.batch != nullptr && io.batch[s] == 0.0f) continue;
     const double n1 = io.nat1[s], n2 = io.nat2[s];
     SiteResult r = site_step<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], n1, n2, power,
                                      ensure_psd != 0);
@@ -85,7 +88,8 @@ __global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams 
     double t1, t2;
     energy_terms<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], io.nat1[s], io.nat2[s], power,
                          t1, t2);
-    s1 += t1;
+    // the data term runs over the mini-batch only (inference.py:130-325); the pseudo term over every site
+    if (io.batch == nullptr || io.batch[s] != 0.0f) s1 += t1;
     s2 += t2;
   }
   io.part1[j] = s1;
@@ -341,7 +345,8 @@ int bn_site_reparametrise(const bn_plan* plan, const double* nat1, const double*
 
 int bn_site_update(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
                    int32_t gh_points, double lr, double power, int32_t ensure_psd, const float* y,
-                   const float* post_mean, const float* post_var, double* nat1, double* nat2, float* pseudo_y,
+                   const float* post_mean, const float* post_var, const float* batch, double* nat1, double* nat2,
+                   float* pseudo_y,
                    float* pseudo_var, float* out_mean, float* out_jac, float* out_hess, double* diff1, double* diff2,
                    void* workspace, void* stream) {
   if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
@@ -357,7 +362,7 @@ int bn_site_update(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik
   double* p1 = site_partials(plan, kern, workspace, 0);
   double* p2 = site_partials(plan, kern, workspace, 1);
   if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
-  SiteIO io{y, post_mean, post_var, nat1, nat2, pseudo_y, pseudo_var, out_mean, out_jac, out_hess, p1, p2};
+  SiteIO io{y, post_mean, post_var, nat1, nat2, pseudo_y, pseudo_var, out_mean, out_jac, out_hess, p1, p2, batch};
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
   switch (method) {
@@ -381,7 +386,8 @@ int bn_site_update(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik
 
 int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik_desc* lik, int32_t method,
                     int32_t gh_points, double power, const float* y, const float* post_mean, const float* post_var,
-                    const double* nat1, const double* nat2, double* term1, double* term2, void* workspace,
+                    const float* batch, const double* nat1, const double* nat2, double* term1, double* term2,
+                    void* workspace,
                     void* stream) {
   if (plan == nullptr) return fail(BN_EINVAL, "plan is NULL");
   if (!y || !post_mean || !post_var || !nat1 || !nat2 || !term1 || !term2 || !workspace)
@@ -395,7 +401,7 @@ int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_li
   double* p2 = site_partials(plan, kern, workspace, 1);
   if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
   SiteIO io{y, post_mean, post_var, const_cast<double*>(nat1), const_cast<double*>(nat2), nullptr, nullptr,
-            nullptr, nullptr, nullptr, p1, p2};
+            nullptr, nullptr, nullptr, p1, p2, batch};
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
   switch (method) {

@@ -466,6 +466,7 @@ struct StSiteIO {
   float* o_mean;          // optional [K,T,R] returned site terms
   float* o_jac;
   float* o_hess;
+  const float* batch;     // optional [K,T]: nonzero = step in the mini-batch (NULL: every step)
 };
 
 // In-place lower Cholesky of an n x n SPD matrix a (LDS, stride ld), the upper part
@@ -559,6 +560,9 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
   constexpr bool SITES = MODE != BN_ST_PROJECT;
   const double* gP = io.pP + kt * M * M;
   double* gW = io.Lw + kt * M * M;
+  // mini-batch (inference.py:65-90, group_natural_params basemodels.py:195-199): a step
+  // outside it keeps its sites and adds no data term; its projection is still formed
+  const bool in_batch = io.batch == nullptr || io.batch[kt] != 0.0f;
   if (SITES) {
     for (int e = tid; e < M; e += kT) sm[e] = io.pm[kt * M + e];
   }
@@ -583,7 +587,8 @@ __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GH
           vf = fma(c, sX[tid * LC + j], vf);
         }
         const double yv = (double)io.y[s];
-        if (UPDATE) {
+        if (!in_batch) {
+        } else if (UPDATE) {
           const double n1 = io.nat1[s], n2 = io.nat2[s];
           // EP: (pm, pP) is the cavity of the inducing states, so (mf, vf) is the cavity at the data
           SiteResult sr = METHOD == M_EP ? ep_site_from_cavity(lik, gh, yv, mf, vf, power, mask_pseudo != 0,
@@ -1992,9 +1997,9 @@ int bn_st_conditional(const bn_st_dims* dims, const bn_st_hypers* hyp, int32_t s
 
 int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, int32_t mode, int32_t gh_points,
                 double lr, int32_t ensure_psd, const float* y, const double* B, const double* cdiag,
-                const double* post_mean, const double* post_cov, double* nat1, double* nat2, double* Lw, double* zw,
-                double* ldl, double* part0, double* part1, float* out_mean, float* out_jac, float* out_hess,
-                void* stream) {
+                const double* post_mean, const double* post_cov, const float* batch, double* nat1, double* nat2,
+                double* Lw, double* zw, double* ldl, double* part0, double* part1, float* out_mean, float* out_jac,
+                float* out_hess, void* stream) {
   StDims dm;
   if (int rc = check_dims(dims, dm)) return rc;
   if (!lik || !y || !B || !cdiag || !post_mean || !post_cov || !nat1 || !nat2 || !Lw || !zw || !ldl || !part0 ||
@@ -2009,7 +2014,8 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
   GHTable gh;
   if (int rc = make_gh_table(gh_points, gh)) return rc;
   if (int rc = set_smem_limits()) return rc;
-  StSiteIO io{y, B, cdiag, post_mean, post_cov, nat1, nat2, Lw, zw, ldl, part0, part1, out_mean, out_jac, out_hess};
+  StSiteIO io{y,    B,    cdiag, post_mean, post_cov, nat1,    nat2,     Lw,
+              zw,   ldl,  part0, part1,     out_mean, out_jac, out_hess, batch};
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(dm.T, dm.K);
   BN_TIMED(KID_ST_SITES, st);
@@ -2059,7 +2065,8 @@ int bn_st_ep_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t mode,
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(dm.T, dm.K);
   StCavityIO cio{y, post_mean, post_cov, Lw, zw, ldl, cav_mean, cav_cov, part1};
-  StSiteIO io{y, B, cdiag, cav_mean, cav_cov, nat1, nat2, Lw, zw, ldl, part0, part1, out_mean, out_jac, out_hess};
+  StSiteIO io{y,  B,   cdiag, cav_mean, cav_cov,  nat1,    nat2,     Lw,
+              zw, ldl, part0, part1,    out_mean, out_jac, out_hess, nullptr};
   BN_TIMED(KID_ST_SITES, st);
   if (mode == BN_ST_UPDATE) {
     hipLaunchKernelGGL((k_st_ep_cavity<false>), grid, dim3(kT), kCavitySmem, st, dm, cio, power, mask_pseudo);

@@ -267,9 +267,28 @@ class MarkovGaussianProcess:
         return self.plan.scatter(self._pv[None]).to(torch.float64).reshape(-1, 1, 1)
 
     # ---- inference (inference.py:65-90) ----------------------------------------------
+    def _batch_mask(self, batch_ind):
+        """Tiled mask of a mini-batch (indices into the model's time-sorted data; for a
+        model over several series, into their concatenation in series order), or None
+        for the whole data set (inference.py:67-68)."""
+        if batch_ind is None:
+            return None
+        idx = np.asarray(batch_ind).reshape(-1)
+        if idx.shape[0] == self.num_data:
+            return None
+        if idx.size == 0:
+            raise ValueError("empty mini-batch")
+        if idx.min() < 0 or idx.max() >= self.num_data:
+            raise IndexError("batch_ind out of range")
+        m = torch.zeros(self.num_data, dtype=torch.float32, device=self.device)
+        m[torch.as_tensor(idx, dtype=torch.int64, device=self.device)] = 1.0
+        return self.plan.gather(m)[0]
+
     def inference(self, lr=1.0, batch_ind=None, cubature=None, ensure_psd=True, return_terms=True, **kwargs):
-        if batch_ind is not None and len(batch_ind) != self.num_data:
-            raise NotImplementedError("mini-batched site updates are not implemented on the HIP path")
+        """inference.py:65-90.  With batch_ind, only the sites of those data points are
+        updated (the others keep their values, group_natural_params), diff1 / diff2 are
+        means over all N sites, and the returned terms are those of the batch [B, 1, 1]."""
+        batch = self._batch_mask(batch_ind)
         self._ensure_posterior()
         P = self.plan
         dev = self.device
@@ -283,7 +302,8 @@ class MarkovGaussianProcess:
         nat.check(nat.lib().bn_site_update(
             P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, cub.num_points(cubature), float(lr),
             self.power, 1 if ensure_psd else 0, nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv),
-            nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._py), nat.ptr(self._pr), nat.ptr(om),
+            nat.ptr(batch), nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._py), nat.ptr(self._pr),
+            nat.ptr(om),
             nat.ptr(oj), nat.ptr(oh), nat.ptr(self._d1), nat.ptr(self._d2), nat.ptr(P.workspace(self._desc)),
             nat.stream_ptr()))
         self._sites_version += 1
@@ -292,15 +312,22 @@ class MarkovGaussianProcess:
         diff2 = self._d2.sum() / self.num_data
         if return_terms:
             terms = tuple(P.scatter(a[None]).to(torch.float64).reshape(-1, 1, 1) for a in (om, oj, oh))
+            if batch is not None:
+                sel = torch.as_tensor(np.asarray(batch_ind).reshape(-1), dtype=torch.int64, device=dev)
+                terms = tuple(a[sel] for a in terms)
         else:
             terms = (None, None, None)
         return terms, (diff1, diff2)
 
     # ---- energy -------------------------------------------------------------------
-    def energy_per_series(self, cubature=None):
+    def energy_per_series(self, cubature=None, batch_ind=None):
         """Energy of every series (the reference's energy() for each one).  As in the
         reference, the stored posterior is used as is and only log Z (the filter
-        log-likelihood of the sites under the current hypers) is recomputed."""
+        log-likelihood of the sites under the current hypers) is recomputed.  With
+        batch_ind the data term sums the batch and is scaled by N / B (inference.py:
+        134-138, 203-207, 292-296); the pseudo terms and log Z keep every site."""
+        batch = self._batch_mask(batch_ind)
+        scale = 1.0 if batch is None else self.num_data / float(np.asarray(batch_ind).size)
         self._ensure_loglik()
         P = self.plan
         lik = self.likelihood.desc()
@@ -308,18 +335,16 @@ class MarkovGaussianProcess:
         t2 = torch.empty_like(t1)
         nat.check(nat.lib().bn_energy_terms(
             P.handle, ctypes.byref(self._desc), ctypes.byref(lik), self.method, cub.num_points(cubature), self.power,
-            nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(self._nat1), nat.ptr(self._nat2),
-            nat.ptr(t1), nat.ptr(t2), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
+            nat.ptr(self._y), nat.ptr(self._pm), nat.ptr(self._pv), nat.ptr(batch), nat.ptr(self._nat1),
+            nat.ptr(self._nat2), nat.ptr(t1), nat.ptr(t2), nat.ptr(P.workspace(self._desc)), nat.stream_ptr()))
         logZ = self._loglik
         if self.method == nat.BN_EP:
-            return -(logZ + (t1 - t2) / self.power)
-        return -(t1 - (t2 - logZ))
+            return -(logZ + (scale * t1 - t2) / self.power)
+        return -(scale * t1 - (t2 - logZ))
 
     def energy(self, batch_ind=None, cubature=None, **kwargs):
         """inference.py:130-154 (Newton), 197-222 (VI), 286-325 (EP)."""
-        if batch_ind is not None and len(batch_ind) != self.num_data:
-            raise NotImplementedError("mini-batched energies are not implemented on the HIP path")
-        return self.energy_per_series(cubature).sum()
+        return self.energy_per_series(cubature, batch_ind).sum()
 
 
     # ---- prediction (basemodels.py:766-816, 165-193) ----------------------------------------

@@ -151,15 +151,31 @@ class SpaceTimeMarkovGP:
         return h
 
     # ---- sites ---------------------------------------------------------------------------
-    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None):
+    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None, batch=None):
         lik = self.likelihood.desc()
         om, oj, oh = terms if terms is not None else (None, None, None)
         nat.check(nat.lib().bn_st_sites(
             ctypes.byref(self._dims), ctypes.byref(lik), self.method, mode, cub.num_points(cubature), float(lr),
             1 if ensure_psd else 0, nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag), nat.ptr(self._pm),
-            nat.ptr(self._pP), nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw), nat.ptr(self._zw),
-            nat.ptr(self._ldl), nat.ptr(self._p0), nat.ptr(self._p1), nat.ptr(om), nat.ptr(oj), nat.ptr(oh),
-            nat.stream_ptr()))
+            nat.ptr(self._pP), nat.ptr(batch), nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw),
+            nat.ptr(self._zw), nat.ptr(self._ldl), nat.ptr(self._p0), nat.ptr(self._p1), nat.ptr(om), nat.ptr(oj),
+            nat.ptr(oh), nat.stream_ptr()))
+
+    def _batch_mask(self, batch_ind):
+        """[K, T] mask of a mini-batch of time steps (indices into the time-sorted steps,
+        the same for every run), or None for all of them (inference.py:67-68)."""
+        if batch_ind is None:
+            return None
+        idx = np.asarray(batch_ind).reshape(-1)
+        if idx.shape[0] == self.num_data:
+            return None
+        if idx.size == 0:
+            raise ValueError("empty mini-batch")
+        if idx.min() < 0 or idx.max() >= self.num_data:
+            raise IndexError("batch_ind out of range")
+        m = torch.zeros((self.K, self.T), dtype=torch.float32, device=self.device)
+        m[:, torch.as_tensor(idx, dtype=torch.int64, device=self.device)] = 1.0
+        return m
 
     def _sum(self, per_step, out):
         nat.check(nat.lib().bn_st_sum(self.K, self.T, nat.ptr(per_step), nat.ptr(out), nat.stream_ptr()))
@@ -247,16 +263,16 @@ class SpaceTimeMarkovGP:
     def inference(self, lr=1.0, batch_ind=None, cubature=None, ensure_psd=True, return_terms=True, **kwargs):
         """One site update + posterior update.  Returns ((mean, jacobian, hessian), (diff1,
         diff2)) with the site terms per location, [T, n_loc] each (the hessian's diagonal),
-        and the mean absolute change of the natural parameters."""
-        if batch_ind is not None and len(batch_ind) != self.num_data:
-            raise NotImplementedError("mini-batched site updates are not implemented on the HIP path")
+        and the mean absolute change of the natural parameters.  batch_ind: a mini-batch
+        of time steps; only their sites change, and the terms are those steps' [B, n_loc]."""
+        batch = self._batch_mask(batch_ind)
         self._ensure_conditional()
         self._ensure_posterior()
         terms = None
         if return_terms:
             terms = tuple(torch.empty((self.K, self.T, self.n_loc), dtype=torch.float32, device=self.device)
                           for _ in range(3))
-        self._sites(nat.BN_ST_UPDATE, lr, cubature, ensure_psd, terms)
+        self._sites(nat.BN_ST_UPDATE, lr, cubature, ensure_psd, terms, batch)
         self._sites_version += 1
         self._proj_key = (self._cond_key, self._sites_version)
         # inference.py:79-80 averages over nat1 [T, R, 1] and nat2 [T, R, R] (whose
@@ -265,21 +281,27 @@ class SpaceTimeMarkovGP:
         d1 = self._sum(self._p0, self._s0) / n
         d2 = self._sum(self._p1, self._s1) / (n * self.n_loc)
         self.update_posterior()
-        out_terms = tuple(self._out(a.to(torch.float64)) for a in terms) if terms else (None, None, None)
+        out_terms = tuple(a.to(torch.float64) for a in terms) if terms else (None, None, None)
+        if terms and batch is not None:
+            sel = torch.as_tensor(np.asarray(batch_ind).reshape(-1), dtype=torch.int64, device=self.device)
+            out_terms = tuple(a[:, sel] for a in out_terms)
+        out_terms = tuple(self._out(a) for a in out_terms) if terms else out_terms
         return out_terms, (self._out(d1.clone()), self._out(d2.clone()))
 
     # ---- energy ------------------------------------------------------------------------------
-    def energy_per_run(self, cubature=None):
+    def energy_per_run(self, cubature=None, batch_ind=None):
+        """Energy of every run; with batch_ind the data term sums those time steps, scaled
+        by T / B (inference.py:130-154, 197-222)."""
+        batch = self._batch_mask(batch_ind)
+        scale = 1.0 if batch is None else self.num_data / float(np.asarray(batch_ind).size)
         self._ensure_loglik()
-        self._sites(nat.BN_ST_ENERGY, cubature=cubature)
+        self._sites(nat.BN_ST_ENERGY, cubature=cubature, batch=batch)
         ell = self._sum(self._p0, self._s0).clone()
         ed = self._sum(self._p1, self._s1).clone()
-        return -(ell - (ed - self._loglik))
+        return -(scale * ell - (ed - self._loglik))
 
     def energy(self, batch_ind=None, cubature=None, **kwargs):
-        if batch_ind is not None and len(batch_ind) != self.num_data:
-            raise NotImplementedError("mini-batched energies are not implemented on the HIP path")
-        e = self.energy_per_run(cubature)
+        e = self.energy_per_run(cubature, batch_ind)
         return e[0] if self._single else e.sum()
 
     # ---- hyperparameter gradients ------------------------------------------------------
@@ -453,17 +475,19 @@ class SpaceTimeExpectationPropagationGP(SpaceTimeMarkovGP):
         self._cm = torch.empty((self.K, self.T, self.M), **f64)
         self._cP = torch.empty((self.K, self.T, self.M, self.M), **f64)
 
-    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None):
+    def _sites(self, mode, lr=1.0, cubature=None, ensure_psd=True, terms=None, batch=None):
         lik = self.likelihood.desc()
         om, oj, oh = terms if terms is not None else (None, None, None)
         if mode == nat.BN_ST_PROJECT:
             # the projected pseudo observation does not depend on the inference method
             nat.check(nat.lib().bn_st_sites(
                 ctypes.byref(self._dims), ctypes.byref(lik), nat.BN_VI, mode, cub.num_points(cubature), 1.0, 1,
-                nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag), nat.ptr(self._pm), nat.ptr(self._pP),
+                nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag), nat.ptr(self._pm), nat.ptr(self._pP), None,
                 nat.ptr(self._nat1), nat.ptr(self._nat2), nat.ptr(self._Lw), nat.ptr(self._zw), nat.ptr(self._ldl),
                 nat.ptr(self._p0), nat.ptr(self._p1), None, None, None, nat.stream_ptr()))
             return
+        if batch is not None:
+            raise NotImplementedError("mini-batched EP on the space-time models is not implemented")
         nat.check(nat.lib().bn_st_ep_sites(
             ctypes.byref(self._dims), ctypes.byref(lik), mode, cub.num_points(cubature), float(lr),
             1 if ensure_psd else 0, self.power, nat.ptr(self._y), nat.ptr(self._B), nat.ptr(self._cdiag),
@@ -477,7 +501,9 @@ class SpaceTimeExpectationPropagationGP(SpaceTimeMarkovGP):
         [T, M, M] ([K, ...] for several runs)."""
         return self._out(self._cm), self._out(self._cP)
 
-    def energy_per_run(self, cubature=None):
+    def energy_per_run(self, cubature=None, batch_ind=None):
+        if self._batch_mask(batch_ind) is not None:
+            raise NotImplementedError("mini-batched EP on the space-time models is not implemented")
         if self.n_loc > 1:
             raise NotImplementedError("masking in spatio-temporal models not implemented for EP")
         self._ensure_loglik()

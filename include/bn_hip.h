@@ -238,12 +238,15 @@ int bn_lik_predict(const bn_lik_desc *lik, int32_t gh_points, int64_t n, const d
  * out_mean / out_jac / out_hess: the (mean, jacobian, hessian) inference()
  * returns, or all NULL.  diff1 / diff2 [n_series]: sums of |nat_new - nat_old|
  * before damping (inference.py:79-81 divides by N).  gh_points: Gauss-Hermite
- * nodes for the cubature likelihoods (reference default 20). */
+ * nodes for the cubature likelihoods (reference default 20).  batch: optional tiled
+ * mask (nonzero = in the mini-batch of inference(batch_ind=...)): sites outside it are
+ * left as they are and add nothing to diff1 / diff2 (group_natural_params,
+ * basemodels.py:195-199; inference.py:65-90); NULL updates every site. */
 int bn_site_update(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
                    int32_t gh_points, double lr, double power, int32_t ensure_psd, const float *y,
-                   const float *post_mean, const float *post_var, double *nat1, double *nat2, float *pseudo_y,
-                   float *pseudo_var, float *out_mean, float *out_jac, float *out_hess, double *diff1,
-                   double *diff2, void *workspace, void *stream);
+                   const float *post_mean, const float *post_var, const float *batch, double *nat1, double *nat2,
+                   float *pseudo_y, float *pseudo_var, float *out_mean, float *out_jac, float *out_hess,
+                   double *diff1, double *diff2, void *workspace, void *stream);
 
 /* --- energy terms per series (inference.py:130-154, 197-222, 286-325):
  * term1 = VI: sum E_q[log p(y|f)] | Newton: sum log p(y|m) | EP: sum log Z
@@ -252,11 +255,13 @@ int bn_site_update(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik
  *         (basemodels.py:247-262).  Masked sites contribute 0.
  * With logZ the filter log-likelihood of the same sites: energy =
  * -(term1 - (term2 - logZ)) for VI / Newton, -(logZ + (term1 - term2) / power)
- * for EP. */
+ * for EP.  batch: optional tiled mask of energy(batch_ind=...): term1 sums the
+ * mini-batch only (the caller scales it by N / batch size), term2 every site;
+ * NULL = every site. */
 int bn_energy_terms(const bn_plan *plan, const bn_kernel_desc *kern, const bn_lik_desc *lik, int32_t method,
                     int32_t gh_points, double power, const float *y, const float *post_mean, const float *post_var,
-                    const double *nat1, const double *nat2, double *term1, double *term2, void *workspace,
-                    void *stream);
+                    const float *batch, const double *nat1, const double *nat2, double *term1, double *term2,
+                    void *workspace, void *stream);
 
 /* d term1 / d(likelihood hyperparameter) per series with the posterior and sites fixed
  * (the likelihood part of objax.GradValues(model.energy); d energy = -this):
@@ -356,12 +361,15 @@ int bn_st_conditional(const bn_st_dims *dims, const bn_st_hypers *hyp, int32_t s
  * E_q[log p(y|f)] (VI) or log p(y|m) (Newton); part1 = per-step
  * gaussian_expected_log_lik of the pseudo observation (utils.py:510-531) from
  * Lw, zw, ldl.  mode BN_ST_PROJECT: Lw, zw, ldl of the current sites only
- * (post_* unused).  Methods BN_VI and BN_NEWTON.  out_* optional [K,T,R]. */
+ * (post_* unused).  Methods BN_VI and BN_NEWTON.  out_* optional [K,T,R].
+ * batch: optional [K,T] mask of the mini-batch of time steps (inference(batch_ind=...),
+ * energy(batch_ind=...)): steps outside it keep their sites (UPDATE) and add nothing to
+ * part0 (ENERGY; the caller scales by T / batch size); NULL = every step. */
 int bn_st_sites(const bn_st_dims *dims, const bn_lik_desc *lik, int32_t method, int32_t mode, int32_t gh_points,
                 double lr, int32_t ensure_psd, const float *y, const double *B, const double *cdiag,
-                const double *post_mean, const double *post_cov, double *nat1, double *nat2, double *Lw, double *zw,
-                double *ldl, double *part0, double *part1, float *out_mean, float *out_jac, float *out_hess,
-                void *stream);
+                const double *post_mean, const double *post_cov, const float *batch, double *nat1, double *nat2,
+                double *Lw, double *zw, double *ldl, double *part0, double *part1, float *out_mean, float *out_jac,
+                float *out_hess, void *stream);
 
 /* Power EP on the space-time models (ExpectationPropagation, inference.py:238-325, with
  * the cavity of the projected sites, basemodels.py:230-262 and 667-674).  Per (run,

@@ -375,29 +375,45 @@ class MarkovGP:
             return cav_mean_f, dlel, d2lel
         raise NotImplementedError(kind)
 
-    def inference(self, lr=1.0, ensure_psd=True):
-        """inference.py:65-90"""
+    def inference(self, lr=1.0, ensure_psd=True, batch_ind=None):
+        """inference.py:65-90.  batch_ind (indices into the time-sorted data): the site
+        map runs on those points only and group_natural_params (basemodels.py:195-199)
+        writes them into the full site arrays.  The site map is elementwise in the data
+        points, so it is evaluated on all of them and the batch is selected."""
+        if batch_ind is not None and np.asarray(batch_ind).shape[0] == self.num_data:
+            batch_ind = None
         self.update_posterior()
         mean, jac, hess = self.update_variational_params(ensure_psd=ensure_psd)
+        if batch_ind is not None:
+            b = np.asarray(batch_ind)
+            mean, jac, hess = mean[b], jac[b], hess[b]
         nat1_n, nat2_n = newton_update(mean, jac, hess)
+        if batch_ind is not None:
+            n1, n2 = self.nat1.copy(), self.nat2.copy()
+            n1[b], n2[b] = nat1_n, nat2_n
+            nat1_n, nat2_n = n1, n2
         diff1 = np.mean(np.abs(nat1_n - self.nat1))
         diff2 = np.mean(np.abs(nat2_n - self.nat2))
         self.update_nat_params((1 - lr) * self.nat1 + lr * nat1_n, (1 - lr) * self.nat2 + lr * nat2_n)
         self.update_posterior()
         return (mean, jac, hess), (diff1, diff2)
 
-    def energy(self):
+    def energy(self, batch_ind=None):
+        """inference.py:130-154, 197-222, 286-325; with batch_ind the data term sums the
+        batch and is scaled by N / B, the KL / pseudo terms keep every site."""
         kind = self.inference_kind
+        sel = np.arange(self.num_data) if batch_ind is None else np.asarray(batch_ind)
+        scale = self.num_data / sel.shape[0]
         if kind == "VI":
             # inference.py:197-222
             mean_f, cov_f = self.conditional_posterior_to_data()
             ell, _, _ = self._variational_expectation(mean_f, cov_f)
-            return -(np.nansum(ell) - self.compute_kl())
+            return -(scale * np.nansum(ell[sel]) - self.compute_kl())
         if kind == "Newton":
             # inference.py:130-154
             mean_f, _ = self.conditional_posterior_to_data()
             ll, _, _ = self._log_likelihood_gradients(mean_f)
-            return -(np.nansum(ll) - self.compute_kl())
+            return -(scale * np.nansum(ll[sel]) - self.compute_kl())
         if kind == "EP":
             # inference.py:286-325 with basemodels.py:247-262
             if self.mask_y.ndim == 2 and self.mask_y.shape[1] > 1:
@@ -414,7 +430,7 @@ class MarkovGP:
             cav_mean_f, cav_cov_f = self.conditional_posterior_to_data(cav_mean, cav_cov)
             lel, _, _ = self._moment_match(cav_mean_f, cav_cov_f)
             lel = np.where(np.squeeze(self.mask_y), 0.0, np.squeeze(lel))
-            return -(lZ + 1.0 / power * (np.nansum(lel) - np.nansum(lel_pseudo)))
+            return -(lZ + 1.0 / power * (scale * np.nansum(np.atleast_1d(lel)[sel]) - np.nansum(lel_pseudo)))
         raise NotImplementedError(kind)
 
 
