@@ -54,19 +54,35 @@ struct Spec {
   }
 };
 
-// Closed-form A = expm(F dt) of one component (kernels.py:156-163, 216-224, 273-286).
+// True when `p` holds on every active lane of the wavefront (the host build: `p`).
+BN_HD bool lanes_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __all(p) != 0;
+#else
+  return p;
+#endif
+}
+BN_HD bool lanes_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __any(p) != 0;
+#else
+  return p;
+#endif
+}
+
+// Closed-form A = expm(F dt) of one component (kernels.py:156-163, 216-224, 273-286),
+// given e = exp(-lam dt).
 template <int K>
-BN_HD void comp_transition(double lam, double dt, double* A) {
+BN_HD void comp_transition_e(double lam, double dt, double e, double* A) {
   if (K == 1) {
-    A[0] = exp(-dt * lam);
+    A[0] = e;
   } else if (K == 2) {
-    double e = exp(-dt * lam);
     A[0] = e * (dt * lam + 1.0);
     A[1] = e * dt;
     A[2] = e * (-dt * lam * lam);
     A[3] = e * (-dt * lam + 1.0);
   } else {
-    double dl = dt * lam, e = exp(-dl);
+    const double dl = dt * lam;
     A[0] = e * (dt * lam * (0.5 * dl + 1.0) + 1.0);
     A[1] = e * (dt * (dl + 1.0));
     A[2] = e * (dt * 0.5 * dt);
@@ -77,6 +93,11 @@ BN_HD void comp_transition(double lam, double dt, double* A) {
     A[7] = e * (dt * lam * lam * (dl - 3.0));
     A[8] = e * (dt * lam * (0.5 * dl - 2.0) + 1.0);
   }
+}
+
+template <int K>
+BN_HD void comp_transition(double lam, double dt, double* A) {
+  comp_transition_e<K>(lam, dt, exp(-(lam * dt)), A);
 }
 
 // Stationary covariance of one component (kernels.py:144-146, 207-210, 288-293).
@@ -165,22 +186,24 @@ struct Model {
 
   template <int K, int O, class T>
   BN_HD static void comp_block(double lam, double var, double dt, T A[D][D], T Q[D][D]) {
+    const double x = lam * dt;
+    const double e = exp(-x);  // shared by A and Q's direct form
     double a[9];
-    comp_transition<K>(lam, dt, a);
+    comp_transition_e<K>(lam, dt, e, a);
     BN_UNROLL for (int i = 0; i < K; ++i)
       BN_UNROLL for (int j = 0; j < K; ++j) A[O + i][O + j] = (T)a[i * K + j];
-    const double x = lam * dt;
     const bool small = x < kQX0;
-    const double e = exp(-x);
+    // which Q forms the wavefront needs: a form no active lane uses is not evaluated
+    const bool need_s = lanes_any(small), need_p = lanes_any(!small);
     const double e2 = e * e;
     int idx = 0;
     double lp_i = var;  // var * lam^(i + j), built incrementally
     BN_UNROLL for (int i = 0; i < K; ++i) {
       double lp = lp_i;
       BN_UNROLL for (int j = i; j < K; ++j) {
-        // Both forms from coefficient rows every lane of the wave reads at the same
-        // address (scalar loads, independent of the lane's x), fully unrolled, then a
-        // per-lane select.  Small x: F_ij = x^(2K-1-i-j) G_ij(x); else
+        // The forms some lane of the wavefront needs, from coefficient rows every lane
+        // reads at the same address (scalar loads, independent of the lane's x), fully
+        // unrolled, then a per-lane select.  Small x: F_ij = x^(2K-1-i-j) G_ij(x); else
         // F_ij = C_ij - e^{-2x} P_ij(x).  A per-lane choice of table turns every
         // coefficient into a dependent vector load on the recurrence's critical path.
         // The series rows are zero-padded to their declared width, which leaves the
@@ -188,12 +211,18 @@ struct Model {
         const double* ts = qtab<K>(idx, true);
         const double* tp = qtab<K>(idx, false);
         constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
-        double vs = ts[WS - 1];
-        BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = vs * x + ts[n];
-        double vp = tp[WP - 1];
-        BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = vp * x + tp[n];
-        BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) vs *= x;
-        const double v = small ? vs : qc<K>(idx) - e2 * vp;
+        double vs = 0.0, vp = 0.0;
+        if (need_s) {
+          vs = ts[WS - 1];
+          BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = vs * x + ts[n];
+          BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) vs *= x;
+        }
+        if (need_p) {
+          vp = tp[WP - 1];
+          BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = vp * x + tp[n];
+          vp = qc<K>(idx) - e2 * vp;
+        }
+        const double v = small ? vs : vp;
         const T q = (T)(lp * v);
         Q[O + i][O + j] = q;
         Q[O + j][O + i] = q;

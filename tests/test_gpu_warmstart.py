@@ -87,3 +87,30 @@ def test_warm_start_reaches_plain_energy_in_fewer_iterations():
     it_w, e_w, _ = W.run_to_convergence(warm, tol=1e-7)
     assert abs(float(e_w[0]) - float(e_p[0])) <= 1e-5 * abs(float(e_p[0]))
     assert it_w < it_p, (it_w, it_p)
+
+
+def test_shipped_network_loads_and_warm_starts_a_held_out_run():
+    """bayesnewton_amd/pretrained/spacetime_cox_warmstart.npz through the package loader
+    (no pickles): the parameter layout, and on a held-out Cox run (seed 1002, T = 200,
+    R = 256, M = 64, the bench's lr 0.2) the same energy as the plain start in fewer
+    iterations."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, spacetime as S, warmstart as W, workloads
+    net, record = W.WarmStartMLP.pretrained()
+    assert net.params.numel() == W.P and net.norm.numel() == 2 * W.F + 4
+    z = _grid(8)
+
+    def model():
+        d = workloads.spacetime_cox(seed=1002, T=200)
+        kern = K.SpatioTemporalKernel(K.Matern32(variance=1.0, lengthscale=10.0),
+                                      K.Matern32(variance=1.0, lengthscale=0.2), z=z, sparse=True, conditional="Full")
+        return S.SpaceTimeVariationalGP(kern, L.Poisson(binsize=2.0), d["t"], d["y"], d["r"])
+
+    plain, warm = model(), model()
+    it_p, e_p, _ = W.run_to_convergence(plain, tol=1e-7, max_iter=400, lr=0.2)
+    net.init_sites(warm)
+    it_w, e_w, _ = W.run_to_convergence(warm, tol=1e-7, max_iter=400, lr=0.2)
+    print(f"[warmstart] shipped network: plain {it_p} iterations, warm {it_w}; energies {float(e_p[0])!r} "
+          f"{float(e_w[0])!r}", flush=True)
+    assert np.isfinite(float(e_p[0])) and np.isfinite(float(e_w[0]))
+    assert abs(float(e_w[0]) - float(e_p[0])) <= 1e-5 * abs(float(e_p[0]))
+    assert it_w < it_p, (it_w, it_p)
