@@ -69,6 +69,15 @@ BN_HD bool lanes_any(bool p) {
   return p;
 #endif
 }
+// 1 / sqrt(x): the device's reciprocal square root (one refined sequence in place of a
+// square root and a division; within an ulp or two of the two), 1 / sqrt on the host.
+BN_HD double recip_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrt(x);
+#else
+  return 1.0 / std::sqrt(x);
+#endif
+}
 
 // Closed-form A = expm(F dt) of one component (kernels.py:156-163, 216-224, 273-286),
 // given e = exp(-lam dt).
@@ -302,8 +311,8 @@ BN_HD bool chol_spd(double A[D][D]) {
     double s = A[j][j];
     BN_UNROLL for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
     ok = ok && (s > 0.0);
-    double l = sqrt(s);
-    double il = 1.0 / l;
+    double il = recip_sqrt(s);
+    double l = s * il;
     A[j][j] = l;
     BN_UNROLL for (int i = j + 1; i < D; ++i) {
       double t = A[i][j];
@@ -317,22 +326,28 @@ BN_HD bool chol_spd(double A[D][D]) {
 // X <- L^-1 X (forward substitution), L lower FP64, X has D rows and NC columns.
 template <int D, int NC, class T>
 BN_HD void lower_solve(const double L[D][D], T X[D][NC]) {
+  // one division per pivot, not per right-hand side (an FP64 division is a ten
+  // instruction sequence; the product by the reciprocal differs by at most an ulp)
+  double il[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) il[i] = 1.0 / L[i][i];
   BN_UNROLL for (int c = 0; c < NC; ++c)
     BN_UNROLL for (int i = 0; i < D; ++i) {
       double s = (double)X[i][c];
       BN_UNROLL for (int k = 0; k < i; ++k) s -= L[i][k] * (double)X[k][c];
-      X[i][c] = (T)(s / L[i][i]);
+      X[i][c] = (T)(s * il[i]);
     }
 }
 
 // X <- L'^-1 X (backward substitution with L').
 template <int D, int NC, class T>
 BN_HD void upper_t_solve(const double L[D][D], T X[D][NC]) {
+  double il[D];
+  BN_UNROLL for (int i = 0; i < D; ++i) il[i] = 1.0 / L[i][i];
   BN_UNROLL for (int c = 0; c < NC; ++c)
     BN_UNROLL for (int i = D - 1; i >= 0; --i) {
       double s = (double)X[i][c];
       BN_UNROLL for (int k = i + 1; k < D; ++k) s -= L[k][i] * (double)X[k][c];
-      X[i][c] = (T)(s / L[i][i]);
+      X[i][c] = (T)(s * il[i]);
     }
 }
 
@@ -414,54 +429,115 @@ BN_HD void combine_factor(const T Sig1[D][D], const T J2[D][D], T Z[D][D], T Y[D
     BN_UNROLL for (int j = 0; j < D; ++j) { Z[i][j] = (T)Zd[i][j]; Y[i][j] = (T)Yd[i][j]; }
 }
 
-// e = e1 (earlier) followed by e2 (later): parallel_filtering_operator, ops.py:203-219.
+// Scheduling fence between the phases of a long unrolled body: keeps the compiler from
+// hoisting a later phase's loads above it (which lengthens their live ranges).
+BN_HD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+// The earlier operand of fcombine_from held in registers.  An operand type gives the
+// fields of an element by name: sig and jm as full symmetric matrices.
 template <int D, class T>
-BN_HD void fcombine(const FElem<D, T>& e1, const FElem<D, T>& e2, FElem<D, T>& out) {
-  T Z[D][D], Y[D][D];
-  combine_factor<D, T>(e1.Sig, e2.J, Z, Y);
-  // T Phi1 = Phi1 - Z'(Y Phi1)
-  T YP[D][D], TP[D][D];
-  mat_mul<D, T>(Y, e1.Phi, YP);
-  BN_UNROLL for (int i = 0; i < D; ++i)
-    BN_UNROLL for (int j = 0; j < D; ++j) {
-      T s = e1.Phi[i][j];
-      BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * YP[k][j];
-      TP[i][j] = s;
+struct RegOperand {
+  const FElem<D, T>& e;
+  BN_HD void phi(T P[D][D]) const { BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) P[i][j] = e.Phi[i][j]; }
+  BN_HD void beta(T b[D]) const { BN_UNROLL for (int i = 0; i < D; ++i) b[i] = e.beta[i]; }
+  BN_HD void sig(T S[D][D]) const { BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) S[i][j] = e.Sig[i][j]; }
+  BN_HD void jm(T J[D][D]) const { BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int j = 0; j < D; ++j) J[i][j] = e.J[i][j]; }
+  BN_HD void eta(T v[D]) const { BN_UNROLL for (int i = 0; i < D; ++i) v[i] = e.eta[i]; }
+};
+
+// e = e1 (earlier) followed by e2 (later): parallel_filtering_operator, ops.py:203-219.
+// Three phases, each reading only the fields of e1 it needs (through the operand type
+// L), so a caller that streams e1 from memory holds about half an element less:
+//   1. Z, Y from Sig1 and J2; w = beta1 + Sig1 eta2; q = eta2 - J2 beta1;
+//   2. from Phi1, J1, eta1: T Phi1, Tw, eta = Phi1' Ttq + eta1 and
+//      J = Phi1' J2 Phi1 - (Y Phi1)'(Y Phi1) + J1 (e1 and J2 are then dead);
+//   3. beta = Phi2 Tw + beta2, Phi = Phi2 T Phi1, Sig = (Phi2 Z')(Phi2 Z')' + Sig2.
+// out must not alias e2.
+template <int D, class T, class L>
+BN_HD void fcombine_from(const L& e1, const FElem<D, T>& e2, FElem<D, T>& out) {
+  T Z[D][D], Y[D][D], w[D], q[D];
+  {
+    T Sig1[D][D], beta1[D];
+    e1.sig(Sig1);
+    e1.beta(beta1);
+    combine_factor<D, T>(Sig1, e2.J, Z, Y);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = beta1[i];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += Sig1[i][k] * e2.eta[k];
+      w[i] = s;
+      T u = e2.eta[i];
+      BN_UNROLL for (int k = 0; k < D; ++k) u -= e2.J[i][k] * beta1[k];
+      q[i] = u;
     }
-  // w = beta1 + Sig1 eta2 ; Tw = w - Z'(Y w)
-  T w[D], yw[D], Tw[D];
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = e1.beta[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Sig[i][k] * e2.eta[k];
-    w[i] = s;
   }
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = T(0);
-    BN_UNROLL for (int k = 0; k < D; ++k) s += Y[i][k] * w[k];
-    yw[i] = s;
+  sched_fence();
+  T TP[D][D], Tw[D];
+  {
+    T Phi1[D][D];
+    e1.phi(Phi1);
+    // T Phi1 = Phi1 - Z'(Y Phi1)
+    T YP[D][D];
+    mat_mul<D, T>(Y, Phi1, YP);
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = 0; j < D; ++j) {
+        T s = Phi1[i][j];
+        BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * YP[k][j];
+        TP[i][j] = s;
+      }
+    // Tw = w - Z'(Y w)
+    T yw[D];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += Y[i][k] * w[k];
+      yw[i] = s;
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = w[i];
+      BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * yw[k];
+      Tw[i] = s;
+    }
+    // Ttq = q - Y'(Z q) ; eta = Phi1' Ttq + eta1
+    T zq[D], Ttq[D], eta1[D];
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = T(0);
+      BN_UNROLL for (int k = 0; k < D; ++k) s += Z[i][k] * q[k];
+      zq[i] = s;
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = q[i];
+      BN_UNROLL for (int k = 0; k < D; ++k) s -= Y[k][i] * zq[k];
+      Ttq[i] = s;
+    }
+    e1.eta(eta1);
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      T s = eta1[i];
+      BN_UNROLL for (int k = 0; k < D; ++k) s += Phi1[k][i] * Ttq[k];
+      out.eta[i] = s;
+    }
+    // J = Phi1' J2 Phi1 - (Y Phi1)'(Y Phi1) + J1
+    T JP[D][D], J1[D][D];
+    mat_mul<D, T>(e2.J, Phi1, JP);
+    e1.jm(J1);
+    BN_UNROLL for (int i = 0; i < D; ++i)
+      BN_UNROLL for (int j = i; j < D; ++j) {
+        T s = J1[i][j];
+        BN_UNROLL for (int k = 0; k < D; ++k) s += Phi1[k][i] * JP[k][j] - YP[k][i] * YP[k][j];
+        out.J[i][j] = s;
+        out.J[j][i] = s;
+      }
   }
+  sched_fence();
+  // beta = Phi2 Tw + beta2 ; Phi = Phi2 TP
   BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = w[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s -= Z[k][i] * yw[k];
-    Tw[i] = s;
+    T s = e2.beta[i];
+    BN_UNROLL for (int k = 0; k < D; ++k) s += e2.Phi[i][k] * Tw[k];
+    out.beta[i] = s;
   }
-  // q = eta2 - J2 beta1 ; Ttq = q - Y'(Z q)
-  T q[D], zq[D], Ttq[D];
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = e2.eta[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s -= e2.J[i][k] * e1.beta[k];
-    q[i] = s;
-  }
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = T(0);
-    BN_UNROLL for (int k = 0; k < D; ++k) s += Z[i][k] * q[k];
-    zq[i] = s;
-  }
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = q[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s -= Y[k][i] * zq[k];
-    Ttq[i] = s;
-  }
+  mat_mul<D, T>(e2.Phi, TP, out.Phi);
   // R = Phi2 Z' ; Sig = R R' + Sig2
   T R[D][D];
   BN_UNROLL for (int i = 0; i < D; ++i)
@@ -477,29 +553,11 @@ BN_HD void fcombine(const FElem<D, T>& e1, const FElem<D, T>& e2, FElem<D, T>& o
       out.Sig[i][j] = s;
       out.Sig[j][i] = s;
     }
-  // beta = Phi2 Tw + beta2 ; Phi = Phi2 TP
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = e2.beta[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s += e2.Phi[i][k] * Tw[k];
-    out.beta[i] = s;
-  }
-  mat_mul<D, T>(e2.Phi, TP, out.Phi);
-  // eta = Phi1' Ttq + eta1
-  BN_UNROLL for (int i = 0; i < D; ++i) {
-    T s = e1.eta[i];
-    BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Phi[k][i] * Ttq[k];
-    out.eta[i] = s;
-  }
-  // J = Phi1' J2 Phi1 - (Y Phi1)'(Y Phi1) + J1
-  T JP[D][D];
-  mat_mul<D, T>(e2.J, e1.Phi, JP);
-  BN_UNROLL for (int i = 0; i < D; ++i)
-    BN_UNROLL for (int j = i; j < D; ++j) {
-      T s = e1.J[i][j];
-      BN_UNROLL for (int k = 0; k < D; ++k) s += e1.Phi[k][i] * JP[k][j] - YP[k][i] * YP[k][j];
-      out.J[i][j] = s;
-      out.J[j][i] = s;
-    }
+}
+
+template <int D, class T>
+BN_HD void fcombine(const FElem<D, T>& e1, const FElem<D, T>& e2, FElem<D, T>& out) {
+  fcombine_from<D, T>(RegOperand<D, T>{e1}, e2, out);
 }
 
 // (m, P) <- state (m, P) followed by element e: the combine with Phi1 = 0.

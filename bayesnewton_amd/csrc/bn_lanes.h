@@ -45,6 +45,9 @@ struct PlanView {
   const int32_t* series_ntiles;
   const int32_t* series_first_chunk;
   const int32_t* series_nchunks;
+  int32_t n_groups;            // scan groups of the tile scans
+  const int32_t* group_tiles;  // kGroupMax tiles per group, -1 for an empty place
+  const uint8_t* group_w;      // width of each group: its tiles have at most kTile / w chunks
 
   BN_HD int64_t slot(int32_t j, int32_t k) const {
     return (((int64_t)(j >> gshift) * chunk_len + k) << gshift) + (j & ((1 << gshift) - 1));
@@ -105,6 +108,26 @@ template <class E>
 BN_HD void soa_load(E& e, CSoA s, int64_t i) {
   e.load([&](int f) { return s.base[(int64_t)f * s.stride + i]; });
 }
+
+// The earlier operand of fcombine_from read from SoA storage (FElem's packed order:
+// Phi, beta, Sig upper, J upper, eta) field group by field group, when the combine's
+// phase needs it, instead of whole before the combine.
+template <int D, class T>
+struct SoaOperand {
+  CSoA s;
+  int64_t i;
+  static constexpr int NS = D * (D + 1) / 2;
+  BN_HD T at(int f) const { return (T)s.base[(int64_t)f * s.stride + i]; }
+  BN_HD void phi(T P[D][D]) const { BN_UNROLL for (int r = 0; r < D; ++r) BN_UNROLL for (int c = 0; c < D; ++c) P[r][c] = at(r * D + c); }
+  BN_HD void beta(T b[D]) const { BN_UNROLL for (int r = 0; r < D; ++r) b[r] = at(D * D + r); }
+  BN_HD void sym(int f0, T S[D][D]) const {
+    int f = f0;
+    BN_UNROLL for (int r = 0; r < D; ++r) BN_UNROLL for (int c = r; c < D; ++c) { const T v = at(f++); S[r][c] = v; S[c][r] = v; }
+  }
+  BN_HD void sig(T S[D][D]) const { sym(D * D + D, S); }
+  BN_HD void jm(T J[D][D]) const { sym(D * D + D + NS, J); }
+  BN_HD void eta(T v[D]) const { BN_UNROLL for (int r = 0; r < D; ++r) v[r] = at(D * D + D + 2 * NS + r); }
+};
 
 // Where the filtered moments go / come from.  packed: D mean planes then NS
 // packed-upper covariance planes; full: D mean planes + D*D covariance planes.

@@ -60,61 +60,78 @@ __global__ __launch_bounds__(kBlock, 2) void k_filter_reduce(PlanView pl, HyperV
 }
 
 // Tile scan with two elements per thread: each thread combines its pair, a
-// Hillis-Steele scan runs over the <= 64 pair totals in LDS, and the first element
-// of each pair takes the previous pair's prefix.  About 3.5 combines per element
-// instead of log2(128) = 7 for a scan over the elements themselves.
+// Hillis-Steele scan runs over the pair totals in LDS, and the first element of each
+// pair takes the previous pair's prefix.  About 3.5 combines per element instead of
+// log2(128) = 7 for a scan over the elements themselves.  A workgroup scans one scan
+// group of the plan: up to 8 short tiles side by side, each on its own slice of the
+// lanes and of the LDS pair slots (most series fit in one short tile, and packing them
+// cuts the workgroups, which run one wavefront per SIMD, from one per tile).
 template <int D>
 __global__ __launch_bounds__(kScan / 2) void k_fscan_tile(PlanView pl, SoA agg, SoA tagg) {
   extern __shared__ agg_t lds[];
   using E = FElem<D, CT>;
   constexpr int NP = kScan / 2;
-  const int32_t t = blockIdx.x;
-  const int32_t tid = threadIdx.x;
-  const int32_t first = pl.tile_first[t];
-  const int32_t count = pl.tile_count[t];
-  if ((pl.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && count == 1) return;
-  const int32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+  const int32_t g = blockIdx.x;
+  const int32_t w = pl.group_w[g];
+  const int32_t lp = NP / w;                // lanes (pair slots) per tile
+  const int32_t h = threadIdx.x / lp;       // place of this lane's tile in the group
+  const int32_t tid = threadIdx.x - h * lp;  // lane within the tile
+  const int32_t base = h * lp;               // the tile's first LDS pair slot
+  const int32_t t = pl.group_tiles[g * kGroupMax + h];
+  const int32_t first = t >= 0 ? pl.tile_first[t] : 0;
+  const int32_t count = t >= 0 ? pl.tile_count[t] : 0;
   const int32_t np = (count + 1) / 2;
-  E e0, pr;
-  if (i0 < count) soa_load(e0, CSoA{agg.base, agg.stride}, first + i0);
-  else e0.set_identity_elem();
+  int32_t npmax = 0;  // every lane runs the group's longest scan (the barriers are shared)
+  for (int32_t u = 0; u < w; ++u) {
+    const int32_t tu = pl.group_tiles[g * kGroupMax + u];
+    if (tu >= 0) npmax = max(npmax, (pl.tile_count[tu] + 1) / 2);
+  }
+  const int32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+  const CSoA ag{agg.base, agg.stride};
+  E pr;
   if (i1 < count) {
     E e1;
-    soa_load(e1, CSoA{agg.base, agg.stride}, first + i1);
-    fcombine<D, CT>(e0, e1, pr);
+    soa_load(e1, ag, first + i1);
+    fcombine_from<D, CT>(SoaOperand<D, CT>{ag, first + i0}, e1, pr);
+  } else if (i0 < count) {
+    soa_load(pr, ag, first + i0);
   } else {
-    pr = e0;
+    pr.set_identity_elem();
   }
   const SoA ls{lds, NP};
   const CSoA lc{lds, NP};
-  for (int32_t off = 1; off < np; off <<= 1) {
-    soa_store(pr, ls, tid);
+  for (int32_t off = 1; off < npmax; off <<= 1) {
+    soa_store(pr, ls, base + tid);
     __syncthreads();
     if (tid >= off && tid < np) {
-      E p, o;
-      soa_load(p, lc, tid - off);
-      fcombine<D, CT>(p, pr, o);
+      E o;
+      fcombine_from<D, CT>(SoaOperand<D, CT>{lc, base + tid - off}, pr, o);
       pr = o;
     }
     __syncthreads();
   }
-  soa_store(pr, ls, tid);
+  soa_store(pr, ls, base + tid);
   __syncthreads();
-  if (i0 < count) {
-    E out0;
-    if (tid == 0) {
-      out0 = e0;
-    } else {
-      E p;
-      soa_load(p, lc, tid - 1);
-      fcombine<D, CT>(p, e0, out0);
-    }
-    soa_store(out0, agg, first + i0);
-    if (i0 == count - 1) soa_store(out0, tagg, t);
-  }
+  // the pair's second output is the inclusive prefix itself; store it first so that
+  // the prefix is no longer live during the first output's combine
   if (i1 < count) {
     soa_store(pr, agg, first + i1);
     if (i1 == count - 1) soa_store(pr, tagg, t);
+  }
+  if (i0 < count) {
+    // the pair's first element is read again here rather than held through the scan
+    // (an element is 65 doubles); the opaque index keeps the load below the scan
+    int32_t at = first + i0;
+    asm volatile("" : "+v"(at));
+    E e0, out0;
+    soa_load(e0, ag, at);
+    if (tid == 0) {
+      out0 = e0;
+    } else {
+      fcombine_from<D, CT>(SoaOperand<D, CT>{lc, base + tid - 1}, e0, out0);
+    }
+    soa_store(out0, agg, first + i0);
+    if (i0 == count - 1) soa_store(out0, tagg, t);
   }
 }
 
@@ -136,9 +153,8 @@ __global__ __launch_bounds__(kScan) void k_fscan_series(PlanView pl, SoA tagg, S
     soa_store(e, ls, tid);
     __syncthreads();
     if (tid >= off && tid < nt) {
-      E p, o;
-      soa_load(p, lc, tid - off);
-      fcombine<D, CT>(p, e, o);
+      E o;
+      fcombine_from<D, CT>(SoaOperand<D, CT>{lc, tid - off}, e, o);
       e = o;
     }
     __syncthreads();
@@ -164,33 +180,45 @@ __global__ __launch_bounds__(kBlock) void k_smoother_reduce(PlanView pl, HyperVi
   lane_smoother_reduce<K1, K2, CT>(pl, hv, dts, fin, sagg, j);
 }
 
+// Suffix scan of the smoother elements of each tile, one element per thread, over the
+// scan groups of the plan as in k_fscan_tile (a tile of a group of width w on
+// kScan / w lanes).
 template <int D>
 __global__ __launch_bounds__(kScan) void k_sscan_tile(PlanView pl, SoA agg, SoA tagg) {
   extern __shared__ agg_t lds[];
   using E = SElem<D, CT>;
-  const int32_t t = blockIdx.x;
-  const int32_t tid = threadIdx.x;
-  const int32_t first = pl.tile_first[t];
-  const int32_t count = pl.tile_count[t];
-  if ((pl.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && count == 1) return;
+  const int32_t g = blockIdx.x;
+  const int32_t w = pl.group_w[g];
+  const int32_t lp = kScan / w;
+  const int32_t h = threadIdx.x / lp;
+  const int32_t tid = threadIdx.x - h * lp;
+  const int32_t base = h * lp;
+  const int32_t t = pl.group_tiles[g * kGroupMax + h];
+  const int32_t first = t >= 0 ? pl.tile_first[t] : 0;
+  const int32_t count = t >= 0 ? pl.tile_count[t] : 0;
+  int32_t cmax = 0;  // every lane runs the group's longest scan (the barriers are shared)
+  for (int32_t u = 0; u < w; ++u) {
+    const int32_t tu = pl.group_tiles[g * kGroupMax + u];
+    if (tu >= 0) cmax = max(cmax, pl.tile_count[tu]);
+  }
   E e;
   if (tid < count) soa_load(e, CSoA{agg.base, agg.stride}, first + tid);
   else e.set_identity_elem();
   const SoA ls{lds, kScan};
   const CSoA lc{lds, kScan};
-  for (int32_t off = 1; off < count; off <<= 1) {
-    soa_store(e, ls, tid);
+  for (int32_t off = 1; off < cmax; off <<= 1) {
+    soa_store(e, ls, base + tid);
     __syncthreads();
     if (tid + off < count) {
       E p, o;
-      soa_load(p, lc, tid + off);
+      soa_load(p, lc, base + tid + off);
       scombine<D, CT>(e, p, o);
       e = o;
     }
     __syncthreads();
   }
   if (tid < count) soa_store(e, agg, first + tid);
-  if (tid == 0) soa_store(e, tagg, t);
+  if (tid == 0 && count > 0) soa_store(e, tagg, t);
 }
 
 template <int D>
@@ -718,7 +746,8 @@ void launch_filter_scan(const bn_plan* plan, HyperView hv, const float* dt, cons
   { BN_TIMED(KID_FILTER_REDUCE, st);
   hipLaunchKernelGGL((k_filter_reduce<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, b.fagg); }
   { BN_TIMED(KID_FSCAN_TILE, st);
-  hipLaunchKernelGGL((k_fscan_tile<D>), dim3(p.n_tiles), dim3(kScan / 2), ldsF / 2, st, plan->view, b.fagg,
+  if (plan->view.n_groups > 0)
+    hipLaunchKernelGGL((k_fscan_tile<D>), dim3(plan->view.n_groups), dim3(kScan / 2), ldsF / 2, st, plan->view, b.fagg,
                      b.ftagg); }
   { BN_TIMED(KID_FSCAN_SERIES, st);
   hipLaunchKernelGGL((k_fscan_series<D>), dim3(p.n_series), dim3(kScan), ldsF, st, plan->view, b.ftagg, b.ftcarry); }
@@ -730,7 +759,9 @@ void launch_smoother_scan(const bn_plan* plan, const Bufs& b, hipStream_t st) {
   if (!needs_scan(p)) return;
   const size_t ldsS = sizeof(agg_t) * SElem<D, CT>::NF * kScan;
   { BN_TIMED(KID_SSCAN_TILE, st);
-  hipLaunchKernelGGL((k_sscan_tile<D>), dim3(p.n_tiles), dim3(kScan), ldsS, st, plan->view, b.sagg, b.stagg); }
+  if (plan->view.n_groups > 0)
+    hipLaunchKernelGGL((k_sscan_tile<D>), dim3(plan->view.n_groups), dim3(kScan), ldsS, st, plan->view, b.sagg,
+                       b.stagg); }
   { BN_TIMED(KID_SSCAN_SERIES, st);
   hipLaunchKernelGGL((k_sscan_series<D>), dim3(p.n_series), dim3(kScan), ldsS, st, plan->view, b.stagg, b.stcarry); }
 }
@@ -1178,7 +1209,8 @@ int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_
       {h.tile_count.data(), h.tile_count.size() * 4, 0},     {h.tile_series.data(), h.tile_series.size() * 4, 0},
       {h.tile_flags.data(), h.tile_flags.size(), 0},         {h.series_first_tile.data(), h.series_first_tile.size() * 4, 0},
       {h.series_ntiles.data(), h.series_ntiles.size() * 4, 0}, {h.series_first_chunk.data(), h.series_first_chunk.size() * 4, 0},
-      {h.series_nchunks.data(), h.series_nchunks.size() * 4, 0}, {nullptr, 0, 0}, {nullptr, 0, 0}};
+      {h.series_nchunks.data(), h.series_nchunks.size() * 4, 0}, {h.group_tiles.data(), h.group_tiles.size() * 4, 0},
+      {h.group_w.data(), h.group_w.size(), 0}};
   size_t total = 0;
   for (auto& pc : pcs) {
     pc.off = total;
@@ -1221,6 +1253,9 @@ int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_
   v.series_ntiles = (const int32_t*)(d + pcs[10].off);
   v.series_first_chunk = (const int32_t*)(d + pcs[11].off);
   v.series_nchunks = (const int32_t*)(d + pcs[12].off);
+  v.n_groups = (int32_t)h.group_w.size();
+  v.group_tiles = (const int32_t*)(d + pcs[13].off);
+  v.group_w = (const uint8_t*)(d + pcs[14].off);
   *out = p;
   return BN_OK;
 }

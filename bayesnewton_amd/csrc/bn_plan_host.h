@@ -20,8 +20,39 @@ struct PlanHost {
   std::vector<int32_t> series_first_tile, series_ntiles, series_first_chunk, series_nchunks;
   std::vector<uint8_t> chunk_flags, tile_flags;
   std::vector<int64_t> chunk_start;
+  // Scan groups of the tile scans: workgroup g scans the up to kGroupMax tiles
+  // group_tiles[g * kGroupMax + h] (-1 for an empty place) side by side, each on
+  // 1 / group_w[g] of the lanes, so a tile of a group of width w has at most kTile / w
+  // chunks.  Tiles of a one-chunk series need no scan and are in no group.
+  std::vector<int32_t> group_tiles;
+  std::vector<uint8_t> group_w;
 };
 
+constexpr int kGroupMax = 8;
+
+// Packs the tiles into scan groups by size: width 8 for tiles of at most kTile / 8
+// chunks, then 4 and 2, width 1 for the rest.  The width-1 groups (the deepest scans)
+// come first so they start in the first round of workgroups.
+inline void build_scan_groups(PlanHost& p) {
+  p.group_tiles.clear();
+  p.group_w.clear();
+  std::vector<int32_t> bucket[4];  // widths 1, 2, 4, 8
+  for (int32_t t = 0; t < (int32_t)p.tile_count.size(); ++t) {
+    const int32_t c = p.tile_count[t];
+    if (p.tile_flags[t] == 3 && c == 1) continue;  // first and last tile of a one-chunk series
+    int b = 0;
+    while (b < 3 && c <= kTile / (2 << b)) ++b;
+    bucket[b].push_back(t);
+  }
+  for (int b = 0; b < 4; ++b) {
+    const int w = 1 << b;
+    for (size_t i = 0; i < bucket[b].size(); i += w) {
+      for (int h = 0; h < kGroupMax; ++h)
+        p.group_tiles.push_back(h < w && i + h < bucket[b].size() ? bucket[b][i + h] : -1);
+      p.group_w.push_back((uint8_t)w);
+    }
+  }
+}
 inline bool build_plan(const int64_t* offsets, int32_t n_series, int32_t chunk_len, PlanHost& p, std::string& err) {
   if (n_series < 1) { err = "n_series must be >= 1"; return false; }
   if (chunk_len < 1 || chunk_len > (1 << 24)) { err = "chunk_len must be in [1, 2^24]"; return false; }
@@ -69,6 +100,7 @@ inline bool build_plan(const int64_t* offsets, int32_t n_series, int32_t chunk_l
   }
   p.n_chunks = (int32_t)p.chunk_series.size();
   p.n_tiles = (int32_t)p.tile_first.size();
+  build_scan_groups(p);
   // Empty series inside a batch are fine (no chunk, log-likelihood 0); a batch
   // with no step at all would launch empty grids, so it is refused here.
   if (p.n_chunks == 0) { err = "offsets describe no steps (every series is empty)"; return false; }

@@ -62,6 +62,9 @@ PlanView view_of(const PlanHost& p) {
   v.series_ntiles = p.series_ntiles.data();
   v.series_first_chunk = p.series_first_chunk.data();
   v.series_nchunks = p.series_nchunks.data();
+  v.n_groups = (int32_t)p.group_w.size();
+  v.group_tiles = p.group_tiles.data();
+  v.group_w = p.group_w.data();
   return v;
 }
 
@@ -385,6 +388,38 @@ int main() {
       for (int32_t b = 0; b < p.n_series; ++b)
         ok = ok && (p.series_nchunks[b] == 0) == (off[b + 1] == off[b]);
       check(ok, "plan slots cover every step once (chunk_len " + std::to_string(cl) + ")");
+      // scan groups: every tile that needs a scan in exactly one group, no other tile in
+      // any, a tile of a group of width w holds at most kTile / w chunks
+      bool gok = p.group_tiles.size() == (size_t)kGroupMax * p.group_w.size() && v.n_groups == (int32_t)p.group_w.size();
+      std::vector<int> in_group((size_t)p.n_tiles, 0);
+      for (size_t g = 0; gok && g < p.group_w.size(); ++g) {
+        const int w = p.group_w[g];
+        gok = gok && (w == 1 || w == 2 || w == 4 || w == 8);
+        for (int h = 0; gok && h < kGroupMax; ++h) {
+          const int32_t t = p.group_tiles[g * kGroupMax + h];
+          if (t < 0) continue;
+          gok = gok && h < w && t < p.n_tiles && p.tile_count[t] <= kTile / w;
+          if (gok) ++in_group[(size_t)t];
+        }
+      }
+      for (int32_t t = 0; gok && t < p.n_tiles; ++t) {
+        const bool one = (p.tile_flags[t] & (TL_FIRST | TL_LAST)) == (TL_FIRST | TL_LAST) && p.tile_count[t] == 1;
+        gok = in_group[(size_t)t] == (one ? 0 : 1);
+      }
+      check(gok, "scan groups cover every tile once (chunk_len " + std::to_string(cl) + ")");
+    }
+    // many short series pack eight tiles to a group, long ones one
+    {
+      std::vector<int64_t> o2(41, 0);
+      for (int b = 1; b <= 40; ++b) o2[b] = o2[b - 1] + (b <= 36 ? 10 : 1000);
+      check(build_plan(o2.data(), 40, 1, p, err), "plan built (mixed lengths)");
+      int n8 = 0, n1 = 0;
+      for (uint8_t w : p.group_w) { n8 += w == 8; n1 += w == 1; }
+      std::printf("mixed lengths: %d tiles in %zu groups (%d of width 8, %d of width 1)\n", p.n_tiles, p.group_w.size(),
+                  n8, n1);
+      // 36 one-tile series of 10 chunks (width 8: 5 groups), 4 series of 8 tiles whose
+      // last has 104 chunks (width 1)
+      check(n8 == 5 && n1 == 32 && p.group_w.size() == 37, "short series share groups");
     }
   }
   std::printf(g_fail ? "LANES_HOST_FAILED %d\n" : "LANES_HOST_OK\n", g_fail);
