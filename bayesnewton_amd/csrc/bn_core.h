@@ -69,6 +69,18 @@ BN_HD bool lanes_any(bool p) {
   return p;
 #endif
 }
+// a * x + c for a wave-uniform c (held in scalar registers): one VOP3 FMA reading c
+// from its scalar registers, where the compiler's own choice copies c into vector
+// registers first to use the two-operand accumulate form.
+BN_HD double fma_uniform_addend(double a, double x, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(x), "s"(c));
+  return r;
+#else
+  return std::fma(a, x, c);
+#endif
+}
 // 1 / sqrt(x): the device's reciprocal square root (one refined sequence in place of a
 // square root and a division; within an ulp or two of the two), 1 / sqrt on the host.
 BN_HD double recip_sqrt(double x) {
@@ -205,6 +217,13 @@ struct Model {
     // which Q forms the wavefront needs: a form no active lane uses is not evaluated
     const bool need_s = lanes_any(small), need_p = lanes_any(!small);
     const double e2 = e * e;
+    // An opaque zero offset keeps the coefficient loads at their use: hoisted out of a
+    // caller's step loop, the tables outgrow the scalar registers and spill into
+    // vector-register lanes, read back with one v_readlane per use.
+    int zo = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(zo));
+#endif
     int idx = 0;
     double lp_i = var;  // var * lam^(i + j), built incrementally
     BN_UNROLL for (int i = 0; i < K; ++i) {
@@ -217,18 +236,18 @@ struct Model {
         // coefficient into a dependent vector load on the recurrence's critical path.
         // The series rows are zero-padded to their declared width, which leaves the
         // Horner value unchanged.
-        const double* ts = qtab<K>(idx, true);
-        const double* tp = qtab<K>(idx, false);
+        const double* ts = qtab<K>(idx, true) + zo;
+        const double* tp = qtab<K>(idx, false) + zo;
         constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
         double vs = 0.0, vp = 0.0;
         if (need_s) {
           vs = ts[WS - 1];
-          BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = vs * x + ts[n];
+          BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = fma_uniform_addend(vs, x, ts[n]);
           BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) vs *= x;
         }
         if (need_p) {
           vp = tp[WP - 1];
-          BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = vp * x + tp[n];
+          BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = fma_uniform_addend(vp, x, tp[n]);
           vp = qc<K>(idx) - e2 * vp;
         }
         const double v = small ? vs : vp;
