@@ -530,8 +530,27 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
   double ell = 0.0;
   int32_t st = 0;
   auto pk = [](int i, int c) { return i <= c ? S::sidx(i, c) : S::sidx(c, i); };
+  // the next step's inputs are loaded one step ahead, off the recurrence's critical path
+  float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;
+  bool nx_masked = false;
+  if (L > 0) {
+    const int64_t s0 = pl.slot(j, 0);
+    nx_dt = dt[s0];
+    nx_y = y[s0];
+    nx_r = r[s0];
+    nx_masked = mask != nullptr && mask[s0];
+  }
   for (int32_t k = 0; k < L; ++k) {
     const int64_t s = pl.slot(j, k);
+    const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+    const bool masked = nx_masked;
+    if (k + 1 < L) {
+      const int64_t s1 = pl.slot(j, k + 1);
+      nx_dt = dt[s1];
+      nx_y = y[s1];
+      nx_r = r[s1];
+      nx_masked = mask != nullptr && mask[s1];
+    }
     if (k == 0 && (fl & CH_FIRST)) {
       BN_UNROLL for (int i = 0; i < D; ++i) {
         m[i] = T(0);
@@ -540,7 +559,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     } else {
       // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'
       T A[D][D];
-      transition_only<K1, K2, T>(mdl, dt[s], A);
+      transition_only<K1, K2, T>(mdl, dtk, A);
       T nm[D];
       BN_UNROLL for (int i = 0; i < D; ++i) {
         T t = T(0);
@@ -583,7 +602,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
       hP[c] = t;
     }
-    double Sd = (double)r[s];
+    double Sd = (double)rk;
     T hm = T(0);
     BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
     if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
@@ -591,11 +610,11 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
       out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
     }
-    const double vd = (double)y[s] - (double)hm;
+    const double vd = (double)yk - (double)hm;
     const double iSd = 1.0 / Sd;
     const T vS = (T)(vd * iSd);
     const T iS = (T)iSd;
-    if (!(mask != nullptr && mask[s])) ell += -0.5 * (vd * vd * iSd + kLog2Pi + log(Sd));
+    if (!masked) ell += -0.5 * (vd * vd * iSd + kLog2Pi + log(Sd));
     if (want_s && k > 0) {
       // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
       T ch[D];
@@ -898,25 +917,30 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     back((double)dt[sn], lt, Lt);
   }
   bool smooth_ok = true;
-  float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;  // next (earlier) step's inputs, loaded one step ahead
+  // the next (earlier) step's inputs and filter rows, loaded one step ahead
+  float nx_dt = 0.f, nx_y = 0.f, nx_r = 0.f;
+  hrow_t nx_h[D + 1];
+  BN_UNROLL for (int c = 0; c <= D; ++c) nx_h[c] = hrow_t(0);
   if (L > 0) {
     const int64_t s0 = pl.slot(j, L - 1);
     nx_dt = dt[s0];
     nx_y = y[s0];
     nx_r = r[s0];
+    BN_UNROLL for (int c = 0; c <= D; ++c) nx_h[c] = hrow[c * ns + s0];
   }
   for (int32_t k = L - 1; k >= 0; --k) {
     const int64_t s = pl.slot(j, k);
     const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+    T hP[D], kv[D];
+    BN_UNROLL for (int c = 0; c < D; ++c) hP[c] = (T)nx_h[c];
+    const T hm = (T)nx_h[D];
     if (k > 0) {
       const int64_t s1 = pl.slot(j, k - 1);
       nx_dt = dt[s1];
       nx_y = y[s1];
       nx_r = r[s1];
+      BN_UNROLL for (int c = 0; c <= D; ++c) nx_h[c] = hrow[c * ns + s1];
     }
-    T hP[D], kv[D];
-    BN_UNROLL for (int c = 0; c < D; ++c) hP[c] = (T)hrow[c * ns + s];
-    const T hm = (T)hrow[D * ns + s];
     T hph = T(0);
     BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) hph += hP[i];
     const double Sd = (double)rk + (double)hph;

@@ -162,11 +162,12 @@ __global__ __launch_bounds__(kScan) void k_fscan_series(PlanView pl, SoA tagg, S
   if (tid < nt - 1) soa_store(e, carry, ft + tid + 1);
 }
 
+// Two wavefronts per SIMD up to state dimension 5 (the one-step-ahead input loads fit
+// in 256 registers with a few spills); larger states keep the register file to one.
 template <int K1, int K2>
-__global__ __launch_bounds__(kBlock) void k_filter_pass(PlanView pl, HyperView hv, const float* __restrict__ dt,
-                                                        const float* __restrict__ y, const float* __restrict__ r,
-                                                        const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry,
-                                                        FilterPassOut out) {
+__global__ __launch_bounds__(kBlock, (K1 + K2 <= 5 ? 2 : 1)) void k_filter_pass(
+    PlanView pl, HyperView hv, const float* __restrict__ dt, const float* __restrict__ y, const float* __restrict__ r,
+    const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry, FilterPassOut out) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
   lane_filter_pass<K1, K2, CT>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
