@@ -35,6 +35,7 @@ namespace bn {
 #endif
 
 static constexpr double kLog2Pi = 1.8378770664093453;
+static constexpr double kLn2 = 0.6931471805599453;
 #ifndef BN_SEMICHOL_RTOL
 #define BN_SEMICHOL_RTOL 1e-9
 #endif

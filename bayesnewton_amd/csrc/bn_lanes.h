@@ -527,7 +527,10 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
   }
   const bool want_s = out.sagg.base != nullptr;
   T mu[D], V[NS], C[D][D];  // fixed-point smoother of x_s
-  double ell = 0.0;
+  // log S is summed as the log of a running product kept as mantissa * 2^exponent
+  // (two frexp per step in place of a log; the mantissa is renormalised every step)
+  double ell = 0.0, s_mant = 1.0;
+  int32_t s_exp = 0;
   int32_t st = 0;
   auto pk = [](int i, int c) { return i <= c ? S::sidx(i, c) : S::sidx(c, i); };
   // the next step's inputs are loaded one step ahead, off the recurrence's critical path
@@ -614,7 +617,12 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     const double iSd = 1.0 / Sd;
     const T vS = (T)(vd * iSd);
     const T iS = (T)iSd;
-    if (!masked) ell += -0.5 * (vd * vd * iSd + kLog2Pi + log(Sd));
+    if (!masked) {
+      ell += -0.5 * (vd * vd * iSd + kLog2Pi);
+      int e1, e2;
+      s_mant = frexp(s_mant * frexp(Sd, &e1), &e2);
+      s_exp += e1 + e2;
+    }
     if (want_s && k > 0) {
       // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
       T ch[D];
@@ -686,6 +694,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     }
     soa_store(acc, out.sagg, j);
   }
+  ell -= 0.5 * (log(s_mant) + (double)s_exp * kLn2);
   out.ell[j] = ell;
   if (out.status != nullptr) {
     bool fin = finite_d(ell);
@@ -944,8 +953,9 @@ BN_HD void lane_smoother_mbf(const PlanView& pl, const HyperView& hv, const floa
     T hph = T(0);
     BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) hph += hP[i];
     const double Sd = (double)rk + (double)hph;
-    const T iS = (T)(1.0 / Sd);
-    const T vS = (T)(((double)yk - (double)hm) / Sd);
+    const double iSd = 1.0 / Sd;
+    const T iS = (T)iSd;
+    const T vS = (T)(((double)yk - (double)hm) * iSd);
     BN_UNROLL for (int i = 0; i < D; ++i) kv[i] = hP[i] * iS;
     T kl = T(0);
     BN_UNROLL for (int i = 0; i < D; ++i) kl += kv[i] * lh[i];
