@@ -313,13 +313,14 @@ BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const flo
     const T v = (T)((double)yk - (double)hb);
     const T iSf = (T)iS;
     const T vS = v * iSf;
+    // rank-one updates with the 1 / S folded into one factor first (one FMA per entry)
     BN_UNROLL for (int i = 0; i < D; ++i) {
       eta[i] += u[i] * vS;
-      const T ki = hP[i] * iSf;
+      const T ki = hP[i] * iSf, ui = u[i] * iSf;
       beta[i] += ki * v;
       BN_UNROLL for (int c = i; c < D; ++c) {
-        J[S::sidx(i, c)] += u[i] * u[c] * iSf;
-        Sig[S::sidx(i, c)] -= hP[i] * hP[c] * iSf;
+        J[S::sidx(i, c)] += ui * u[c];
+        Sig[S::sidx(i, c)] -= ki * hP[c];
       }
       BN_UNROLL for (int c = 0; c < D; ++c) Phi[i][c] -= ki * u[c];
     }
@@ -633,13 +634,15 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       }
       BN_UNROLL for (int i = 0; i < D; ++i) {
         mu[i] += ch[i] * vS;
-        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ch[i] * ch[c] * iS;
-        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ch[i] * hP[c] * iS;
+        const T ci = ch[i] * iS;
+        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ci * ch[c];
+        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ci * hP[c];
       }
     }
     BN_UNROLL for (int i = 0; i < D; ++i) {
       m[i] += hP[i] * vS;
-      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= hP[i] * hP[c] * iS;
+      const T ki = hP[i] * iS;
+      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= ki * hP[c];
     }
     if (want_s && k == 0) {
       BN_UNROLL for (int i = 0; i < D; ++i) {
