@@ -225,34 +225,49 @@ struct Model {
 #if defined(__HIP_DEVICE_COMPILE__)
     asm volatile("" : "+s"(zo));
 #endif
+    // The forms some lane of the wavefront needs, from coefficient rows every lane reads
+    // at the same address (scalar loads, independent of the lane's x), fully unrolled,
+    // then a per-lane select.  Small x: F_ij = x^(2K-1-i-j) G_ij(x); else
+    // F_ij = C_ij - e^{-2x} P_ij(x).  A per-lane choice of table turns every coefficient
+    // into a dependent vector load on the recurrence's critical path.  The series rows
+    // are zero-padded to their declared width, which leaves the Horner value unchanged.
+    // The entries' Horner chains are interleaved (independent FMAs back to back).
+    constexpr int NQ = K * (K + 1) / 2;
+    constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
+    auto series = [&](double (&o)[NQ]) {
+      BN_UNROLL for (int q = 0; q < NQ; ++q) o[q] = qtab<K>(q, true)[zo + WS - 1];
+      BN_UNROLL for (int n = WS - 2; n >= 0; --n)
+        BN_UNROLL for (int q = 0; q < NQ; ++q) o[q] = fma_uniform_addend(o[q], x, qtab<K>(q, true)[zo + n]);
+      double xp[2 * K];  // x^m
+      xp[0] = 1.0;
+      BN_UNROLL for (int m = 1; m < 2 * K; ++m) xp[m] = (m & 1) ? xp[m - 1] * x : xp[m / 2] * xp[m / 2];
+      int q = 0;
+      BN_UNROLL for (int i = 0; i < K; ++i)
+        BN_UNROLL for (int j = i; j < K; ++j) { o[q] *= xp[2 * K - 1 - i - j]; ++q; }
+    };
+    auto direct = [&](double (&o)[NQ]) {
+      BN_UNROLL for (int q = 0; q < NQ; ++q) o[q] = qtab<K>(q, false)[zo + WP - 1];
+      BN_UNROLL for (int n = WP - 2; n >= 0; --n)
+        BN_UNROLL for (int q = 0; q < NQ; ++q) o[q] = fma_uniform_addend(o[q], x, qtab<K>(q, false)[zo + n]);
+      BN_UNROLL for (int q = 0; q < NQ; ++q) o[q] = qc<K>(q) - e2 * o[q];
+    };
+    double v[NQ];
+    if (need_p) {
+      direct(v);
+      if (need_s) {
+        double w[NQ];
+        series(w);
+        BN_UNROLL for (int q = 0; q < NQ; ++q) v[q] = small ? w[q] : v[q];
+      }
+    } else {
+      series(v);
+    }
     int idx = 0;
     double lp_i = var;  // var * lam^(i + j), built incrementally
     BN_UNROLL for (int i = 0; i < K; ++i) {
       double lp = lp_i;
       BN_UNROLL for (int j = i; j < K; ++j) {
-        // The forms some lane of the wavefront needs, from coefficient rows every lane
-        // reads at the same address (scalar loads, independent of the lane's x), fully
-        // unrolled, then a per-lane select.  Small x: F_ij = x^(2K-1-i-j) G_ij(x); else
-        // F_ij = C_ij - e^{-2x} P_ij(x).  A per-lane choice of table turns every
-        // coefficient into a dependent vector load on the recurrence's critical path.
-        // The series rows are zero-padded to their declared width, which leaves the
-        // Horner value unchanged.
-        const double* ts = qtab<K>(idx, true) + zo;
-        const double* tp = qtab<K>(idx, false) + zo;
-        constexpr int WS = qser_width<K>(), WP = qpolydeg<K>();
-        double vs = 0.0, vp = 0.0;
-        if (need_s) {
-          vs = ts[WS - 1];
-          BN_UNROLL for (int n = WS - 2; n >= 0; --n) vs = fma_uniform_addend(vs, x, ts[n]);
-          BN_UNROLL for (int q = 0; q < 2 * K - 1 - i - j; ++q) vs *= x;
-        }
-        if (need_p) {
-          vp = tp[WP - 1];
-          BN_UNROLL for (int n = WP - 2; n >= 0; --n) vp = fma_uniform_addend(vp, x, tp[n]);
-          vp = qc<K>(idx) - e2 * vp;
-        }
-        const double v = small ? vs : vp;
-        const T q = (T)(lp * v);
+        const T q = (T)(lp * v[idx]);
         Q[O + i][O + j] = q;
         Q[O + j][O + i] = q;
         lp *= lam;

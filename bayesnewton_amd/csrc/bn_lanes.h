@@ -258,9 +258,13 @@ BN_HD void lane_filter_reduce(const PlanView& pl, const HyperView& hv, const flo
       nx_y = y[s1];
       nx_r = r[s1];
     }
-    if (!(k == 0 && (fl & CH_FIRST))) {
+    {
+      // The series' first step has no predict: a zero step there (A = I and Q = 0
+      // exactly) keeps the body free of a divergent branch, whose merge costs a register
+      // copy of every updated value per step.
+      const float dte = (k == 0 && (fl & CH_FIRST)) ? 0.f : dtk;
       T A[D][D], Q[D][D];
-      mdl.discretise(dtk, A, Q);
+      mdl.discretise(dte, A, Q);
       // beta = A beta, Phi = A Phi (block-diagonal A; new rows of a block from old ones)
       T nb[D];
       BN_UNROLL for (int i = 0; i < D; ++i) {
@@ -523,8 +527,11 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     filter_incoming<D, T>(pl, fagg, fcarry, j, m, Pf);
     BN_UNROLL for (int i = 0; i < D; ++i) BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = Pf[i][c];
   } else {
-    BN_UNROLL for (int i = 0; i < D; ++i) m[i] = T(0);
-    BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = T(0);
+    // the series' first step: predicted state (0, Pinf), reached below by a zero step
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] = T(0);
+      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = (T)mdl.Pinf[i][c];
+    }
   }
   const bool want_s = out.sagg.base != nullptr;
   T mu[D], V[NS], C[D][D];  // fixed-point smoother of x_s
@@ -555,15 +562,14 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       nx_r = r[s1];
       nx_masked = mask != nullptr && mask[s1];
     }
-    if (k == 0 && (fl & CH_FIRST)) {
-      BN_UNROLL for (int i = 0; i < D; ++i) {
-        m[i] = T(0);
-        BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = (T)mdl.Pinf[i][c];
-      }
-    } else {
-      // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'
+    {
+      // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'.
+      // The series' first step has no predict: a zero step there (A = I exactly) keeps
+      // the body free of a divergent branch, whose merge costs a register copy of every
+      // updated value per step.
+      const float dte = (k == 0 && (fl & CH_FIRST)) ? 0.f : dtk;
       T A[D][D];
-      transition_only<K1, K2, T>(mdl, dtk, A);
+      transition_only<K1, K2, T>(mdl, dte, A);
       T nm[D];
       BN_UNROLL for (int i = 0; i < D; ++i) {
         T t = T(0);
