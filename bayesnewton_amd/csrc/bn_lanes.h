@@ -16,6 +16,8 @@
 // Everything here is __host__ __device__: the HIP kernels call these bodies with
 // their lane index, and a host build runs them in plain loops.
 #pragma once
+#include <type_traits>
+
 #include "bn_core.h"
 
 namespace bn {
@@ -508,7 +510,9 @@ struct FilterPassOut {
 // joint of (x_s, x_{e+1}) given y_1..e then gives E = C- Pp^-1, g = mu - E mp,
 // L = V - E C-', the element of ops.py:318-335 composed over the chunk, with
 // one solve per chunk.  The smoother uses dt of the NEXT step (basemodels.py:700).
-template <int K1, int K2, class T>
+// WANT_S: build the smoother element (out.sagg is set); without it the fixed-point
+// smoother is not run.
+template <int K1, int K2, class T, bool WANT_S>
 BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                             const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
                             CSoA fagg, CSoA fcarry, const FilterPassOut& out, int32_t j) {
@@ -533,7 +537,7 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
       BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] = (T)mdl.Pinf[i][c];
     }
   }
-  const bool want_s = out.sagg.base != nullptr;
+  constexpr bool want_s = WANT_S;
   T mu[D], V[NS], C[D][D];  // fixed-point smoother of x_s
   // log S is summed as the log of a running product kept as mantissa * 2^exponent
   // (two frexp per step in place of a log; the mantissa is renormalised every step)
@@ -551,114 +555,120 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
     nx_r = r[s0];
     nx_masked = mask != nullptr && mask[s0];
   }
-  for (int32_t k = 0; k < L; ++k) {
-    const int64_t s = pl.slot(j, k);
-    const float dtk = nx_dt, yk = nx_y, rk = nx_r;
-    const bool masked = nx_masked;
-    if (k + 1 < L) {
-      const int64_t s1 = pl.slot(j, k + 1);
-      nx_dt = dt[s1];
-      nx_y = y[s1];
-      nx_r = r[s1];
-      nx_masked = mask != nullptr && mask[s1];
-    }
-    {
-      // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'.
-      // The series' first step has no predict: a zero step there (A = I exactly) keeps
-      // the body free of a divergent branch, whose merge costs a register copy of every
-      // updated value per step.
-      const float dte = (k == 0 && (fl & CH_FIRST)) ? 0.f : dtk;
-      T A[D][D];
-      transition_only<K1, K2, T>(mdl, dte, A);
-      T nm[D];
-      BN_UNROLL for (int i = 0; i < D; ++i) {
-        T t = T(0);
-        BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * m[q];
-        nm[i] = t;
+  // One step of the chunk.  The chunk's first step is a separate instance (FIRST), so
+  // that no step carries a branch on the step index: the merge after such a branch
+  // costs a register copy of every value it updates.
+  auto step = [&](int32_t k, auto first) {
+    constexpr bool FIRST = decltype(first)::value;
+      const int64_t s = pl.slot(j, k);
+      const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+      const bool masked = nx_masked;
+      if (k + 1 < L) {
+        const int64_t s1 = pl.slot(j, k + 1);
+        nx_dt = dt[s1];
+        nx_y = y[s1];
+        nx_r = r[s1];
+        nx_masked = mask != nullptr && mask[s1];
       }
-      BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
-      T np_[NS];
-      BN_UNROLL for (int i = 0; i < D; ++i)
-        BN_UNROLL for (int c = i; c < D; ++c) {
-          T t = S::nz(i, c) ? (T)mdl.Pinf[i][c] : T(0);
-          BN_UNROLL for (int p = 0; p < D; ++p) {
-            if (!S::nz(i, p)) continue;
-            T u = T(0);
-            BN_UNROLL for (int q = 0; q < D; ++q)
-              if (S::nz(c, q)) u += (P[pk(p, q)] - (S::nz(p, q) ? (T)mdl.Pinf[p][q] : T(0))) * A[c][q];
-            t += A[i][p] * u;
-          }
-          np_[S::sidx(i, c)] = t;
-        }
-      BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = np_[q];
-      if (want_s && k > 0) {
+      {
+        // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'.
+        // The series' first step has no predict: a zero step there (A = I exactly) keeps
+        // the body free of a divergent branch, whose merge costs a register copy of every
+        // updated value per step.
+        const float dte = (FIRST && (fl & CH_FIRST)) ? 0.f : dtk;
+        T A[D][D];
+        transition_only<K1, K2, T>(mdl, dte, A);
+        T nm[D];
         BN_UNROLL for (int i = 0; i < D; ++i) {
-          T row[D];
-          BN_UNROLL for (int c = 0; c < D; ++c) {
-            T t = T(0);
-            BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
-            row[c] = t;
+          T t = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * m[q];
+          nm[i] = t;
+        }
+        BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
+        T np_[NS];
+        BN_UNROLL for (int i = 0; i < D; ++i)
+          BN_UNROLL for (int c = i; c < D; ++c) {
+            T t = S::nz(i, c) ? (T)mdl.Pinf[i][c] : T(0);
+            BN_UNROLL for (int p = 0; p < D; ++p) {
+              if (!S::nz(i, p)) continue;
+              T u = T(0);
+              BN_UNROLL for (int q = 0; q < D; ++q)
+                if (S::nz(c, q)) u += (P[pk(p, q)] - (S::nz(p, q) ? (T)mdl.Pinf[p][q] : T(0))) * A[c][q];
+              t += A[i][p] * u;
+            }
+            np_[S::sidx(i, c)] = t;
           }
-          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = row[c];
+        BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = np_[q];
+        if constexpr (want_s && !FIRST) {
+          BN_UNROLL for (int i = 0; i < D; ++i) {
+            T row[D];
+            BN_UNROLL for (int c = 0; c < D; ++c) {
+              T t = T(0);
+              BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
+              row[c] = t;
+            }
+            BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = row[c];
+          }
         }
       }
-    }
-    if (out.state.mean != nullptr && out.predict && (out.hrow == nullptr || k == 0))
-      write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
-    // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
-    T hP[D];
-    BN_UNROLL for (int c = 0; c < D; ++c) {
-      T t = T(0);
-      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
-      hP[c] = t;
-    }
-    double Sd = (double)rk;
-    T hm = T(0);
-    BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
-    if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
-    if (out.hrow != nullptr) {
-      BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
-      out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
-    }
-    const double vd = (double)yk - (double)hm;
-    const double iSd = 1.0 / Sd;
-    const T vS = (T)(vd * iSd);
-    const T iS = (T)iSd;
-    if (!masked) {
-      ell += -0.5 * (vd * vd * iSd + kLog2Pi);
-      int e1, e2;
-      s_mant = frexp(s_mant * frexp(Sd, &e1), &e2);
-      s_exp += e1 + e2;
-    }
-    if (want_s && k > 0) {
-      // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
-      T ch[D];
-      BN_UNROLL for (int i = 0; i < D; ++i) {
+      if (out.state.mean != nullptr && out.predict && (out.hrow == nullptr || FIRST))
+        write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+      // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
+      T hP[D];
+      BN_UNROLL for (int c = 0; c < D; ++c) {
         T t = T(0);
-        BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += C[i][q];
-        ch[i] = t;
+        BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
+        hP[c] = t;
+      }
+      double Sd = (double)rk;
+      T hm = T(0);
+      BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
+      if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
+      if (out.hrow != nullptr) {
+        BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
+        out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
+      }
+      const double vd = (double)yk - (double)hm;
+      const double iSd = 1.0 / Sd;
+      const T vS = (T)(vd * iSd);
+      const T iS = (T)iSd;
+      if (!masked) {
+        ell += -0.5 * (vd * vd * iSd + kLog2Pi);
+        int e1, e2;
+        s_mant = frexp(s_mant * frexp(Sd, &e1), &e2);
+        s_exp += e1 + e2;
+      }
+      if constexpr (want_s && !FIRST) {
+        // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
+        T ch[D];
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          T t = T(0);
+          BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += C[i][q];
+          ch[i] = t;
+        }
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          mu[i] += ch[i] * vS;
+          const T ci = ch[i] * iS;
+          BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ci * ch[c];
+          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ci * hP[c];
+        }
       }
       BN_UNROLL for (int i = 0; i < D; ++i) {
-        mu[i] += ch[i] * vS;
-        const T ci = ch[i] * iS;
-        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ci * ch[c];
-        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ci * hP[c];
+        m[i] += hP[i] * vS;
+        const T ki = hP[i] * iS;
+        BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= ki * hP[c];
       }
-    }
-    BN_UNROLL for (int i = 0; i < D; ++i) {
-      m[i] += hP[i] * vS;
-      const T ki = hP[i] * iS;
-      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= ki * hP[c];
-    }
-    if (want_s && k == 0) {
-      BN_UNROLL for (int i = 0; i < D; ++i) {
-        mu[i] = m[i];
-        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] = P[S::sidx(i, c)];
-        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = P[pk(i, c)];
+      if constexpr (want_s && FIRST) {
+        BN_UNROLL for (int i = 0; i < D; ++i) {
+          mu[i] = m[i];
+          BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] = P[S::sidx(i, c)];
+          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = P[pk(i, c)];
+        }
       }
-    }
-    if (out.state.mean != nullptr && !out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
-  }
+      if (out.state.mean != nullptr && !out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);  };
+  if (L > 0) step(0, std::true_type{});
+  for (int32_t k = 1; k < L; ++k) step(k, std::false_type{});
+
   if (want_s) {
     SElem<D, T> acc;
     if (fl & CH_LAST) {

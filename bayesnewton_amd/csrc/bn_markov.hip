@@ -164,13 +164,13 @@ __global__ __launch_bounds__(kScan) void k_fscan_series(PlanView pl, SoA tagg, S
 
 // Two wavefronts per SIMD up to state dimension 5 (the one-step-ahead input loads fit
 // in 256 registers with a few spills); larger states keep the register file to one.
-template <int K1, int K2>
+template <int K1, int K2, bool WANT_S>
 __global__ __launch_bounds__(kBlock, (K1 + K2 <= 5 ? 2 : 1)) void k_filter_pass(
     PlanView pl, HyperView hv, const float* __restrict__ dt, const float* __restrict__ y, const float* __restrict__ r,
     const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry, FilterPassOut out) {
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= pl.n_chunks) return;
-  lane_filter_pass<K1, K2, CT>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
+  lane_filter_pass<K1, K2, CT, WANT_S>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j);
 }
 
 template <int K1, int K2>
@@ -786,8 +786,8 @@ int run_filter_smoother(const bn_plan* plan, HyperView hv, const float* dt, cons
   fo.status = status != nullptr ? b.chst : nullptr;
   {
     BN_TIMED(KID_FILTER_PASS, st);
-    hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
-                       cs(b.fagg), cs(b.ftcarry), fo);
+    hipLaunchKernelGGL((k_filter_pass<K1, K2, true>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
+                       mask, cs(b.fagg), cs(b.ftcarry), fo);
   }
   launch_smoother_scan<D>(plan, b, st);
   {
@@ -819,8 +819,8 @@ int run_kalman_filter(const bn_plan* plan, HyperView hv, const float* dt, const 
   fo.sagg = SoA{nullptr, 0};
   fo.ell = b.ell;
   fo.status = status != nullptr ? b.chst : nullptr;
-  hipLaunchKernelGGL((k_filter_pass<K1, K2>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask,
-                     cs(b.fagg), cs(b.ftcarry), fo);
+  hipLaunchKernelGGL((k_filter_pass<K1, K2, false>), lane_grid(p), dim3(kBlock), 0, st, plan->view, hv, dt, y, r,
+                     mask, cs(b.fagg), cs(b.ftcarry), fo);
   hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik,
                      (const int32_t*)fo.status, status);
   BN_HIP_CHECK(hipGetLastError());

@@ -152,7 +152,7 @@ void host_pass(const PlanHost& p, const std::vector<double>& ls, const std::vect
   fo.ell = ell.data();
   fo.status = chst.data();
   for (int32_t j = 0; j < p.n_chunks; ++j)
-    lane_filter_pass<K1, K2, double>(pl, hv, dt.data(), y.data(), r.data(), mask.data(), CSoA{fagg.data(), nc},
+    lane_filter_pass<K1, K2, double, true>(pl, hv, dt.data(), y.data(), r.data(), mask.data(), CSoA{fagg.data(), nc},
                                      CSoA{ftcarry.data(), nt}, fo, j);
   if (p.needs_scan) {
     for (int32_t t = 0; t < p.n_tiles; ++t) {  // tile-local inclusive suffixes
