@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -419,10 +420,15 @@ struct PredictIO {
 constexpr double kPredictEdge = 1e10;   // utils.py / basemodels.py:795-796 sentinel times
 constexpr double kPredictJitter = 1e-8;  // utils.py:207
 
-template <int K1, int K2>
-__global__ __launch_bounds__(kBlock) void k_predict(PlanView pl, HyperView hv, PredictIO io) {
-  using S = Spec<K1, K2>;
-  constexpr int D = S::D;
+// Prediction runs in three launches.  k_predict_locate finds each test time's series and
+// neighbour count (two binary searches, at the occupancy of a small kernel) and a sort
+// key: the slot of the training state it reads the most (the left neighbour; the right
+// one before the series' first training time).  A radix sort orders the test times by
+// that key, and k_predict works in that order, so that neighbouring lanes read
+// neighbouring slots of the state planes (in input order each lane's reads of the 55
+// state values each touched a cache line of their own).
+__global__ __launch_bounds__(kBlock) void k_predict_locate(PlanView pl, PredictIO io, uint32_t* __restrict__ key,
+                                                           uint32_t* __restrict__ idx, int64_t* __restrict__ loc) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n_test) return;
   // series of test point i: the last b with test_off[b] <= i
@@ -441,7 +447,34 @@ __global__ __launch_bounds__(kBlock) void k_predict(PlanView pl, HyperView hv, P
     const int64_t mid = (a + c) >> 1;
     if (io.x_train[t0 + mid] < xs) a = mid + 1; else c = mid;
   }
-  const int64_t cnt = a;
+  loc[i] = ((int64_t)b << 32) | (int64_t)(uint32_t)a;
+  if (key != nullptr) {
+    uint32_t k = (uint32_t)pl.n_slots;  // no training state: after every slot
+    if (n > 0) {
+      const int64_t r = a > 0 ? a - 1 : 0;
+      k = (uint32_t)pl.slot(pl.series_first_chunk[b] + (int32_t)(r / pl.chunk_len), (int32_t)(r % pl.chunk_len));
+    }
+    key[i] = k;
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// One thread per test time, in the order `order` gives (NULL: input order).
+template <int K1, int K2>
+__global__ __launch_bounds__(kBlock) void k_predict(PlanView pl, HyperView hv, PredictIO io,
+                                                    const uint32_t* __restrict__ order,
+                                                    const int64_t* __restrict__ loc) {
+  using S = Spec<K1, K2>;
+  constexpr int D = S::D;
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= io.n_test) return;
+  const int64_t i = order != nullptr ? (int64_t)order[p] : p;
+  const int64_t li = loc[i];
+  const int32_t b = (int32_t)(li >> 32);
+  const int64_t cnt = (int64_t)(uint32_t)(li & 0xffffffff);
+  const double xs = io.x_test[i];
+  const int64_t t0 = io.train_off[b];
+  const int64_t n = io.train_off[b + 1] - t0;
   const Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
   const int32_t fc = pl.series_first_chunk[b];
   auto slot_of = [&](int64_t k) {
@@ -854,10 +887,40 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
 
 template <int K1, int K2>
 int run_predict(const bn_plan* plan, HyperView hv, PredictIO io, hipStream_t st) {
-  if (io.n_test > 0)
-    hipLaunchKernelGGL((k_predict<K1, K2>), dim3((unsigned)((io.n_test + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
-                       plan->view, hv, io);
-  BN_HIP_CHECK(hipGetLastError());
+  if (io.n_test == 0) return BN_OK;
+  const int64_t n = io.n_test;
+  // the sort keys are slots (32 bits) and the values test indices (32 bits)
+  const bool sorted = plan->host.n_slots < (int64_t)UINT32_MAX && n < (int64_t)UINT32_MAX;
+  unsigned bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)plan->host.n_slots) ++bits;
+  size_t sort_bytes = 0;
+  if (sorted)
+    BN_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)n, 0, bits, st));
+  // stream-ordered scratch: loc [n] int64, keys and indices [n] x 2 uint32, sort space
+  const size_t off_keys = (size_t)align256(n * 8), off_idx = off_keys + (size_t)align256(n * 4 * 2);
+  const size_t off_sort = off_idx + (size_t)align256(n * 4 * 2);
+  const size_t total = sorted ? off_sort + (size_t)align256((int64_t)sort_bytes) : (size_t)align256(n * 8);
+  char* buf = nullptr;
+  BN_HIP_CHECK(hipMallocAsync((void**)&buf, total, st));
+  int64_t* loc = (int64_t*)buf;
+  uint32_t* keys = sorted ? (uint32_t*)(buf + off_keys) : nullptr;
+  uint32_t* idx = sorted ? (uint32_t*)(buf + off_idx) : nullptr;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_predict_locate, grid, dim3(kBlock), 0, st, plan->view, io, keys, idx, loc);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && sorted) {
+    size_t sb = sort_bytes;
+    e = rocprim::radix_sort_pairs(buf + off_sort, sb, keys, keys + n, idx, idx + n, (size_t)n, 0, bits, st);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((k_predict<K1, K2>), grid, dim3(kBlock), 0, st, plan->view, hv, io,
+                       sorted ? (const uint32_t*)(idx + n) : nullptr, (const int64_t*)loc);
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(buf, st);
+  if (e != hipSuccess) return fail(BN_EHIP, std::string("bn_predict: ") + hipGetErrorString(e));
+  BN_HIP_CHECK(ef);
   return BN_OK;
 }
 

@@ -628,8 +628,9 @@ def bench_predict(args):
         "wall_ms_per_call": wall * 1e3,
         "full_state_filter_smoother_s": full_s,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_predict", "alg_bytes_per_test_time": PRED_BYTES,
-                     "test_times_per_launch": n, "avg_launch_ms": kern_ms},
+                     "traffic": None, "scope": "one bn_predict call: k_predict_locate, radix sort by state slot, "
+                     "k_predict", "alg_bytes_per_test_time": PRED_BYTES, "test_times_per_call": n,
+                     "ms_per_call": kern_ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
