@@ -32,13 +32,15 @@ struct LikParams {
   int32_t kind;
   double p0, p1;  // Gaussian: variance | Student-t: scale, df | Poisson: binsize
   double c0;      // Student-t log-density constant
+  double t0;      // Student-t: 1 / (scale^2 df), so that a node costs no division
 };
 
 inline LikParams make_lik(int32_t kind, double p0, double p1) {
-  LikParams L{kind, p0, p1, 0.0};
+  LikParams L{kind, p0, p1, 0.0, 0.0};
   if (kind == LIK_STUDENTS_T) {
     const double df = p1, s = p0;
     L.c0 = lgamma((df + 1.0) * 0.5) - lgamma(df * 0.5) - 0.5 * (log(s * s) + log(df) + log(M_PI));
+    L.t0 = 1.0 / (s * s * df);
   }
   return L;
 }
@@ -54,15 +56,15 @@ BN_HD double log_lik(const LikParams& L, double y, double f) {
       return -0.5 * log(2.0 * M_PI * L.p0) - 0.5 * (y - f) * (y - f) / L.p0;
     case LIK_BERNOULLI_PROBIT: {
       const double p = norm_cdf(f) * (1.0 - 2.0 * kProbitEps) + kProbitEps;
-      return log(y == 1.0 ? p : 1.0 - p);
+      return log_fast(y == 1.0 ? p : 1.0 - p);
     }
     case LIK_POISSON: {
       const double mu = exp(f) * L.p0;
       return y * log(mu) - mu - lgamma(y + 1.0);
     }
     default: {
-      const double r = (y - f) / L.p0;
-      return L.c0 - 0.5 * (L.p1 + 1.0) * log(1.0 + r * r / L.p1);
+      const double r = y - f;
+      return L.c0 - 0.5 * (L.p1 + 1.0) * log_fast(1.0 + r * r * L.t0);
     }
   }
 }

@@ -1,13 +1,17 @@
 // HIP kernels and C ABI of the site updates and energy terms of the Markov-GP
-// inference step (inference.py:21-325) on the plan's tiled layout.  One lane
-// walks one chunk (the same coalesced access pattern as the filter kernels)
-// and leaves per-chunk FP64 partial sums; a block per series reduces them in a
-// fixed order.
+// inference step (inference.py:21-325) on the plan's tiled layout.  A workgroup
+// takes 64 chunks, one per lane, and its four wavefronts take every fourth step of
+// them (the same coalesced access pattern as the filter kernels, four times the
+// wavefronts); the per-chunk FP64 partial sums are combined in LDS, and a block per
+// series reduces them in a fixed order.  The kernels are instantiated per likelihood:
+// with only its branch compiled, the site step needs a third of the registers of one
+// compiled for all likelihoods, and the wavefronts fit several to a SIMD.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bn_internal.h"
@@ -18,6 +22,33 @@ using namespace bn;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kLanes = 64;                // chunks per workgroup, one per lane
+constexpr int kPhases = kBlock / kLanes;  // wavefronts per workgroup, each every kPhases-th step
+
+// Sum over the workgroup's wavefronts of a per-chunk partial (fixed order); the result
+// is valid in the first wavefront.  Every thread of the workgroup calls it.
+__device__ double phase_sum(double v, double* red) {
+  const int lane = threadIdx.x % kLanes, w = threadIdx.x / kLanes;
+  red[w * kLanes + lane] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (w == 0)
+    for (int q = 0; q < kPhases; ++q) t += red[q * kLanes + lane];
+  __syncthreads();  // red is reused by the next sum
+  return t;
+}
+
+// Runs f with the likelihood kind as a compile-time constant.
+template <class F>
+int with_kind(int32_t kind, F&& f) {
+  switch (kind) {
+    case LIK_GAUSSIAN: f(std::integral_constant<int, LIK_GAUSSIAN>{}); return BN_OK;
+    case LIK_BERNOULLI_PROBIT: f(std::integral_constant<int, LIK_BERNOULLI_PROBIT>{}); return BN_OK;
+    case LIK_POISSON: f(std::integral_constant<int, LIK_POISSON>{}); return BN_OK;
+    case LIK_STUDENTS_T: f(std::integral_constant<int, LIK_STUDENTS_T>{}); return BN_OK;
+    default: return set_error(BN_EINVAL, "unknown likelihood kind");
+  }
+}
 
 int fail(int code, const std::string& m) { return set_error(code, m); }
 
@@ -43,14 +74,16 @@ struct SiteIO {
   const float* batch;  // optional per slot: nonzero = in the mini-batch (NULL: every site)
 };
 
-template <int METHOD>
+template <int METHOD, int KIND>
 __global__ __launch_bounds__(kBlock) void k_site_update(PlanView pl, LikParams L, GHTable gh, SiteIO io, double lr,
                                                         double power, int ensure_psd) {
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= pl.n_chunks) return;
-  const int32_t len = pl.chunk_len_[j];
+  __shared__ double red[kBlock];
+  L.kind = KIND;
+  const int32_t j = blockIdx.x * kLanes + (int32_t)(threadIdx.x % kLanes);
+  const int32_t w = (int32_t)(threadIdx.x / kLanes);
+  const int32_t len = j < pl.n_chunks ? pl.chunk_len_[j] : 0;
   double d1 = 0.0, d2 = 0.0;
-  for (int32_t k = 0; k < len; ++k) {
+  for (int32_t k = w; k < len; k += kPhases) {
     const int64_t s = pl.slot(j, k);
     // group_natural_params (basemodels.py:195-199): sites outside the batch keep their values
     if (io.batch != nullptr && io.batch[s] == 0.0f) continue;
@@ -72,18 +105,24 @@ __global__ __launch_bounds__(kBlock) void k_site_update(PlanView pl, LikParams L
       io.o_hess[s] = (float)r.hess;
     }
   }
-  io.part1[j] = d1;
-  io.part2[j] = d2;
+  d1 = phase_sum(d1, red);
+  d2 = phase_sum(d2, red);
+  if (w == 0 && j < pl.n_chunks) {
+    io.part1[j] = d1;
+    io.part2[j] = d2;
+  }
 }
 
-template <int METHOD>
+template <int METHOD, int KIND>
 __global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams L, GHTable gh, SiteIO io,
                                                          double power) {
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= pl.n_chunks) return;
-  const int32_t len = pl.chunk_len_[j];
+  __shared__ double red[kBlock];
+  L.kind = KIND;
+  const int32_t j = blockIdx.x * kLanes + (int32_t)(threadIdx.x % kLanes);
+  const int32_t w = (int32_t)(threadIdx.x / kLanes);
+  const int32_t len = j < pl.n_chunks ? pl.chunk_len_[j] : 0;
   double s1 = 0.0, s2 = 0.0;
-  for (int32_t k = 0; k < len; ++k) {
+  for (int32_t k = w; k < len; k += kPhases) {
     const int64_t s = pl.slot(j, k);
     double t1, t2;
     energy_terms<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], io.nat1[s], io.nat2[s], power,
@@ -92,22 +131,29 @@ __global__ __launch_bounds__(kBlock) void k_energy_terms(PlanView pl, LikParams 
     if (io.batch == nullptr || io.batch[s] != 0.0f) s1 += t1;
     s2 += t2;
   }
-  io.part1[j] = s1;
-  io.part2[j] = s2;
+  s1 = phase_sum(s1, red);
+  s2 = phase_sum(s2, red);
+  if (w == 0 && j < pl.n_chunks) {
+    io.part1[j] = s1;
+    io.part2[j] = s2;
+  }
 }
 
-template <int METHOD>
+template <int METHOD, int KIND>
 __global__ __launch_bounds__(kBlock) void k_lik_grad(PlanView pl, LikParams L, GHTable gh, SiteIO io, double power) {
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= pl.n_chunks) return;
-  const int32_t len = pl.chunk_len_[j];
+  __shared__ double red[kBlock];
+  L.kind = KIND;
+  const int32_t j = blockIdx.x * kLanes + (int32_t)(threadIdx.x % kLanes);
+  const int32_t w = (int32_t)(threadIdx.x / kLanes);
+  const int32_t len = j < pl.n_chunks ? pl.chunk_len_[j] : 0;
   double s1 = 0.0;
-  for (int32_t k = 0; k < len; ++k) {
+  for (int32_t k = w; k < len; k += kPhases) {
     const int64_t s = pl.slot(j, k);
     const double n1 = METHOD == M_EP ? io.nat1[s] : 0.0, n2 = METHOD == M_EP ? io.nat2[s] : 0.0;
     s1 += dterm1_dparam<METHOD>(L, gh, (double)io.y[s], (double)io.pm[s], (double)io.pv[s], n1, n2, power);
   }
-  io.part1[j] = s1;
+  s1 = phase_sum(s1, red);
+  if (w == 0 && j < pl.n_chunks) io.part1[j] = s1;
 }
 
 __global__ void k_reparametrise(PlanView pl, const double* __restrict__ nat1, const double* __restrict__ nat2,
@@ -364,21 +410,22 @@ int bn_site_update(const bn_plan* plan, const bn_kernel_desc* kern, const bn_lik
   if (!p1 || !p2) return fail(BN_EINVAL, "bad kernel descriptor for the workspace");
   SiteIO io{y, post_mean, post_var, nat1, nat2, pseudo_y, pseudo_var, out_mean, out_jac, out_hess, p1, p2, batch};
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
-  switch (method) {
-    case BN_VI:
-      hipLaunchKernelGGL(k_site_update<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power, ensure_psd);
-      break;
-    case BN_NEWTON:
-      hipLaunchKernelGGL(k_site_update<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power,
-                         ensure_psd);
-      break;
-    case BN_EP:
-      hipLaunchKernelGGL(k_site_update<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power, ensure_psd);
-      break;
-    default:
-      return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
-  }
+  const dim3 grid((plan->host.n_chunks + kLanes - 1) / kLanes);
+  if (method != BN_VI && method != BN_NEWTON && method != BN_EP)
+    return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
+  if (int rc = with_kind(L.kind, [&](auto kd) {
+        constexpr int KD = decltype(kd)::value;
+        if (method == BN_VI)
+          hipLaunchKernelGGL((k_site_update<M_VI, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power,
+                             ensure_psd);
+        else if (method == BN_NEWTON)
+          hipLaunchKernelGGL((k_site_update<M_NEWTON, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr,
+                             power, ensure_psd);
+        else
+          hipLaunchKernelGGL((k_site_update<M_EP, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, lr, power,
+                             ensure_psd);
+      }))
+    return rc;
   hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, diff1, diff2);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;
@@ -403,20 +450,19 @@ int bn_energy_terms(const bn_plan* plan, const bn_kernel_desc* kern, const bn_li
   SiteIO io{y, post_mean, post_var, const_cast<double*>(nat1), const_cast<double*>(nat2), nullptr, nullptr,
             nullptr, nullptr, nullptr, p1, p2, batch};
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
-  switch (method) {
-    case BN_VI:
-      hipLaunchKernelGGL(k_energy_terms<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-      break;
-    case BN_NEWTON:
-      hipLaunchKernelGGL(k_energy_terms<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-      break;
-    case BN_EP:
-      hipLaunchKernelGGL(k_energy_terms<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-      break;
-    default:
-      return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
-  }
+  const dim3 grid((plan->host.n_chunks + kLanes - 1) / kLanes);
+  if (method != BN_VI && method != BN_NEWTON && method != BN_EP)
+    return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
+  if (int rc = with_kind(L.kind, [&](auto kd) {
+        constexpr int KD = decltype(kd)::value;
+        if (method == BN_VI)
+          hipLaunchKernelGGL((k_energy_terms<M_VI, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+        else if (method == BN_NEWTON)
+          hipLaunchKernelGGL((k_energy_terms<M_NEWTON, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+        else
+          hipLaunchKernelGGL((k_energy_terms<M_EP, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+      }))
+    return rc;
   hipLaunchKernelGGL(k_series_sum2, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, p2, term1, term2);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;
@@ -438,15 +484,19 @@ int bn_lik_param_grad(const bn_plan* plan, const bn_kernel_desc* kern, const bn_
   SiteIO io{y, post_mean, post_var, const_cast<double*>(nat1), const_cast<double*>(nat2), nullptr, nullptr,
             nullptr, nullptr, nullptr, p1, nullptr};
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((plan->host.n_chunks + kBlock - 1) / kBlock);
-  if (method == BN_VI)
-    hipLaunchKernelGGL(k_lik_grad<M_VI>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-  else if (method == BN_NEWTON)
-    hipLaunchKernelGGL(k_lik_grad<M_NEWTON>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-  else if (method == BN_EP)
-    hipLaunchKernelGGL(k_lik_grad<M_EP>, grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
-  else
+  const dim3 grid((plan->host.n_chunks + kLanes - 1) / kLanes);
+  if (method != BN_VI && method != BN_NEWTON && method != BN_EP)
     return fail(BN_EINVAL, "method must be BN_VI, BN_NEWTON or BN_EP");
+  if (int rc = with_kind(L.kind, [&](auto kd) {
+        constexpr int KD = decltype(kd)::value;
+        if (method == BN_VI)
+          hipLaunchKernelGGL((k_lik_grad<M_VI, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+        else if (method == BN_NEWTON)
+          hipLaunchKernelGGL((k_lik_grad<M_NEWTON, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+        else
+          hipLaunchKernelGGL((k_lik_grad<M_EP, KD>), grid, dim3(kBlock), 0, st, plan->view, L, gh, io, power);
+      }))
+    return rc;
   hipLaunchKernelGGL(k_series_sum1, dim3(plan->host.n_series), dim3(kBlock), 0, st, plan->view, p1, grad);
   BN_HIP_CHECK2(hipGetLastError());
   return BN_OK;

@@ -12,6 +12,7 @@
 //     other series of the batch keep status 0 and their oracle values.
 // tests/test_cpu_lanes.py builds it with -fsanitize=address,undefined and runs it.
 // Prints one line per case; exit status 0 when every check holds.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -421,6 +422,27 @@ int main() {
       // last has 104 chunks (width 1)
       check(n8 == 5 && n1 == 32 && p.group_w.size() == 37, "short series share groups");
     }
+  }
+  // log_fast (bn_core.h) against the library log: at most 4 ulp on finite x > 0 (three
+  // ranges: the full exponent range, the Student-t argument 1 + r^2 / nu, around 1, and a
+  // subnormal), and log's values for zero, negatives, infinities and NaN
+  {
+    std::mt19937_64 g(7);
+    double worst = 0.0;
+    auto ulps = [](double a, double b) {
+      const double sp = std::nextafter(std::fabs(b), INFINITY) - std::fabs(b);
+      return b == 0.0 ? std::fabs(a) : std::fabs(a - b) / sp;
+    };
+    std::uniform_real_distribution<double> u1(-700.0, 700.0), u2(0.0, 40.0), u3(0.5, 2.0);
+    for (int i = 0; i < 200000; ++i) {
+      const double xs[3] = {std::exp(u1(g)), 1.0 + u2(g), u3(g)};
+      for (double x : xs) worst = std::max(worst, ulps(log_fast(x), std::log(x)));
+    }
+    std::printf("log_fast: worst %.2f ulp\n", worst);
+    worst = std::max(worst, ulps(log_fast(1e-310), std::log(1e-310)));  // subnormal
+    bool same = log_fast(0.0) == -INFINITY && std::isnan(log_fast(-1.0)) && log_fast(INFINITY) == INFINITY &&
+                std::isnan(log_fast(-INFINITY)) && std::isnan(log_fast(NAN)) && log_fast(1.0) == 0.0;
+    check(worst <= 4.0 && same, "log_fast within 4 ulp of log, library values off the normal range");
   }
   std::printf(g_fail ? "LANES_HOST_FAILED %d\n" : "LANES_HOST_OK\n", g_fail);
   return g_fail ? 1 : 0;
