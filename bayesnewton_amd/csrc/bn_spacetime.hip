@@ -535,10 +535,14 @@ constexpr int kRC = 32;  // rows of B per chunk
 constexpr int kLC = kMaxM + 1;  // LDS row stride of a chunk: consecutive rows start in different banks
 static_assert(2 * kRC * kLC >= kMaxM * kMaxM, "the two chunk buffers hold an M x M matrix");
 
-template <int METHOD, int MODE>
+// KIND: the likelihood as a compile-time constant (0: the projection, which has none);
+// with only its branch compiled, the site step spills a fraction of what it does with
+// every likelihood's code in the kernel.
+template <int METHOD, int MODE, int KIND>
 __global__ __launch_bounds__(kT, 3) void k_st_sites(StDims dm, LikParams lik, GHTable gh, StSiteIO io, double lr,
                                                  int ensure_psd, double power, int mask_pseudo) {
   extern __shared__ double smem[];
+  if (KIND != 0) lik.kind = KIND;
   const int M = dm.M, R = dm.R;
   // LDS is kept near 53 KB (three workgroups per CU): the posterior covariance is read
   // from global memory by the GEMM, the projection W accumulates in its output slot Lw
@@ -1945,9 +1949,14 @@ int set_smem_limits() {
                                    (int)kGainsSmem));
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_gains<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kGainsSmem));
-#define ST_SITES_ATTR(MT, MD)                                                                              \
-  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_sites<MT, MD>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+#define ST_SITES_ATTR1(MT, MD, KD)                                                                             \
+  ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_sites<MT, MD, KD>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                    (int)kSitesSmem))
+#define ST_SITES_ATTR(MT, MD)                 \
+  ST_SITES_ATTR1(MT, MD, LIK_GAUSSIAN);        \
+  ST_SITES_ATTR1(MT, MD, LIK_BERNOULLI_PROBIT); \
+  ST_SITES_ATTR1(MT, MD, LIK_POISSON);         \
+  ST_SITES_ATTR1(MT, MD, LIK_STUDENTS_T)
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_mf_pseudo, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kPseudoSmem));
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_ih_dare<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1960,19 +1969,43 @@ int set_smem_limits() {
                                    (int)kSmootherSmem));
   ST_SITES_ATTR(M_VI, BN_ST_UPDATE);
   ST_SITES_ATTR(M_VI, BN_ST_ENERGY);
-  ST_SITES_ATTR(M_VI, BN_ST_PROJECT);
+  ST_SITES_ATTR1(M_VI, BN_ST_PROJECT, 0);
   ST_SITES_ATTR(M_NEWTON, BN_ST_UPDATE);
   ST_SITES_ATTR(M_NEWTON, BN_ST_ENERGY);
-  ST_SITES_ATTR(M_NEWTON, BN_ST_PROJECT);
   ST_SITES_ATTR(M_EP, BN_ST_UPDATE);
   ST_SITES_ATTR(M_EP, BN_ST_ENERGY);
 #undef ST_SITES_ATTR
+#undef ST_SITES_ATTR1
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_ep_cavity<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kCavitySmem));
   ST_HIP_CHECK(hipFuncSetAttribute((const void*)k_st_ep_cavity<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)kCavitySmem));
   done = 1;
   return BN_OK;
+}
+
+// k_st_sites<MT, MD> for the likelihood kind L.kind (checked by check_lik).
+template <int MT, int MD>
+void launch_st_sites(dim3 grid, hipStream_t st, StDims dm, LikParams L, GHTable gh, StSiteIO io, double lr,
+                     int ensure_psd, double power, int mask_pseudo) {
+  switch (L.kind) {
+    case LIK_GAUSSIAN:
+      hipLaunchKernelGGL((k_st_sites<MT, MD, LIK_GAUSSIAN>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                         ensure_psd, power, mask_pseudo);
+      break;
+    case LIK_BERNOULLI_PROBIT:
+      hipLaunchKernelGGL((k_st_sites<MT, MD, LIK_BERNOULLI_PROBIT>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io,
+                         lr, ensure_psd, power, mask_pseudo);
+      break;
+    case LIK_POISSON:
+      hipLaunchKernelGGL((k_st_sites<MT, MD, LIK_POISSON>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                         ensure_psd, power, mask_pseudo);
+      break;
+    default:
+      hipLaunchKernelGGL((k_st_sites<MT, MD, LIK_STUDENTS_T>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                         ensure_psd, power, mask_pseudo);
+      break;
+  }
 }
 
 }  // namespace
@@ -2019,18 +2052,17 @@ int bn_st_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t method, 
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid(dm.T, dm.K);
   BN_TIMED(KID_ST_SITES, st);
-#define ST_SITES(MT, MD) \
-  hipLaunchKernelGGL((k_st_sites<MT, MD>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0)
-  if (method == BN_VI) {
-    if (mode == BN_ST_UPDATE) ST_SITES(M_VI, BN_ST_UPDATE);
-    else if (mode == BN_ST_ENERGY) ST_SITES(M_VI, BN_ST_ENERGY);
-    else ST_SITES(M_VI, BN_ST_PROJECT);
+  if (mode == BN_ST_PROJECT) {
+    // the projection is method- and likelihood-free
+    hipLaunchKernelGGL((k_st_sites<M_VI, BN_ST_PROJECT, 0>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
+                       ensure_psd, 1.0, 0);
+  } else if (method == BN_VI) {
+    if (mode == BN_ST_UPDATE) launch_st_sites<M_VI, BN_ST_UPDATE>(grid, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0);
+    else launch_st_sites<M_VI, BN_ST_ENERGY>(grid, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0);
   } else {
-    if (mode == BN_ST_UPDATE) ST_SITES(M_NEWTON, BN_ST_UPDATE);
-    else if (mode == BN_ST_ENERGY) ST_SITES(M_NEWTON, BN_ST_ENERGY);
-    else ST_SITES(M_NEWTON, BN_ST_PROJECT);
+    if (mode == BN_ST_UPDATE) launch_st_sites<M_NEWTON, BN_ST_UPDATE>(grid, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0);
+    else launch_st_sites<M_NEWTON, BN_ST_ENERGY>(grid, st, dm, L, gh, io, lr, ensure_psd, 1.0, 0);
   }
-#undef ST_SITES
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
 }
@@ -2070,12 +2102,10 @@ int bn_st_ep_sites(const bn_st_dims* dims, const bn_lik_desc* lik, int32_t mode,
   BN_TIMED(KID_ST_SITES, st);
   if (mode == BN_ST_UPDATE) {
     hipLaunchKernelGGL((k_st_ep_cavity<false>), grid, dim3(kT), kCavitySmem, st, dm, cio, power, mask_pseudo);
-    hipLaunchKernelGGL((k_st_sites<M_EP, BN_ST_UPDATE>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
-                       ensure_psd, power, mask_pseudo);
+    launch_st_sites<M_EP, BN_ST_UPDATE>(grid, st, dm, L, gh, io, lr, ensure_psd, power, mask_pseudo);
   } else {
     hipLaunchKernelGGL((k_st_ep_cavity<true>), grid, dim3(kT), kCavitySmem, st, dm, cio, power, mask_pseudo);
-    hipLaunchKernelGGL((k_st_sites<M_EP, BN_ST_ENERGY>), grid, dim3(kT), kSitesSmem, st, dm, L, gh, io, lr,
-                       ensure_psd, power, mask_pseudo);
+    launch_st_sites<M_EP, BN_ST_ENERGY>(grid, st, dm, L, gh, io, lr, ensure_psd, power, mask_pseudo);
   }
   ST_HIP_CHECK(hipGetLastError());
   return BN_OK;
