@@ -560,112 +560,112 @@ BN_HD void lane_filter_pass(const PlanView& pl, const HyperView& hv, const float
   // costs a register copy of every value it updates.
   auto step = [&](int32_t k, auto first) {
     constexpr bool FIRST = decltype(first)::value;
-      const int64_t s = pl.slot(j, k);
-      const float dtk = nx_dt, yk = nx_y, rk = nx_r;
-      const bool masked = nx_masked;
-      if (k + 1 < L) {
-        const int64_t s1 = pl.slot(j, k + 1);
-        nx_dt = dt[s1];
-        nx_y = y[s1];
-        nx_r = r[s1];
-        nx_masked = mask != nullptr && mask[s1];
-      }
-      {
-        // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'.
-        // The series' first step has no predict: a zero step there (A = I exactly) keeps
-        // the body free of a divergent branch, whose merge costs a register copy of every
-        // updated value per step.
-        const float dte = (FIRST && (fl & CH_FIRST)) ? 0.f : dtk;
-        T A[D][D];
-        transition_only<K1, K2, T>(mdl, dte, A);
-        T nm[D];
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          T t = T(0);
-          BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * m[q];
-          nm[i] = t;
-        }
-        BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
-        T np_[NS];
-        BN_UNROLL for (int i = 0; i < D; ++i)
-          BN_UNROLL for (int c = i; c < D; ++c) {
-            T t = S::nz(i, c) ? (T)mdl.Pinf[i][c] : T(0);
-            BN_UNROLL for (int p = 0; p < D; ++p) {
-              if (!S::nz(i, p)) continue;
-              T u = T(0);
-              BN_UNROLL for (int q = 0; q < D; ++q)
-                if (S::nz(c, q)) u += (P[pk(p, q)] - (S::nz(p, q) ? (T)mdl.Pinf[p][q] : T(0))) * A[c][q];
-              t += A[i][p] * u;
-            }
-            np_[S::sidx(i, c)] = t;
-          }
-        BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = np_[q];
-        if constexpr (want_s && !FIRST) {
-          BN_UNROLL for (int i = 0; i < D; ++i) {
-            T row[D];
-            BN_UNROLL for (int c = 0; c < D; ++c) {
-              T t = T(0);
-              BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
-              row[c] = t;
-            }
-            BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = row[c];
-          }
-        }
-      }
-      if (out.state.mean != nullptr && out.predict && (out.hrow == nullptr || FIRST))
-        write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
-      // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
-      T hP[D];
-      BN_UNROLL for (int c = 0; c < D; ++c) {
+    const int64_t s = pl.slot(j, k);
+    const float dtk = nx_dt, yk = nx_y, rk = nx_r;
+    const bool masked = nx_masked;
+    if (k + 1 < L) {
+      const int64_t s1 = pl.slot(j, k + 1);
+      nx_dt = dt[s1];
+      nx_y = y[s1];
+      nx_r = r[s1];
+      nx_masked = mask != nullptr && mask[s1];
+    }
+    {
+      // predict in place: m = A m, P = A (P - Pinf) A' + Pinf (Q-free form), C = C A'.
+      // The series' first step has no predict: a zero step there (A = I exactly) keeps
+      // the body free of a divergent branch, whose merge costs a register copy of every
+      // updated value per step.
+      const float dte = (FIRST && (fl & CH_FIRST)) ? 0.f : dtk;
+      T A[D][D];
+      transition_only<K1, K2, T>(mdl, dte, A);
+      T nm[D];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
         T t = T(0);
-        BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
-        hP[c] = t;
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(i, q)) t += A[i][q] * m[q];
+        nm[i] = t;
       }
-      double Sd = (double)rk;
-      T hm = T(0);
-      BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
-      if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
-      if (out.hrow != nullptr) {
-        BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
-        out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
-      }
-      const double vd = (double)yk - (double)hm;
-      const double iSd = 1.0 / Sd;
-      const T vS = (T)(vd * iSd);
-      const T iS = (T)iSd;
-      if (!masked) {
-        ell += -0.5 * (vd * vd * iSd + kLog2Pi);
-        int e1, e2;
-        s_mant = frexp(s_mant * frexp(Sd, &e1), &e2);
-        s_exp += e1 + e2;
-      }
+      BN_UNROLL for (int i = 0; i < D; ++i) m[i] = nm[i];
+      T np_[NS];
+      BN_UNROLL for (int i = 0; i < D; ++i)
+        BN_UNROLL for (int c = i; c < D; ++c) {
+          T t = S::nz(i, c) ? (T)mdl.Pinf[i][c] : T(0);
+          BN_UNROLL for (int p = 0; p < D; ++p) {
+            if (!S::nz(i, p)) continue;
+            T u = T(0);
+            BN_UNROLL for (int q = 0; q < D; ++q)
+              if (S::nz(c, q)) u += (P[pk(p, q)] - (S::nz(p, q) ? (T)mdl.Pinf[p][q] : T(0))) * A[c][q];
+            t += A[i][p] * u;
+          }
+          np_[S::sidx(i, c)] = t;
+        }
+      BN_UNROLL for (int q = 0; q < NS; ++q) P[q] = np_[q];
       if constexpr (want_s && !FIRST) {
-        // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
-        T ch[D];
         BN_UNROLL for (int i = 0; i < D; ++i) {
-          T t = T(0);
-          BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += C[i][q];
-          ch[i] = t;
+          T row[D];
+          BN_UNROLL for (int c = 0; c < D; ++c) {
+            T t = T(0);
+            BN_UNROLL for (int q = 0; q < D; ++q) if (S::nz(c, q)) t += C[i][q] * A[c][q];
+            row[c] = t;
+          }
+          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = row[c];
         }
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          mu[i] += ch[i] * vS;
-          const T ci = ch[i] * iS;
-          BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ci * ch[c];
-          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ci * hP[c];
-        }
+      }
+    }
+    if (out.state.mean != nullptr && out.predict && (out.hrow == nullptr || FIRST))
+      write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);
+    // measurement update (ops.py:163-171): hP = H P-, S = H P- H' + r, v = y - H m-
+    T hP[D];
+    BN_UNROLL for (int c = 0; c < D; ++c) {
+      T t = T(0);
+      BN_UNROLL for (int i = 0; i < D; ++i) if (S::h(i)) t += P[pk(i, c)];
+      hP[c] = t;
+    }
+    double Sd = (double)rk;
+    T hm = T(0);
+    BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) { Sd += (double)hP[q]; hm += m[q]; }
+    if (!(Sd > 0.0)) st |= ST_INNOV_NONPD;
+    if (out.hrow != nullptr) {
+      BN_UNROLL for (int q = 0; q < D; ++q) out.hrow[q * pl.n_slots + s] = (hrow_t)hP[q];
+      out.hrow[D * pl.n_slots + s] = (hrow_t)hm;
+    }
+    const double vd = (double)yk - (double)hm;
+    const double iSd = 1.0 / Sd;
+    const T vS = (T)(vd * iSd);
+    const T iS = (T)iSd;
+    if (!masked) {
+      ell += -0.5 * (vd * vd * iSd + kLog2Pi);
+      int e1, e2;
+      s_mant = frexp(s_mant * frexp(Sd, &e1), &e2);
+      s_exp += e1 + e2;
+    }
+    if constexpr (want_s && !FIRST) {
+      // fixed-point smoother: ch = C- H'; mu += ch v / S; V -= ch ch' / S; C = C- - ch hP' / S
+      T ch[D];
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        T t = T(0);
+        BN_UNROLL for (int q = 0; q < D; ++q) if (S::h(q)) t += C[i][q];
+        ch[i] = t;
       }
       BN_UNROLL for (int i = 0; i < D; ++i) {
-        m[i] += hP[i] * vS;
-        const T ki = hP[i] * iS;
-        BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= ki * hP[c];
+        mu[i] += ch[i] * vS;
+        const T ci = ch[i] * iS;
+        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] -= ci * ch[c];
+        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] -= ci * hP[c];
       }
-      if constexpr (want_s && FIRST) {
-        BN_UNROLL for (int i = 0; i < D; ++i) {
-          mu[i] = m[i];
-          BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] = P[S::sidx(i, c)];
-          BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = P[pk(i, c)];
-        }
+    }
+    BN_UNROLL for (int i = 0; i < D; ++i) {
+      m[i] += hP[i] * vS;
+      const T ki = hP[i] * iS;
+      BN_UNROLL for (int c = i; c < D; ++c) P[S::sidx(i, c)] -= ki * hP[c];
+    }
+    if constexpr (want_s && FIRST) {
+      BN_UNROLL for (int i = 0; i < D; ++i) {
+        mu[i] = m[i];
+        BN_UNROLL for (int c = i; c < D; ++c) V[S::sidx(i, c)] = P[S::sidx(i, c)];
+        BN_UNROLL for (int c = 0; c < D; ++c) C[i][c] = P[pk(i, c)];
       }
-      if (out.state.mean != nullptr && !out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);  };
+    }
+    if (out.state.mean != nullptr && !out.predict) write_state_packed<K1, K2, T>(out.state, pl.n_slots, s, m, P);  };
   if (L > 0) step(0, std::true_type{});
   for (int32_t k = 1; k < L; ++k) step(k, std::false_type{});
 
@@ -809,11 +809,15 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
   T sm[D], sP[D][D];
   if (!(fl & CH_LAST)) smoother_incoming<D, T>(pl, sagg, scarry, j, sm, sP);
   const int64_t ns = pl.n_slots;
-  for (int32_t k = L - 1; k >= 0; --k) {
+  // One step, walking backwards.  The chunk's last step is a separate instance (LAST):
+  // only there can the step be the series' terminal one, and no other step carries that
+  // branch, whose merge would cost a register copy of the smoothed state per step.
+  auto step = [&](int32_t k, auto last) {
+    constexpr bool LAST = decltype(last)::value;
     const int64_t s = pl.slot(j, k);
     T fm[D], fP[D][D], G[D][D];
     read_state<K1, K2, T>(io.fin, ns, s, fm, fP);
-    const bool terminal = (k == L - 1) && (fl & CH_LAST);
+    const bool terminal = LAST && (fl & CH_LAST);
     double dtn;
     if (io.shifted) dtn = io.dt[s];
     else dtn = terminal ? 0.0 : (double)io.dt[k + 1 < L ? pl.slot(j, k + 1) : pl.slot(j + 1, 0)];
@@ -862,7 +866,10 @@ BN_HD void lane_smoother_pass(const PlanView& pl, const HyperView& hv, CSoA sagg
       BN_UNROLL for (int i = 0; i < D; ++i)
         BN_UNROLL for (int c = 0; c < D; ++c) io.gains[(i * D + c) * ns + s] = (OT)G[i][c];
     }
-  }
+  };
+  if (L > 0) step(L - 1, std::true_type{});
+  for (int32_t k = L - 2; k >= 0; --k) step(k, std::false_type{});
+
 }
 
 // smoother_mbf: the smoothed marginals H m, H P H' of one chunk by the modified
