@@ -209,7 +209,9 @@ int bn_rts_smoother(const bn_plan *plan, const bn_kernel_desc *kern, const doubl
  * offsets; x_train: the training times the plan was built on (sorted within each
  * series); x_test: test times of each series (any order); means / covs / gains: the
  * smoother's full output (bn_rts_smoother with BN_RETURN_FULL, tiled planes, FP64).
- * Outputs out_mean / out_var [n_test], FP64. */
+ * Outputs out_mean / out_var [n_test], FP64.  The call takes stream-ordered scratch
+ * (hipMallocAsync / hipFreeAsync on `stream`, about 24 bytes per test time) for the
+ * test times' neighbour indices and their sort by the state slot they read. */
 int bn_predict(const bn_plan *plan, const bn_kernel_desc *kern, const double *lengthscale, const double *variance,
                int32_t hyper_stride, const int64_t *train_offsets, const double *x_train, const double *means,
                const double *covs, const double *gains, const int64_t *test_offsets, const double *x_test,
