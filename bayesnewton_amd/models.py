@@ -393,7 +393,8 @@ class MarkovGaussianProcess:
             X = [X]
         if len(X) != self.num_series:
             raise ValueError(f"expected test inputs for {self.num_series} series")
-        xs = [np.asarray(x, dtype=np.float64).reshape(len(x), -1)[:, 0] for x in X]
+        xs = [np.asarray(x, dtype=np.float64) for x in X]
+        xs = [x.reshape(len(x), -1)[:, 0] if x.size else np.zeros(0) for x in xs]  # a series may have none
         off = np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.int64)
         xt = np.concatenate(xs) if off[-1] > 0 else np.zeros(0)
         return (torch.as_tensor(xt, dtype=torch.float64, device=self.device),

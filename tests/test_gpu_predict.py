@@ -108,6 +108,34 @@ def test_predict_multi_series_matches_per_series_oracle():
         np.testing.assert_allclose(gv[off[b]:off[b + 1]], ov, rtol=TOL, atol=TOL * np.max(ov))
 
 
+def test_predict_sorted_order_with_a_series_without_test_times():
+    """bn_predict sorts the test times by the state slot they read; outputs come back in
+    input order, for series with unsorted test times, a series with none, and test
+    times far outside the data (the sentinel states)."""
+    from bayesnewton_amd import kernels as K, likelihoods as L, models as M
+    rng = np.random.default_rng(9)
+    lens = [40, 7, 90, 3]
+    xs = [np.sort(rng.choice(20 * 256, size=n, replace=False) / 256.0) for n in lens]
+    ys = [_f32(np.cos(0.7 * x) + 0.1 * rng.standard_normal(len(x))) for x in xs]
+    g = M.MarkovVariationalGP.from_series(K.Matern52(variance=1.0, lengthscale=1.5), L.Gaussian(variance=0.01), xs,
+                                          ys, chunk_len=8)
+    g.inference(lr=1.0)
+    tests = [rng.permutation(np.concatenate([rng.uniform(x[0] - 2, x[-1] + 2, 30), [x[-1] + 1e3, x[0] - 1e3]]))
+             if b != 1 else np.zeros(0) for b, x in enumerate(xs)]
+    gm, gv = g.predict(tests)
+    gm, gv = gm.cpu().numpy(), gv.cpu().numpy()
+    off = np.concatenate([[0], np.cumsum([len(t) for t in tests])])
+    assert gm.shape[0] == off[-1]
+    for b, (x, y, t) in enumerate(zip(xs, ys, tests)):
+        if len(t) == 0:
+            continue
+        o = OM.MarkovGP(OK.Matern52(1.0, 1.5), OL.Gaussian(0.01), x, y, inference="VI")
+        o.inference(lr=1.0)
+        om, ov = o.predict(t[:, None])
+        np.testing.assert_allclose(gm[off[b]:off[b + 1]], om, rtol=TOL, atol=TOL * np.max(np.abs(om)))
+        np.testing.assert_allclose(gv[off[b]:off[b + 1]], ov, rtol=TOL, atol=TOL * np.max(ov))
+
+
 @pytest.mark.parametrize("g,T,nr,kind", [(3, 20, 16, "VI"), (2, 15, 9, "Newton"), (8, 12, 256, "VI")])
 def test_spacetime_predict_matches_oracle(g, T, nr, kind):
     """Space-time prediction (basemodels.py:766-816): filter, full smoother with the
