@@ -1266,7 +1266,7 @@ int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_
   const PlanHost& h = p->host;
   // one device allocation for every table, 256-byte aligned pieces
   struct Piece { const void* src; size_t bytes; size_t off; };
-  Piece pcs[15] = {
+  Piece pcs[16] = {
       {h.chunk_series.data(), h.chunk_series.size() * 4, 0}, {h.chunk_len_.data(), h.chunk_len_.size() * 4, 0},
       {h.chunk_flags.data(), h.chunk_flags.size(), 0},       {h.chunk_tile.data(), h.chunk_tile.size() * 4, 0},
       {h.chunk_start.data(), h.chunk_start.size() * 8, 0},   {h.tile_first.data(), h.tile_first.size() * 4, 0},
@@ -1274,7 +1274,7 @@ int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_
       {h.tile_flags.data(), h.tile_flags.size(), 0},         {h.series_first_tile.data(), h.series_first_tile.size() * 4, 0},
       {h.series_ntiles.data(), h.series_ntiles.size() * 4, 0}, {h.series_first_chunk.data(), h.series_first_chunk.size() * 4, 0},
       {h.series_nchunks.data(), h.series_nchunks.size() * 4, 0}, {h.group_tiles.data(), h.group_tiles.size() * 4, 0},
-      {h.group_w.data(), h.group_w.size(), 0}};
+      {h.group_w.data(), h.group_w.size(), 0},               {h.wave_maxlen.data(), h.wave_maxlen.size() * 4, 0}};
   size_t total = 0;
   for (auto& pc : pcs) {
     pc.off = total;
@@ -1320,6 +1320,7 @@ int bn_plan_create(const int64_t* offsets_host, int32_t n_series, int32_t chunk_
   v.n_groups = (int32_t)h.group_w.size();
   v.group_tiles = (const int32_t*)(d + pcs[13].off);
   v.group_w = (const uint8_t*)(d + pcs[14].off);
+  v.wave_maxlen = (const int32_t*)(d + pcs[15].off);
   *out = p;
   return BN_OK;
 }

@@ -26,6 +26,9 @@ struct PlanHost {
   // chunks.  Tiles of a one-chunk series need no scan and are in no group.
   std::vector<int32_t> group_tiles;
   std::vector<uint8_t> group_w;
+  // Longest chunk among each 64 consecutive chunks (the lanes of one wavefront of the
+  // lane-per-chunk kernels): the backward walks run their steps in step with it.
+  std::vector<int32_t> wave_maxlen;
 };
 
 constexpr int kGroupMax = 8;
@@ -101,6 +104,9 @@ inline bool build_plan(const int64_t* offsets, int32_t n_series, int32_t chunk_l
   p.n_chunks = (int32_t)p.chunk_series.size();
   p.n_tiles = (int32_t)p.tile_first.size();
   build_scan_groups(p);
+  p.wave_maxlen.assign((size_t)(p.n_chunks + 63) / 64, 0);
+  for (int32_t j = 0; j < p.n_chunks; ++j)
+    p.wave_maxlen[(size_t)(j >> 6)] = std::max(p.wave_maxlen[(size_t)(j >> 6)], p.chunk_len_[(size_t)j]);
   // Empty series inside a batch are fine (no chunk, log-likelihood 0); a batch
   // with no step at all would launch empty grids, so it is refused here.
   if (p.n_chunks == 0) { err = "offsets describe no steps (every series is empty)"; return false; }

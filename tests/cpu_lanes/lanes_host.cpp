@@ -66,6 +66,7 @@ PlanView view_of(const PlanHost& p) {
   v.n_groups = (int32_t)p.group_w.size();
   v.group_tiles = p.group_tiles.data();
   v.group_w = p.group_w.data();
+  v.wave_maxlen = p.wave_maxlen.data();
   return v;
 }
 
@@ -408,6 +409,13 @@ int main() {
         gok = in_group[(size_t)t] == (one ? 0 : 1);
       }
       check(gok, "scan groups cover every tile once (chunk_len " + std::to_string(cl) + ")");
+      bool wok = p.wave_maxlen.size() == (size_t)(p.n_chunks + 63) / 64;
+      for (int32_t j = 0; wok && j < p.n_chunks; ++j) {
+        int32_t mx = 0;
+        for (int32_t u = j & ~63; u < std::min(p.n_chunks, (j & ~63) + 64); ++u) mx = std::max(mx, p.chunk_len_[u]);
+        wok = p.wave_maxlen[(size_t)(j >> 6)] == mx;
+      }
+      check(wok, "wave_maxlen is the longest chunk of each 64 (chunk_len " + std::to_string(cl) + ")");
     }
     // many short series pack eight tiles to a group, long ones one
     {
