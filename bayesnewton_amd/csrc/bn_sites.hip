@@ -324,10 +324,12 @@ int check_lik(const bn_lik_desc* d, LikParams& L) { return make_lik_checked(d, L
 // with Z clipped at 1e-8).  Conditional moments (likelihoods.py): Bernoulli (p, p - p^2)
 // with the jittered probit link, Poisson (binsize e^f, binsize e^f), Student-t
 // (f, scale^2 df / (df - 2)).
+template <int KIND>
 __global__ void k_lik_predict(LikParams L, GHTable gh, int64_t n, const double* __restrict__ mean_f,
                               const double* __restrict__ var_f, const double* __restrict__ y,
                               double* __restrict__ mean_y, double* __restrict__ var_y,
                               double* __restrict__ log_density) {
+  L.kind = KIND;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = mean_f[i], v = var_f[i];
@@ -511,9 +513,13 @@ int bn_lik_predict(const bn_lik_desc* lik, int32_t gh_points, int64_t n, const d
   if (n < 0) return fail(BN_EINVAL, "n must be >= 0");
   if (n > 0 && (!mean_f || !var_f)) return fail(BN_EINVAL, "NULL mean_f / var_f");
   if (n > 0 && log_density && !y) return fail(BN_EINVAL, "log_density needs y");
-  if (n > 0)
-    hipLaunchKernelGGL(k_lik_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, L, gh, n,
-                       mean_f, var_f, y, mean_y, var_y, log_density);
+  if (n > 0) {
+    if (int rc = with_kind(L.kind, [&](auto kd) {
+          hipLaunchKernelGGL((k_lik_predict<decltype(kd)::value>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                             (hipStream_t)stream, L, gh, n, mean_f, var_f, y, mean_y, var_y, log_density);
+        }))
+      return rc;
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(BN_EHIP, std::string("bn_lik_predict: ") + hipGetErrorString(e));
   return BN_OK;
