@@ -289,11 +289,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv
 // The chunk elements e = (M, O, f, c) of a series compose associatively (chunk order):
 //   (n o a) = (Mn Ma, Mn Oa + On, fa + Ma' fn, ca + cn + Oa' fn),
 // and the gradient is c of the whole series' composition (its incoming tangent is 0).
-// Two levels: k_grad_group composes groups of kGradGroup consecutive chunks of a series
-// in parallel (one workgroup per group, the result replaces the group's first chunk),
-// k_grad_total composes a series' group elements.  Layout of e: M column-major (NZ*NZ),
-// O (NH columns of NZ), f (NZ), c (NH); field q of chunk j at agg[q * stride + j].
-constexpr int kGradGroup = 64;
+// A tree of groups: each k_grad_group launch composes groups of kGradGroup consecutive
+// elements of a level (elements `span` chunks apart; one workgroup per group, the result
+// replaces the group's first element), the next level's span is kGradGroup times larger,
+// and k_grad_total composes the at most kGradGroup elements of the last level.  The
+// dependent chain of a long series is then kGradGroup compositions per level, not the
+// series' chunk count / 64.  Layout of e: M column-major (NZ*NZ), O (NH columns of NZ),
+// f (NZ), c (NH); field q of chunk j at agg[q * stride + j].
+constexpr int kGradGroup = 16;
 
 template <int NZ, int NH>
 __device__ void compose_elem(const double* __restrict__ a, const double* __restrict__ n, double* __restrict__ out) {
@@ -359,24 +362,25 @@ __device__ const double* compose_run(const double* __restrict__ agg, int64_t str
 template <int K1, int K2>
 __global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_grad_group(PlanView pl,
                                                                                       double* __restrict__ agg,
-                                                                                      int64_t stride) {
+                                                                                      int64_t stride, int32_t span) {
   using GD = GradDims<K1, K2>;
   constexpr int NT = GD::NF <= 128 ? 128 : 256;
   __shared__ double buf[3 * GD::NF];
   const int32_t b = blockIdx.x;
-  const int32_t first = blockIdx.y * kGradGroup;
   const int32_t nc = pl.series_nchunks[b];
-  if (first >= nc) return;
-  const int64_t j0 = (int64_t)pl.series_first_chunk[b] + first;
-  const int32_t cnt = min(kGradGroup, nc - first);
-  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, j0, 1, cnt, buf);
+  const int32_t ne = (nc + span - 1) / span;  // elements of this level in the series
+  const int32_t first = blockIdx.y * kGradGroup;
+  if (first >= ne) return;
+  const int64_t j0 = (int64_t)pl.series_first_chunk[b] + (int64_t)first * span;
+  const int32_t cnt = min(kGradGroup, ne - first);
+  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, j0, span, cnt, buf);
   for (int e = threadIdx.x; e < GD::NF; e += NT) agg[(int64_t)e * stride + j0] = ac[e];
 }
 
 template <int K1, int K2>
 __global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_grad_total(PlanView pl,
                                                                                       const double* __restrict__ agg,
-                                                                                      int64_t stride,
+                                                                                      int64_t stride, int32_t span,
                                                                                       double* __restrict__ grad) {
   using GD = GradDims<K1, K2>;
   constexpr int NT = GD::NF <= 128 ? 128 : 256;
@@ -388,8 +392,8 @@ __global__ __launch_bounds__((GradDims<K1, K2>::NF <= 128 ? 128 : 256)) void k_g
     if ((int)threadIdx.x < GD::NH) grad[(int64_t)b * GD::NH + threadIdx.x] = 0.0;
     return;
   }
-  const int32_t ng = (nc + kGradGroup - 1) / kGradGroup;
-  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, pl.series_first_chunk[b], kGradGroup, ng, buf);
+  const int32_t ng = (nc + span - 1) / span;  // elements of the last level
+  const double* ac = compose_run<GD::NZ, GD::NH, NT>(agg, stride, pl.series_first_chunk[b], span, ng, buf);
   if ((int)threadIdx.x < GD::NH) grad[(int64_t)b * GD::NH + threadIdx.x] = ac[FC + threadIdx.x];
 }
 
@@ -874,11 +878,16 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   constexpr int NT = GradDims<K1, K2>::NF <= 128 ? 128 : 256;
   int32_t max_nc = 1;
   for (int32_t v : p.series_nchunks) max_nc = std::max(max_nc, v);
-  const unsigned max_groups = (unsigned)((max_nc + kGradGroup - 1) / kGradGroup);
-  hipLaunchKernelGGL((k_grad_group<K1, K2>), dim3((unsigned)p.n_series, max_groups), dim3(NT), 0, st, plan->view,
-                     (double*)gws, (int64_t)p.n_chunks_pad);
+  int32_t span = 1;  // chunks between consecutive elements of the current level
+  while ((max_nc + span - 1) / span > kGradGroup) {
+    const int32_t ne = (max_nc + span - 1) / span;
+    const unsigned groups = (unsigned)((ne + kGradGroup - 1) / kGradGroup);
+    hipLaunchKernelGGL((k_grad_group<K1, K2>), dim3((unsigned)p.n_series, groups), dim3(NT), 0, st, plan->view,
+                       (double*)gws, (int64_t)p.n_chunks_pad, span);
+    span *= kGradGroup;
+  }
   hipLaunchKernelGGL((k_grad_total<K1, K2>), dim3((unsigned)p.n_series), dim3(NT), 0, st, plan->view,
-                     (const double*)gws, (int64_t)p.n_chunks_pad, grad);
+                     (const double*)gws, (int64_t)p.n_chunks_pad, span, grad);
   hipLaunchKernelGGL(k_series_sum, dim3(p.n_series), dim3(kBlock), 0, st, plan->view, b.ell, loglik, nullptr,
                        (int32_t*)nullptr);
   BN_HIP_CHECK(hipGetLastError());

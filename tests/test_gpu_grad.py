@@ -183,8 +183,10 @@ def test_autograd_function_and_adam_steps():
 
 def test_batched_gradients_match_single_series():
     """Per-series energies and gradients of a batch of uneven series (1 to 2,100 steps,
-    16-step chunks: one to three 64-chunk groups per series) equal those of each series
-    run alone, whose gradient the finite-difference tests above pin."""
+    16-step chunks: one to 132 chunks, composed in one or two levels of 16-element groups)
+    equal those of each series run alone, whose gradient the finite-difference tests above
+    pin.  The longest series at 4-step chunks (525 chunks: three levels, the last one
+    ragged) has the same gradient as at 16-step chunks."""
     from bayesnewton_amd import kernels as K, likelihoods as L, models as M
     rng = np.random.default_rng(5)
     lens = [1, 50, 700, 2100, 33]
@@ -205,3 +207,11 @@ def test_batched_gradients_match_single_series():
         assert abs(float(Eb[i]) - float(E1[0])) <= 1e-9 * max(1.0, abs(float(E1[0]))), i
         np.testing.assert_allclose(gkb[i].cpu().numpy(), gk1[0].cpu().numpy(), rtol=1e-9, atol=1e-9)
         np.testing.assert_allclose(float(glb[i]), float(gl1[0]), rtol=1e-9, atol=1e-9)
+        if lens[i] == 2100:
+            g4 = M.MarkovVariationalGP(kern(), L.StudentsT(scale=0.25, df=3.0), x, y, chunk_len=4)
+            g4.inference(lr=1.0)
+            E4, gk4, gl4 = g4.energy_and_grad_per_series()
+            print(f"[grad] 525 chunks: energy {float(E4[0])!r} vs {float(E1[0])!r}", flush=True)
+            assert abs(float(E4[0]) - float(E1[0])) <= 1e-9 * max(1.0, abs(float(E1[0])))
+            np.testing.assert_allclose(gk4[0].cpu().numpy(), gk1[0].cpu().numpy(), rtol=1e-8, atol=1e-9)
+            np.testing.assert_allclose(float(gl4[0]), float(gl1[0]), rtol=1e-8, atol=1e-9)
