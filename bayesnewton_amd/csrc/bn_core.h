@@ -499,6 +499,19 @@ BN_HD void combine_factor(const T Sig1[D][D], const T J2[D][D], T Z[D][D], T Y[D
     BN_UNROLL for (int j = 0; j < D; ++j) { Z[i][j] = (T)Zd[i][j]; Y[i][j] = (T)Yd[i][j]; }
 }
 
+// Sum of a[t], a[t + nt], a[t + 2 nt], ... below n: eight independent loads and partial
+// sums per round, so a thread striding over a long series waits one memory latency per
+// eight elements rather than per element.  For n <= t + nt the result is a[t] exactly.
+BN_HD double strided_sum(const double* __restrict__ a, int32_t n, int32_t t, int32_t nt) {
+  double acc[8];
+  BN_UNROLL for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+  int32_t i = t;
+  for (; i + 7 * nt < n; i += 8 * nt)
+    BN_UNROLL for (int u = 0; u < 8; ++u) acc[u] += a[i + u * nt];
+  for (; i < n; i += nt) acc[0] += a[i];
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
 // Scheduling fence between the phases of a long unrolled body: keeps the compiler from
 // hoisting a later phase's loads above it (which lengthens their live ranges).
 BN_HD void sched_fence() {

@@ -18,11 +18,22 @@
 
 namespace bn {
 
+// lam^p for a small integer p (|p| < 2K), ilam = 1 / lam: a product chain (p is a
+// constant once the callers' loops unroll) instead of a pow call per matrix entry.
+BN_HD double int_pow(double lam, double ilam, int p) {
+  const double f = p < 0 ? ilam : lam;
+  const int n = p < 0 ? -p : p;
+  double r = 1.0;
+  for (int q = 0; q < n; ++q) r *= f;
+  return r;
+}
+
 // dA/dlambda of one component: A_ij = lam^(i-j) e^{-x} a_ij(x), x = lam dt, so
 //   dA_ij/dlam = (i-j) lam^(i-j-1) e^{-x} a_ij + lam^(i-j) e^{-x} dt (a_ij' - a_ij)
+// e = exp(-x) from the caller.
 template <int K>
-BN_HD void comp_dA_dlam(double lam, double dt, double* dA) {
-  const double x = lam * dt, e = exp(-x);
+BN_HD void comp_dA_dlam(double lam, double ilam, double dt, double e, double* dA) {
+  const double x = lam * dt;
   double a[9], da[9];
   if (K == 1) {
     a[0] = 1.0; da[0] = 0.0;
@@ -37,26 +48,26 @@ BN_HD void comp_dA_dlam(double lam, double dt, double* dA) {
     da[3] = -x;      da[4] = 1.0 - 2.0 * x; da[5] = 1.0 - x;
     da[6] = x - 1.0; da[7] = 2.0 * x - 3.0; da[8] = -2.0 + x;
   }
-  for (int i = 0; i < K; ++i)
-    for (int j = 0; j < K; ++j) {
+  BN_UNROLL for (int i = 0; i < K; ++i)
+    BN_UNROLL for (int j = 0; j < K; ++j) {
       const int p = i - j;
-      const double lp = pow(lam, (double)p);
-      dA[i * K + j] = (p != 0 ? p * lp / lam * e * a[i * K + j] : 0.0) + lp * e * dt * (da[i * K + j] - a[i * K + j]);
+      const double lp = int_pow(lam, ilam, p);
+      dA[i * K + j] = (p != 0 ? p * lp * ilam * e * a[i * K + j] : 0.0) + lp * e * dt * (da[i * K + j] - a[i * K + j]);
     }
 }
 
 // dQ/dlambda of one component (packed upper, K(K+1)/2 entries) from
 // Q_ij = var lam^(i+j) F_ij(x):  dQ_ij/dlam = var (i+j) lam^(i+j-1) F + var lam^(i+j) dt F'(x),
 // with F = x^m G(x) (series, x < X0) or C - e^{-2x} P(x) (above), as in
-// Model::comp_block.  Also returns Q itself.
+// Model::comp_block.  Also returns Q itself.  e = exp(-x) from the caller.
 template <int K>
-BN_HD void comp_dQ_dlam(double lam, double var, double dt, double* Qp, double* dQp) {
+BN_HD void comp_dQ_dlam(double lam, double ilam, double var, double dt, double e, double* Qp, double* dQp) {
   const double x = lam * dt;
   const bool small = x < kQX0;
-  const double e2 = exp(-2.0 * x);
+  const double e2 = e * e;
   int idx = 0;
-  for (int i = 0; i < K; ++i)
-    for (int j = i; j < K; ++j) {
+  BN_UNROLL for (int i = 0; i < K; ++i)
+    BN_UNROLL for (int j = i; j < K; ++j) {
       // both forms from wave-uniform coefficient rows, selected per lane (as in
       // Model::comp_block): no per-lane table choice, no dependent vector loads
       const double* ts = qtab<K>(idx, true);
@@ -77,9 +88,9 @@ BN_HD void comp_dQ_dlam(double lam, double var, double dt, double* Qp, double* d
       for (int q = 0; q < m - 1; ++q) xm1 *= x;
       const double F = small ? xm1 * x * vs : qc<K>(idx) - e2 * vp;
       const double dF = small ? xm1 * (m * vs + x * dvs) : e2 * (2.0 * vp - dvp);
-      const double lp = pow(lam, (double)(i + j));
+      const double lp = int_pow(lam, ilam, i + j);
       Qp[idx] = var * lp * F;
-      dQp[idx] = var * ((i + j) * lp / lam * F + lp * dt * dF);
+      dQp[idx] = var * ((i + j) * lp * ilam * F + lp * dt * dF);
       ++idx;
     }
 }
@@ -198,8 +209,9 @@ template <int K, int O, int D>
 BN_HD void comp_step_col(double lam, double var, double ls, double dt, const double* Pinf_full, int kind,
                          double A[D][D], double Q[D][D], double* dA, double* dQ, double* dPinf) {
   double a[9], q[6], dq[6];
-  comp_transition<K>(lam, dt, a);
-  comp_dQ_dlam<K>(lam, var, dt, q, dq);
+  const double e = exp(-(lam * dt)), ilam = 1.0 / lam;
+  comp_transition_e<K>(lam, dt, e, a);
+  comp_dQ_dlam<K>(lam, ilam, var, dt, e, q, dq);
   const double dlam_dls = -lam / ls;
   int idx = 0;
   for (int i = 0; i < K; ++i)
@@ -212,14 +224,14 @@ BN_HD void comp_step_col(double lam, double var, double ls, double dt, const dou
       ++idx;
     }
   double da[9];
-  if (kind == 0) comp_dA_dlam<K>(lam, dt, da);
+  if (kind == 0) comp_dA_dlam<K>(lam, ilam, dt, e, da);
   for (int i = 0; i < K; ++i)
     for (int j = 0; j < K; ++j) {
       A[O + i][O + j] = a[i * K + j];
       if (kind >= 0) {
         const double p = Pinf_full[(O + i) * D + O + j];
         dA[(O + i) * D + O + j] = kind == 0 ? da[i * K + j] * dlam_dls : 0.0;
-        dPinf[(O + i) * D + O + j] = kind == 0 ? (i + j) * p / lam * dlam_dls : p / var;
+        dPinf[(O + i) * D + O + j] = kind == 0 ? (i + j) * p * ilam * dlam_dls : p / var;
       }
     }
 }
@@ -308,7 +320,7 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
     for (int q = 0; q < D; ++q) if (S::h(q)) { sd += bb[q]; hm += mp[q]; }
     const double v = (double)y[s] - hm;
     const bool masked = mask != nullptr && mask[s];
-    if (!masked) ell += -0.5 * (v * v / sd + kLog2Pi + log(sd));
+    if (h == 0 && !masked) ell += -0.5 * (v * v / sd + kLog2Pi + log(sd));  // one column stores it
     acc += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, h >= 0 ? dA : nullptr, h >= 0 ? dQ : nullptr,
                                 h >= 0 ? dPinf : nullptr, z);
     // primal update

@@ -605,12 +605,10 @@ __global__ __launch_bounds__(kBlock) void k_series_sum(PlanView pl, const double
   const int32_t b = blockIdx.x;
   const int32_t c0 = pl.series_first_chunk[b];
   const int32_t nc = pl.series_nchunks[b];
-  double s = 0.0;
+  const double s = strided_sum(per_chunk + c0, nc, threadIdx.x, kBlock);
   int32_t f = 0;
-  for (int32_t i = threadIdx.x; i < nc; i += kBlock) {
-    s += per_chunk[c0 + i];
-    if (chst != nullptr) f |= chst[c0 + i];
-  }
+  if (chst != nullptr)
+    for (int32_t i = threadIdx.x; i < nc; i += kBlock) f |= chst[c0 + i];
   red[threadIdx.x] = s;
   ored[threadIdx.x] = f;
   __syncthreads();

@@ -170,9 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_series_sum1(PlanView pl, const doubl
   const int32_t s = blockIdx.x;
   const int32_t c0 = pl.series_first_chunk[s];
   const int32_t nc = pl.series_nchunks[s];
-  double x = 0.0;
-  for (int32_t i = threadIdx.x; i < nc; i += kBlock) x += a[c0 + i];
-  ra[threadIdx.x] = x;
+  ra[threadIdx.x] = strided_sum(a + c0, nc, threadIdx.x, kBlock);
   __syncthreads();
   for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
     if ((int32_t)threadIdx.x < w) ra[threadIdx.x] += ra[threadIdx.x + w];
@@ -188,13 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_series_sum2(PlanView pl, const doubl
   const int32_t s = blockIdx.x;
   const int32_t c0 = pl.series_first_chunk[s];
   const int32_t nc = pl.series_nchunks[s];
-  double x = 0.0, y = 0.0;
-  for (int32_t i = threadIdx.x; i < nc; i += kBlock) {
-    x += a[c0 + i];
-    y += b[c0 + i];
-  }
-  ra[threadIdx.x] = x;
-  rb[threadIdx.x] = y;
+  ra[threadIdx.x] = strided_sum(a + c0, nc, threadIdx.x, kBlock);
+  rb[threadIdx.x] = strided_sum(b + c0, nc, threadIdx.x, kBlock);
   __syncthreads();
   for (int32_t w = kBlock / 2; w > 0; w >>= 1) {
     if ((int32_t)threadIdx.x < w) {

@@ -452,6 +452,23 @@ int main() {
                 std::isnan(log_fast(-INFINITY)) && std::isnan(log_fast(NAN)) && log_fast(1.0) == 0.0;
     check(worst <= 4.0 && same, "log_fast within 4 ulp of log, library values off the normal range");
   }
+  // strided_sum (bn_core.h): every element of a thread's stride counted once, at lengths
+  // around the 8-element rounds (integers, so every order is exact), and a[t] itself when
+  // the thread holds one element
+  {
+    std::vector<double> a(5000);
+    for (size_t i = 0; i < a.size(); ++i) a[i] = (double)((i * 7919) % 1009) - 500.0;
+    bool ok = true;
+    for (int32_t n : {0, 1, 255, 256, 257, 2047, 2048, 2049, 4999}) {
+      for (int32_t t : {0, 1, 100, 255}) {
+        double want = 0.0;
+        for (int32_t i = t; i < n; i += 256) want += a[i];
+        ok = ok && strided_sum(a.data(), n, t, 256) == want;
+      }
+    }
+    ok = ok && strided_sum(a.data(), 200, 7, 256) == a[7];
+    check(ok, "strided_sum counts each strided element once");
+  }
   std::printf(g_fail ? "LANES_HOST_FAILED %d\n" : "LANES_HOST_OK\n", g_fail);
   return g_fail ? 1 : 0;
 }
