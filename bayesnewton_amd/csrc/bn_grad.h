@@ -240,7 +240,10 @@ BN_HD void comp_step_col(double lam, double var, double ls, double dt, const dou
 // unit tangent (col < NZ: column col of M and entry col of f) or a hyperparameter
 // tangent started at zero (h = col - NZ: column h of O and entry h of c).  The columns
 // of a chunk are independent given the primal trajectory, which every lane recomputes.
-template <int K1, int K2>
+// HK is the column's class, fixed at compile time so that each instance carries only its
+// own source terms: -1 a unit tangent (no sources), 0 a lengthscale, 1 a variance
+// (h = 2 comp + HK).
+template <int K1, int K2, int HK>
 BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                          const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
                          CSoA fagg, CSoA fcarry, const GradOut& out, int32_t j, int col) {
@@ -252,13 +255,13 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
   const int32_t b = pl.chunk_series[j];
   const int32_t L = pl.chunk_len_[j];
   const bool first_chunk = (pl.chunk_flags[j] & CH_FIRST) != 0;
-  const int h = col - NZ;
-  if (h < 0 && first_chunk) {  // a series' first chunk has no incoming tangent: M = 0, f = 0
+  const int h = HK < 0 ? -1 : col - NZ;
+  if (HK < 0 && first_chunk) {  // a series' first chunk has no incoming tangent: M = 0, f = 0
     for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(col * NZ + q) * st + j] = 0.0;
     out.agg[(int64_t)(FF + col) * st + j] = 0.0;
     return;
   }
-  const int comp = h >= 0 ? h / 2 : -1, kind = h >= 0 ? h % 2 : -1;
+  const int comp = HK >= 0 ? (K2 > 0 ? h / 2 : 0) : -1;
   Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
   const int64_t ho = (int64_t)b * hv.stride;
   const double ls1 = hv.ls[ho], var1 = hv.var[ho];
@@ -283,9 +286,9 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
       for (int c = 0; c < D; ++c) { A[i][c] = 0.0; Q[i][c] = 0.0; }
     for (int q = 0; q < D * D; ++q) { dA[q] = 0.0; dQ[q] = 0.0; dPinf[q] = 0.0; }
     const double dts = first ? 0.0 : (double)dt[s];
-    comp_step_col<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, comp == 0 ? kind : -1, A, Q, dA, dQ, dPinf);
+    comp_step_col<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, comp == 0 ? HK : -1, A, Q, dA, dQ, dPinf);
     if (K2 > 0)
-      comp_step_col<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, comp == 1 ? kind : -1, A, Q, dA,
+      comp_step_col<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, comp == 1 ? HK : -1, A, Q, dA,
                                               dQ, dPinf);
     // primal prediction
     double mp[D], Pp[D][D];
@@ -321,14 +324,14 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
     const double v = (double)y[s] - hm;
     const bool masked = mask != nullptr && mask[s];
     if (h == 0 && !masked) ell += -0.5 * (v * v / sd + kLog2Pi + log(sd));  // one column stores it
-    acc += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, h >= 0 ? dA : nullptr, h >= 0 ? dQ : nullptr,
-                                h >= 0 ? dPinf : nullptr, z);
+    acc += tangent_step<K1, K2>(A, m, P, bb, sd, v, masked, first, HK >= 0 ? dA : nullptr,
+                                HK >= 0 ? dQ : nullptr, HK >= 0 ? dPinf : nullptr, z);
     // primal update
     for (int i = 0; i < D; ++i) m[i] = mp[i] + bb[i] * v / sd;
     for (int i = 0; i < D; ++i)
       for (int c = i; c < D; ++c) P[i][c] = P[c][i] = Pp[i][c] - bb[i] * bb[c] / sd;
   }
-  if (h < 0) {
+  if (HK < 0) {
     for (int q = 0; q < NZ; ++q) out.agg[(int64_t)(col * NZ + q) * st + j] = z[q];
     out.agg[(int64_t)(FF + col) * st + j] = acc;
   } else {

@@ -272,18 +272,23 @@ __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView 
   lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, chst, j);
 }
 
-template <int K1, int K2>
+// One launch per column class HK (lane_grad_col): the unit tangents (NZ columns), the
+// lengthscales, the variances (one column per kernel component each).
+template <int K1, int K2, int HK>
 __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                        const float* __restrict__ y, const float* __restrict__ r,
                                                        const uint8_t* __restrict__ mask, CSoA fagg, CSoA fcarry,
                                                        GradOut out) {
-  // lane l -> column l / stride of chunk l % stride: a wave covers consecutive chunks of one column
-  constexpr int NC = GradDims<K1, K2>::NZ + GradDims<K1, K2>::NH;
+  // lane l -> column u = l / stride of the class, chunk l % stride: a wave covers
+  // consecutive chunks of one column
+  using GD = GradDims<K1, K2>;
+  constexpr int NU = HK < 0 ? GD::NZ : GD::NH / 2;
   const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int col = (int)(l / out.stride);
-  const int32_t j = (int32_t)(l - (int64_t)col * out.stride);
-  if (col >= NC || j >= pl.n_chunks) return;
-  lane_grad_col<K1, K2>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, col);
+  const int u = (int)(l / out.stride);
+  const int32_t j = (int32_t)(l - (int64_t)u * out.stride);
+  if (u >= NU || j >= pl.n_chunks) return;
+  const int col = HK < 0 ? u : GD::NZ + 2 * u + HK;
+  lane_grad_col<K1, K2, HK>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, col);
 }
 
 // The chunk elements e = (M, O, f, c) of a series compose associatively (chunk order):
@@ -870,8 +875,12 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   Bufs b = ws_bufs(p, D, ws);
   launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
   GradOut go{(double*)gws, p.n_chunks_pad, b.ell};
-  constexpr int64_t NC = GradDims<K1, K2>::NZ + GradDims<K1, K2>::NH;
-  hipLaunchKernelGGL((k_grad_chunk<K1, K2>), dim3((unsigned)((NC * p.n_chunks_pad + kBlock - 1) / kBlock)),
+  constexpr int64_t NZ = GradDims<K1, K2>::NZ, NH2 = GradDims<K1, K2>::NH / 2;
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2, -1>), dim3((unsigned)((NZ * p.n_chunks_pad + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2, 0>), dim3((unsigned)((NH2 * p.n_chunks_pad + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2, 1>), dim3((unsigned)((NH2 * p.n_chunks_pad + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
   constexpr int NT = GradDims<K1, K2>::NF <= 128 ? 128 : 256;
   int32_t max_nc = 1;
