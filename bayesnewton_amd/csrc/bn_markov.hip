@@ -272,8 +272,8 @@ __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView 
   lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, chst, j);
 }
 
-// One launch per column class HK (lane_grad_col): the unit tangents (NZ columns), the
-// lengthscales, the variances (one column per kernel component each).
+// Two launches (column classes of lane_grad_col): the NZ unit tangents (HK = -1) and the
+// NH hyperparameter tangents (HK = 2).
 template <int K1, int K2, int HK>
 __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                        const float* __restrict__ y, const float* __restrict__ r,
@@ -282,12 +282,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv
   // lane l -> column u = l / stride of the class, chunk l % stride: a wave covers
   // consecutive chunks of one column
   using GD = GradDims<K1, K2>;
-  constexpr int NU = HK < 0 ? GD::NZ : GD::NH / 2;
+  constexpr int NU = HK < 0 ? GD::NZ : (HK == 2 ? GD::NH : GD::NH / 2);
   const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int u = (int)(l / out.stride);
   const int32_t j = (int32_t)(l - (int64_t)u * out.stride);
   if (u >= NU || j >= pl.n_chunks) return;
-  const int col = HK < 0 ? u : GD::NZ + 2 * u + HK;
+  const int col = HK < 0 ? u : (HK == 2 ? GD::NZ + u : GD::NZ + 2 * u + HK);
   lane_grad_col<K1, K2, HK>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, col);
 }
 
@@ -875,12 +875,10 @@ int run_loglik_grad(const bn_plan* plan, HyperView hv, const float* dt, const fl
   Bufs b = ws_bufs(p, D, ws);
   launch_filter_scan<K1, K2>(plan, hv, dt, y, r, b, st);
   GradOut go{(double*)gws, p.n_chunks_pad, b.ell};
-  constexpr int64_t NZ = GradDims<K1, K2>::NZ, NH2 = GradDims<K1, K2>::NH / 2;
+  constexpr int64_t NZ = GradDims<K1, K2>::NZ, NH = GradDims<K1, K2>::NH;
   hipLaunchKernelGGL((k_grad_chunk<K1, K2, -1>), dim3((unsigned)((NZ * p.n_chunks_pad + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
-  hipLaunchKernelGGL((k_grad_chunk<K1, K2, 0>), dim3((unsigned)((NH2 * p.n_chunks_pad + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
-  hipLaunchKernelGGL((k_grad_chunk<K1, K2, 1>), dim3((unsigned)((NH2 * p.n_chunks_pad + kBlock - 1) / kBlock)),
+  hipLaunchKernelGGL((k_grad_chunk<K1, K2, 2>), dim3((unsigned)((NH * p.n_chunks_pad + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, st, plan->view, hv, dt, y, r, mask, cs(b.fagg), cs(b.ftcarry), go);
   constexpr int NT = GradDims<K1, K2>::NF <= 128 ? 128 : 256;
   int32_t max_nc = 1;
