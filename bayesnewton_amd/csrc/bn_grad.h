@@ -242,7 +242,7 @@ BN_HD void comp_step_col(double lam, double var, double ls, double dt, const dou
 // of a chunk are independent given the primal trajectory, which every lane recomputes.
 // HK is the column's class, fixed at compile time so that each instance carries only its
 // own source terms: -1 a unit tangent (no sources), 0 a lengthscale, 1 a variance
-// (h = 2 comp + HK), 2 any hyperparameter (kind h % 2 at run time).
+// (h = 2 comp + HK).
 template <int K1, int K2, int HK>
 BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* __restrict__ dt,
                          const float* __restrict__ y, const float* __restrict__ r, const uint8_t* __restrict__ mask,
@@ -262,7 +262,6 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
     return;
   }
   const int comp = HK >= 0 ? (K2 > 0 ? h / 2 : 0) : -1;
-  const int kind = HK == 2 ? h % 2 : HK;
   Model<K1, K2> mdl = load_model<K1, K2>(hv, b);
   const int64_t ho = (int64_t)b * hv.stride;
   const double ls1 = hv.ls[ho], var1 = hv.var[ho];
@@ -287,9 +286,9 @@ BN_HD void lane_grad_col(const PlanView& pl, const HyperView& hv, const float* _
       for (int c = 0; c < D; ++c) { A[i][c] = 0.0; Q[i][c] = 0.0; }
     for (int q = 0; q < D * D; ++q) { dA[q] = 0.0; dQ[q] = 0.0; dPinf[q] = 0.0; }
     const double dts = first ? 0.0 : (double)dt[s];
-    comp_step_col<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, comp == 0 ? kind : -1, A, Q, dA, dQ, dPinf);
+    comp_step_col<K1, 0, D>(mdl.lam1, var1, ls1, dts, Pinf_full, comp == 0 ? HK : -1, A, Q, dA, dQ, dPinf);
     if (K2 > 0)
-      comp_step_col<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, comp == 1 ? kind : -1, A, Q, dA,
+      comp_step_col<(K2 > 0 ? K2 : 1), K1, D>(mdl.lam2, var2, ls2, dts, Pinf_full, comp == 1 ? HK : -1, A, Q, dA,
                                               dQ, dPinf);
     // primal prediction
     double mp[D], Pp[D][D];

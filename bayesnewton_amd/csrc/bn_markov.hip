@@ -272,8 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_smoother_mbf(PlanView pl, HyperView 
   lane_smoother_mbf<K1, K2, CT>(pl, hv, dt, y, r, sagg, scarry, fin, hrow, pmean, pvar, chst, j);
 }
 
-// Two launches (column classes of lane_grad_col): the NZ unit tangents (HK = -1) and the
-// NH hyperparameter tangents (HK = 2).
+// Two launches over the column classes of lane_grad_col: the NZ unit tangents (HK = -1)
+// and the NH hyperparameter tangents (HK = 2: each wave takes the lengthscale or the
+// variance instance, so both run in one launch and neither carries the other's terms).
 template <int K1, int K2, int HK>
 __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv, const float* __restrict__ dt,
                                                        const float* __restrict__ y, const float* __restrict__ r,
@@ -287,8 +288,13 @@ __global__ __launch_bounds__(kBlock) void k_grad_chunk(PlanView pl, HyperView hv
   const int u = (int)(l / out.stride);
   const int32_t j = (int32_t)(l - (int64_t)u * out.stride);
   if (u >= NU || j >= pl.n_chunks) return;
-  const int col = HK < 0 ? u : (HK == 2 ? GD::NZ + u : GD::NZ + 2 * u + HK);
-  lane_grad_col<K1, K2, HK>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, col);
+  if constexpr (HK < 0) {
+    lane_grad_col<K1, K2, -1>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, u);
+  } else if ((u & 1) == 0) {  // column NZ + u: lengthscale for even u, variance for odd
+    lane_grad_col<K1, K2, 0>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, GD::NZ + u);
+  } else {
+    lane_grad_col<K1, K2, 1>(pl, hv, dt, y, r, mask, fagg, fcarry, out, j, GD::NZ + u);
+  }
 }
 
 // The chunk elements e = (M, O, f, c) of a series compose associatively (chunk order):
