@@ -512,6 +512,17 @@ BN_HD double strided_sum(const double* __restrict__ a, int32_t n, int32_t t, int
   return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
+// Bitwise or of the same strided elements of a flag array, eight loads per round.
+BN_HD int32_t strided_or(const int32_t* __restrict__ a, int32_t n, int32_t t, int32_t nt) {
+  int32_t acc[8];
+  BN_UNROLL for (int u = 0; u < 8; ++u) acc[u] = 0;
+  int32_t i = t;
+  for (; i + 7 * nt < n; i += 8 * nt)
+    BN_UNROLL for (int u = 0; u < 8; ++u) acc[u] |= a[i + u * nt];
+  for (; i < n; i += nt) acc[0] |= a[i];
+  return (acc[0] | acc[1] | acc[2] | acc[3]) | (acc[4] | acc[5] | acc[6] | acc[7]);
+}
+
 // Scheduling fence between the phases of a long unrolled body: keeps the compiler from
 // hoisting a later phase's loads above it (which lengthens their live ranges).
 BN_HD void sched_fence() {

@@ -617,9 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_series_sum(PlanView pl, const double
   const int32_t c0 = pl.series_first_chunk[b];
   const int32_t nc = pl.series_nchunks[b];
   const double s = strided_sum(per_chunk + c0, nc, threadIdx.x, kBlock);
-  int32_t f = 0;
-  if (chst != nullptr)
-    for (int32_t i = threadIdx.x; i < nc; i += kBlock) f |= chst[c0 + i];
+  const int32_t f = chst != nullptr ? strided_or(chst + c0, nc, threadIdx.x, kBlock) : 0;
   red[threadIdx.x] = s;
   ored[threadIdx.x] = f;
   __syncthreads();

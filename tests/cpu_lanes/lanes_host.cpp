@@ -468,6 +468,18 @@ int main() {
     }
     ok = ok && strided_sum(a.data(), 200, 7, 256) == a[7];
     check(ok, "strided_sum counts each strided element once");
+    // strided_or: one set bit per element, so a dropped or extra element shows
+    std::vector<int32_t> fl(4000);
+    for (size_t i = 0; i < fl.size(); ++i) fl[i] = (int32_t)(1u << (i / 256 % 31));
+    bool oks = true;
+    for (int32_t n : {0, 1, 300, 2047, 2048, 2049, 3999}) {
+      for (int32_t t : {0, 5, 255}) {
+        int32_t want = 0;
+        for (int32_t i = t; i < n; i += 256) want |= fl[i];
+        oks = oks && strided_or(fl.data(), n, t, 256) == want;
+      }
+    }
+    check(oks, "strided_or covers each strided element");
   }
   std::printf(g_fail ? "LANES_HOST_FAILED %d\n" : "LANES_HOST_OK\n", g_fail);
   return g_fail ? 1 : 0;
